@@ -1,0 +1,286 @@
+"""ctypes binding of libe13.so, the gfx950 TNF-enumeration + simulate-and-filter
+library (C ABI in include/e13.h).
+
+The reference keeps this path inside src/e13search.c (run, explore,
+run_delete) and src/e13gpu.py (expand kernel and its host loop).  This module
+mirrors that interface for Python callers: `setup()` is setup()/set_sym(),
+`Context.search()` is run_delete(), `Context.simulate_batch()` is run_full()
+over many tables, and the survivor/summary formatting follows report() and
+the summary line.  Every call goes through the HIP library; there is no CPU
+fallback, and a missing library raises.
+"""
+import ctypes
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(HERE, "lib")
+LIB_PATH = os.path.join(LIB_DIR, "libe13.so")
+CLI_PATH = os.path.join(LIB_DIR, "e13search_gpu")
+CSRC = os.path.join(HERE, "csrc")
+
+NS = 9
+MAXCELLS = 32
+DEAD, SURV = -1, -2
+HALT, FORCED = -1, -2
+T14S = (0, 2, 22, 90, 268, 944, 3434, 13450, 52068, 208864)  # e13search.c:70
+
+ERRORS = {
+    0: "ok", -1: "invalid argument", -2: "HIP runtime error", -3: "out of memory",
+    -4: "forced cells exceed fmax", -5: "symmetric state is named by a fixed cell or has one",
+    -6: "start state in the symmetric states", -7: "node stack overflow", -8: "no HIP device",
+}
+
+
+class E13Error(RuntimeError):
+    def __init__(self, code, what=""):
+        self.code = code
+        super().__init__(f"{what}: {ERRORS.get(code, 'error')} ({code})" if what else ERRORS.get(code, str(code)))
+
+
+class Problem(ctypes.Structure):
+    _fields_ = [("nq", ctypes.c_int32), ("X", ctypes.c_int32), ("budget", ctypes.c_int32),
+                ("sym_lo", ctypes.c_int32), ("sym_hi", ctypes.c_int32), ("sym_n0", ctypes.c_int32),
+                ("nforced", ctypes.c_int32), ("tab", ctypes.c_int8 * MAXCELLS)]
+
+
+class Counts(ctypes.Structure):
+    _fields_ = [("leaves", ctypes.c_uint64), ("steps", ctypes.c_uint64), ("survivors", ctypes.c_uint64),
+                ("dead_at_stage", ctypes.c_uint64 * (NS + 1))]
+
+    def as_dict(self):
+        return {"leaves": self.leaves, "steps": self.steps, "survivors": self.survivors,
+                "hist": list(self.dead_at_stage)}
+
+
+class Survivor(ctypes.Structure):
+    _fields_ = [("tab", ctypes.c_int8 * MAXCELLS), ("read_mask", ctypes.c_uint32),
+                ("st0", ctypes.c_int32), ("head0", ctypes.c_int32), ("abs_ok", ctypes.c_int32),
+                ("rel_ok", ctypes.c_int32), ("steps", ctypes.c_int64), ("ch", ctypes.c_uint64 * 3),
+                ("recheck_ok", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
+class Leaf(ctypes.Structure):
+    _fields_ = [("result", ctypes.c_int32), ("stage", ctypes.c_int32), ("steps", ctypes.c_int32),
+                ("st0", ctypes.c_int32), ("head0", ctypes.c_int32), ("abs_ok", ctypes.c_int32),
+                ("rel_ok", ctypes.c_int32), ("event", ctypes.c_int32), ("ch", ctypes.c_uint64 * 3)]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("form", ctypes.c_int32), ("slice", ctypes.c_int32), ("wg_threads", ctypes.c_int32),
+                ("wgs_per_cu", ctypes.c_int32), ("cap", ctypes.c_int32), ("batch", ctypes.c_int32),
+                ("gate", ctypes.c_int32), ("refill_tries", ctypes.c_int32), ("stack_nodes", ctypes.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("kernel_ms", ctypes.c_double), ("launches", ctypes.c_int64), ("rounds", ctypes.c_int64),
+                ("lane_slots", ctypes.c_int64), ("peak_stack", ctypes.c_int64), ("slice", ctypes.c_int32),
+                ("wg_threads", ctypes.c_int32), ("grid", ctypes.c_int32), ("gate", ctypes.c_int32),
+                ("form", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
+EXPORTS = ("e13_open", "e13_close", "e13_strerror", "e13_set_tuning", "e13_get_tuning", "e13_get_stats",
+           "e13_keys", "e13_parse_machine", "e13_setup", "e13_search", "e13_simulate_batch",
+           "e13_simulate_batch_device", "e13_sync", "e13_recheck", "e13_format_survivor", "e13_format_hash",
+           "e13_synth_table")
+
+_lib = None
+
+
+def build(quiet=True):
+    """Compile libe13.so and the e13search_gpu command line for gfx950 (hipcc)."""
+    out = subprocess.run(["make", "-C", CSRC, "-j8"], capture_output=True, text=True)
+    if out.returncode != 0:
+        raise RuntimeError("building libe13.so failed:\n" + out.stdout[-4000:] + out.stderr[-4000:])
+    if not quiet:
+        print(out.stdout)
+
+
+def lib():
+    """The loaded libe13.so (raises when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it first (make -C {CSRC})")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    L.e13_open.argtypes = [ctypes.c_int, P(Tuning), P(ctypes.c_void_p)]
+    L.e13_close.argtypes = [ctypes.c_void_p]
+    L.e13_close.restype = None
+    L.e13_strerror.restype = ctypes.c_char_p
+    L.e13_set_tuning.argtypes = [ctypes.c_void_p, P(Tuning)]
+    L.e13_get_tuning.argtypes = [ctypes.c_void_p, P(Tuning)]
+    L.e13_get_stats.argtypes = [ctypes.c_void_p, P(Stats)]
+    L.e13_keys.argtypes = [P(ctypes.c_uint64), P(ctypes.c_uint64)]
+    L.e13_parse_machine.argtypes = [ctypes.c_char_p, P(ctypes.c_int32), ctypes.c_int]
+    L.e13_setup.argtypes = [P(ctypes.c_int32), ctypes.c_int, ctypes.c_int, ctypes.c_int, P(ctypes.c_uint8),
+                            ctypes.c_int, ctypes.c_int, P(Problem)]
+    L.e13_search.argtypes = [ctypes.c_void_p, P(Problem), P(Counts), P(Survivor), ctypes.c_uint64,
+                             P(ctypes.c_uint64)]
+    L.e13_simulate_batch.argtypes = [ctypes.c_void_p, P(ctypes.c_int8), ctypes.c_int, ctypes.c_int,
+                                     P(ctypes.c_int32), P(Leaf)]
+    L.e13_simulate_batch_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_void_p, ctypes.c_void_p]
+    L.e13_sync.argtypes = [ctypes.c_void_p]
+    L.e13_recheck.argtypes = [P(ctypes.c_int8), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                              ctypes.c_int, ctypes.c_int64, P(ctypes.c_uint32)]
+    L.e13_format_survivor.argtypes = [P(Problem), P(Survivor), ctypes.c_char_p, ctypes.c_int]
+    L.e13_format_hash.argtypes = [P(Problem), P(Survivor), ctypes.c_char_p, ctypes.c_int]
+    L.e13_synth_table.argtypes = [ctypes.c_uint64, ctypes.c_uint64, P(ctypes.c_int8), P(ctypes.c_int32)]
+    L.e13_synth_table.restype = None
+    _lib = L
+    return L
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise E13Error(rc, what)
+
+
+def keys():
+    """KEY[0..1363] and TGT[0..9] (e13search.c:437-444)."""
+    k = (ctypes.c_uint64 * 1364)()
+    t = (ctypes.c_uint64 * (NS + 1))()
+    lib().e13_keys(k, t)
+    return list(k), list(t)
+
+
+def parse_machine(text):
+    """Machine text -> list of codes (load/parse_cell, e13search.c:105-121)."""
+    base = (ctypes.c_int32 * 32)()
+    nq = lib().e13_parse_machine(text.encode(), base, 16)
+    if nq < 0:
+        raise E13Error(nq, "parse_machine")
+    return list(base[:2 * nq])
+
+
+def cell_index(name):
+    """'D0' -> 6 (cell = 2 * state + symbol, states A = 0 ...)."""
+    return 2 * (ord(name[0]) - ord("A")) + int(name[1])
+
+
+def setup(base, X, fmax=0, free_cells=None, sym=None):
+    """setup() + set_sym() (e13search.c:317-355, 423-426).  free_cells: names or
+    indices of base cells (free mode); sym: (LO, HI) base states or None."""
+    nq_base = len(base) // 2
+    b = (ctypes.c_int32 * 32)(*base)
+    fb = None
+    if free_cells is not None:
+        fb = (ctypes.c_uint8 * 32)()
+        for c in free_cells:
+            fb[cell_index(c) if isinstance(c, str) else int(c)] = 1
+    lo, hi = sym if sym else (-1, -1)
+    p = Problem()
+    rc = lib().e13_setup(b, nq_base, X, fmax, fb, lo, hi, ctypes.byref(p))
+    if rc != 0:
+        raise E13Error(rc, f"setup X={X}")
+    return p
+
+
+def format_survivor(p, s):
+    buf = ctypes.create_string_buffer(1024)
+    lib().e13_format_survivor(ctypes.byref(p), ctypes.byref(s), buf, 1024)
+    return buf.value.decode()
+
+
+def format_hash(p, s):
+    buf = ctypes.create_string_buffer(256)
+    lib().e13_format_hash(ctypes.byref(p), ctypes.byref(s), buf, 256)
+    return buf.value.decode()
+
+
+def summary_line(p, counts):
+    """The parity fields of the reference summary (e13search.c:413-416)."""
+    return (f"X={p.X} budget={p.budget} leaves={counts.leaves} survivors={counts.survivors} "
+            f"steps={counts.steps} dead-at-stage(0..9): " + " ".join(str(x) for x in counts.dead_at_stage))
+
+
+def synth_table(seed0, k):
+    t = (ctypes.c_int8 * 28)()
+    cap = ctypes.c_int32()
+    lib().e13_synth_table(seed0, k, t, ctypes.byref(cap))
+    return list(t), cap.value
+
+
+def recheck(tab, nq, st0, head0, abs_ok, rel_ok, steps):
+    t = (ctypes.c_int8 * 32)(*([int(x) for x in tab] + [HALT] * (32 - len(tab))))
+    m = ctypes.c_uint32()
+    ok = lib().e13_recheck(t, nq, st0, head0, abs_ok, rel_ok, steps, ctypes.byref(m))
+    return ok, m.value
+
+
+class Context:
+    """One HIP device context (e13_open/e13_close)."""
+
+    def __init__(self, device=0, **tuning):
+        self._h = ctypes.c_void_p()
+        t = Tuning(**tuning)
+        _check(lib().e13_open(device, ctypes.byref(t), ctypes.byref(self._h)), "e13_open")
+
+    def close(self):
+        if self._h:
+            lib().e13_close(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_tuning(self, **kw):
+        _check(lib().e13_set_tuning(self._h, ctypes.byref(Tuning(**kw))), "e13_set_tuning")
+
+    def tuning(self):
+        t = Tuning()
+        _check(lib().e13_get_tuning(self._h, ctypes.byref(t)), "e13_get_tuning")
+        return t.as_dict()
+
+    def stats(self):
+        s = Stats()
+        _check(lib().e13_get_stats(self._h, ctypes.byref(s)), "e13_get_stats")
+        return s.as_dict()
+
+    def search(self, p, surv_cap=200000):
+        """run_delete() for one problem: (Counts, [Survivor], n_survivors)."""
+        c = Counts()
+        sv = (Survivor * surv_cap)()
+        ns = ctypes.c_uint64()
+        _check(lib().e13_search(self._h, ctypes.byref(p), ctypes.byref(c), sv, surv_cap, ctypes.byref(ns)),
+               f"e13_search X={p.X}")
+        return c, [sv[i] for i in range(min(ns.value, surv_cap))], ns.value
+
+    def simulate_batch(self, tabs, caps=None):
+        """run_full() on each table (list of equal-length code lists)."""
+        n = len(tabs)
+        ncells = len(tabs[0])
+        flat = (ctypes.c_int8 * (n * ncells))(*[int(v) for t in tabs for v in t])
+        cp = (ctypes.c_int32 * n)(*caps) if caps is not None else None
+        out = (Leaf * n)()
+        _check(lib().e13_simulate_batch(self._h, flat, ncells, n, cp, out), "e13_simulate_batch")
+        return out
+
+    def simulate_batch_device(self, tabs_ptr, nq, n, caps_ptr, out_ptr):
+        _check(lib().e13_simulate_batch_device(self._h, ctypes.c_void_p(tabs_ptr), nq, n,
+                                               ctypes.c_void_p(caps_ptr) if caps_ptr else None,
+                                               ctypes.c_void_p(out_ptr)), "e13_simulate_batch_device")
+
+    def sync(self):
+        _check(lib().e13_sync(self._h), "e13_sync")

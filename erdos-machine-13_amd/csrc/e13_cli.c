@@ -1,0 +1,225 @@
+/* e13_cli.c -- e13search_gpu: the reference's command line over libe13.so.
+ *
+ *   e13search_gpu base    machine.txt
+ *   e13search_gpu mutant  machine.txt
+ *   e13search_gpu delete  machine.txt FMAX X...
+ *   e13search_gpu free    machine.txt X CELL...
+ *   e13search_gpu freesym machine.txt X LO HI CELL...
+ *
+ * Argument meaning, stdout/stderr split and line formats follow
+ * src/e13search.c main() (:435-472), report() (:184-200) and the summary at
+ * :413-416; the GPU summary follows src/e13gpu.py:456-459.  After each
+ * SURVIVOR line comes a HASH line with the counter hashes, and after each
+ * summary a digest line (order-independent over the survivor and hash lines).
+ * Survivors whose plain re-check disagrees are flagged on stderr.
+ *
+ * Environment: E13_FORM (1 plain, 2 batched), E13_SLICE, E13_CAP, E13_BATCH,
+ * E13_GATE (1 count, 2 hash), E13_WG, E13_WGS_PER_CU, E13_DEVICE.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "e13.h"
+
+static double now_s(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+static int env_int(const char *name) {
+  const char *v = getenv(name);
+  return v ? atoi(v) : 0;
+}
+
+static uint64_t mix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ULL;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+  return x ^ (x >> 31);
+}
+
+static uint64_t text_hash(const char *s) {
+  uint64_t h = 0xcbf29ce484222325ULL;
+  for (; *s; s++) h = (h ^ (uint8_t)*s) * 0x100000001b3ULL;
+  return mix64(h);
+}
+
+static char *read_file(const char *path) {
+  FILE *f = fopen(path, "rb");
+  if (!f) return NULL;
+  fseek(f, 0, SEEK_END);
+  long n = ftell(f);
+  fseek(f, 0, SEEK_SET);
+  char *b = (char *)malloc((size_t)n + 1);
+  if (!b) {
+    fclose(f);
+    return NULL;
+  }
+  size_t got = fread(b, 1, (size_t)n, f);
+  b[got] = 0;
+  fclose(f);
+  return b;
+}
+
+static int run_one(e13_ctx *ctx, const int32_t *base, int nq_base, int X, int fmax, const uint8_t *freebase,
+                   int lo, int hi) {
+  e13_problem p;
+  int rc = e13_setup(base, nq_base, X, fmax, freebase, lo, hi, &p);
+  if (rc == E13_EFORCED) {
+    fprintf(stderr, "X=%d forced=%d > fmax\n", X, p.nforced); /* e13search.c:353 */
+    return 0;
+  }
+  if (rc == E13_ESYM) {
+    fprintf(stderr, "symmetric state is named by a fixed cell or has one\n");
+    exit(1);
+  }
+  if (rc == E13_ESYMSTART) {
+    fprintf(stderr, "start state in the symmetric states\n");
+    exit(1);
+  }
+  if (rc) {
+    fprintf(stderr, "setup failed: %s\n", e13_strerror(rc));
+    exit(1);
+  }
+  const uint64_t cap = 200000;
+  e13_survivor *sv = (e13_survivor *)calloc(cap, sizeof *sv);
+  e13_counts cnt;
+  uint64_t ns = 0;
+  e13_stats st0, st1;
+  e13_get_stats(ctx, &st0);
+  double w0 = now_s();
+  rc = e13_search(ctx, &p, &cnt, sv, cap, &ns);
+  double wall = now_s() - w0;
+  if (rc) {
+    fprintf(stderr, "search failed: %s\n", e13_strerror(rc));
+    exit(1);
+  }
+  e13_get_stats(ctx, &st1);
+  uint64_t digest = 0;
+  char line[1024], hl[256];
+  for (uint64_t k = 0; k < ns && k < cap; k++) {
+    e13_format_survivor(&p, &sv[k], line, sizeof line);
+    e13_format_hash(&p, &sv[k], hl, sizeof hl);
+    printf("%s\n%s\n", line, hl);
+    if (!sv[k].recheck_ok) fprintf(stderr, "RECHECK FAILED: %s\n", line);
+    char l2[1100], h2[300];
+    snprintf(l2, sizeof l2, "%s\n", line);
+    snprintf(h2, sizeof h2, "%s\n", hl);
+    digest += text_hash(l2) ^ mix64(text_hash(h2));
+  }
+  fflush(stdout);
+  fprintf(stderr, "X=%d budget=%d gpu leaves=%llu survivors=%llu steps=%llu wall=%.1fs dead-at-stage(0..%d):", X,
+          p.budget, (unsigned long long)cnt.leaves, (unsigned long long)cnt.survivors,
+          (unsigned long long)cnt.steps, wall, E13_NS);
+  for (int j = 0; j <= E13_NS; j++) fprintf(stderr, " %llu", (unsigned long long)cnt.dead_at_stage[j]);
+  fprintf(stderr, "\n  launches=%lld peak stack=%lld nodes kernel=%.1fms %.2e steps/s\nX=%d digest=%016llx\n",
+          (long long)(st1.launches - st0.launches), (long long)st1.peak_stack, st1.kernel_ms - st0.kernel_ms,
+          wall > 0 ? (double)cnt.steps / wall : 0.0, X, (unsigned long long)digest);
+  if (ns > cap) fprintf(stderr, "WARNING: %llu survivors, only %llu printed\n", (unsigned long long)ns,
+                        (unsigned long long)cap);
+  fflush(stderr);
+  free(sv);
+  return 0;
+}
+
+int main(int argc, char **argv) {
+  if (argc < 3) {
+    fprintf(stderr, "usage: e13search_gpu base|mutant|delete|free|freesym machine.txt [...]\n");
+    return 1;
+  }
+  char *text = read_file(argv[2]);
+  if (!text) {
+    fprintf(stderr, "cannot open %s\n", argv[2]);
+    return 1;
+  }
+  int32_t base[32];
+  int nq = e13_parse_machine(text, base, 16);
+  free(text);
+  if (nq < 1) {
+    fprintf(stderr, "cannot parse %s\n", argv[2]);
+    return 1;
+  }
+  e13_tuning tu;
+  memset(&tu, 0, sizeof tu);
+  tu.form = env_int("E13_FORM");
+  tu.slice = env_int("E13_SLICE");
+  tu.cap = env_int("E13_CAP");
+  tu.batch = env_int("E13_BATCH");
+  tu.gate = env_int("E13_GATE");
+  tu.wg_threads = env_int("E13_WG");
+  tu.wgs_per_cu = env_int("E13_WGS_PER_CU");
+  e13_ctx *ctx = NULL;
+  int rc = e13_open(env_int("E13_DEVICE"), &tu, &ctx);
+  if (rc) {
+    fprintf(stderr, "e13_open: %s\n", e13_strerror(rc));
+    return 1;
+  }
+  const char *mode = argv[1];
+  if (!strcmp(mode, "base") || !strcmp(mode, "mutant")) {
+    int ncells = 2 * nq, n = 0;
+    int8_t *tabs = (int8_t *)malloc((size_t)ncells * (size_t)(ncells * (4 * nq + 1) + 1));
+    int *ci = (int *)malloc(sizeof(int) * (size_t)(ncells * (4 * nq + 1) + 1));
+    int *cv = (int *)malloc(sizeof(int) * (size_t)(ncells * (4 * nq + 1) + 1));
+    if (!strcmp(mode, "base")) {
+      for (int i = 0; i < ncells; i++) tabs[i] = (int8_t)base[i];
+      n = 1;
+    } else { /* every single-cell change (e13search.c:451-460) */
+      for (int i = 0; i < ncells; i++)
+        for (int v = -1; v < 4 * nq; v++) {
+          if (v == base[i]) continue;
+          for (int j = 0; j < ncells; j++) tabs[(size_t)n * ncells + j] = (int8_t)base[j];
+          tabs[(size_t)n * ncells + i] = (int8_t)v;
+          ci[n] = i;
+          cv[n] = v;
+          n++;
+        }
+    }
+    e13_leaf *lv = (e13_leaf *)calloc((size_t)n, sizeof *lv);
+    rc = e13_simulate_batch(ctx, tabs, ncells, n, NULL, lv);
+    if (rc) {
+      fprintf(stderr, "simulate: %s\n", e13_strerror(rc));
+      return 1;
+    }
+    if (!strcmp(mode, "base"))
+      printf("base states=%d result=%s stage=%d st0=%d head0=%d abs=%d rel=%d steps=%d\n", nq,
+             lv[0].result == E13_SURV ? "SURV" : "DEAD", lv[0].stage, lv[0].st0, lv[0].head0, lv[0].abs_ok,
+             lv[0].rel_ok, lv[0].steps);
+    else
+      for (int k = 0; k < n; k++) printf("%d %d %d %d\n", ci[k], cv[k], lv[k].stage, lv[k].result == E13_SURV);
+    free(lv);
+    free(tabs);
+    free(ci);
+    free(cv);
+  } else if (!strcmp(mode, "delete")) {
+    if (argc < 4) return 1;
+    int fmax = atoi(argv[3]);
+    for (int i = 4; i < argc; i++) run_one(ctx, base, nq, atoi(argv[i]), fmax, NULL, -1, -1);
+  } else if (!strcmp(mode, "free") || !strcmp(mode, "freesym")) {
+    if (argc < 4) return 1;
+    uint8_t fb[32] = {0};
+    int X = atoi(argv[3]), a0 = 4, lo = -1, hi = -1;
+    if (!strcmp(mode, "freesym")) {
+      if (argc < 6) return 1;
+      lo = atoi(argv[4]);
+      hi = atoi(argv[5]);
+      a0 = 6;
+    }
+    for (int i = a0; i < argc; i++) {
+      int q = argv[i][0] - 'A', s = argv[i][1] - '0';
+      if (q < 0 || q >= nq || (s != 0 && s != 1)) {
+        fprintf(stderr, "bad cell %s\n", argv[i]);
+        return 1;
+      }
+      fb[2 * q + s] = 1;
+    }
+    run_one(ctx, base, nq, X, 0, fb, lo, hi);
+  } else {
+    fprintf(stderr, "unknown mode %s\n", mode);
+    return 1;
+  }
+  e13_close(ctx);
+  return 0;
+}

@@ -1,0 +1,105 @@
+/* e13_dev.h -- records shared by the C host and the gfx950 kernels.
+ *
+ * Tape layout.  Bit q of a lane's tape holds cell q - 63, so the live window
+ * of the reference (head in -63..1298, e13search.c:173) is q in [0, 1361]
+ * and one unsigned compare checks it.  Cells -63..-3 (the counter, hashed
+ * into hc at e13search.c:170) are q 0..60; cells >= -2 (the block, hashed
+ * into hb) are q >= 61.  43 words of 32 bits hold the window.
+ *
+ * Table codes (one byte per cell, cell = 2*state + symbol):
+ *   0..59  a move, (next << 2) | (dir << 1) | write, dir 1 = R (e13search.c:72)
+ *   61     an unread cell that may still be edited (delete mode, budget > 0)
+ *   62     halt
+ *   63     a free cell not yet chosen (FORCED, e13search.c:66)
+ */
+#ifndef E13_DEV_H
+#define E13_DEV_H
+
+#include <stdint.h>
+
+#define E13_NS 9
+#define E13_TW 43           /* tape words per machine */
+#define E13_TW_PAD 44       /* words stored in a node record */
+#define E13_QMAX 1361       /* largest live q */
+#define E13_QOFF 63         /* q = cell + 63 */
+#define E13_QBLK 61         /* first q of the block region (cell -2) */
+#define E13_MAXCELLS 32
+#define E13_MAXNQ 15        /* states: codes must stay below 60 */
+
+#define E13_C_EDIT 61
+#define E13_C_HALT 62
+#define E13_C_FREE 63
+
+/* A stopped machine: the state right before the read of cell `pend`, or
+   (pend < 0) a machine paused by the per-launch step cap.  304 bytes. */
+typedef struct E13Node {
+  int32_t q, st, t, nt;            /* head (q), state, steps from the blank tape, next target stage */
+  int32_t stage, st0, q0, flags;   /* last accepted stage, stage-1 state and head; flags: bit0 abs_ok,
+                                      bit1 rel_ok, bits 2..4 cs[0..2] */
+  int32_t ones, budget, nintro, pend; /* ones at q >= 61, edits left, symmetric states in use, cell to bind */
+  int32_t front, depth, nbound, pad;  /* subtree id (cost logging), binding depth, cells bound so far */
+  uint64_t hb;                     /* block hash (hash-gate build only) */
+  uint64_t ch[3];                  /* counter hashes at the accepted stages */
+  uint8_t tab[E13_MAXCELLS];
+  uint32_t tape[E13_TW_PAD];
+} E13Node;
+
+/* Survivor as the kernel records it. */
+typedef struct E13SurvRec {
+  uint8_t tab[E13_MAXCELLS];
+  int32_t st0, head0, abs_ok, rel_ok, steps, pad;
+  uint64_t ch[3];
+} E13SurvRec;
+
+/* Result of one complete table run from the blank tape. */
+typedef struct E13LeafRec {
+  int32_t result; /* -1 dead, -2 survivor (e13search.c:64-65) */
+  int32_t stage, steps, st0, head0, abs_ok, rel_ok, pad;
+  uint64_t ch[3];
+} E13LeafRec;
+
+/* Device counters.  Totals are 64-bit; the per-launch work cursor and slot
+   counters are 32-bit. */
+typedef struct E13Counters {
+  uint64_t leaves, steps, survivors;
+  uint64_t hist[E13_NS + 1];
+  uint64_t lane_slots;  /* lanes x rounds executed (for lane utilisation) */
+  uint64_t rounds;      /* wave rounds */
+  uint32_t item;        /* next work item of the current launch */
+  uint32_t nout;        /* children written by the current launch */
+  uint32_t nsurv;       /* survivor slots taken (may exceed the capacity) */
+  uint32_t overflow;    /* set when a child did not fit the output buffer */
+  uint64_t pad[4];
+} E13Counters;
+
+/* Per-problem constants of the expansion kernel. */
+typedef struct E13Problem {
+  int32_t nq, nslot, X, sym_lo, sym_hi;
+  int32_t dl[E13_NS + 1];   /* per-stage deadlines, index 1..9 */
+  int8_t base[E13_MAXCELLS]; /* codes after setup(): move, -1 halt, -2 free */
+} E13Problem;
+
+typedef struct E13ExpandArgs {
+  const E13Node *in;
+  E13Node *out;
+  E13Counters *ctr;
+  E13SurvRec *surv;
+  const uint64_t *key;    /* Zobrist keys KEY[0..1363] */
+  const uint64_t *tgt;    /* TGT[0..9] */
+  const E13Problem *pb;   /* device copy of the problem constants */
+  uint32_t nin, out_cap, surv_cap, nslot;
+  int32_t slice, cap, refill_tries, nq;
+} E13ExpandArgs;
+
+typedef struct E13BatchArgs {
+  const uint8_t *tabs;    /* n x 32 codes (0..59 move, 62 halt) */
+  const int32_t *caps;    /* n caps replacing every stage deadline, or NULL */
+  E13LeafRec *out;
+  E13Counters *ctr;
+  const uint64_t *key;
+  const uint64_t *tgt;
+  const int32_t *dl;      /* device copy of the stage deadlines dl[0..9] */
+  int32_t n, nq, slice, pad;
+} E13BatchArgs;
+
+#endif

@@ -1,0 +1,716 @@
+/* e13_host.c -- C host of libe13.so: problem setup, the search driver around
+ * the gfx950 expansion kernel, complete-table batches, the plain survivor
+ * re-check and the reference's output lines.
+ *
+ * References: src/e13search.c (setup :317-355, set_sym :423-426, run_delete
+ * :357-420, report :184-200, run_full :428-433) and the GPU driver loop of
+ * src/e13gpu.py:381-437.
+ */
+#define __HIP_PLATFORM_AMD__ 1
+#include <hip/hip_runtime_api.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "e13.h"
+#include "e13_dev.h"
+
+/* Launch wrappers (e13_kernels.hip). */
+size_t e13k_lds_bytes(int wg_threads, int gate);
+int e13k_launch_expand(const E13ExpandArgs *a, int grid, int wg_threads, int gate, int refill, hipStream_t s);
+int e13k_launch_batch(const E13BatchArgs *a, int grid, int wg_threads, int gate, hipStream_t s);
+
+static const long T14S[E13_NS + 1] = {0, 2, 22, 90, 268, 944, 3434, 13450, 52068, 208864}; /* e13search.c:70 */
+
+struct e13_ctx {
+  int device, ncu;
+  size_t lds_max; /* largest dynamic LDS a workgroup may use */
+  hipStream_t stream;
+  hipEvent_t ev0, ev1;
+  e13_tuning tune;
+  e13_stats stats;
+  E13Counters *d_ctr;
+  uint64_t *d_key, *d_tgt;
+  E13Problem *d_pb;
+  int32_t *d_dl;
+  E13Node *d_stack;
+  size_t stack_cap;
+  E13Node *d_out;
+  size_t out_cap;
+  E13SurvRec *d_surv;
+  size_t surv_cap;
+  void *d_tabs, *d_caps, *d_leaves;
+  size_t batch_cap;
+};
+
+const char *e13_strerror(int code) {
+  switch (code) {
+  case E13_OK: return "ok";
+  case E13_EINVAL: return "invalid argument";
+  case E13_EHIP: return "HIP runtime error";
+  case E13_ENOMEM: return "out of memory";
+  case E13_EFORCED: return "forced cells exceed fmax";
+  case E13_ESYM: return "symmetric state is named by a fixed cell or has one";
+  case E13_ESYMSTART: return "start state in the symmetric states";
+  case E13_ESTACK: return "node stack overflow";
+  case E13_ENODEV: return "no HIP device";
+  default: return "unknown error";
+  }
+}
+
+#define HIPCHK(x)                                                                                                     \
+  do {                                                                                                                \
+    hipError_t e_ = (x);                                                                                              \
+    if (e_ != hipSuccess) {                                                                                           \
+      fprintf(stderr, "libe13: %s failed: %s (%s:%d)\n", #x, hipGetErrorString(e_), __FILE__, __LINE__);             \
+      return E13_EHIP;                                                                                                \
+    }                                                                                                                 \
+  } while (0)
+
+/* ------------------------------------------------------------------------ */
+/* Zobrist tables (e13search.c:98-103, 437-444).                             */
+/* ------------------------------------------------------------------------ */
+static uint64_t splitmix(uint64_t *s) {
+  uint64_t z = (*s += 0x9E3779B97F4A7C15ULL);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+int e13_keys(uint64_t *key, uint64_t *tgt) {
+  uint64_t seed = 20261028ULL;
+  for (int i = 0; i < 1364; i++) key[i] = splitmix(&seed);
+  tgt[0] = 0;
+  for (int n = 1; n <= E13_NS; n++) {
+    uint64_t h = 0;
+    for (int c = -1; c <= (1 << n) - 2; c++) h ^= key[c + 64];
+    tgt[n] = h;
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Machine text and setup.                                                   */
+/* ------------------------------------------------------------------------ */
+static int parse_cell(const char *s, int len) { /* e13search.c:105-108 */
+  if (len >= 1 && s[0] == '-') return E13_HALT;
+  if (len < 3 || (s[0] != '0' && s[0] != '1') || (s[1] != 'L' && s[1] != 'R')) return -100;
+  int n = 0;
+  for (int i = 2; i < len; i++) {
+    if (s[i] < '0' || s[i] > '9') return -100;
+    n = 10 * n + (s[i] - '0');
+  }
+  return (n << 2) | ((s[1] == 'R') << 1) | (s[0] - '0');
+}
+
+int e13_parse_machine(const char *text, int32_t *base, int max_states) {
+  int ncell = 0;
+  const char *p = text;
+  while (*p) {
+    while (*p == ' ' || *p == '\t' || *p == '\n' || *p == '\r') p++;
+    if (!*p) break;
+    const char *s = p;
+    while (*p && *p != ' ' && *p != '\t' && *p != '\n' && *p != '\r') p++;
+    int code = parse_cell(s, (int)(p - s));
+    if (code == -100) return E13_EINVAL;
+    if (ncell >= 2 * max_states) break;
+    base[ncell++] = code;
+  }
+  if (ncell % 2) return E13_EINVAL;
+  return ncell / 2;
+}
+
+int e13_setup(const int32_t *base, int nq_base, int X, int fmax, const uint8_t *freebase, int sym_lo, int sym_hi,
+              e13_problem *out) {
+  if (!base || !out || nq_base < 1 || nq_base > 16 || X < -1 || X >= nq_base) return E13_EINVAL;
+  memset(out, 0, sizeof *out);
+  int nforced = 0, row = 0;
+  for (int q = 0; q < nq_base; q++) {
+    if (q == X) continue;
+    for (int a = 0; a < 2; a++) {
+      int code = base[2 * q + a];
+      if (freebase && freebase[2 * q + a]) {
+        code = E13_FORCED;
+      } else if (code >= 0 && (code >> 2) == X) {
+        code = E13_FORCED;
+        nforced++;
+      } else if (code >= 0) {
+        int nx = code >> 2;
+        if (X >= 0 && nx > X) nx--;
+        code = (nx << 2) | (code & 3);
+      }
+      out->tab[2 * row + a] = (int8_t)code;
+    }
+    row++;
+  }
+  for (int i = 2 * row; i < E13_MAXCELLS; i++) out->tab[i] = E13_FORCED;
+  out->nq = row;
+  out->X = X;
+  out->nforced = nforced;
+  out->sym_lo = out->sym_hi = -1;
+  out->sym_n0 = 0;
+  if (sym_lo >= 0) { /* set_sym (e13search.c:423-426) + the checks at :336-351 */
+    int lo = sym_lo - (sym_lo > X), hi = sym_hi - (sym_hi >= X);
+    if (lo < 0 || hi < lo || hi >= row) return E13_EINVAL;
+    uint8_t named[16] = {0};
+    for (int i = 0; i < 2 * row; i++) {
+      int c = out->tab[i];
+      if (c >= 0 && (c >> 2) >= lo && (c >> 2) <= hi) named[(c >> 2) - lo] = 1;
+    }
+    int n0 = 0;
+    while (n0 <= hi - lo && named[n0]) n0++;
+    for (int q = lo + n0; q <= hi; q++)
+      if (named[q - lo] || out->tab[2 * q] != E13_FORCED || out->tab[2 * q + 1] != E13_FORCED) return E13_ESYM;
+    if (lo == 0) return E13_ESYMSTART;
+    out->sym_lo = lo;
+    out->sym_hi = hi;
+    out->sym_n0 = n0;
+  }
+  if (freebase) fmax = nforced; /* e13search.c:352 */
+  if (nforced > fmax) {
+    out->budget = -1;
+    return E13_EFORCED;
+  }
+  out->budget = fmax - nforced;
+  return E13_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Context.                                                                  */
+/* ------------------------------------------------------------------------ */
+static void tuning_defaults(e13_tuning *t) {
+  t->form = 2;
+  t->slice = 32;
+  t->wg_threads = 256;
+  t->wgs_per_cu = 2;
+  t->cap = 4096;
+  t->batch = 1 << 17;
+  t->gate = 1;
+  t->refill_tries = 2;
+  t->stack_nodes = 1LL << 25;
+}
+
+static void tuning_merge(e13_tuning *dst, const e13_tuning *src) {
+  if (!src) return;
+  if (src->form > 0) dst->form = src->form;
+  if (src->slice > 0) dst->slice = src->slice;
+  if (src->wg_threads > 0) dst->wg_threads = src->wg_threads;
+  if (src->wgs_per_cu > 0) dst->wgs_per_cu = src->wgs_per_cu;
+  if (src->cap > 0) dst->cap = src->cap;
+  if (src->batch > 0) dst->batch = src->batch;
+  if (src->gate > 0) dst->gate = src->gate;
+  if (src->refill_tries > 0) dst->refill_tries = src->refill_tries;
+  if (src->stack_nodes > 0) dst->stack_nodes = src->stack_nodes;
+}
+
+static int tuning_valid(const e13_tuning *t) {
+  if (t->form < 1 || t->form > 2) return 0;
+  if (t->slice < 1 || t->slice > 1 << 16) return 0;
+  if (t->wg_threads < 64 || t->wg_threads > 256 || t->wg_threads % 64) return 0;
+  if (t->wgs_per_cu < 1 || t->wgs_per_cu > 16) return 0;
+  if (t->cap < 1) return 0;
+  if (t->batch < 1 || t->batch > (1 << 22)) return 0;
+  if (t->gate < 1 || t->gate > 2) return 0;
+  if (t->refill_tries < 1 || t->refill_tries > 64) return 0;
+  if (t->stack_nodes < 1024) return 0;
+  return 1;
+}
+
+int e13_open(int device, const e13_tuning *t, e13_ctx **out) {
+  if (!out) return E13_EINVAL;
+  *out = NULL;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return E13_ENODEV;
+  if (device < 0 || device >= ndev) return E13_EINVAL;
+  e13_ctx *c = (e13_ctx *)calloc(1, sizeof *c);
+  if (!c) return E13_ENOMEM;
+  tuning_defaults(&c->tune);
+  tuning_merge(&c->tune, t);
+  if (!tuning_valid(&c->tune)) {
+    free(c);
+    return E13_EINVAL;
+  }
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess) {
+    free(c);
+    return E13_EHIP;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+    free(c);
+    return E13_EHIP;
+  }
+  c->ncu = prop.multiProcessorCount;
+  c->lds_max = prop.sharedMemPerBlock;
+  if (hipStreamCreate(&c->stream) != hipSuccess || hipEventCreate(&c->ev0) != hipSuccess ||
+      hipEventCreate(&c->ev1) != hipSuccess) {
+    free(c);
+    return E13_EHIP;
+  }
+  uint64_t key[1364], tgt[E13_NS + 1];
+  e13_keys(key, tgt);
+  int32_t dl[E13_NS + 1];
+  dl[0] = 0;
+  for (int n = 1; n <= E13_NS; n++) dl[n] = (int32_t)(4 * T14S[n] + 64); /* BUDGET_MULT*T14S[n]+64 */
+  if (hipMalloc((void **)&c->d_ctr, sizeof(E13Counters)) != hipSuccess ||
+      hipMalloc((void **)&c->d_key, sizeof key) != hipSuccess ||
+      hipMalloc((void **)&c->d_tgt, sizeof tgt) != hipSuccess ||
+      hipMalloc((void **)&c->d_pb, sizeof(E13Problem)) != hipSuccess ||
+      hipMalloc((void **)&c->d_dl, sizeof dl) != hipSuccess) {
+    e13_close(c);
+    return E13_ENOMEM;
+  }
+  if (hipMemcpy(c->d_key, key, sizeof key, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->d_tgt, tgt, sizeof tgt, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->d_dl, dl, sizeof dl, hipMemcpyHostToDevice) != hipSuccess) {
+    e13_close(c);
+    return E13_EHIP;
+  }
+  *out = c;
+  return E13_OK;
+}
+
+void e13_close(e13_ctx *c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  hipFree(c->d_ctr);
+  hipFree(c->d_key);
+  hipFree(c->d_tgt);
+  hipFree(c->d_pb);
+  hipFree(c->d_dl);
+  hipFree(c->d_stack);
+  hipFree(c->d_out);
+  hipFree(c->d_surv);
+  hipFree(c->d_tabs);
+  hipFree(c->d_caps);
+  hipFree(c->d_leaves);
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->stream) hipStreamDestroy(c->stream);
+  free(c);
+}
+
+int e13_set_tuning(e13_ctx *c, const e13_tuning *t) {
+  if (!c) return E13_EINVAL;
+  e13_tuning n = c->tune;
+  tuning_merge(&n, t);
+  if (!tuning_valid(&n)) return E13_EINVAL;
+  c->tune = n;
+  return E13_OK;
+}
+
+int e13_get_tuning(e13_ctx *c, e13_tuning *t) {
+  if (!c || !t) return E13_EINVAL;
+  *t = c->tune;
+  return E13_OK;
+}
+
+int e13_get_stats(e13_ctx *c, e13_stats *s) {
+  if (!c || !s) return E13_EINVAL;
+  *s = c->stats;
+  return E13_OK;
+}
+
+int e13_sync(e13_ctx *c) {
+  if (!c) return E13_EINVAL;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return E13_OK;
+}
+
+/* Workgroup size that fits the LDS limit (drops waves, never below one). */
+static int fit_wg(const e13_ctx *c, int wg, int gate_hash) {
+  while (wg > 64 && e13k_lds_bytes(wg, gate_hash) > c->lds_max) wg -= 64;
+  return wg;
+}
+
+static int ensure_buf(void **p, size_t *cap, size_t need, size_t elem) {
+  if (*cap >= need) return E13_OK;
+  hipFree(*p);
+  *p = NULL;
+  *cap = 0;
+  if (hipMalloc(p, need * elem) != hipSuccess) return E13_ENOMEM;
+  *cap = need;
+  return E13_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Search.                                                                   */
+/* ------------------------------------------------------------------------ */
+static void fill_problem(const e13_problem *p, E13Problem *pb) {
+  memset(pb, 0, sizeof *pb);
+  pb->nq = p->nq;
+  pb->nslot = 4 * p->nq + 1;
+  pb->X = p->X;
+  pb->sym_lo = p->sym_lo;
+  pb->sym_hi = p->sym_hi;
+  for (int n = 1; n <= E13_NS; n++) pb->dl[n] = (int32_t)(4 * T14S[n] + 64); /* BUDGET_MULT*T14S+64 */
+  for (int i = 0; i < E13_MAXCELLS; i++) pb->base[i] = i < 2 * p->nq ? p->tab[i] : E13_FORCED;
+}
+
+/* The blank-tape root (sim_init, e13search.c:123-129; init_root, e13gpu.py:276-283). */
+static void root_node(const e13_problem *p, E13Node *nd) {
+  memset(nd, 0, sizeof *nd);
+  nd->q = E13_QOFF;
+  nd->st = 0;
+  nd->t = 0;
+  nd->nt = 1;
+  nd->stage = 0;
+  nd->st0 = -1;
+  nd->q0 = E13_QOFF;
+  nd->flags = 3;
+  nd->budget = p->budget;
+  nd->nintro = p->sym_n0;
+  nd->pend = -1;
+  for (int i = 0; i < E13_MAXCELLS; i++) {
+    int c = i < 2 * p->nq ? p->tab[i] : E13_FORCED;
+    uint8_t code;
+    if (c == E13_FORCED) code = E13_C_FREE;
+    else if (p->budget > 0) code = E13_C_EDIT; /* every unread cell branches while edits are left */
+    else code = c < 0 ? E13_C_HALT : (uint8_t)c;
+    nd->tab[i] = code;
+  }
+}
+
+static int check_problem(const e13_problem *p) {
+  if (!p || p->nq < 1 || p->nq > E13_MAXNQ || p->budget < 0) return 0;
+  for (int i = 0; i < 2 * p->nq; i++) {
+    int c = p->tab[i];
+    if (c < E13_FORCED || c >= 4 * p->nq) return 0;
+  }
+  if (p->sym_lo >= 0 && (p->sym_hi < p->sym_lo || p->sym_hi >= p->nq)) return 0;
+  return 1;
+}
+
+int e13_search(e13_ctx *c, const e13_problem *p, e13_counts *out, e13_survivor *surv, uint64_t surv_cap,
+               uint64_t *n_surv) {
+  if (!c || !out || !check_problem(p)) return E13_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  const e13_tuning *tu = &c->tune;
+  const int nslot = 4 * p->nq + 1;
+  const size_t batch = (size_t)tu->batch;
+  int rc;
+  if ((rc = ensure_buf((void **)&c->d_stack, &c->stack_cap, (size_t)tu->stack_nodes, sizeof(E13Node))))
+    return rc;
+  if ((rc = ensure_buf((void **)&c->d_out, &c->out_cap, batch * (size_t)nslot, sizeof(E13Node)))) return rc;
+  size_t want_surv = surv_cap > 4096 ? (size_t)surv_cap : 4096;
+  if ((rc = ensure_buf((void **)&c->d_surv, &c->surv_cap, want_surv, sizeof(E13SurvRec)))) return rc;
+
+  E13ExpandArgs a;
+  memset(&a, 0, sizeof a);
+  E13Problem pb;
+  fill_problem(p, &pb);
+  HIPCHK(hipMemcpyAsync(c->d_pb, &pb, sizeof pb, hipMemcpyHostToDevice, c->stream));
+  a.pb = c->d_pb;
+  a.nslot = (uint32_t)nslot;
+  a.nq = p->nq;
+  a.out = c->d_out;
+  a.ctr = c->d_ctr;
+  a.surv = c->d_surv;
+  a.key = c->d_key;
+  a.tgt = c->d_tgt;
+  a.out_cap = (uint32_t)(batch * (size_t)nslot);
+  a.surv_cap = (uint32_t)c->surv_cap;
+  a.slice = tu->slice;
+  a.cap = tu->cap;
+  a.refill_tries = tu->refill_tries;
+
+  E13Node root;
+  root_node(p, &root);
+  HIPCHK(hipMemsetAsync(c->d_ctr, 0, sizeof(E13Counters), c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_stack, &root, sizeof root, hipMemcpyHostToDevice, c->stream));
+  size_t top = 1, peak = 1;
+  const int refill = tu->form == 2;
+  const int wg = fit_wg(c, tu->wg_threads, tu->gate == 2);
+  if (e13k_lds_bytes(wg, tu->gate == 2) > c->lds_max) return E13_EINVAL;
+  int grid_used = 0;
+  while (top > 0) {
+    size_t nin = top < batch ? top : batch;
+    size_t base_i = top - nin;
+    a.in = c->d_stack + base_i;
+    a.nin = (uint32_t)nin;
+    /* item cursor and child count restart at 0 each launch */
+    HIPCHK(hipMemsetAsync(&c->d_ctr->item, 0, 2 * sizeof(uint32_t), c->stream));
+    int grid;
+    if (refill) {
+      grid = c->ncu * tu->wgs_per_cu;
+    } else {
+      size_t items = nin * (size_t)nslot;
+      grid = (int)((items + (size_t)wg - 1) / (size_t)wg);
+    }
+    grid_used = grid;
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    int lr = e13k_launch_expand(&a, grid, wg, tu->gate == 2, refill, c->stream);
+    if (lr != 0) {
+      fprintf(stderr, "libe13: expand launch failed: %s\n", hipGetErrorString((hipError_t)lr));
+      return E13_EHIP;
+    }
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    uint32_t io[4];
+    HIPCHK(hipMemcpyAsync(io, &c->d_ctr->item, sizeof io, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->stats.kernel_ms += ms;
+    c->stats.launches++;
+    uint32_t nout = io[1];
+    if (io[3]) return E13_ESTACK; /* output buffer overflow (cannot happen: one child per item) */
+    if (base_i + nout > c->stack_cap) return E13_ESTACK;
+    if (nout)
+      HIPCHK(hipMemcpyAsync(c->d_stack + base_i, c->d_out, (size_t)nout * sizeof(E13Node), hipMemcpyDeviceToDevice,
+                            c->stream));
+    top = base_i + nout;
+    if (top > peak) peak = top;
+  }
+  E13Counters h;
+  HIPCHK(hipMemcpyAsync(&h, c->d_ctr, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  out->leaves = h.leaves;
+  out->steps = h.steps;
+  out->survivors = h.survivors;
+  for (int j = 0; j <= E13_NS; j++) out->dead_at_stage[j] = h.hist[j];
+  c->stats.rounds += (int64_t)h.rounds;
+  c->stats.lane_slots += (int64_t)h.lane_slots;
+  if ((int64_t)peak > c->stats.peak_stack) c->stats.peak_stack = (int64_t)peak;
+  c->stats.slice = tu->slice;
+  c->stats.wg_threads = wg;
+  c->stats.grid = grid_used;
+  c->stats.gate = tu->gate;
+  c->stats.form = tu->form;
+
+  uint64_t ns = h.nsurv;
+  if (n_surv) *n_surv = ns;
+  uint64_t keep = ns < c->surv_cap ? ns : c->surv_cap;
+  if (surv && surv_cap && keep) {
+    uint64_t nk = keep < surv_cap ? keep : surv_cap;
+    E13SurvRec *rec = (E13SurvRec *)malloc(nk * sizeof(E13SurvRec));
+    if (!rec) return E13_ENOMEM;
+    HIPCHK(hipMemcpy(rec, c->d_surv, nk * sizeof(E13SurvRec), hipMemcpyDeviceToHost));
+    for (uint64_t k = 0; k < nk; k++) {
+      e13_survivor *s = &surv[k];
+      memset(s, 0, sizeof *s);
+      for (int i = 0; i < E13_MAXCELLS; i++) {
+        int code = rec[k].tab[i];
+        int v;
+        if (i >= 2 * p->nq) v = E13_FORCED;
+        else if (code == E13_C_EDIT) v = p->tab[i]; /* never read: still the base value */
+        else if (code == E13_C_HALT) v = E13_HALT;
+        else if (code == E13_C_FREE) v = E13_FORCED;
+        else v = code;
+        s->tab[i] = (int8_t)v;
+      }
+      s->st0 = rec[k].st0;
+      s->head0 = rec[k].head0;
+      s->abs_ok = rec[k].abs_ok;
+      s->rel_ok = rec[k].rel_ok;
+      s->steps = rec[k].steps;
+      s->ch[0] = rec[k].ch[0];
+      s->ch[1] = rec[k].ch[1];
+      s->ch[2] = rec[k].ch[2];
+      uint32_t mask = 0;
+      s->recheck_ok = e13_recheck(s->tab, p->nq, s->st0, s->head0, s->abs_ok, s->rel_ok, s->steps, &mask);
+      s->read_mask = mask;
+    }
+    free(rec);
+  }
+  return E13_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Complete tables.                                                          */
+/* ------------------------------------------------------------------------ */
+static int batch_launch(e13_ctx *c, const void *tabs_dev, int nq, int n, const void *caps_dev, void *out_dev) {
+  E13BatchArgs a;
+  memset(&a, 0, sizeof a);
+  a.tabs = (const uint8_t *)tabs_dev;
+  a.caps = (const int32_t *)caps_dev;
+  a.out = (E13LeafRec *)out_dev;
+  a.ctr = c->d_ctr;
+  a.key = c->d_key;
+  a.tgt = c->d_tgt;
+  a.n = n;
+  a.nq = nq;
+  a.slice = c->tune.slice;
+  a.dl = c->d_dl;
+  HIPCHK(hipMemsetAsync(c->d_ctr, 0, sizeof(E13Counters), c->stream));
+  const int wg = fit_wg(c, c->tune.wg_threads, c->tune.gate == 2);
+  if (e13k_lds_bytes(wg, c->tune.gate == 2) > c->lds_max) return E13_EINVAL;
+  int grid = c->ncu * c->tune.wgs_per_cu;
+  long need = ((long)n + wg - 1) / wg;
+  if (need < grid) grid = (int)(need > 0 ? need : 1);
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  int lr = e13k_launch_batch(&a, grid, wg, c->tune.gate == 2, c->stream);
+  if (lr != 0) {
+    fprintf(stderr, "libe13: batch launch failed: %s\n", hipGetErrorString((hipError_t)lr));
+    return E13_EHIP;
+  }
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  c->stats.launches++;
+  c->stats.grid = grid;
+  c->stats.wg_threads = wg;
+  c->stats.slice = c->tune.slice;
+  c->stats.gate = c->tune.gate;
+  return E13_OK;
+}
+
+int e13_simulate_batch_device(e13_ctx *c, const void *tabs_dev, int nq, int n, const void *caps_dev, void *out_dev) {
+  if (!c || !tabs_dev || !out_dev || n < 1 || nq < 1 || nq > E13_MAXNQ) return E13_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  return batch_launch(c, tabs_dev, nq, n, caps_dev, out_dev);
+}
+
+int e13_simulate_batch(e13_ctx *c, const int8_t *tabs, int ncells, int n, const int32_t *caps, e13_leaf *out) {
+  if (!c || !tabs || !out || n < 1 || ncells < 2 || ncells % 2 || ncells > 2 * E13_MAXNQ) return E13_EINVAL;
+  const int nq = ncells / 2;
+  HIPCHK(hipSetDevice(c->device));
+  uint8_t *h = (uint8_t *)malloc((size_t)n * E13_MAXCELLS);
+  if (!h) return E13_ENOMEM;
+  for (int k = 0; k < n; k++) {
+    for (int i = 0; i < E13_MAXCELLS; i++) {
+      int v = i < ncells ? tabs[(size_t)k * ncells + i] : E13_HALT;
+      if (v < -1 || v >= 4 * nq) {
+        free(h);
+        return E13_EINVAL;
+      }
+      h[(size_t)k * E13_MAXCELLS + i] = v < 0 ? E13_C_HALT : (uint8_t)v;
+    }
+  }
+  int rc;
+  size_t cap_before = c->batch_cap;
+  if ((rc = ensure_buf(&c->d_tabs, &c->batch_cap, (size_t)n, E13_MAXCELLS))) {
+    free(h);
+    return rc;
+  }
+  if (c->batch_cap != cap_before || !c->d_caps || !c->d_leaves) {
+    hipFree(c->d_caps);
+    hipFree(c->d_leaves);
+    c->d_caps = c->d_leaves = NULL;
+    if (hipMalloc(&c->d_caps, c->batch_cap * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc(&c->d_leaves, c->batch_cap * sizeof(E13LeafRec)) != hipSuccess) {
+      free(h);
+      return E13_ENOMEM;
+    }
+  }
+  HIPCHK(hipMemcpyAsync(c->d_tabs, h, (size_t)n * E13_MAXCELLS, hipMemcpyHostToDevice, c->stream));
+  if (caps) HIPCHK(hipMemcpyAsync(c->d_caps, caps, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  rc = batch_launch(c, c->d_tabs, nq, n, caps ? c->d_caps : NULL, c->d_leaves);
+  if (rc) {
+    free(h);
+    return rc;
+  }
+  E13LeafRec *lr = (E13LeafRec *)malloc((size_t)n * sizeof(E13LeafRec));
+  if (!lr) {
+    free(h);
+    return E13_ENOMEM;
+  }
+  HIPCHK(hipMemcpyAsync(lr, c->d_leaves, (size_t)n * sizeof(E13LeafRec), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) c->stats.kernel_ms += ms;
+  for (int k = 0; k < n; k++) {
+    out[k].result = lr[k].result;
+    out[k].stage = lr[k].stage;
+    out[k].steps = lr[k].steps;
+    out[k].st0 = lr[k].st0;
+    out[k].head0 = lr[k].head0;
+    out[k].abs_ok = lr[k].abs_ok;
+    out[k].rel_ok = lr[k].rel_ok;
+    out[k].event = lr[k].pad;
+    for (int j = 0; j < 3; j++) out[k].ch[j] = lr[k].ch[j];
+  }
+  free(lr);
+  free(h);
+  return E13_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Plain re-check (after recheck.c:57-92, stages 1..9, search deadlines).     */
+/* ------------------------------------------------------------------------ */
+int e13_recheck(const int8_t *tab, int nq, int st0, int head0, int abs_ok, int rel_ok, int64_t steps,
+                uint32_t *read_mask) {
+  enum { L = 64, R = 1300, T = L + R };
+  uint8_t tape[T];
+  uint8_t ctr[E13_NS + 1][L];
+  memset(tape, 0, sizeof tape);
+  const int rel = !abs_ok && rel_ok;
+  int head = 0, st = 0, ones = 0;
+  int64_t t = 0;
+  uint32_t mask = 0;
+  if (read_mask) *read_mask = 0;
+  if (!tab || nq < 1 || nq > 16) return 0;
+  for (int n = 1; n <= E13_NS; n++) {
+    const int blk = 1 << n, want = rel ? head0 + blk - 2 : head0;
+    const int64_t limit = 4 * (int64_t)T14S[n] + 64;
+    for (;;) {
+      if (st == st0 && head == want && ones == blk) {
+        int all = 1;
+        for (int k = -1; k <= blk - 2; k++)
+          if (!tape[k + L]) {
+            all = 0;
+            break;
+          }
+        if (all) break;
+      }
+      if (t > limit) return 0;
+      const int sym = tape[head + L], cell = 2 * st + sym;
+      mask |= 1u << cell;
+      const int code = tab[cell];
+      if (code < 0) return 0; /* halt, or a never-chosen cell */
+      const int w = code & 1;
+      if (w != sym) {
+        tape[head + L] = (uint8_t)w;
+        if (head >= -2) ones += w ? 1 : -1;
+      }
+      head += (code & 2) ? 1 : -1;
+      if (head <= -L || head >= R - 1) return 0;
+      st = code >> 2;
+      if (st >= nq) return 0;
+      t++;
+    }
+    for (int k = 0; k < L; k++) ctr[n][k] = k < L - 2 ? tape[k] : 0; /* cells -64..-3 */
+    const int cls = n <= 2 ? 0 : (n <= 8 ? 1 : 2), first = cls == 0 ? 1 : (cls == 1 ? 3 : 9);
+    if (n != first && memcmp(ctr[n], ctr[first], L)) return 0;
+  }
+  if (!memcmp(ctr[1], ctr[3], L) || !memcmp(ctr[3], ctr[9], L) || !memcmp(ctr[1], ctr[9], L)) return 0;
+  if (t != steps) return 0;
+  if (read_mask) *read_mask = mask;
+  return 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Output lines.                                                             */
+/* ------------------------------------------------------------------------ */
+int e13_format_survivor(const e13_problem *p, const e13_survivor *s, char *buf, int buflen) {
+  if (!p || !s || !buf || buflen < 256) return E13_EINVAL;
+  int n = snprintf(buf, (size_t)buflen, "SURVIVOR X=%d st0=%d head0=%d abs=%d rel=%d steps=%lld table=", p->X, s->st0,
+                   s->head0, s->abs_ok, s->rel_ok, (long long)s->steps);
+  for (int q = 0; q < p->nq && n < buflen - 16; q++) {
+    for (int a = 0; a < 2; a++) {
+      int i = 2 * q + a, code = s->tab[i];
+      if (!((s->read_mask >> i) & 1)) buf[n++] = '?';
+      if (code < 0)
+        n += snprintf(buf + n, (size_t)(buflen - n), "---");
+      else
+        n += snprintf(buf + n, (size_t)(buflen - n), "%d%c%d", code & 1, (code & 2) ? 'R' : 'L', code >> 2);
+      if (a == 0) buf[n++] = ' ';
+    }
+    if (q + 1 < p->nq) buf[n++] = '|';
+  }
+  buf[n] = 0;
+  return n;
+}
+
+int e13_format_hash(const e13_problem *p, const e13_survivor *s, char *buf, int buflen) {
+  if (!p || !s || !buf || buflen < 96) return E13_EINVAL;
+  return snprintf(buf, (size_t)buflen, "HASH X=%d ch0=%016llx ch1=%016llx ch2=%016llx", p->X,
+                  (unsigned long long)s->ch[0], (unsigned long long)s->ch[1], (unsigned long long)s->ch[2]);
+}
+
+void e13_synth_table(uint64_t seed0, uint64_t k, int8_t *tab28, int32_t *cap) {
+  uint64_t st = seed0 + k;
+  for (int i = 0; i < 28; i++) tab28[i] = (int8_t)(splitmix(&st) % 56);
+  tab28[splitmix(&st) % 28] = E13_HALT;
+  double u = (double)(splitmix(&st) >> 11) * (1.0 / 9007199254740992.0);
+  *cap = (int32_t)floor(10.0 * pow(2e4, u));
+}

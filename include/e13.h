@@ -1,0 +1,156 @@
+/* e13.h -- C ABI of the MI355X (gfx950) TNF-enumeration + simulate-and-filter
+ * library, libe13.so.  Plain C types only.
+ *
+ * The reference keeps this path in static functions of src/e13search.c plus a
+ * CuPy kernel (src/e13gpu.py); it has no FFI.  Each entry point below names
+ * the reference function or call it replaces.
+ *
+ * Conventions: functions return 0 or a negative E13_E* code and never exit;
+ * the caller owns every host buffer, the context owns device memory; one
+ * context per host thread (not re-entrant), one HIP stream inside.
+ */
+#ifndef E13_H
+#define E13_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define E13_NS 9
+#define E13_MAXCELLS 32
+
+/* Error codes. */
+#define E13_OK 0
+#define E13_EINVAL (-1)     /* bad argument or shape */
+#define E13_EHIP (-2)       /* HIP runtime error */
+#define E13_ENOMEM (-3)     /* device or host allocation failed */
+#define E13_EFORCED (-4)    /* forced cells exceed FMAX (e13search.c:353) */
+#define E13_ESYM (-5)       /* a symmetric state is named by a fixed cell or has one (e13search.c:346) */
+#define E13_ESYMSTART (-6)  /* start state in the symmetric states (e13search.c:349) */
+#define E13_ESTACK (-7)     /* node stack capacity exceeded (e13gpu.py:420-421) */
+#define E13_ENODEV (-8)     /* no HIP device */
+
+/* Result codes of a run (e13search.c:64-65). */
+#define E13_DEAD (-1)
+#define E13_SURV (-2)
+
+/* Table codes: (next << 2) | (dir << 1) | write, dir 1 = R; -1 halt; -2 a
+   cell still to be chosen (FORCED) -- e13search.c:66, 72. */
+#define E13_HALT (-1)
+#define E13_FORCED (-2)
+
+/* A search after setup()/set_sym() (e13search.c:315-355, 422-426). */
+typedef struct {
+  int32_t nq;       /* states after the deletion */
+  int32_t X;        /* deleted base state, -1 none (printed as X=) */
+  int32_t budget;   /* edits left after the forced cells (0 in free mode) */
+  int32_t sym_lo, sym_hi, sym_n0; /* symmetric states, renumbered; -1 = off */
+  int32_t nforced;  /* rewritten rules into X */
+  int8_t tab[E13_MAXCELLS];
+} e13_problem;
+
+typedef struct {
+  uint64_t leaves, steps, survivors;
+  uint64_t dead_at_stage[E13_NS + 1];
+} e13_counts;
+
+typedef struct {
+  int8_t tab[E13_MAXCELLS]; /* -1: halt, or a free cell never chosen */
+  uint32_t read_mask;       /* bit i: cell i read before the stage-9 start ('?' when 0) */
+  int32_t st0, head0, abs_ok, rel_ok;
+  int64_t steps;
+  uint64_t ch[3];           /* counter hashes at the stage starts, classes 0..2 */
+  int32_t recheck_ok;       /* 1: the plain unhashed re-check agrees */
+  int32_t pad;
+} e13_survivor;
+
+typedef struct {
+  int32_t result; /* E13_DEAD or E13_SURV */
+  int32_t stage, steps, st0, head0, abs_ok, rel_ok, event;
+  uint64_t ch[3];
+} e13_leaf;
+
+/* Launch and schedule settings.  Any field <= 0 takes the default (or the
+   autotuned value when one is stored for this device). */
+typedef struct {
+  int32_t form;        /* 1 plain (one item per lane, host loop), 2 batched (persistent lanes) */
+  int32_t slice;       /* steps per round between lane refills */
+  int32_t wg_threads;  /* 64 .. 256 */
+  int32_t wgs_per_cu;  /* persistent grid = CUs x this */
+  int32_t cap;         /* step cap per machine per launch */
+  int32_t batch;       /* input nodes per launch */
+  int32_t gate;        /* 1 ones-count gate, 2 reference Zobrist hash gate */
+  int32_t refill_tries;
+  int64_t stack_nodes; /* node stack capacity */
+} e13_tuning;
+
+typedef struct {
+  double kernel_ms;      /* summed HIP-event time of the expansion/batch kernels */
+  int64_t launches;
+  int64_t rounds;        /* wave rounds */
+  int64_t lane_slots;    /* active lanes summed over rounds */
+  int64_t peak_stack;
+  int32_t slice, wg_threads, grid, gate, form, pad;
+} e13_stats;
+
+typedef struct e13_ctx e13_ctx;
+
+/* Device context (no reference counterpart: the reference's CuPy module setup,
+   e13gpu.py:318-347). */
+int e13_open(int device, const e13_tuning *t, e13_ctx **out);
+void e13_close(e13_ctx *ctx);
+const char *e13_strerror(int code);
+int e13_set_tuning(e13_ctx *ctx, const e13_tuning *t);
+int e13_get_tuning(e13_ctx *ctx, e13_tuning *t);
+int e13_get_stats(e13_ctx *ctx, e13_stats *s);
+
+/* Zobrist tables: KEY[0..1363], TGT[0..9] (e13search.c:437-444). */
+int e13_keys(uint64_t *key, uint64_t *tgt);
+
+/* Machine text -> codes; returns the number of states or a negative code
+   (load/parse_cell, e13search.c:105-121).  `text` holds the file contents. */
+int e13_parse_machine(const char *text, int32_t *base, int max_states);
+
+/* setup() + set_sym() (e13search.c:317-355, 423-426).  freebase == NULL:
+   delete mode with budget fmax; else free mode.  sym_lo < 0: no symmetry. */
+int e13_setup(const int32_t *base, int nq_base, int X, int fmax, const uint8_t *freebase, int sym_lo, int sym_hi,
+              e13_problem *out);
+
+/* The search: drop-in for run_delete() (e13search.c:357-420) and for the GPU
+   driver loop of e13gpu.py:381-437.  Counts are exact; survivors beyond
+   surv_cap are counted in *n_surv but not stored. */
+int e13_search(e13_ctx *ctx, const e13_problem *p, e13_counts *out, e13_survivor *surv, uint64_t surv_cap,
+               uint64_t *n_surv);
+
+/* Complete tables from the blank tape: run_full() (e13search.c:428-433) over a
+   batch.  tabs: n x ncells codes (>= 0 move, -1 halt); caps: NULL for the
+   reference deadlines, else one cap per table replacing every stage deadline. */
+int e13_simulate_batch(e13_ctx *ctx, const int8_t *tabs, int ncells, int n, const int32_t *caps, e13_leaf *out);
+
+/* Same, on device buffers already resident in HBM (tabs_dev: n x 32 bytes,
+   codes 0..59 or 62 halt; caps_dev or NULL; out_dev n records).  Stream-ordered
+   on the context stream; returns after the launch is queued. */
+int e13_simulate_batch_device(e13_ctx *ctx, const void *tabs_dev, int nq, int n, const void *caps_dev, void *out_dev);
+int e13_sync(e13_ctx *ctx);
+
+/* Plain, unhashed re-check of a survivor over stages 1..9 with the search's
+   deadlines and window (after recheck.c:57-92): 1 consistent, 0 not.  The
+   cells read before the stage-9 start go to *read_mask. */
+int e13_recheck(const int8_t *tab, int nq, int st0, int head0, int abs_ok, int rel_ok, int64_t steps,
+                uint32_t *read_mask);
+
+/* The SURVIVOR line of report() (e13search.c:184-200) and the extra HASH line;
+   returns the length written (without the NUL). */
+int e13_format_survivor(const e13_problem *p, const e13_survivor *s, char *buf, int buflen);
+int e13_format_hash(const e13_problem *p, const e13_survivor *s, char *buf, int buflen);
+
+/* Synthetic stress table k: splitmix64 from seed0 + k; 28 cells v = next() % 56;
+   halt cell next() % 28; cap floor(10 * 2e4^u), u = (next() >> 11) * 2^-53. */
+void e13_synth_table(uint64_t seed0, uint64_t k, int8_t *tab28, int32_t *cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,212 @@
+"""The C host of libe13.so on the CPU: exported symbols, Zobrist tables,
+machine parsing, setup(), the plain survivor re-check, the output lines and
+the synthetic generator.  No kernel is launched here."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from conftest import (REF_DUMP, REF_RESULTS, ROOT, XS, load_pkg, oracle_lib, parse_summaries, read_machine,
+                      read_text)
+
+
+@pytest.fixture(scope="module")
+def e13():
+    return load_pkg()
+
+
+def header_functions():
+    text = read_text(ROOT, "include", "e13.h")
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(e13_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_function(e13):
+    L = e13.lib()
+    names = header_functions()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(e13.EXPORTS)
+
+
+def test_keys_match_reference_dump(e13):
+    key, tgt = e13.keys()
+    vals = [int(l, 16) for l in read_text(REF_DUMP, "keys_tgt.txt").split()]
+    assert key == vals[:1364]
+    assert tgt[1:] == vals[1364:]
+
+
+def test_parse_machine(e13):
+    base = e13.parse_machine(read_machine())
+    assert len(base) == 28
+    assert base[0] == (1 << 2) | 0 | 1      # 1LB
+    assert base[25] == -1                    # M1 = ---
+    assert base[27] == (12 << 2) | 0 | 0     # 0LM
+    with pytest.raises(e13.E13Error):
+        e13.parse_machine("1L1 2R0\n")
+
+
+class EoProblem(ctypes.Structure):
+    _fields_ = [("nq_base", ctypes.c_int), ("base", ctypes.c_int * 32), ("X", ctypes.c_int), ("fmax", ctypes.c_int),
+                ("freemode", ctypes.c_int), ("freebase", ctypes.c_uint8 * 32), ("sym_lo", ctypes.c_int),
+                ("sym_hi", ctypes.c_int), ("nq", ctypes.c_int), ("budget", ctypes.c_int), ("sym_n0", ctypes.c_int),
+                ("tab", ctypes.c_int * 32)]
+
+
+def oracle_setup(base, X, fmax, free=None, sym=None):
+    L = oracle_lib()
+    p = EoProblem()
+    p.nq_base = len(base) // 2
+    for i, v in enumerate(base):
+        p.base[i] = v
+    p.X, p.fmax = X, fmax
+    p.sym_lo = p.sym_hi = -1
+    if free is not None:
+        p.freemode = 1
+        for c in free:
+            p.freebase[2 * (ord(c[0]) - 65) + int(c[1])] = 1
+    if sym:
+        L.eo_set_sym(ctypes.byref(p), sym[0], sym[1])
+    rc = L.eo_setup(ctypes.byref(p))
+    return rc, p
+
+
+@pytest.mark.parametrize("fmax", [2, 3, 4])
+def test_setup_delete_matches_oracle_and_logs(e13, fmax):
+    base = e13.parse_machine(read_machine())
+    logs = read_text(REF_RESULTS, f"1_local_edits_log_f{min(fmax, 4)}.txt")
+    forced = {int(m.group(1)): int(m.group(2)) for m in re.finditer(r"X=(\d+) forced=(\d+)", logs)}
+    for X in XS:
+        p = e13.setup(base, X, fmax)
+        rc, o = oracle_setup(base, X, fmax)
+        assert rc == 0
+        assert p.nq == o.nq == 13 and p.budget == o.budget and p.sym_n0 == o.sym_n0
+        assert list(p.tab[:26]) == list(o.tab[:26])
+        assert p.nforced == forced[X]
+        assert p.budget == fmax - forced[X]
+
+
+def test_setup_free_sym_matches_oracle(e13):
+    base = e13.parse_machine(read_machine())
+    cells = "D0 G0 K0 K1 L1 M1".split()
+    p = e13.setup(base, 13, free_cells=cells, sym=(10, 13))
+    rc, o = oracle_setup(base, 13, 0, cells, (10, 13))
+    assert rc == 0
+    assert (p.sym_lo, p.sym_hi, p.sym_n0, p.budget) == (o.sym_lo, o.sym_hi, o.sym_n0, o.budget) == (10, 12, 0, 0)
+    assert list(p.tab[:26]) == list(o.tab[:26])
+    # V4 layout (results/4_division_6/v4/g_v4.log: sym=3..8 n0=3)
+    cells4 = "F0 F1 G0 G1 H0 H1 I0 I1".split()
+    p4 = e13.setup(base, 9, free_cells=cells4, sym=(3, 9))
+    assert (p4.sym_lo, p4.sym_hi, p4.sym_n0) == (3, 8, 3)
+
+
+def test_setup_errors(e13):
+    base = e13.parse_machine(read_machine())
+    with pytest.raises(e13.E13Error) as ei:
+        e13.setup(base, 2, 1)          # two rules into C, FMAX 1
+    assert ei.value.code == -4
+    with pytest.raises(e13.E13Error) as ei:
+        e13.setup(base, 13, free_cells=["D0"], sym=(10, 13))   # K is named by fixed cells
+    assert ei.value.code == -5
+
+
+def fixture_survivors():
+    out = []
+    for f in ("1_local_edits_surv_f2.txt", "1_local_edits_surv_f3.txt", "1_local_edits_surv_f4.txt",
+              "2_counter_3_surv_modsym.txt", "validation_split_test.txt", "validation_split_test_free.txt"):
+        out += [l.strip() for l in read_text(REF_RESULTS, f).splitlines() if l.startswith("SURVIVOR")]
+    return out
+
+
+def parse_line(e13, line):
+    f = dict(kv.split("=", 1) for kv in line.split(" table=")[0].split()[1:])
+    cells = []
+    for row in line.split("table=")[1].split("|"):
+        cells += row.split()
+    tab, unread = [], 0
+    for i, c in enumerate(cells):
+        if c.startswith("?"):
+            unread |= 1 << i
+            c = c[1:]
+        tab.append(-1 if c == "---" else (int(c[2:]) << 2) | ((c[1] == "R") << 1) | int(c[0]))
+    return f, tab, unread
+
+
+def test_recheck_every_fixture_survivor(e13):
+    lines = fixture_survivors()
+    assert len(lines) >= 130
+    for line in lines:
+        f, tab, unread = parse_line(e13, line)
+        nq = len(tab) // 2
+        ok, mask = e13.recheck(tab, nq, int(f["st0"]), int(f["head0"]), int(f["abs"]), int(f["rel"]),
+                               int(f["steps"]))
+        assert ok == 1, line
+        full = (1 << len(tab)) - 1
+        assert mask == full & ~unread, line
+
+
+def test_recheck_rejects_wrong_claims(e13):
+    f, tab, _ = parse_line(e13, fixture_survivors()[0])
+    nq = len(tab) // 2
+    args = (int(f["st0"]), int(f["head0"]), int(f["abs"]), int(f["rel"]), int(f["steps"]))
+    assert e13.recheck(tab, nq, *args)[0] == 1
+    assert e13.recheck(tab, nq, args[0], args[1], args[2], args[3], args[4] + 1)[0] == 0
+    assert e13.recheck(tab, nq, (args[0] + 1) % nq, *args[1:])[0] == 0
+
+
+def test_format_survivor_reproduces_fixture_lines(e13):
+    base = e13.parse_machine(read_machine())
+    for line in fixture_survivors():
+        if "split" in line:
+            continue
+        f, tab, unread = parse_line(e13, line)
+        X = int(f["X"])
+        nq = len(tab) // 2
+        p = e13.Problem()
+        p.nq, p.X = nq, X
+        s = e13.Survivor()
+        for i, v in enumerate(tab):
+            s.tab[i] = v
+        s.read_mask = ((1 << len(tab)) - 1) & ~unread
+        s.st0, s.head0, s.abs_ok, s.rel_ok, s.steps = (int(f[k]) for k in ("st0", "head0", "abs", "rel", "steps"))
+        assert e13.format_survivor(p, s) == line
+
+
+def test_synth_tables_match_oracle(e13):
+    L = oracle_lib()
+    L.eo_synth_table.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int),
+                                 ctypes.POINTER(ctypes.c_int32)]
+    caps = []
+    for k in range(2000):
+        t = (ctypes.c_int * 32)()
+        c = ctypes.c_int32()
+        L.eo_synth_table(0x5EED, k, t, ctypes.byref(c))
+        mine, cap = e13.synth_table(0x5EED, k)
+        assert mine == list(t[:28]) and cap == c.value
+        assert sum(1 for v in mine if v == -1) >= 1
+        caps.append(cap)
+    assert min(caps) >= 10 and max(caps) <= 200000
+    # log-uniform: about a quarter of the caps below 10 * 2e4**0.25 = 119
+    frac = sum(1 for c in caps if c < 119) / len(caps)
+    assert 0.2 < frac < 0.3
+
+
+def test_open_without_device_fails_cleanly(e13):
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(e13.E13Error) as ei:
+        e13.Context(0)
+    assert ei.value.code == -8
+
+
+def test_product_path_has_no_oracle_import():
+    """The package and its sources never reference the oracle."""
+    pkg = os.path.join(ROOT, "erdos-machine-13_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for fn in files:
+            if fn.endswith((".py", ".c", ".h", ".hip", "Makefile")):
+                text = read_text(dirpath, fn)
+                assert "oracle" not in text.lower() or fn == "__init__.py" and "oracle" not in text, fn
