@@ -57,7 +57,7 @@ class Survivor(ctypes.Structure):
     _fields_ = [("tab", ctypes.c_int8 * MAXCELLS), ("read_mask", ctypes.c_uint32),
                 ("st0", ctypes.c_int32), ("head0", ctypes.c_int32), ("abs_ok", ctypes.c_int32),
                 ("rel_ok", ctypes.c_int32), ("steps", ctypes.c_int64), ("ch", ctypes.c_uint64 * 3),
-                ("recheck_ok", ctypes.c_int32), ("pad", ctypes.c_int32)]
+                ("recheck_ok", ctypes.c_int32), ("prob", ctypes.c_int32)]
 
 
 class Leaf(ctypes.Structure):
@@ -79,14 +79,15 @@ class Stats(ctypes.Structure):
     _fields_ = [("kernel_ms", ctypes.c_double), ("launches", ctypes.c_int64), ("rounds", ctypes.c_int64),
                 ("lane_slots", ctypes.c_int64), ("peak_stack", ctypes.c_int64), ("slice", ctypes.c_int32),
                 ("wg_threads", ctypes.c_int32), ("grid", ctypes.c_int32), ("gate", ctypes.c_int32),
-                ("form", ctypes.c_int32), ("pad", ctypes.c_int32)]
+                ("form", ctypes.c_int32), ("pad", ctypes.c_int32), ("donations", ctypes.c_int64),
+                ("pool_takes", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
 
 
 EXPORTS = ("e13_open", "e13_close", "e13_strerror", "e13_set_tuning", "e13_get_tuning", "e13_get_stats",
-           "e13_keys", "e13_parse_machine", "e13_setup", "e13_search", "e13_simulate_batch",
+           "e13_keys", "e13_parse_machine", "e13_setup", "e13_search", "e13_search_many", "e13_simulate_batch",
            "e13_simulate_batch_device", "e13_sync", "e13_recheck", "e13_format_survivor", "e13_format_hash",
            "e13_synth_table")
 
@@ -124,6 +125,8 @@ def lib():
                             ctypes.c_int, ctypes.c_int, P(Problem)]
     L.e13_search.argtypes = [ctypes.c_void_p, P(Problem), P(Counts), P(Survivor), ctypes.c_uint64,
                              P(ctypes.c_uint64)]
+    L.e13_search_many.argtypes = [ctypes.c_void_p, P(Problem), ctypes.c_int, P(Counts), P(Survivor),
+                                  ctypes.c_uint64, P(ctypes.c_uint64)]
     L.e13_simulate_batch.argtypes = [ctypes.c_void_p, P(ctypes.c_int8), ctypes.c_int, ctypes.c_int,
                                      P(ctypes.c_int32), P(Leaf)]
     L.e13_simulate_batch_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
@@ -266,6 +269,16 @@ class Context:
         _check(lib().e13_search(self._h, ctypes.byref(p), ctypes.byref(c), sv, surv_cap, ctypes.byref(ns)),
                f"e13_search X={p.X}")
         return c, [sv[i] for i in range(min(ns.value, surv_cap))], ns.value
+
+    def search_many(self, problems, surv_cap=200000):
+        """Several problems in one persistent launch: ([Counts], [Survivor], n)."""
+        n = len(problems)
+        ps = (Problem * n)(*problems)
+        cs = (Counts * n)()
+        sv = (Survivor * surv_cap)()
+        ns = ctypes.c_uint64()
+        _check(lib().e13_search_many(self._h, ps, n, cs, sv, surv_cap, ctypes.byref(ns)), "e13_search_many")
+        return [cs[i] for i in range(n)], [sv[i] for i in range(min(ns.value, surv_cap))], ns.value
 
     def simulate_batch(self, tabs, caps=None):
         """run_full() on each table (list of equal-length code lists)."""

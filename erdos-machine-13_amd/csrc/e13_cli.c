@@ -115,12 +115,83 @@ static int run_one(e13_ctx *ctx, const int32_t *base, int nq_base, int X, int fm
           p.budget, (unsigned long long)cnt.leaves, (unsigned long long)cnt.survivors,
           (unsigned long long)cnt.steps, wall, E13_NS);
   for (int j = 0; j <= E13_NS; j++) fprintf(stderr, " %llu", (unsigned long long)cnt.dead_at_stage[j]);
-  fprintf(stderr, "\n  launches=%lld peak stack=%lld nodes kernel=%.1fms %.2e steps/s\nX=%d digest=%016llx\n",
+  fprintf(stderr,
+          "\n  launches=%lld peak stack=%lld nodes kernel=%.1fms %.2e steps/s rounds=%lld lane-slots=%lld "
+          "donations=%lld pool-takes=%lld\nX=%d digest=%016llx\n",
           (long long)(st1.launches - st0.launches), (long long)st1.peak_stack, st1.kernel_ms - st0.kernel_ms,
-          wall > 0 ? (double)cnt.steps / wall : 0.0, X, (unsigned long long)digest);
+          wall > 0 ? (double)cnt.steps / wall : 0.0, (long long)(st1.rounds - st0.rounds),
+          (long long)(st1.lane_slots - st0.lane_slots), (long long)(st1.donations - st0.donations),
+          (long long)(st1.pool_takes - st0.pool_takes), X, (unsigned long long)digest);
   if (ns > cap) fprintf(stderr, "WARNING: %llu survivors, only %llu printed\n", (unsigned long long)ns,
                         (unsigned long long)cap);
   fflush(stderr);
+  free(sv);
+  return 0;
+}
+
+/* Every X of a delete search in one persistent launch; the same lines as
+   run_one, per X, in argument order. */
+static int run_many(e13_ctx *ctx, const int32_t *base, int nq_base, int nx, char **xs, int fmax) {
+  e13_problem ps[16];
+  int xv[16], np = 0;
+  for (int i = 0; i < nx; i++) {
+    int X = atoi(xs[i]);
+    e13_problem p;
+    int rc = e13_setup(base, nq_base, X, fmax, NULL, -1, -1, &p);
+    if (rc == E13_EFORCED) {
+      fprintf(stderr, "X=%d forced=%d > fmax\n", X, p.nforced);
+      continue;
+    }
+    if (rc || np == 16) {
+      fprintf(stderr, "setup failed: %s\n", e13_strerror(rc ? rc : E13_EINVAL));
+      exit(1);
+    }
+    xv[np] = X;
+    ps[np++] = p;
+  }
+  if (!np) return 0;
+  const uint64_t cap = 200000;
+  e13_survivor *sv = (e13_survivor *)calloc(cap, sizeof *sv);
+  e13_counts cnt[16];
+  uint64_t ns = 0;
+  e13_stats st0, st1;
+  e13_get_stats(ctx, &st0);
+  double w0 = now_s();
+  int rc = e13_search_many(ctx, ps, np, cnt, sv, cap, &ns);
+  double wall = now_s() - w0;
+  if (rc) {
+    fprintf(stderr, "search failed: %s\n", e13_strerror(rc));
+    exit(1);
+  }
+  e13_get_stats(ctx, &st1);
+  uint64_t digest[16] = {0}, tot = 0;
+  char line[1024], hl[256];
+  for (int k = 0; k < np; k++)
+    for (uint64_t j = 0; j < ns && j < cap; j++) {
+      if (sv[j].prob != k) continue;
+      e13_format_survivor(&ps[k], &sv[j], line, sizeof line);
+      e13_format_hash(&ps[k], &sv[j], hl, sizeof hl);
+      printf("%s\n%s\n", line, hl);
+      if (!sv[j].recheck_ok) fprintf(stderr, "RECHECK FAILED: %s\n", line);
+      char l2[1100], h2[300];
+      snprintf(l2, sizeof l2, "%s\n", line);
+      snprintf(h2, sizeof h2, "%s\n", hl);
+      digest[k] += text_hash(l2) ^ mix64(text_hash(h2));
+    }
+  fflush(stdout);
+  for (int k = 0; k < np; k++) {
+    fprintf(stderr, "X=%d budget=%d gpu leaves=%llu survivors=%llu steps=%llu wall=%.1fs dead-at-stage(0..%d):", xv[k],
+            ps[k].budget, (unsigned long long)cnt[k].leaves, (unsigned long long)cnt[k].survivors,
+            (unsigned long long)cnt[k].steps, wall, E13_NS);
+    for (int j = 0; j <= E13_NS; j++) fprintf(stderr, " %llu", (unsigned long long)cnt[k].dead_at_stage[j]);
+    fprintf(stderr, "\nX=%d digest=%016llx\n", xv[k], (unsigned long long)digest[k]);
+    tot += cnt[k].steps;
+  }
+  fprintf(stderr, "  one launch for %d problems: kernel=%.1fms %.2e steps/s, rounds=%lld lane-slots=%lld "
+                  "donations=%lld pool-takes=%lld\n",
+          np, st1.kernel_ms - st0.kernel_ms, wall > 0 ? (double)tot / wall : 0.0, (long long)(st1.rounds - st0.rounds),
+          (long long)(st1.lane_slots - st0.lane_slots), (long long)(st1.donations - st0.donations),
+          (long long)(st1.pool_takes - st0.pool_takes));
   free(sv);
   return 0;
 }
@@ -196,7 +267,12 @@ int main(int argc, char **argv) {
   } else if (!strcmp(mode, "delete")) {
     if (argc < 4) return 1;
     int fmax = atoi(argv[3]);
-    for (int i = 4; i < argc; i++) run_one(ctx, base, nq, atoi(argv[i]), fmax, NULL, -1, -1);
+    e13_tuning cur;
+    e13_get_tuning(ctx, &cur);
+    if (cur.form == 3 && argc - 4 > 1)
+      run_many(ctx, base, nq, argc - 4, argv + 4, fmax);
+    else
+      for (int i = 4; i < argc; i++) run_one(ctx, base, nq, atoi(argv[i]), fmax, NULL, -1, -1);
   } else if (!strcmp(mode, "free") || !strcmp(mode, "freesym")) {
     if (argc < 4) return 1;
     uint8_t fb[32] = {0};

@@ -31,23 +31,28 @@
 #define E13_C_FREE 63
 
 /* A stopped machine: the state right before the read of cell `pend`, or
-   (pend < 0) a machine paused by the per-launch step cap.  304 bytes. */
+   (pend < 0) a machine paused by the per-launch step cap.  304 bytes = 76
+   words; the word offsets are used by the kernels' cooperative copies. */
 typedef struct E13Node {
   int32_t q, st, t, nt;            /* head (q), state, steps from the blank tape, next target stage */
   int32_t stage, st0, q0, flags;   /* last accepted stage, stage-1 state and head; flags: bit0 abs_ok,
                                       bit1 rel_ok, bits 2..4 cs[0..2] */
   int32_t ones, budget, nintro, pend; /* ones at q >= 61, edits left, symmetric states in use, cell to bind */
-  int32_t front, depth, nbound, pad;  /* subtree id (cost logging), binding depth, cells bound so far */
+  int32_t prob, depth, nbound, next_slot; /* problem index, binding depth, cells bound so far, first child
+                                             slot not yet handed out */
   uint64_t hb;                     /* block hash (hash-gate build only) */
   uint64_t ch[3];                  /* counter hashes at the accepted stages */
-  uint8_t tab[E13_MAXCELLS];
-  uint32_t tape[E13_TW_PAD];
+  uint8_t tab[E13_MAXCELLS];       /* word 24 */
+  uint32_t tape[E13_TW_PAD];       /* word 32 */
 } E13Node;
+#define E13_NODE_WORDS 76
+#define E13_NODE_W_TAB 24
+#define E13_NODE_W_TAPE 32
 
 /* Survivor as the kernel records it. */
 typedef struct E13SurvRec {
   uint8_t tab[E13_MAXCELLS];
-  int32_t st0, head0, abs_ok, rel_ok, steps, pad;
+  int32_t st0, head0, abs_ok, rel_ok, steps, prob;
   uint64_t ch[3];
 } E13SurvRec;
 
@@ -90,6 +95,36 @@ typedef struct E13ExpandArgs {
   uint32_t nin, out_cap, surv_cap, nslot;
   int32_t slice, cap, refill_tries, nq;
 } E13ExpandArgs;
+
+/* Work-pool and termination state of the persistent search (one per search). */
+typedef struct E13Sched {
+  uint64_t head, tail;  /* pool ring: nodes claimed / nodes reserved (monotone) */
+  int32_t busy;         /* waves holding work */
+  int32_t error;        /* nonzero: abort (1 private stack full, 2 pool wait timed out, 3 idle watchdog) */
+  int32_t donations, pool_takes;
+  int32_t exited;       /* waves that have left */
+  int32_t pad0;
+  uint64_t pad[3];
+} E13Sched;
+
+#define E13_MAXPROB 16
+#define E13_PCNT 16       /* per-problem counters: leaves, steps, survivors, hist[0..9] */
+
+typedef struct E13SearchArgs {
+  E13Node *stacks;        /* nwaves x stk_cap private stacks */
+  E13Node *pool;          /* pool_cap donated nodes */
+  uint64_t *pool_seq;     /* pool_cap sequence words: slot j holds node j when pool_seq = j + 1 */
+  E13Sched *sch;
+  const E13Problem *pbs;  /* np problems */
+  uint64_t *pcnt;         /* np x E13_PCNT */
+  E13Counters *ctr;       /* survivor slots, rounds, lane slots */
+  E13SurvRec *surv;
+  const uint64_t *key;
+  const uint64_t *tgt;
+  int32_t np, slice, stk_cap, pool_cap;
+  int32_t refill_tries, donate_every, pool_min_idle, total_waves;
+  uint32_t surv_cap, spin_limit, idle_limit, pad0;
+} E13SearchArgs;
 
 typedef struct E13BatchArgs {
   const uint8_t *tabs;    /* n x 32 codes (0..59 move, 62 halt) */

@@ -20,6 +20,8 @@
 size_t e13k_lds_bytes(int wg_threads, int gate);
 int e13k_launch_expand(const E13ExpandArgs *a, int grid, int wg_threads, int gate, int refill, hipStream_t s);
 int e13k_launch_batch(const E13BatchArgs *a, int grid, int wg_threads, int gate, hipStream_t s);
+size_t e13k_search_lds_bytes(int wg_threads, int gate, int np);
+int e13k_launch_search(const E13SearchArgs *a, int grid, int wg_threads, int gate, hipStream_t s);
 
 static const long T14S[E13_NS + 1] = {0, 2, 22, 90, 268, 944, 3434, 13450, 52068, 208864}; /* e13search.c:70 */
 
@@ -42,6 +44,15 @@ struct e13_ctx {
   size_t surv_cap;
   void *d_tabs, *d_caps, *d_leaves;
   size_t batch_cap;
+  /* persistent search */
+  E13Node *d_stacks;
+  size_t stacks_cap; /* nodes */
+  E13Node *d_pool;
+  uint64_t *d_pool_seq;
+  size_t pool_cap;
+  E13Sched *d_sch;
+  E13Problem *d_pbs;
+  uint64_t *d_pcnt;
 };
 
 const char *e13_strerror(int code) {
@@ -180,7 +191,7 @@ int e13_setup(const int32_t *base, int nq_base, int X, int fmax, const uint8_t *
 /* Context.                                                                  */
 /* ------------------------------------------------------------------------ */
 static void tuning_defaults(e13_tuning *t) {
-  t->form = 2;
+  t->form = 3;
   t->slice = 32;
   t->wg_threads = 256;
   t->wgs_per_cu = 2;
@@ -205,7 +216,7 @@ static void tuning_merge(e13_tuning *dst, const e13_tuning *src) {
 }
 
 static int tuning_valid(const e13_tuning *t) {
-  if (t->form < 1 || t->form > 2) return 0;
+  if (t->form < 1 || t->form > 3) return 0;
   if (t->slice < 1 || t->slice > 1 << 16) return 0;
   if (t->wg_threads < 64 || t->wg_threads > 256 || t->wg_threads % 64) return 0;
   if (t->wgs_per_cu < 1 || t->wgs_per_cu > 16) return 0;
@@ -286,6 +297,12 @@ void e13_close(e13_ctx *c) {
   hipFree(c->d_tabs);
   hipFree(c->d_caps);
   hipFree(c->d_leaves);
+  hipFree(c->d_stacks);
+  hipFree(c->d_pool);
+  hipFree(c->d_pool_seq);
+  hipFree(c->d_sch);
+  hipFree(c->d_pbs);
+  hipFree(c->d_pcnt);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -373,6 +390,8 @@ static void root_node(const e13_problem *p, E13Node *nd) {
   }
 }
 
+static void finish_survivor(const e13_problem *p, const E13SurvRec *r, e13_survivor *s, int prob);
+
 static int check_problem(const e13_problem *p) {
   if (!p || p->nq < 1 || p->nq > E13_MAXNQ || p->budget < 0) return 0;
   for (int i = 0; i < 2 * p->nq; i++) {
@@ -386,6 +405,7 @@ static int check_problem(const e13_problem *p) {
 int e13_search(e13_ctx *c, const e13_problem *p, e13_counts *out, e13_survivor *surv, uint64_t surv_cap,
                uint64_t *n_surv) {
   if (!c || !out || !check_problem(p)) return E13_EINVAL;
+  if (c->tune.form == 3) return e13_search_many(c, p, 1, out, surv, surv_cap, n_surv);
   HIPCHK(hipSetDevice(c->device));
   const e13_tuning *tu = &c->tune;
   const int nslot = 4 * p->nq + 1;
@@ -487,30 +507,172 @@ int e13_search(e13_ctx *c, const e13_problem *p, e13_counts *out, e13_survivor *
     E13SurvRec *rec = (E13SurvRec *)malloc(nk * sizeof(E13SurvRec));
     if (!rec) return E13_ENOMEM;
     HIPCHK(hipMemcpy(rec, c->d_surv, nk * sizeof(E13SurvRec), hipMemcpyDeviceToHost));
+    for (uint64_t k = 0; k < nk; k++) finish_survivor(p, &rec[k], &surv[k], 0);
+    free(rec);
+  }
+  return E13_OK;
+}
+
+/* Survivor records -> e13_survivor, with the plain re-check and read mask. */
+static void finish_survivor(const e13_problem *p, const E13SurvRec *r, e13_survivor *s, int prob) {
+  memset(s, 0, sizeof *s);
+  for (int i = 0; i < E13_MAXCELLS; i++) {
+    int code = r->tab[i], v;
+    if (i >= 2 * p->nq) v = E13_FORCED;
+    else if (code == E13_C_EDIT) v = p->tab[i]; /* never read: still the base value */
+    else if (code == E13_C_HALT) v = E13_HALT;
+    else if (code == E13_C_FREE) v = E13_FORCED;
+    else v = code;
+    s->tab[i] = (int8_t)v;
+  }
+  s->st0 = r->st0;
+  s->head0 = r->head0;
+  s->abs_ok = r->abs_ok;
+  s->rel_ok = r->rel_ok;
+  s->steps = r->steps;
+  s->ch[0] = r->ch[0];
+  s->ch[1] = r->ch[1];
+  s->ch[2] = r->ch[2];
+  s->prob = prob;
+  uint32_t mask = 0;
+  s->recheck_ok = e13_recheck(s->tab, p->nq, s->st0, s->head0, s->abs_ok, s->rel_ok, s->steps, &mask);
+  s->read_mask = mask;
+}
+
+/* The persistent form: every problem in one launch (see k_search). */
+int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, e13_survivor *surv,
+                    uint64_t surv_cap, uint64_t *n_surv) {
+  if (!c || !ps || !out || np < 1 || np > E13_MAXPROB) return E13_EINVAL;
+  for (int k = 0; k < np; k++)
+    if (!check_problem(&ps[k])) return E13_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  const e13_tuning *tu = &c->tune;
+  int wg = tu->wg_threads;
+  while (wg > 64 && e13k_search_lds_bytes(wg, tu->gate == 2, np) > c->lds_max) wg -= 64;
+  if (e13k_search_lds_bytes(wg, tu->gate == 2, np) > c->lds_max) return E13_EINVAL;
+  const int grid = c->ncu * tu->wgs_per_cu;
+  const int total_waves = grid * (wg / 64);
+  const int stk_cap = tu->stack_nodes >= 64 * (int64_t)total_waves ? (int)(tu->stack_nodes / total_waves) : 64;
+  const size_t need_stacks = (size_t)total_waves * (size_t)stk_cap;
+  const size_t pool_cap = 1 << 16;
+  int rc;
+  if ((rc = ensure_buf((void **)&c->d_stacks, &c->stacks_cap, need_stacks, sizeof(E13Node)))) return rc;
+  if (c->pool_cap < pool_cap) {
+    hipFree(c->d_pool);
+    hipFree(c->d_pool_seq);
+    c->d_pool = NULL;
+    c->d_pool_seq = NULL;
+    c->pool_cap = 0;
+    if (hipMalloc((void **)&c->d_pool, pool_cap * sizeof(E13Node)) != hipSuccess ||
+        hipMalloc((void **)&c->d_pool_seq, pool_cap * sizeof(uint64_t)) != hipSuccess)
+      return E13_ENOMEM;
+    c->pool_cap = pool_cap;
+  }
+  if (!c->d_sch && hipMalloc((void **)&c->d_sch, sizeof(E13Sched)) != hipSuccess) return E13_ENOMEM;
+  if (!c->d_pbs && hipMalloc((void **)&c->d_pbs, E13_MAXPROB * sizeof(E13Problem)) != hipSuccess) return E13_ENOMEM;
+  if (!c->d_pcnt && hipMalloc((void **)&c->d_pcnt, E13_MAXPROB * E13_PCNT * sizeof(uint64_t)) != hipSuccess)
+    return E13_ENOMEM;
+  size_t want_surv = surv_cap > 4096 ? (size_t)surv_cap : 4096;
+  if ((rc = ensure_buf((void **)&c->d_surv, &c->surv_cap, want_surv, sizeof(E13SurvRec)))) return rc;
+
+  /* problems, roots in pool slots 0..np-1, slot sequence words */
+  E13Problem pbs[E13_MAXPROB];
+  E13Node roots[E13_MAXPROB];
+  for (int k = 0; k < np; k++) {
+    fill_problem(&ps[k], &pbs[k]);
+    root_node(&ps[k], &roots[k]);
+    roots[k].prob = k;
+  }
+  uint64_t *seq = (uint64_t *)malloc(pool_cap * sizeof(uint64_t));
+  if (!seq) return E13_ENOMEM;
+  for (size_t i = 0; i < pool_cap; i++) seq[i] = i < (size_t)np ? i + 1 : i;
+  E13Sched sch;
+  memset(&sch, 0, sizeof sch);
+  sch.tail = (uint64_t)np;
+  HIPCHK(hipMemcpyAsync(c->d_pbs, pbs, (size_t)np * sizeof(E13Problem), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_pool, roots, (size_t)np * sizeof(E13Node), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_pool_seq, seq, pool_cap * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_sch, &sch, sizeof sch, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemsetAsync(c->d_pcnt, 0, (size_t)np * E13_PCNT * sizeof(uint64_t), c->stream));
+  HIPCHK(hipMemsetAsync(c->d_ctr, 0, sizeof(E13Counters), c->stream));
+
+  E13SearchArgs a;
+  memset(&a, 0, sizeof a);
+  a.stacks = c->d_stacks;
+  a.pool = c->d_pool;
+  a.pool_seq = c->d_pool_seq;
+  a.sch = c->d_sch;
+  a.pbs = c->d_pbs;
+  a.pcnt = c->d_pcnt;
+  a.ctr = c->d_ctr;
+  a.surv = c->d_surv;
+  a.key = c->d_key;
+  a.tgt = c->d_tgt;
+  a.np = np;
+  a.slice = tu->slice;
+  a.stk_cap = stk_cap;
+  a.pool_cap = (int32_t)pool_cap;
+  a.refill_tries = tu->refill_tries;
+  a.donate_every = 1;
+  a.pool_min_idle = 16;
+  a.total_waves = total_waves;
+  a.surv_cap = (uint32_t)c->surv_cap;
+  a.spin_limit = 1u << 22;
+  a.idle_limit = 1u << 22;
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  int lr = e13k_launch_search(&a, grid, wg, tu->gate == 2, c->stream);
+  if (lr != 0) {
+    free(seq);
+    fprintf(stderr, "libe13: search launch failed: %s\n", hipGetErrorString((hipError_t)lr));
+    return E13_EHIP;
+  }
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  uint64_t pcnt[E13_MAXPROB * E13_PCNT];
+  E13Counters h;
+  HIPCHK(hipMemcpyAsync(pcnt, c->d_pcnt, (size_t)np * E13_PCNT * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipMemcpyAsync(&h, c->d_ctr, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(&sch, c->d_sch, sizeof sch, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  free(seq);
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->stats.kernel_ms += ms;
+  c->stats.launches++;
+  c->stats.rounds += (int64_t)h.rounds;
+  c->stats.lane_slots += (int64_t)h.lane_slots;
+  c->stats.slice = tu->slice;
+  c->stats.wg_threads = wg;
+  c->stats.grid = grid;
+  c->stats.gate = tu->gate;
+  c->stats.form = 3;
+  c->stats.donations += sch.donations;
+  c->stats.pool_takes += sch.pool_takes;
+  if (sch.error) {
+    fprintf(stderr, "libe13: persistent search aborted (%s)\n",
+            sch.error == 1 ? "a private node stack is full"
+                           : (sch.error == 2 ? "a pool slot was not filled in time" : "idle watchdog"));
+    return E13_ESTACK;
+  }
+  for (int k = 0; k < np; k++) {
+    const uint64_t *q = pcnt + k * E13_PCNT;
+    out[k].leaves = q[0];
+    out[k].steps = q[1];
+    out[k].survivors = q[2];
+    for (int j = 0; j <= E13_NS; j++) out[k].dead_at_stage[j] = q[3 + j];
+  }
+  uint64_t ns = h.nsurv;
+  if (n_surv) *n_surv = ns;
+  uint64_t keep = ns < c->surv_cap ? ns : c->surv_cap;
+  if (surv && surv_cap && keep) {
+    uint64_t nk = keep < surv_cap ? keep : surv_cap;
+    E13SurvRec *rec = (E13SurvRec *)malloc(nk * sizeof(E13SurvRec));
+    if (!rec) return E13_ENOMEM;
+    HIPCHK(hipMemcpy(rec, c->d_surv, nk * sizeof(E13SurvRec), hipMemcpyDeviceToHost));
     for (uint64_t k = 0; k < nk; k++) {
-      e13_survivor *s = &surv[k];
-      memset(s, 0, sizeof *s);
-      for (int i = 0; i < E13_MAXCELLS; i++) {
-        int code = rec[k].tab[i];
-        int v;
-        if (i >= 2 * p->nq) v = E13_FORCED;
-        else if (code == E13_C_EDIT) v = p->tab[i]; /* never read: still the base value */
-        else if (code == E13_C_HALT) v = E13_HALT;
-        else if (code == E13_C_FREE) v = E13_FORCED;
-        else v = code;
-        s->tab[i] = (int8_t)v;
-      }
-      s->st0 = rec[k].st0;
-      s->head0 = rec[k].head0;
-      s->abs_ok = rec[k].abs_ok;
-      s->rel_ok = rec[k].rel_ok;
-      s->steps = rec[k].steps;
-      s->ch[0] = rec[k].ch[0];
-      s->ch[1] = rec[k].ch[1];
-      s->ch[2] = rec[k].ch[2];
-      uint32_t mask = 0;
-      s->recheck_ok = e13_recheck(s->tab, p->nq, s->st0, s->head0, s->abs_ok, s->rel_ok, s->steps, &mask);
-      s->read_mask = mask;
+      int pr = rec[k].prob;
+      if (pr < 0 || pr >= np) pr = 0;
+      finish_survivor(&ps[pr], &rec[k], &surv[k], pr);
     }
     free(rec);
   }

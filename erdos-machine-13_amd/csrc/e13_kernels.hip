@@ -391,7 +391,7 @@ __global__ void __launch_bounds__(256) k_expand(E13ExpandArgs a) {
             t0 = L.t;
             budget = nb;
             nintro = nin2;
-            front = nd->front;
+            front = nd->prob;
             depth = nd->depth + (pend >= 0);
             nbound = nd->nbound + (pend >= 0);
             build_rows(c, nd->tab, a.nq, pend, val, budget, a.pb->base);
@@ -441,7 +441,7 @@ __global__ void __launch_bounds__(256) k_expand(E13ExpandArgs a) {
           o->abs_ok = L.flags & 1;
           o->rel_ok = (L.flags >> 1) & 1;
           o->steps = L.t;
-          o->pad = 0;
+          o->prob = front;
           o->ch[0] = L.ch0;
           o->ch[1] = L.ch1;
           o->ch[2] = L.ch2;
@@ -462,10 +462,10 @@ __global__ void __launch_bounds__(256) k_expand(E13ExpandArgs a) {
           o->budget = budget;
           o->nintro = nintro;
           o->pend = ev == EV_STOP ? evcell : -1;
-          o->front = front;
+          o->prob = front;
           o->depth = depth;
           o->nbound = nbound;
-          o->pad = 0;
+          o->next_slot = 0;
           o->hb = L.hb;
           o->ch[0] = L.ch0;
           o->ch[1] = L.ch1;
@@ -596,6 +596,492 @@ __global__ void __launch_bounds__(256) k_batch(E13BatchArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Persistent whole search (one launch per search).  Each wave owns a private
+// depth-first node stack in HBM.  The node on top is staged once in LDS and
+// its child slots are handed to idle lanes; a lane that stops at an unbound
+// cell pushes its machine as a new node.  Idle waves take nodes from a
+// global pool that busy waves feed with their oldest nodes.  The search ends
+// when no wave holds work and the pool is empty; every wait is bounded and
+// any failure sets an abort flag that all waves watch.
+// ---------------------------------------------------------------------------
+namespace {
+
+#define STG_WORDS 112 // staged node (76) + rows as stored (16) + rows with edits exhausted (16) + 4 spare
+
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ long long uni64(long long v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((unsigned long long)v >> 32));
+  return (long long)(((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+__device__ __forceinline__ void wave_mem_sync() { // this wave's global stores visible to its own later loads
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ E13Node *stk_node(const E13SearchArgs &a, int wid, long long idx) {
+  return a.stacks + (size_t)wid * (size_t)a.stk_cap + (size_t)(idx % a.stk_cap);
+}
+
+__device__ __forceinline__ uint32_t ld_agent(const uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(uint32_t *p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t ld_agent64(const uint64_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent64(uint64_t *p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int ld_agent_i(const int32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Wait until every memory operation this wave issued has completed.  Data
+// shared across CUs is written and read only with agent-scope (sc1) accesses,
+// which are coherent without cache maintenance, so a completed store is
+// visible to any later agent-scope load: a drain before a flag store makes
+// the flag a publication of everything before it (no buffer_wbl2/buffer_inv).
+__device__ __forceinline__ void vm_drain() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// Encode both row sets of the staged node: A as stored, B with unread
+// editable cells bound to their base values (a child that spends the last edit).
+__device__ void stage_rows(uint32_t *stg, const E13Problem *pbs, int lane) {
+  wave_lds_sync();
+  const int prob = uni((int)stg[12]);
+  const E13Problem *pb = pbs + prob;
+  const int nq = pb->nq;
+  if (lane < 16) {
+    uint32_t A = 0, B = 0;
+    if (lane < nq) {
+      const uint8_t *tab = (const uint8_t *)(stg + E13_NODE_W_TAB);
+      int c0 = tab[2 * lane], c1 = tab[2 * lane + 1];
+      A = enc_entry(c0) | (enc_entry(c1) << 16);
+      if (c0 == E13_C_EDIT) c0 = pb->base[2 * lane] < 0 ? E13_C_HALT : pb->base[2 * lane];
+      if (c1 == E13_C_EDIT) c1 = pb->base[2 * lane + 1] < 0 ? E13_C_HALT : pb->base[2 * lane + 1];
+      B = enc_entry(c0) | (enc_entry(c1) << 16);
+    }
+    stg[76 + lane] = A;
+    stg[92 + lane] = B;
+  }
+  wave_lds_sync();
+}
+
+// Stage node `nd` (76 words) from global memory into LDS words stg[0..75].
+__device__ void stage_node(uint32_t *stg, const uint32_t *nd, const E13Problem *pbs, int lane) {
+  const uint32_t v0 = nd[lane];
+  const uint32_t v1 = lane < E13_NODE_WORDS - 64 ? nd[64 + lane] : 0u;
+  stg[lane] = v0;
+  if (lane < E13_NODE_WORDS - 64) stg[64 + lane] = v1;
+  stage_rows(stg, pbs, lane);
+}
+
+// This lane's machine as a node record in LDS (the staging area): the same
+// words write_node stores, so the newest child needs no reload from HBM.
+__device__ __forceinline__ void stage_own(const Ctx &c, uint32_t *stg, const Lane &L, int budget, int nintro, int pend,
+                                          int prob, int depth, int nbound, int nq) {
+  stg[0] = (uint32_t)L.q;
+  stg[1] = (uint32_t)((L.row - c.rbl) >> 8);
+  stg[2] = (uint32_t)L.t;
+  stg[3] = (uint32_t)L.nt;
+  stg[4] = (uint32_t)L.stage;
+  stg[5] = (uint32_t)L.st0;
+  stg[6] = (uint32_t)L.q0;
+  stg[7] = (uint32_t)L.flags;
+  stg[8] = (uint32_t)L.ones;
+  stg[9] = (uint32_t)budget;
+  stg[10] = (uint32_t)nintro;
+  stg[11] = (uint32_t)pend;
+  stg[12] = (uint32_t)prob;
+  stg[13] = (uint32_t)depth;
+  stg[14] = (uint32_t)nbound;
+  stg[15] = 0u;
+  stg[16] = (uint32_t)L.hb;
+  stg[17] = (uint32_t)(L.hb >> 32);
+  stg[18] = (uint32_t)L.ch0;
+  stg[19] = (uint32_t)(L.ch0 >> 32);
+  stg[20] = (uint32_t)L.ch1;
+  stg[21] = (uint32_t)(L.ch1 >> 32);
+  stg[22] = (uint32_t)L.ch2;
+  stg[23] = (uint32_t)(L.ch2 >> 32);
+#pragma unroll
+  for (int i = 0; i < E13_MAXCELLS / 4; ++i) stg[E13_NODE_W_TAB + i] = codes_word(c, nq, i);
+  for (int w = 0; w < E13_TW_PAD; ++w)
+    stg[E13_NODE_W_TAPE + w] = w < E13_TW ? lds_ld(c.sb, c.tbl + w * 256) : 0u;
+}
+
+// Copy this lane's machine into node record `o` (plain stores).
+__device__ __forceinline__ void write_node(const Ctx &c, E13Node *o, const Lane &L, int budget, int nintro, int pend,
+                                           int prob, int depth, int nbound, int nq) {
+  o->q = L.q;
+  o->st = (L.row - c.rbl) >> 8;
+  o->t = L.t;
+  o->nt = L.nt;
+  o->stage = L.stage;
+  o->st0 = L.st0;
+  o->q0 = L.q0;
+  o->flags = L.flags;
+  o->ones = L.ones;
+  o->budget = budget;
+  o->nintro = nintro;
+  o->pend = pend;
+  o->prob = prob;
+  o->depth = depth;
+  o->nbound = nbound;
+  o->next_slot = 0;
+  o->hb = L.hb;
+  o->ch[0] = L.ch0;
+  o->ch[1] = L.ch1;
+  o->ch[2] = L.ch2;
+  uint32_t *tw = (uint32_t *)o->tab;
+#pragma unroll
+  for (int i = 0; i < E13_MAXCELLS / 4; ++i) tw[i] = codes_word(c, nq, i);
+  uint4 *dst = (uint4 *)o->tape;
+  for (int w4 = 0; w4 < E13_TW_PAD / 4; ++w4) {
+    uint4 v4;
+    v4.x = lds_ld(c.sb, c.tbl + (4 * w4 + 0) * 256);
+    v4.y = lds_ld(c.sb, c.tbl + (4 * w4 + 1) * 256);
+    v4.z = lds_ld(c.sb, c.tbl + (4 * w4 + 2) * 256);
+    v4.w = (4 * w4 + 3 < E13_TW) ? lds_ld(c.sb, c.tbl + (4 * w4 + 3) * 256) : 0u;
+    dst[w4] = v4;
+  }
+}
+
+} // namespace
+
+template <int GATE>
+__global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
+  extern __shared__ __align__(16) uint32_t smem[];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  const int wid = blockIdx.x * nwaves + wave;
+  Ctx c;
+  c.sb = (char *)smem;
+  c.rbl = wave * ROWS_BYTES + lane * 4;
+  c.tbl = nwaves * ROWS_BYTES + wave * TAPE_BYTES + lane * 4;
+  c.keyb = nwaves * (ROWS_BYTES + TAPE_BYTES);
+  c.key = a.key;
+  c.tgt = a.tgt;
+  c.dl = a.pbs[0].dl;
+  const int after_keys = c.keyb + (GATE == GATE_HASH ? KEYB_WORDS * 8 : 0);
+  uint32_t *stg = (uint32_t *)(c.sb + after_keys) + wave * STG_WORDS;
+  unsigned long long *pc = (unsigned long long *)(c.sb + after_keys + nwaves * STG_WORDS * 4); // np x 16 per WG
+  for (int i = threadIdx.x; i < a.np * E13_PCNT; i += blockDim.x) pc[i] = 0ull;
+  if (GATE == GATE_HASH) fill_keyb(c.sb, c.keyb, a.key);
+  __syncthreads();
+
+  Lane L;
+  L.dlcap = -1;
+  L.tcap = 0x7FFFFFFF;
+  int active = 0, t0 = 0, budget = 0, nintro = 0, prob = 0, depth = 0, nbound = 0, nq = 0;
+  long long top = 0, bot = 0, staged = -1;
+  int cur_slot = 0, busy = 0, round = 0, idle_polls = 0;
+  uint64_t w_slots = 0, w_rounds = 0;
+  unsigned long long w_steps = 0;
+
+  for (;;) {
+    // ---- refill idle lanes from the private stack ----
+    for (int tries = 0; tries < a.refill_tries; ++tries) {
+      const uint64_t need = ballot(!active);
+      if (need == 0 || top == bot) break;
+      const long long tidx = top - 1;
+      if (staged != tidx) {
+        if (staged >= bot && staged < top && lane == 0) stk_node(a, wid, staged)->next_slot = cur_slot;
+        wave_mem_sync();
+        stage_node(stg, (const uint32_t *)stk_node(a, wid, tidx), a.pbs, lane);
+        staged = tidx;
+        cur_slot = uni((int)stg[15]);
+      }
+      const int sprob = uni((int)stg[12]);
+      const E13Problem *pb = a.pbs + sprob;
+      const int nslot = pb->nslot;
+      const int cnt = __popcll(need);
+      const int give = min(cnt, nslot - cur_slot);
+      if (!active) {
+        const int r = (int)rank_in(need);
+        if (r < give) {
+          const int slot = cur_slot + r;
+          const int pend = (int)stg[11];
+          int nb = (int)stg[9], nin2 = (int)stg[10], val = -1, ok = 1;
+          if (pend < 0) {
+            ok = slot == 0;
+          } else {
+            const int orig = pb->base[pend];
+            if (slot == 0) {
+              if (orig == -2) ok = 0;
+              else if (orig == -1) { // keeping a halt: one leaf at the node's stage, 0 steps
+                atomicAdd(&pc[sprob * E13_PCNT + 0], 1ull);
+                atomicAdd(&pc[sprob * E13_PCNT + 3 + (int)stg[4]], 1ull);
+                ok = 0;
+              } else val = orig;
+            } else {
+              const int v = slot - 1;
+              if (!(orig == -2 || nb > 0) || v == orig) ok = 0;
+              else {
+                const int n2 = sym_next(v, nin2, pb->sym_lo, pb->sym_hi);
+                if (n2 < 0) ok = 0;
+                else {
+                  if (orig != -2) nb -= 1;
+                  nin2 = n2;
+                  val = v;
+                }
+              }
+            }
+          }
+          if (ok) {
+            prob = sprob;
+            nq = pb->nq;
+            c.dl = pb->dl;
+            L.q = (int)stg[0];
+            L.row = c.rbl + (int)stg[1] * 256;
+            L.t = (int)stg[2];
+            L.nt = (int)stg[3];
+            L.stage = (int)stg[4];
+            L.st0 = (int)stg[5];
+            L.q0 = (int)stg[6];
+            L.flags = (int)stg[7];
+            L.ones = (int)stg[8];
+            L.hb = (uint64_t)stg[16] | ((uint64_t)stg[17] << 32);
+            L.ch0 = (uint64_t)stg[18] | ((uint64_t)stg[19] << 32);
+            L.ch1 = (uint64_t)stg[20] | ((uint64_t)stg[21] << 32);
+            L.ch2 = (uint64_t)stg[22] | ((uint64_t)stg[23] << 32);
+            arm_stage(L, c);
+            t0 = L.t;
+            budget = nb;
+            nintro = nin2;
+            depth = (int)stg[13] + (pend >= 0);
+            nbound = (int)stg[14] + (pend >= 0);
+            const int rs = (nb == 0 && (int)stg[9] > 0) ? 92 : 76; // edits used up: unread cells keep their value
+            for (int r2 = 0; r2 < nq; ++r2) {
+              uint32_t pr = stg[rs + r2];
+              if (pend >> 1 == r2) {
+                const uint32_t e = enc_entry(val);
+                pr = (pend & 1) ? ((pr & 0xFFFFu) | (e << 16)) : ((pr & 0xFFFF0000u) | e);
+              }
+              lds_st(c.sb, c.rbl + r2 * 256, pr);
+            }
+            for (int w = 0; w < E13_TW; ++w) lds_st(c.sb, c.tbl + w * 256, stg[E13_NODE_W_TAPE + w]);
+            active = 1;
+          }
+        }
+      }
+      cur_slot += give;
+      if (cur_slot >= nslot) { // every child slot handed out: pop
+        top -= 1;
+        staged = -1;
+      }
+    }
+    // ---- an empty private stack: take a node from the pool ----
+    if (top == bot) {
+      const int idle = __popcll(ballot(!active));
+      if (idle >= a.pool_min_idle || idle == 64) {
+        long long got = -1;
+        int inc = 0;
+        if (lane == 0) {
+          uint64_t h = __hip_atomic_load(&a.sch->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          const uint64_t t = __hip_atomic_load(&a.sch->tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (h < t) {
+            if (!busy) { // counted busy before the claim, so the end test never misses a node in flight
+              atomicAdd(&a.sch->busy, 1);
+              vm_drain();
+              inc = 1;
+            }
+            if (__hip_atomic_compare_exchange_strong(&a.sch->head, &h, h + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT))
+              got = (long long)h;
+          }
+        }
+        if (uni(inc)) busy = 1;
+        got = uni64(got);
+        if (got >= 0) {
+          // the donor reserved this slot and is writing it: wait for its sequence word
+          const size_t ps = (size_t)(got % a.pool_cap);
+          uint32_t spins = 0, ready = 0;
+          for (;;) {
+            if (lane == 0) ready = ld_agent64(&a.pool_seq[ps]) == (uint64_t)got + 1;
+            ready = uni((int)ready);
+            if (ready) break;
+            if (++spins > a.spin_limit) {
+              if (lane == 0) atomicMax(&a.sch->error, 2);
+              break;
+            }
+            __builtin_amdgcn_s_sleep(8);
+          }
+          if (!ready) break;
+          const uint32_t *src = (const uint32_t *)(a.pool + ps);
+          uint32_t *dst = (uint32_t *)stk_node(a, wid, top);
+          const uint32_t v0 = ld_agent(src + lane);
+          const uint32_t v1 = lane < E13_NODE_WORDS - 64 ? ld_agent(src + 64 + lane) : 0u;
+          dst[lane] = v0;
+          if (lane < E13_NODE_WORDS - 64) dst[64 + lane] = v1;
+          vm_drain();
+          if (lane == 0) { // the slot is free for the producer of node got + pool_cap
+            st_agent64(&a.pool_seq[ps], (uint64_t)got + (uint64_t)a.pool_cap);
+            atomicAdd(&a.sch->pool_takes, 1);
+          }
+          top += 1;
+          continue; // refill from it at once
+        }
+      }
+    }
+    const uint64_t act = ballot(active);
+    if (act == 0) {
+      if (top != bot) continue;
+      // no work here: leave the busy count, then look for the end
+      if (busy) {
+        vm_drain(); // every donation of this wave is complete before it stops counting as busy
+        if (lane == 0) atomicSub(&a.sch->busy, 1);
+        busy = 0;
+      }
+      int done = 0;
+      if (lane == 0) {
+        // busy first, then the pool: a wave that sees no busy wave and an empty
+        // pool after that can leave; one that sees a node keeps trying to claim
+        const int b = ld_agent_i(&a.sch->busy);
+        vm_drain();
+        const uint64_t h = ld_agent64(&a.sch->head);
+        const uint64_t t = ld_agent64(&a.sch->tail);
+        const int err = ld_agent_i(&a.sch->error);
+        done = ((b == 0 && h >= t) || err) ? 1 : 0;
+      }
+      if (uni(done)) {
+        if (lane == 0) atomicAdd(&a.sch->exited, 1);
+        break;
+      }
+      if (++idle_polls > a.idle_limit) { // watchdog: no work for seconds while others still count busy
+        if (lane == 0) atomicMax(&a.sch->error, 3);
+        break;
+      }
+      if (idle_polls > 8) __builtin_amdgcn_s_sleep(127);
+      else __builtin_amdgcn_s_sleep(16);
+      continue;
+    }
+    idle_polls = 0;
+    if (!busy) { // lanes still running with an empty stack and no pool node: still busy
+      if (lane == 0) atomicAdd(&a.sch->busy, 1);
+      busy = 1;
+    }
+    w_rounds++;
+    w_slots += (uint64_t)__popcll(act);
+    // scheduler state for after the round (the loads complete while the lanes step)
+    int s_busy = 0, s_exited = 0, s_err = 0;
+    if (lane == 0) {
+      s_busy = ld_agent_i(&a.sch->busy);
+      s_exited = ld_agent_i(&a.sch->exited);
+      s_err = ld_agent_i(&a.sch->error);
+    }
+    // ---- a round of steps ----
+    int ev = EV_NONE, evcell = -1;
+    if (active) ev = run_slice<GATE>(L, c, a.slice, evcell);
+    // ---- finished machines ----
+    const bool fin = active && ev != EV_NONE;
+    if (fin) {
+      const unsigned long long d = (unsigned long long)(L.t - t0);
+      w_steps += d;
+      if (ev == EV_DEAD) {
+        atomicAdd(&pc[prob * E13_PCNT + 0], 1ull);
+        atomicAdd(&pc[prob * E13_PCNT + 3 + L.stage], 1ull);
+      } else if (ev == EV_SURV) {
+        atomicAdd(&pc[prob * E13_PCNT + 0], 1ull);
+        atomicAdd(&pc[prob * E13_PCNT + 2], 1ull);
+        const uint32_t sidx = atomicAdd(&a.ctr->nsurv, 1u);
+        if (sidx < a.surv_cap) {
+          E13SurvRec *o = a.surv + sidx;
+          uint32_t *tw = (uint32_t *)o->tab;
+#pragma unroll
+          for (int i = 0; i < E13_MAXCELLS / 4; ++i) tw[i] = codes_word(c, nq, i);
+          o->st0 = L.st0;
+          o->head0 = L.q0 - E13_QOFF;
+          o->abs_ok = L.flags & 1;
+          o->rel_ok = (L.flags >> 1) & 1;
+          o->steps = L.t;
+          o->prob = prob;
+          o->ch[0] = L.ch0;
+          o->ch[1] = L.ch1;
+          o->ch[2] = L.ch2;
+        }
+      }
+      atomicAdd(&pc[prob * E13_PCNT + 1], d);
+    }
+    // ---- live children onto the private stack ----
+    const bool emit = fin && (ev == EV_STOP || ev == EV_CAP);
+    const uint64_t em = ballot(emit);
+    if (em) {
+      const int n = __popcll(em);
+      if (top - bot + n > a.stk_cap) {
+        if (lane == 0) atomicMax(&a.sch->error, 1);
+        break;
+      }
+      if (staged >= bot && staged < top && lane == 0) stk_node(a, wid, staged)->next_slot = cur_slot;
+      if (emit) {
+        const int r = (int)rank_in(em);
+        write_node(c, stk_node(a, wid, top + r), L, budget, nintro, ev == EV_STOP ? evcell : -1, prob, depth, nbound,
+                   nq);
+        if (r == n - 1) stage_own(c, stg, L, budget, nintro, ev == EV_STOP ? evcell : -1, prob, depth, nbound, nq);
+      }
+      stage_rows(stg, a.pbs, lane);
+      top += n;
+      staged = top - 1;
+      cur_slot = 0;
+    }
+    if (fin) active = 0;
+    if (uni(s_err)) break;
+    // ---- feed the pool when waves are hungry: give away the oldest node ----
+    ++round;
+    const int hungry_est = uni(a.total_waves - s_busy - s_exited);
+    if (round % a.donate_every == 0 && top - bot >= 2 && hungry_est > 0) {
+      int give = 0;
+      uint64_t t = 0;
+      if (lane == 0) {
+        const uint64_t h = ld_agent64(&a.sch->head);
+        t = ld_agent64(&a.sch->tail);
+        if ((long long)(t - h) < (long long)hungry_est && t - h < (uint64_t)a.pool_cap &&
+            ld_agent64(&a.pool_seq[t % a.pool_cap]) == t &&
+            __hip_atomic_compare_exchange_strong(&a.sch->tail, &t, t + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT))
+          give = 1;
+      }
+      give = uni(give);
+      if (give) {
+        t = (uint64_t)uni64((long long)t);
+        if (staged == bot) staged = -1;
+        wave_mem_sync();
+        const uint32_t *src = (const uint32_t *)stk_node(a, wid, bot);
+        uint32_t *dst = (uint32_t *)(a.pool + t % a.pool_cap);
+        const uint32_t v0 = src[lane];
+        const uint32_t v1 = lane < E13_NODE_WORDS - 64 ? src[64 + lane] : 0u;
+        st_agent(dst + lane, v0);
+        if (lane < E13_NODE_WORDS - 64) st_agent(dst + 64 + lane, v1);
+        vm_drain(); // the node is complete before its sequence word says so
+        if (lane == 0) {
+          st_agent64(&a.pool_seq[t % a.pool_cap], t + 1);
+          atomicAdd(&a.sch->donations, 1);
+        }
+        bot += 1;
+      }
+    }
+  }
+  // ---- flush ----
+  if (lane == 0) {
+    atomicAdd((unsigned long long *)&a.ctr->rounds, (unsigned long long)w_rounds);
+    atomicAdd((unsigned long long *)&a.ctr->lane_slots, (unsigned long long)w_slots);
+  }
+  const uint64_t s_steps = wave_sum64(w_steps);
+  if (lane == 0) atomicAdd((unsigned long long *)&a.ctr->steps, (unsigned long long)s_steps);
+  __syncthreads();
+  for (int i = threadIdx.x; i < a.np * E13_PCNT; i += blockDim.x)
+    if (pc[i]) atomicAdd((unsigned long long *)&a.pcnt[i], pc[i]);
+}
+
+// ---------------------------------------------------------------------------
 // Thin C ABI over the launches.  Shapes are checked by the caller (host C).
 // ---------------------------------------------------------------------------
 extern "C" {
@@ -614,6 +1100,19 @@ int e13k_launch_expand(const E13ExpandArgs *a, int grid, int wg_threads, int gat
     if (refill) hipLaunchKernelGGL((k_expand<GATE_COUNT, 1>), dim3(grid), dim3(wg_threads), lds, s, *a);
     else hipLaunchKernelGGL((k_expand<GATE_COUNT, 0>), dim3(grid), dim3(wg_threads), lds, s, *a);
   }
+  return (int)hipGetLastError();
+}
+
+size_t e13k_search_lds_bytes(int wg_threads, int gate, int np) {
+  const int nw = wg_threads / WAVE;
+  return (size_t)nw * (ROWS_BYTES + TAPE_BYTES + STG_WORDS * 4) + (gate == GATE_HASH ? KEYB_WORDS * 8 : 0) +
+         (size_t)np * E13_PCNT * 8;
+}
+
+int e13k_launch_search(const E13SearchArgs *a, int grid, int wg_threads, int gate, hipStream_t s) {
+  const size_t lds = e13k_search_lds_bytes(wg_threads, gate, a->np);
+  if (gate == GATE_HASH) hipLaunchKernelGGL((k_search<GATE_HASH>), dim3(grid), dim3(wg_threads), lds, s, *a);
+  else hipLaunchKernelGGL((k_search<GATE_COUNT>), dim3(grid), dim3(wg_threads), lds, s, *a);
   return (int)hipGetLastError();
 }
 

@@ -63,7 +63,7 @@ typedef struct {
   int64_t steps;
   uint64_t ch[3];           /* counter hashes at the stage starts, classes 0..2 */
   int32_t recheck_ok;       /* 1: the plain unhashed re-check agrees */
-  int32_t pad;
+  int32_t prob;             /* index of its problem (e13_search_many) */
 } e13_survivor;
 
 typedef struct {
@@ -75,7 +75,9 @@ typedef struct {
 /* Launch and schedule settings.  Any field <= 0 takes the default (or the
    autotuned value when one is stored for this device). */
 typedef struct {
-  int32_t form;        /* 1 plain (one item per lane, host loop), 2 batched (persistent lanes) */
+  int32_t form;        /* 1 plain: one machine per lane, one launch per batch of nodes, host loop;
+                          2 batched: the same host loop with persistent lanes refilled in the launch;
+                          3 persistent: one launch for the whole search, device work queue */
   int32_t slice;       /* steps per round between lane refills */
   int32_t wg_threads;  /* 64 .. 256 */
   int32_t wgs_per_cu;  /* persistent grid = CUs x this */
@@ -93,6 +95,7 @@ typedef struct {
   int64_t lane_slots;    /* active lanes summed over rounds */
   int64_t peak_stack;
   int32_t slice, wg_threads, grid, gate, form, pad;
+  int64_t donations, pool_takes; /* persistent form: nodes moved between waves */
 } e13_stats;
 
 typedef struct e13_ctx e13_ctx;
@@ -123,6 +126,12 @@ int e13_setup(const int32_t *base, int nq_base, int X, int fmax, const uint8_t *
    surv_cap are counted in *n_surv but not stored. */
 int e13_search(e13_ctx *ctx, const e13_problem *p, e13_counts *out, e13_survivor *surv, uint64_t surv_cap,
                uint64_t *n_surv);
+
+/* Several problems (e.g. every deleted state X) in one persistent launch: the
+   drop-in for a run_delete() loop over X (e13search.c:461-463).  out[k] are
+   the counts of ps[k]; survivors carry their problem index. */
+int e13_search_many(e13_ctx *ctx, const e13_problem *ps, int np, e13_counts *out, e13_survivor *surv,
+                    uint64_t surv_cap, uint64_t *n_surv);
 
 /* Complete tables from the blank tape: run_full() (e13search.c:428-433) over a
    batch.  tabs: n x ncells codes (>= 0 move, -1 halt); caps: NULL for the

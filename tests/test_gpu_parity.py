@@ -115,14 +115,14 @@ def check_against(sums, golden, keys=("leaves", "survivors", "steps", "hist")):
             assert sums[X][k] == g[k], (X, k, sums[X][k], g[k])
 
 
-@pytest.mark.parametrize("form,gate", [(2, 1), (2, 2), (1, 1)])
+@pytest.mark.parametrize("form,gate", [(3, 1), (3, 2), (2, 1), (1, 1)])
 def test_local_edits_fmax2(e13, gpu_ctx, base, form, gate):
     say(f"search: delete FMAX=2 X=0..13 form={form} gate={gate}")
     gpu_ctx.set_tuning(form=form, gate=gate)
     try:
         sums, lines, hashes = search_all(e13, gpu_ctx, base, XS, fmax=2)
     finally:
-        gpu_ctx.set_tuning(form=2, gate=1)
+        gpu_ctx.set_tuning(form=3, gate=1)
     check_against(sums, parse_summaries(read_text(REF_DUMP, "delete_f2.txt")))
     check_against(sums, parse_summaries(read_text(REF_RESULTS, "1_local_edits_log_f2.txt")))
     assert lines == survivor_lines(read_text(REF_RESULTS, "1_local_edits_surv_f2.txt"))
@@ -131,9 +131,36 @@ def test_local_edits_fmax2(e13, gpu_ctx, base, form, gate):
         assert ref[(X, tab)] == ch
 
 
-def test_local_edits_fmax3(e13, gpu_ctx, base):
-    say("search: delete FMAX=3 X=0..13")
-    sums, lines, hashes = search_all(e13, gpu_ctx, base, XS, fmax=3)
+def search_many(e13, ctx, base, Xs, fmax):
+    probs = [e13.setup(base, X, fmax) for X in Xs]
+    cs, sv, ns = ctx.search_many(probs)
+    assert ns == len(sv) == sum(c.survivors for c in cs)
+    sums = {X: {"leaves": c.leaves, "survivors": c.survivors, "steps": c.steps, "hist": list(c.dead_at_stage)}
+            for X, c in zip(Xs, cs)}
+    lines, hashes = [], []
+    for s in sv:
+        assert s.recheck_ok == 1
+        p = probs[s.prob]
+        line = e13.format_survivor(p, s)
+        lines.append(line)
+        hashes.append((p.X, line.split("table=")[1].replace("?", ""), tuple(s.ch)))
+    return sums, sorted(lines), hashes
+
+
+def test_local_edits_fmax2_one_launch(e13, gpu_ctx, base):
+    say("search_many: delete FMAX=2 X=0..13 in one persistent launch")
+    sums, lines, hashes = search_many(e13, gpu_ctx, base, XS, 2)
+    check_against(sums, parse_summaries(read_text(REF_DUMP, "delete_f2.txt")))
+    assert lines == survivor_lines(read_text(REF_RESULTS, "1_local_edits_surv_f2.txt"))
+
+
+@pytest.mark.parametrize("many", [True, False])
+def test_local_edits_fmax3(e13, gpu_ctx, base, many):
+    say(f"search: delete FMAX=3 X=0..13 one_launch={many}")
+    if many:
+        sums, lines, hashes = search_many(e13, gpu_ctx, base, XS, 3)
+    else:
+        sums, lines, hashes = search_all(e13, gpu_ctx, base, XS, fmax=3)
     check_against(sums, parse_summaries(read_text(REF_DUMP, "delete_f3.txt")))
     assert lines == survivor_lines(read_text(REF_RESULTS, "1_local_edits_surv_f3.txt"))
     ref = ref_hashes()
@@ -180,16 +207,21 @@ def test_counter_3state_symmetric(e13, gpu_ctx, base):
         assert ref[(X, tab)] == ch
 
 
-@pytest.mark.parametrize("tune", [dict(slice=1, cap=7), dict(slice=5, cap=97, batch=3), dict(slice=200, cap=100000,
-                                                                                               batch=1 << 20),
-                                  dict(wg_threads=64, wgs_per_cu=1, refill_tries=1), dict(form=1, cap=33, batch=17)])
+@pytest.mark.parametrize("tune", [dict(form=2, slice=1, cap=7), dict(form=2, slice=5, cap=97, batch=3),
+                                  dict(form=2, slice=200, cap=100000, batch=1 << 18),
+                                  dict(form=2, wg_threads=64, wgs_per_cu=1, refill_tries=1),
+                                  dict(form=1, cap=33, batch=17),
+                                  dict(form=3, slice=1), dict(form=3, slice=7, refill_tries=1),
+                                  dict(form=3, slice=300, wg_threads=128, wgs_per_cu=3),
+                                  dict(form=3, wg_threads=64, wgs_per_cu=1, stack_nodes=1 << 16)])
 def test_results_invariant_under_tuning(e13, gpu_ctx, base, tune):
     say(f"search: delete FMAX=2 X=7,8,12 tuning={tune}")
     gpu_ctx.set_tuning(**tune)
     try:
         sums, lines, hashes = search_all(e13, gpu_ctx, base, [7, 8, 12], fmax=2)
     finally:
-        gpu_ctx.set_tuning(form=2, slice=32, cap=4096, batch=1 << 17, wg_threads=256, wgs_per_cu=2, refill_tries=2)
+        gpu_ctx.set_tuning(form=3, slice=32, cap=4096, batch=1 << 17, wg_threads=256, wgs_per_cu=2, refill_tries=2,
+                           stack_nodes=1 << 25)
     golden = parse_summaries(read_text(REF_DUMP, "delete_f2.txt"))
     check_against(sums, {X: golden[X] for X in (7, 8, 12)})
     assert lines == survivor_lines(read_text(REF_RESULTS, "1_local_edits_surv_f2.txt"))
