@@ -7,8 +7,11 @@ dead-at-stage histograms are compared with the reference's (the golden
 fixture tests/golden/ref_dump/delete_f3.txt); a mismatch aborts the run.
 
 value = simulated TM steps per second over the K timed steps (the whole
-search: kernels, the host driver loop and the survivor re-check).  Leaves/s
-and the CPU baseline are reported beside it.
+search: every X in one persistent launch, plus the survivor re-check).
+Leaves/s and the CPU baseline are reported beside it, and the two larger
+searches the build targets -- the symmetric 3-state counter search
+(3,078,656,272 leaves) and the FMAX=4 local-edit search (5,986,617,022
+leaves) -- are run once each with their counts checked.
 
 roofline: the dominant kernel is the expansion kernel.  Its algorithmic work
 is 13 integer lane-ops per simulated step; achieved = steps x 13 / kernel
@@ -52,17 +55,35 @@ def golden():
 
 
 def one_pass(e13, ctx, problems, want):
+    """The whole workload: every X in one search (one persistent launch)."""
+    cs, sv, ns = ctx.search_many([p for _, p in problems], surv_cap=1024)
     leaves = steps = 0
-    for X, p in problems:
-        c, sv, ns = ctx.search(p, surv_cap=1024)
+    for (X, _), c in zip(problems, cs):
         got = (c.leaves, c.survivors, c.steps, list(c.dead_at_stage))
         if got != want[X]:
             raise SystemExit(f"bench: parity failure at X={X}: {got} != {want[X]}")
-        if any(s.recheck_ok != 1 for s in sv):
-            raise SystemExit(f"bench: survivor re-check failed at X={X}")
         leaves += c.leaves
         steps += c.steps
+    if ns != len(sv) or any(s.recheck_ok != 1 for s in sv):
+        raise SystemExit("bench: survivor re-check failed")
     return leaves, steps
+
+
+def extra_workload(e13, ctx, name, problems, want_leaves, want_steps, want_surv):
+    """One timed run of a further north-star search (counts checked)."""
+    ctx.sync()
+    s0 = ctx.stats()
+    t0 = time.perf_counter()
+    cs, sv, ns = ctx.search_many(problems, surv_cap=4096)
+    dt = time.perf_counter() - t0
+    s1 = ctx.stats()
+    leaves = sum(c.leaves for c in cs)
+    steps = sum(c.steps for c in cs)
+    ok = leaves == want_leaves and ns == want_surv and (want_steps is None or steps == want_steps)
+    ok = ok and all(s.recheck_ok == 1 for s in sv)
+    return {"workload": name, "leaves": leaves, "steps": steps, "survivors": ns, "bit_identical_counts": ok,
+            "seconds": dt, "steps_per_s": steps / dt, "leaves_per_s": leaves / dt,
+            "kernel_ms": s1["kernel_ms"] - s0["kernel_ms"]}
 
 
 def cpu_baseline(max_s=120):
@@ -107,6 +128,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extra", action="store_true", help="skip the 3-state counter and FMAX=4 searches")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the search shards by subtree across GPUs, but no multi-GPU "
@@ -141,6 +163,14 @@ def main():
     kernel_s = (s1["kernel_ms"] - s0["kernel_ms"]) / 1e3
     launches = s1["launches"] - s0["launches"]
     achieved = steps * OPS_PER_STEP / kernel_s
+    cyc = [b - a for a, b in zip(s0["cycles"], s1["cycles"])]
+    extra = []
+    if not args.no_extra:
+        counter = e13.setup(base, 13, free_cells="D0 G0 K0 K1 L1 M1".split(), sym=(10, 13))
+        extra.append(extra_workload(e13, ctx, "3-state counter, freesym X=13 10..13 D0 G0 K0 K1 L1 M1", [counter],
+                                    3078656272, 319112357176, 61))
+        f4 = [e13.setup(base, X, 4) for X in XS]
+        extra.append(extra_workload(e13, ctx, "delete FMAX=4 X=0..13", f4, 5986617022, None, 57))
     line = {
         "metric": "simulated TM steps/s, local-edit search FMAX=3 (X=0..13) on erdos_14, bit-identical to the "
                   "reference",
@@ -163,9 +193,12 @@ def main():
         "bit_identical": True,
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_PEAK / 1e12, "unit": "TOP/s",
                      "frac": achieved / VALU_PEAK, "traffic": traffic_from_profiles(),
-                     "kernel": "k_expand", "launches": launches,
+                     "kernel": "k_search" if s1["form"] == 3 else "k_expand", "launches": launches,
                      "kernel_ms_per_launch": kernel_s * 1e3 / max(launches, 1),
-                     "steps_per_s_kernel": steps / kernel_s},
+                     "steps_per_s_kernel": steps / kernel_s,
+                     "wave_cycle_split": {"refill": cyc[0] / max(sum(cyc), 1), "steps": cyc[1] / max(sum(cyc), 1),
+                                          "finish": cyc[2] / max(sum(cyc), 1), "pool_idle": cyc[3] / max(sum(cyc), 1)}},
+        "north_star_workloads": extra,
         "cpu_baseline": None if args.no_cpu else cpu_baseline(),
     }
     ctx.close()

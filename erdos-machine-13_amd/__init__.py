@@ -80,10 +80,13 @@ class Stats(ctypes.Structure):
                 ("lane_slots", ctypes.c_int64), ("peak_stack", ctypes.c_int64), ("slice", ctypes.c_int32),
                 ("wg_threads", ctypes.c_int32), ("grid", ctypes.c_int32), ("gate", ctypes.c_int32),
                 ("form", ctypes.c_int32), ("pad", ctypes.c_int32), ("donations", ctypes.c_int64),
-                ("pool_takes", ctypes.c_int64)]
+                ("pool_takes", ctypes.c_int64), ("cycles", ctypes.c_int64 * 4)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("pad", "cycles")}
+        d["cycles"] = list(self.cycles)
+        return d
+
 
 
 EXPORTS = ("e13_open", "e13_close", "e13_strerror", "e13_set_tuning", "e13_get_tuning", "e13_get_stats",

@@ -47,6 +47,18 @@ static uint64_t text_hash(const char *s) {
   return mix64(h);
 }
 
+static void print_cycles(const e13_stats *a, const e13_stats *b) {
+  double c[4], t = 0;
+  for (int j = 0; j < 4; j++) {
+    c[j] = (double)(b->cycles[j] - a->cycles[j]);
+    t += c[j];
+  }
+  if (t > 0)
+    fprintf(stderr, "  wave cycles: refill %.1f%% steps %.1f%% finish %.1f%% pool/idle %.1f%% (%.3e per round)\n",
+            100 * c[0] / t, 100 * c[1] / t, 100 * c[2] / t, 100 * c[3] / t,
+            (b->rounds - a->rounds) > 0 ? t / (double)(b->rounds - a->rounds) : 0.0);
+}
+
 static char *read_file(const char *path) {
   FILE *f = fopen(path, "rb");
   if (!f) return NULL;
@@ -122,6 +134,7 @@ static int run_one(e13_ctx *ctx, const int32_t *base, int nq_base, int X, int fm
           wall > 0 ? (double)cnt.steps / wall : 0.0, (long long)(st1.rounds - st0.rounds),
           (long long)(st1.lane_slots - st0.lane_slots), (long long)(st1.donations - st0.donations),
           (long long)(st1.pool_takes - st0.pool_takes), X, (unsigned long long)digest);
+  print_cycles(&st0, &st1);
   if (ns > cap) fprintf(stderr, "WARNING: %llu survivors, only %llu printed\n", (unsigned long long)ns,
                         (unsigned long long)cap);
   fflush(stderr);
@@ -192,6 +205,7 @@ static int run_many(e13_ctx *ctx, const int32_t *base, int nq_base, int nx, char
           np, st1.kernel_ms - st0.kernel_ms, wall > 0 ? (double)tot / wall : 0.0, (long long)(st1.rounds - st0.rounds),
           (long long)(st1.lane_slots - st0.lane_slots), (long long)(st1.donations - st0.donations),
           (long long)(st1.pool_takes - st0.pool_takes));
+  print_cycles(&st0, &st1);
   free(sv);
   return 0;
 }

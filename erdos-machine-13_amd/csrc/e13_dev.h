@@ -74,7 +74,7 @@ typedef struct E13Counters {
   uint32_t nout;        /* children written by the current launch */
   uint32_t nsurv;       /* survivor slots taken (may exceed the capacity) */
   uint32_t overflow;    /* set when a child did not fit the output buffer */
-  uint64_t pad[4];
+  uint64_t cyc[4];      /* wave cycles (s_memtime) in refill, stepping, finishing, scheduling */
 } E13Counters;
 
 /* Per-problem constants of the expansion kernel. */
@@ -94,6 +94,7 @@ typedef struct E13ExpandArgs {
   const E13Problem *pb;   /* device copy of the problem constants */
   uint32_t nin, out_cap, surv_cap, nslot;
   int32_t slice, cap, refill_tries, nq;
+  uint32_t chunk, pad0, pad1, pad2; /* work items a wave claims per atomic */
 } E13ExpandArgs;
 
 /* Work-pool and termination state of the persistent search (one per search). */
@@ -123,7 +124,8 @@ typedef struct E13SearchArgs {
   const uint64_t *tgt;
   int32_t np, slice, stk_cap, pool_cap;
   int32_t refill_tries, donate_every, pool_min_idle, total_waves;
-  uint32_t surv_cap, spin_limit, idle_limit, pad0;
+  uint32_t surv_cap, spin_limit, idle_limit, sched_every; /* sched_every: power of two, rounds between
+                                                              reads of the shared scheduler state */
 } E13SearchArgs;
 
 typedef struct E13BatchArgs {

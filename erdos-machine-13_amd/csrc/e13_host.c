@@ -435,6 +435,7 @@ int e13_search(e13_ctx *c, const e13_problem *p, e13_counts *out, e13_survivor *
   a.slice = tu->slice;
   a.cap = tu->cap;
   a.refill_tries = tu->refill_tries;
+  a.chunk = 256;
 
   E13Node root;
   root_node(p, &root);
@@ -475,6 +476,10 @@ int e13_search(e13_ctx *c, const e13_problem *p, e13_counts *out, e13_survivor *
     c->stats.kernel_ms += ms;
     c->stats.launches++;
     uint32_t nout = io[1];
+    if (io[3] & 4u) {
+      fprintf(stderr, "libe13: dynamic LDS does not start at LDS address 0\n");
+      return E13_EHIP;
+    }
     if (io[3]) return E13_ESTACK; /* output buffer overflow (cannot happen: one child per item) */
     if (base_i + nout > c->stack_cap) return E13_ESTACK;
     if (nout)
@@ -492,6 +497,7 @@ int e13_search(e13_ctx *c, const e13_problem *p, e13_counts *out, e13_survivor *
   for (int j = 0; j <= E13_NS; j++) out->dead_at_stage[j] = h.hist[j];
   c->stats.rounds += (int64_t)h.rounds;
   c->stats.lane_slots += (int64_t)h.lane_slots;
+  for (int j = 0; j < 4; j++) c->stats.cycles[j] += (int64_t)h.cyc[j];
   if ((int64_t)peak > c->stats.peak_stack) c->stats.peak_stack = (int64_t)peak;
   c->stats.slice = tu->slice;
   c->stats.wg_threads = wg;
@@ -613,7 +619,8 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
   a.stk_cap = stk_cap;
   a.pool_cap = (int32_t)pool_cap;
   a.refill_tries = tu->refill_tries;
-  a.donate_every = 1;
+  a.donate_every = 4;   /* nodes a wave may give away per scheduler refresh */
+  a.sched_every = 64;   /* rounds between scheduler reads, a power of two */
   a.pool_min_idle = 16;
   a.total_waves = total_waves;
   a.surv_cap = (uint32_t)c->surv_cap;
@@ -641,6 +648,7 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
   c->stats.launches++;
   c->stats.rounds += (int64_t)h.rounds;
   c->stats.lane_slots += (int64_t)h.lane_slots;
+  for (int j = 0; j < 4; j++) c->stats.cycles[j] += (int64_t)h.cyc[j];
   c->stats.slice = tu->slice;
   c->stats.wg_threads = wg;
   c->stats.grid = grid;
@@ -650,8 +658,10 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
   c->stats.pool_takes += sch.pool_takes;
   if (sch.error) {
     fprintf(stderr, "libe13: persistent search aborted (%s)\n",
-            sch.error == 1 ? "a private node stack is full"
-                           : (sch.error == 2 ? "a pool slot was not filled in time" : "idle watchdog"));
+            sch.error == 1   ? "a private node stack is full"
+            : sch.error == 2 ? "a pool slot was not filled in time"
+            : sch.error == 3 ? "idle watchdog"
+                             : "dynamic LDS does not start at LDS address 0");
     return E13_ESTACK;
   }
   for (int k = 0; k < np; k++) {
@@ -767,7 +777,15 @@ int e13_simulate_batch(e13_ctx *c, const int8_t *tabs, int ncells, int n, const 
     return E13_ENOMEM;
   }
   HIPCHK(hipMemcpyAsync(lr, c->d_leaves, (size_t)n * sizeof(E13LeafRec), hipMemcpyDeviceToHost, c->stream));
+  E13Counters hc;
+  HIPCHK(hipMemcpyAsync(&hc, c->d_ctr, sizeof hc, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  if (hc.overflow & 4u) {
+    free(lr);
+    free(h);
+    fprintf(stderr, "libe13: dynamic LDS does not start at LDS address 0\n");
+    return E13_EHIP;
+  }
   float ms = 0;
   if (hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) c->stats.kernel_ms += ms;
   for (int k = 0; k < n; k++) {

@@ -38,8 +38,22 @@ enum { GATE_COUNT = 0, GATE_HASH = 1 };
 
 namespace {
 
-__device__ __forceinline__ uint32_t lds_ld(const char *sb, int off) { return *(const uint32_t *)(sb + off); }
-__device__ __forceinline__ void lds_st(char *sb, int off, uint32_t v) { *(uint32_t *)(sb + off) = v; }
+// LDS words by absolute LDS byte address.  All kernels here use dynamic LDS
+// only, which starts at LDS address 0 (checked at kernel start), so offsets
+// computed from the start of the dynamic area are absolute addresses and need
+// no base added on every access.
+typedef __attribute__((address_space(3))) uint32_t lds_word_t;
+__device__ __forceinline__ uint32_t lds_ld(const char *sb, int off) {
+  (void)sb;
+  return *(const lds_word_t *)(size_t)(uint32_t)off;
+}
+__device__ __forceinline__ void lds_st(char *sb, int off, uint32_t v) {
+  (void)sb;
+  *(lds_word_t *)(size_t)(uint32_t)off = v;
+}
+__device__ __forceinline__ uint32_t lds_base(const void *p) {
+  return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void *)p;
+}
 
 __device__ __forceinline__ uint32_t enc_entry(int c) {
   if (c < 60) return (uint32_t)(c & 3) | ((uint32_t)(c >> 2) << 8);
@@ -164,47 +178,83 @@ __device__ __forceinline__ int stage_event(Lane &L, const Ctx &c) {
   return EV_NONE;
 }
 
+// One step in the reference's exact order (e13search.c:136-175): stage gate,
+// deadline or launch cap, read, unbound/halt stop, write, move, window, t++.
+// The slow path behind the step loop: it always ends the machine or makes the
+// step, so a lane never stalls on a configuration.
+template <int GATE>
+__device__ __forceinline__ int step_once(Lane &L, const Ctx &c, int &evcell) {
+  const bool hit = GATE == GATE_COUNT ? L.ones == L.target : L.hb == L.tgt;
+  if (hit) {
+    const bool pre = GATE == GATE_HASH || ((L.nt == 1) | ((L.row == L.row0) & ((L.q == L.qA) | (L.q == L.qR))));
+    if (pre) {
+      const int ev = stage_event<GATE>(L, c);
+      if (ev) return ev;
+    }
+  }
+  if (L.t >= L.tstop) return L.t > L.deadline ? EV_DEAD : EV_CAP; // deadline (e13search.c:162) or launch cap
+  const uint32_t qb = (uint32_t)L.q & 31u;
+  const int wa = ((L.q >> 5) << 8) + c.tbl;
+  uint32_t word = lds_ld(c.sb, wa);
+  const uint32_t pair = lds_ld(c.sb, L.row);
+  const uint32_t sym = __builtin_amdgcn_ubfe(word, qb, 1u);
+  const uint32_t e = pair >> (sym << 4);
+  const uint32_t nx = e & 0x0F00u;
+  if (nx == 0x0F00u) {
+    if (((e >> 12) & 3u) == 1u) return EV_DEAD; // halt read (e13search.c:166)
+    evcell = ((L.row - c.rbl) >> 7) + (int)sym;  // first read of an unbound cell (e13search.c:164)
+    return EV_STOP;
+  }
+  const uint32_t w = e & 1u;
+  word ^= (w ^ sym) << qb;
+  lds_st(c.sb, wa, word);
+  if (GATE == GATE_COUNT) {
+    L.ones += L.q >= E13_QBLK ? (int)w - (int)sym : 0;
+  } else {
+    const uint64_t kb = *(const uint64_t *)(c.sb + c.keyb + L.q * 8);
+    L.hb ^= kb & (0ull - (uint64_t)(w ^ sym));
+  }
+  L.q += (int)(e & 2u) - 1;
+  if ((unsigned)L.q > (unsigned)E13_QMAX) return EV_DEAD; // off the window; not counted (e13search.c:173)
+  L.row = (int)(nx | (uint32_t)c.rbl);
+  L.t++;
+  return EV_NONE;
+}
+
 // Up to `slice` steps.  Returns an event; for EV_STOP the cell is in evcell.
-// Within a round every running lane takes one step per iteration, so the step
-// count is the round's start count plus the (wave-uniform) iteration index:
-// t needs no per-lane increment, and the deadline/cap test compares a
-// per-lane budget with that index.
+//
+// The loop body has a single exit test, made on the state before the step
+// changes anything: the head left the window on the previous step, the stage
+// gate may fire (ones count and stage-1 state/head prefilter, or the block
+// hash), the deadline/cap is due, or the cell read is unbound or a halt.  Any
+// of them ends the loop and step_once settles the step exactly; everything
+// else runs branch-free.  Within a round every running lane takes one step
+// per iteration, so t is the round's start count plus the (wave-uniform)
+// iteration index and needs no per-lane increment.
 template <int GATE>
 __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int &evcell) {
   const int t_round = L.t;
-  int left = L.tstop - t_round; // steps this lane may still take before the deadline/cap test fires
-  int ev = EV_NONE;
+  const int left = L.tstop - t_round; // iterations before the deadline/cap test is due
+  const bool nt1 = L.nt == 1;
   int r = 0;
   for (; r < slice; ++r) {
-    // stage gate (e13search.c:136)
-    const bool hit = GATE == GATE_COUNT ? L.ones == L.target : L.hb == L.tgt;
-    if (hit) {
-      const bool pre =
-          GATE == GATE_HASH || ((L.nt == 1) | ((L.row == L.row0) & ((L.q == L.qA) | (L.q == L.qR))));
-      if (pre) {
-        L.t = t_round + r;
-        ev = stage_event<GATE>(L, c);
-        if (ev) break;
-        left = L.tstop - t_round;
-      }
-    }
     const uint32_t qb = (uint32_t)L.q & 31u;
-    const int wa = (int)(__builtin_amdgcn_ubfe((uint32_t)L.q, 5u, 6u) << 8) + c.tbl;
+    // q may be one step outside the window here (q = -1 or 1362, caught by the
+    // exit test below): the arithmetic shift keeps that read inside the LDS
+    // allocation (words -1..42 of this lane's tape area)
+    const int wa = (__builtin_amdgcn_sbfe(L.q, 5u, 27u) << 8) + c.tbl;
     uint32_t word = lds_ld(c.sb, wa);
     const uint32_t pair = lds_ld(c.sb, L.row);
     const uint32_t sym = __builtin_amdgcn_ubfe(word, qb, 1u);
     const uint32_t e = pair >> (sym << 4);
     const uint32_t nx = e & 0x0F00u;
-    const bool timeup = r >= left;
-    if (timeup | (nx == 0x0F00u)) {
-      if (timeup) ev = t_round + r > L.deadline ? EV_DEAD : EV_CAP; // deadline (e13search.c:162) or launch cap
-      else if (((e >> 12) & 3u) == 1u) ev = EV_DEAD;                 // halt read (e13search.c:166)
-      else {                                                         // first read of an unbound cell (:164)
-        evcell = ((L.row - c.rbl) >> 7) + (int)sym;
-        ev = EV_STOP;
-      }
-      break;
-    }
+    bool gate;
+    if (GATE == GATE_COUNT)
+      gate = (L.ones == L.target) & (nt1 | ((L.row == L.row0) & ((L.q == L.qA) | (L.q == L.qR))));
+    else
+      gate = L.hb == L.tgt;
+    const bool out = (unsigned)L.q > (unsigned)E13_QMAX;
+    if (out | gate | (r >= left) | (nx == 0x0F00u)) break;
     const uint32_t w = e & 1u;
     word ^= (w ^ sym) << qb;
     lds_st(c.sb, wa, word);
@@ -215,17 +265,28 @@ __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int &
       L.hb ^= kb & (0ull - (uint64_t)(w ^ sym));
     }
     L.q += (int)(e & 2u) - 1;
-    if ((unsigned)L.q > (unsigned)E13_QMAX) { // off the window; the step is not counted (e13search.c:173)
-      ev = EV_DEAD;
-      break;
-    }
     L.row = (int)(nx | (uint32_t)c.rbl);
   }
   L.t = t_round + r;
-  return ev;
+  if (r == slice) return EV_NONE;
+  if ((unsigned)L.q > (unsigned)E13_QMAX) { // the last step left the window: it does not count
+    L.t -= 1;
+    return EV_DEAD;
+  }
+  return step_once<GATE>(L, c, evcell);
 }
 
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ long long uni64(long long v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((unsigned long long)v >> 32));
+  return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
+// Core-clock time stamp for the per-phase cycle counters.
+__device__ __forceinline__ uint64_t stamp() { return __builtin_readcyclecounter(); }
 
 __device__ __forceinline__ uint32_t rank_in(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
@@ -308,6 +369,10 @@ __global__ void __launch_bounds__(256) k_expand(E13ExpandArgs a) {
   c.key = a.key;
   c.tgt = a.tgt;
   c.dl = a.pb->dl;
+  if (lds_base(smem) != 0u) { // the absolute LDS addressing needs the dynamic area at address 0
+    if (threadIdx.x == 0) atomicOr(&a.ctr->overflow, 4u);
+    return;
+  }
   // per-wave stage histogram: 16 words after the tapes and block keys
   uint32_t *whist = (uint32_t *)(c.sb + c.keyb + (GATE == GATE_HASH ? KEYB_WORDS * 8 : 0)) + wave * 16;
   if (lane < 16) whist[lane] = 0;
@@ -323,22 +388,35 @@ __global__ void __launch_bounds__(256) k_expand(E13ExpandArgs a) {
   uint64_t w_slots = 0, w_rounds = 0;
   bool exhausted = false;
   bool first = true;
+  uint32_t ck = 0, ce = 0; // this wave's claimed item range
+  uint64_t cyc0 = 0, cyc1 = 0, cyc2 = 0, ts = stamp();
 
   for (;;) {
     // ---- refill ----
     if (REFILL || first) {
       uint64_t need = ballot(!active);
-      for (int tries = 0; need != 0 && !exhausted && tries < a.refill_tries; ++tries) {
+      for (int tries = 0; need != 0 && tries < a.refill_tries; ++tries) {
         uint32_t k;
         if (REFILL) {
+          // items come from a wave-private range claimed `chunk` at a time:
+          // one atomic on the shared cursor per chunk, not per round
+          if (ck >= ce) {
+            if (exhausted) break;
+            uint32_t b0 = 0;
+            if (lane == 0) b0 = atomicAdd(&a.ctr->item, a.chunk);
+            b0 = (uint32_t)uni((int)b0);
+            if (b0 >= nitems || b0 + a.chunk >= nitems) exhausted = true;
+            ck = b0;
+            ce = min(b0 + a.chunk, nitems);
+            if (ck >= ce) break;
+          }
           const int cnt = __popcll(need);
-          const int leader = __ffsll((long long)need) - 1;
-          uint32_t b0 = 0;
-          if (lane == leader) b0 = atomicAdd(&a.ctr->item, (uint32_t)cnt);
-          b0 = __shfl(b0, leader);
-          if (b0 + (uint32_t)cnt >= nitems) exhausted = true;
-          k = b0 + rank_in(need);
+          const uint32_t give = min((uint32_t)cnt, ce - ck);
+          const uint32_t rk = rank_in(need);
+          k = rk < give ? ck + rk : nitems;
+          ck += give;
         } else {
+          if (tries > 0) break; // one item per lane: never take the same item twice
           k = blockIdx.x * blockDim.x + threadIdx.x;
           exhausted = true;
         }
@@ -412,14 +490,20 @@ __global__ void __launch_bounds__(256) k_expand(E13ExpandArgs a) {
     first = false;
     const uint64_t act = ballot(active);
     if (act == 0) {
-      if (exhausted) break;
+      if (exhausted && ck >= ce) break;
       continue;
     }
     w_rounds++;
     w_slots += (uint64_t)__popcll(act);
+    uint64_t tn = stamp();
+    cyc0 += tn - ts;
+    ts = tn;
     // ---- a round of steps ----
     int ev = EV_NONE, evcell = -1;
     if (active) ev = run_slice<GATE>(L, c, a.slice, evcell);
+    tn = stamp();
+    cyc1 += tn - ts;
+    ts = tn;
     // ---- finished machines ----
     if (active && ev != EV_NONE) {
       active = 0;
@@ -487,9 +571,17 @@ __global__ void __launch_bounds__(256) k_expand(E13ExpandArgs a) {
         }
       }
     }
+    tn = stamp();
+    cyc2 += tn - ts;
+    ts = tn;
     if (!REFILL && ballot(active) == 0) break;
   }
   // ---- flush counters ----
+  if (lane == 0) {
+    atomicAdd((unsigned long long *)&a.ctr->cyc[0], (unsigned long long)cyc0);
+    atomicAdd((unsigned long long *)&a.ctr->cyc[1], (unsigned long long)cyc1);
+    atomicAdd((unsigned long long *)&a.ctr->cyc[2], (unsigned long long)cyc2);
+  }
   const uint64_t s_steps = wave_sum64(my_steps);
   const uint64_t s_leaves = wave_sum64((uint64_t)my_leaves);
   const uint64_t s_surv = wave_sum64((uint64_t)my_surv);
@@ -521,6 +613,10 @@ __global__ void __launch_bounds__(256) k_batch(E13BatchArgs a) {
   c.key = a.key;
   c.tgt = a.tgt;
   c.dl = a.dl;
+  if (lds_base(smem) != 0u) { // the absolute LDS addressing needs the dynamic area at address 0
+    if (threadIdx.x == 0) atomicOr(&a.ctr->overflow, 4u);
+    return;
+  }
   if (GATE == GATE_HASH) fill_keyb(c.sb, c.keyb, a.key);
   __syncthreads();
   Lane L;
@@ -608,12 +704,6 @@ namespace {
 
 #define STG_WORDS 112 // staged node (76) + rows as stored (16) + rows with edits exhausted (16) + 4 spare
 
-__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ __forceinline__ long long uni64(long long v) {
-  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((unsigned long long)v >> 32));
-  return (long long)(((unsigned long long)hi << 32) | lo);
-}
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -771,6 +861,10 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
   c.key = a.key;
   c.tgt = a.tgt;
   c.dl = a.pbs[0].dl;
+  if (lds_base(smem) != 0u) { // the absolute LDS addressing needs the dynamic area at address 0
+    if (threadIdx.x == 0) atomicMax(&a.sch->error, 4);
+    return;
+  }
   const int after_keys = c.keyb + (GATE == GATE_HASH ? KEYB_WORDS * 8 : 0);
   uint32_t *stg = (uint32_t *)(c.sb + after_keys) + wave * STG_WORDS;
   unsigned long long *pc = (unsigned long long *)(c.sb + after_keys + nwaves * STG_WORDS * 4); // np x 16 per WG
@@ -783,9 +877,10 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
   L.tcap = 0x7FFFFFFF;
   int active = 0, t0 = 0, budget = 0, nintro = 0, prob = 0, depth = 0, nbound = 0, nq = 0;
   long long top = 0, bot = 0, staged = -1;
-  int cur_slot = 0, busy = 0, round = 0, idle_polls = 0;
+  int cur_slot = 0, busy = 0, round = 0, idle_polls = 0, hungry = 0;
   uint64_t w_slots = 0, w_rounds = 0;
   unsigned long long w_steps = 0;
+  uint64_t cyc0 = 0, cyc1 = 0, cyc2 = 0, cyc3 = 0, ts = stamp(), tn;
 
   for (;;) {
     // ---- refill idle lanes from the private stack ----
@@ -879,7 +974,10 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
         staged = -1;
       }
     }
-    // ---- an empty private stack: take a node from the pool ----
+    tn = stamp();
+    cyc0 += tn - ts;
+    ts = tn;
+    // ---- an empty private stack: take a node from the pool (time: pool/idle) ----
     if (top == bot) {
       const int idle = __popcll(ballot(!active));
       if (idle >= a.pool_min_idle || idle == 64) {
@@ -960,8 +1058,12 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
         if (lane == 0) atomicMax(&a.sch->error, 3);
         break;
       }
-      if (idle_polls > 8) __builtin_amdgcn_s_sleep(127);
-      else __builtin_amdgcn_s_sleep(16);
+      // exponential backoff: idle waves poll the pool rarely, so busy waves are not slowed
+      const int naps = idle_polls < 7 ? 1 << (idle_polls - 1) : 64;
+      for (int z = 0; z < naps; ++z) __builtin_amdgcn_s_sleep(127);
+      tn = stamp();
+      cyc3 += tn - ts;
+      ts = tn;
       continue;
     }
     idle_polls = 0;
@@ -971,16 +1073,24 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
     }
     w_rounds++;
     w_slots += (uint64_t)__popcll(act);
-    // scheduler state for after the round (the loads complete while the lanes step)
+    // shared scheduler state, read every sched_every rounds (staggered over the
+    // waves so the words are not hot spots); the loads complete while the lanes step
+    const bool refresh = ((round + wid) & (int)(a.sched_every - 1)) == 0;
     int s_busy = 0, s_exited = 0, s_err = 0;
-    if (lane == 0) {
+    if (refresh && lane == 0) {
       s_busy = ld_agent_i(&a.sch->busy);
       s_exited = ld_agent_i(&a.sch->exited);
       s_err = ld_agent_i(&a.sch->error);
     }
-    // ---- a round of steps ----
+    tn = stamp();
+    cyc3 += tn - ts;
+    ts = tn;
+    // ---- a round of steps (time: stepping) ----
     int ev = EV_NONE, evcell = -1;
     if (active) ev = run_slice<GATE>(L, c, a.slice, evcell);
+    tn = stamp();
+    cyc1 += tn - ts;
+    ts = tn;
     // ---- finished machines ----
     const bool fin = active && ev != EV_NONE;
     if (fin) {
@@ -1033,11 +1143,17 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
       cur_slot = 0;
     }
     if (fin) active = 0;
-    if (uni(s_err)) break;
+    tn = stamp();
+    cyc2 += tn - ts;
+    ts = tn;
+    if (refresh) {
+      if (uni(s_err)) break;
+      hungry = uni(a.total_waves - s_busy - s_exited);
+    }
     // ---- feed the pool when waves are hungry: give away the oldest node ----
     ++round;
-    const int hungry_est = uni(a.total_waves - s_busy - s_exited);
-    if (round % a.donate_every == 0 && top - bot >= 2 && hungry_est > 0) {
+    for (int dn = 0; refresh && dn < a.donate_every && top - bot >= 2 && hungry > 0; ++dn) {
+      const int hungry_est = hungry;
       int give = 0;
       uint64_t t = 0;
       if (lane == 0) {
@@ -1066,11 +1182,22 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
           atomicAdd(&a.sch->donations, 1);
         }
         bot += 1;
+        hungry -= 1;
+      } else {
+        break;
       }
     }
+    tn = stamp();
+    cyc3 += tn - ts;
+    ts = tn;
   }
   // ---- flush ----
+  cyc3 += stamp() - ts;
   if (lane == 0) {
+    atomicAdd((unsigned long long *)&a.ctr->cyc[0], (unsigned long long)cyc0);
+    atomicAdd((unsigned long long *)&a.ctr->cyc[1], (unsigned long long)cyc1);
+    atomicAdd((unsigned long long *)&a.ctr->cyc[2], (unsigned long long)cyc2);
+    atomicAdd((unsigned long long *)&a.ctr->cyc[3], (unsigned long long)cyc3);
     atomicAdd((unsigned long long *)&a.ctr->rounds, (unsigned long long)w_rounds);
     atomicAdd((unsigned long long *)&a.ctr->lane_slots, (unsigned long long)w_slots);
   }

@@ -96,6 +96,8 @@ typedef struct {
   int64_t peak_stack;
   int32_t slice, wg_threads, grid, gate, form, pad;
   int64_t donations, pool_takes; /* persistent form: nodes moved between waves */
+  int64_t cycles[4];             /* wave clock cycles spent refilling lanes, stepping, finishing
+                                    machines, and on the work pool / idle (summed over waves) */
 } e13_stats;
 
 typedef struct e13_ctx e13_ctx;
