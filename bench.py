@@ -114,7 +114,7 @@ def cpu_baseline(max_s=120):
 def traffic_from_profiles():
     """HBM bytes per expansion launch from the committed rocprofv3 counter
     summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
-    path = os.path.join(PROFILES, "expand_fmax3_counters.txt")
+    path = os.path.join(PROFILES, "k_search_fmax3_counters.txt")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -129,7 +129,21 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the 3-state counter and FMAX=4 searches")
+    ap.add_argument("--workload", choices=["fmax3", "counter3"], default="fmax3",
+                    help="timed workload (fmax3 is the bench line; counter3 for profiling the kernel)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (rocprof FETCH/WRITE_SIZE calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration << 18  # float32 elements
+        src = torch.ones(n, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n * 4, "copies": 3}))
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the search shards by subtree across GPUs, but no multi-GPU "
               "run is part of it (use --gpus 1)", file=sys.stderr)
@@ -146,6 +160,9 @@ def main():
         base = e13.parse_machine(f.read())
     want = golden()
     problems = [(X, e13.setup(base, X, FMAX)) for X in XS]
+    if args.workload == "counter3":
+        problems = [(13, e13.setup(base, 13, free_cells="D0 G0 K0 K1 L1 M1".split(), sym=(10, 13)))]
+        want = {13: (3078656272, 61, 319112357176, [0, 2957671647, 119843176, 1115937, 22970, 2236, 1, 0, 244, 0])}
     ctx = e13.Context(0)
     for _ in range(args.warmup):
         one_pass(e13, ctx, problems, want)
@@ -170,7 +187,7 @@ def main():
         extra.append(extra_workload(e13, ctx, "3-state counter, freesym X=13 10..13 D0 G0 K0 K1 L1 M1", [counter],
                                     3078656272, 319112357176, 61))
         f4 = [e13.setup(base, X, 4) for X in XS]
-        extra.append(extra_workload(e13, ctx, "delete FMAX=4 X=0..13", f4, 5986617022, None, 57))
+        extra.append(extra_workload(e13, ctx, "delete FMAX=4 X=0..13", f4, 5986617022, 586158995069, 57))
     line = {
         "metric": "simulated TM steps/s, local-edit search FMAX=3 (X=0..13) on erdos_14, bit-identical to the "
                   "reference",
@@ -186,7 +203,9 @@ def main():
         "dtype": "int32",
         "data": "the reference's own search input (erdos_14, deleted states 0..13, <=3 changed cells); no "
                 "synthetic data",
-        "config": {"workload": "delete FMAX=3 X=0..13 (36,383,467 leaves, 4,006,276,021 steps per pass)",
+        "config": {"workload": "delete FMAX=3 X=0..13 (36,383,467 leaves, 4,006,276,021 steps per pass)"
+                   if args.workload == "fmax3" else
+                   "freesym X=13 10..13 D0 G0 K0 K1 L1 M1 (3,078,656,272 leaves, 319,112,357,176 steps)",
                    "form": s1["form"], "slice": s1["slice"], "wg_threads": s1["wg_threads"], "grid": s1["grid"],
                    "gate": "ones-count" if s1["gate"] == 1 else "zobrist-hash"},
         "leaves_per_s": leaves / dt,

@@ -11,7 +11,7 @@
 #                 survivor table.
 #
 # Needs /root/reference (this container only) and `make -C oracle ref`.
-# FMAX=4 takes over an hour on 8 cores: pass --fmax4 to include it.
+# FMAX=4 takes over an hour on 8 cores (62 min here): pass --fmax4 to include it.
 set -euo pipefail
 REF=${REF:-/root/reference}
 HERE=$(cd "$(dirname "$0")" && pwd)

@@ -168,6 +168,21 @@ def test_local_edits_fmax3(e13, gpu_ctx, base, many):
         assert ref[(X, tab)] == ch
 
 
+def test_local_edits_fmax4(e13, gpu_ctx, base):
+    """5,986,617,022 leaves and 586,158,995,069 steps: per-X counts and
+    histograms against the compiled reference's run (ref_dump/delete_f4.txt)
+    and the reference's own log and survivor file."""
+    say("search_many: delete FMAX=4 X=0..13 in one persistent launch")
+    sums, lines, hashes = search_many(e13, gpu_ctx, base, XS, 4)
+    check_against(sums, parse_summaries(read_text(REF_DUMP, "delete_f4.txt")))
+    check_against(sums, parse_summaries(read_text(REF_RESULTS, "1_local_edits_log_f4.txt")))
+    assert sum(v["steps"] for v in sums.values()) == 586158995069
+    assert lines == survivor_lines(read_text(REF_RESULTS, "1_local_edits_surv_f4.txt"))
+    ref = ref_hashes()
+    for X, tab, ch in hashes:
+        assert ref[(X, tab)] == ch
+
+
 def test_split_tests(e13, gpu_ctx):
     split = e13.parse_machine(read_text(REF_RESULTS, "..", "machines", "split_15.txt"))
     say("search: split_15 delete X=14 FMAX=1")
