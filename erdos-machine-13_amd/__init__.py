@@ -66,10 +66,24 @@ class Leaf(ctypes.Structure):
                 ("rel_ok", ctypes.c_int32), ("event", ctypes.c_int32), ("ch", ctypes.c_uint64 * 3)]
 
 
+class BatchSum(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint64), ("digest", ctypes.c_uint64), ("steps", ctypes.c_uint64),
+                ("survivors", ctypes.c_uint64), ("dead_at_stage", ctypes.c_uint64 * 10),
+                ("kernel_ms", ctypes.c_double)]
+
+    def line(self, seed0, k0):
+        """The summary line of the synthetic batch (the e13search_gpu synth line)."""
+        return (f"synth seed0=0x{seed0:x} k0={k0} n={self.n} digest={self.digest:016x} steps={self.steps} "
+                f"survivors={self.survivors} dead-at-stage(0..9): " + " ".join(str(x) for x in self.dead_at_stage))
+
+
 class Tuning(ctypes.Structure):
     _fields_ = [("form", ctypes.c_int32), ("slice", ctypes.c_int32), ("wg_threads", ctypes.c_int32),
                 ("wgs_per_cu", ctypes.c_int32), ("cap", ctypes.c_int32), ("batch", ctypes.c_int32),
-                ("gate", ctypes.c_int32), ("refill_tries", ctypes.c_int32), ("stack_nodes", ctypes.c_int64)]
+                ("gate", ctypes.c_int32), ("refill_tries", ctypes.c_int32), ("stack_nodes", ctypes.c_int64),
+                ("seed_per_wave", ctypes.c_int32), ("sched_every", ctypes.c_int32), ("donate_max", ctypes.c_int32),
+                ("idle_naps", ctypes.c_int32), ("mail_every", ctypes.c_int32), ("trace_events", ctypes.c_int32),
+                ("graph_chain", ctypes.c_int32), ("pad_g", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -80,7 +94,7 @@ class Stats(ctypes.Structure):
                 ("lane_slots", ctypes.c_int64), ("peak_stack", ctypes.c_int64), ("slice", ctypes.c_int32),
                 ("wg_threads", ctypes.c_int32), ("grid", ctypes.c_int32), ("gate", ctypes.c_int32),
                 ("form", ctypes.c_int32), ("pad", ctypes.c_int32), ("donations", ctypes.c_int64),
-                ("pool_takes", ctypes.c_int64), ("cycles", ctypes.c_int64 * 4)]
+                ("pool_takes", ctypes.c_int64), ("cycles", ctypes.c_int64 * 4), ("graphs", ctypes.c_int64)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("pad", "cycles")}
@@ -92,7 +106,7 @@ class Stats(ctypes.Structure):
 EXPORTS = ("e13_open", "e13_close", "e13_strerror", "e13_set_tuning", "e13_get_tuning", "e13_get_stats",
            "e13_keys", "e13_parse_machine", "e13_setup", "e13_search", "e13_search_many", "e13_simulate_batch",
            "e13_simulate_batch_device", "e13_sync", "e13_recheck", "e13_format_survivor", "e13_format_hash",
-           "e13_synth_table")
+           "e13_synth_table", "e13_get_trace", "e13_batch_digest", "e13_synth_run")
 
 _lib = None
 
@@ -141,6 +155,10 @@ def lib():
     L.e13_format_hash.argtypes = [P(Problem), P(Survivor), ctypes.c_char_p, ctypes.c_int]
     L.e13_synth_table.argtypes = [ctypes.c_uint64, ctypes.c_uint64, P(ctypes.c_int8), P(ctypes.c_int32)]
     L.e13_synth_table.restype = None
+    L.e13_batch_digest.argtypes = [P(Leaf), ctypes.c_int64, ctypes.c_uint64, P(BatchSum)]
+    L.e13_synth_run.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int64, P(BatchSum)]
+    L.e13_get_trace.argtypes = [ctypes.c_void_p, P(ctypes.c_uint64), ctypes.c_uint64, P(ctypes.c_int32),
+                                P(ctypes.c_int32)]
     _lib = L
     return L
 
@@ -213,6 +231,15 @@ def synth_table(seed0, k):
     cap = ctypes.c_int32()
     lib().e13_synth_table(seed0, k, t, ctypes.byref(cap))
     return list(t), cap.value
+
+
+def batch_digest(leaves, k0=0):
+    """Totals and order-independent digest of complete-table results (e13_batch_digest)."""
+    n = len(leaves)
+    arr = leaves if isinstance(leaves, ctypes.Array) else (Leaf * n)(*leaves)
+    out = BatchSum()
+    _check(lib().e13_batch_digest(arr, n, k0, ctypes.byref(out)), "e13_batch_digest")
+    return out
 
 
 def recheck(tab, nq, st0, head0, abs_ok, rel_ok, steps):
@@ -297,6 +324,12 @@ class Context:
         _check(lib().e13_simulate_batch_device(self._h, ctypes.c_void_p(tabs_ptr), nq, n,
                                                ctypes.c_void_p(caps_ptr) if caps_ptr else None,
                                                ctypes.c_void_p(out_ptr)), "e13_simulate_batch_device")
+
+    def synth_run(self, seed0, k0, n):
+        """The synthetic stress batch (tables k0..k0+n-1): a BatchSum."""
+        out = BatchSum()
+        _check(lib().e13_synth_run(self._h, seed0, k0, n, ctypes.byref(out)), "e13_synth_run")
+        return out
 
     def sync(self):
         _check(lib().e13_sync(self._h), "e13_sync")

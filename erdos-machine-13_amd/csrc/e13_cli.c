@@ -5,6 +5,7 @@
  *   e13search_gpu delete  machine.txt FMAX X...
  *   e13search_gpu free    machine.txt X CELL...
  *   e13search_gpu freesym machine.txt X LO HI CELL...
+ *   e13search_gpu synth   SEED0 K0 N      (the synthetic stress batch; one summary line)
  *
  * Argument meaning, stdout/stderr split and line formats follow
  * src/e13search.c main() (:435-472), report() (:184-200) and the summary at
@@ -14,7 +15,10 @@
  * Survivors whose plain re-check disagrees are flagged on stderr.
  *
  * Environment: E13_FORM (1 plain, 2 batched), E13_SLICE, E13_CAP, E13_BATCH,
- * E13_GATE (1 count, 2 hash), E13_WG, E13_WGS_PER_CU, E13_DEVICE.
+ * E13_GATE (1 count, 2 hash), E13_WG, E13_WGS_PER_CU, E13_DEVICE,
+ * E13_REFILL_TRIES, E13_SEED_PER_WAVE, E13_SCHED_EVERY, E13_DONATE_MAX,
+ * E13_IDLE_NAPS, E13_MAIL_EVERY, E13_GRAPH_CHAIN; E13_TRACE=N with E13_TRACE_FILE=path writes
+ * the persistent search's wave event log (N events per wave) to path.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -57,6 +61,24 @@ static void print_cycles(const e13_stats *a, const e13_stats *b) {
     fprintf(stderr, "  wave cycles: refill %.1f%% steps %.1f%% finish %.1f%% pool/idle %.1f%% (%.3e per round)\n",
             100 * c[0] / t, 100 * c[1] / t, 100 * c[2] / t, 100 * c[3] / t,
             (b->rounds - a->rounds) > 0 ? t / (double)(b->rounds - a->rounds) : 0.0);
+}
+
+/* The wave event log of the last search, as int32 waves, int32 words per wave,
+   then the words (e13_get_trace). */
+static void dump_trace(e13_ctx *ctx) {
+  const char *path = getenv("E13_TRACE_FILE");
+  int32_t waves = 0, per = 0;
+  if (!path || e13_get_trace(ctx, NULL, 0, &waves, &per) || !waves) return;
+  uint64_t n = (uint64_t)waves * (uint64_t)per;
+  uint64_t *buf = (uint64_t *)malloc(n * sizeof *buf);
+  FILE *f = fopen(path, "wb");
+  if (buf && f && !e13_get_trace(ctx, buf, n, &waves, &per)) {
+    fwrite(&waves, sizeof waves, 1, f);
+    fwrite(&per, sizeof per, 1, f);
+    fwrite(buf, sizeof *buf, n, f);
+  }
+  if (f) fclose(f);
+  free(buf);
 }
 
 static char *read_file(const char *path) {
@@ -135,6 +157,7 @@ static int run_one(e13_ctx *ctx, const int32_t *base, int nq_base, int X, int fm
           (long long)(st1.lane_slots - st0.lane_slots), (long long)(st1.donations - st0.donations),
           (long long)(st1.pool_takes - st0.pool_takes), X, (unsigned long long)digest);
   print_cycles(&st0, &st1);
+  dump_trace(ctx);
   if (ns > cap) fprintf(stderr, "WARNING: %llu survivors, only %llu printed\n", (unsigned long long)ns,
                         (unsigned long long)cap);
   fflush(stderr);
@@ -206,6 +229,7 @@ static int run_many(e13_ctx *ctx, const int32_t *base, int nq_base, int nx, char
           (long long)(st1.lane_slots - st0.lane_slots), (long long)(st1.donations - st0.donations),
           (long long)(st1.pool_takes - st0.pool_takes));
   print_cycles(&st0, &st1);
+  dump_trace(ctx);
   free(sv);
   return 0;
 }
@@ -214,6 +238,40 @@ int main(int argc, char **argv) {
   if (argc < 3) {
     fprintf(stderr, "usage: e13search_gpu base|mutant|delete|free|freesym machine.txt [...]\n");
     return 1;
+  }
+  if (!strcmp(argv[1], "synth")) { /* synth SEED0 K0 N: the synthetic stress batch */
+    e13_tuning st;
+    memset(&st, 0, sizeof st);
+    st.form = env_int("E13_FORM");
+    st.slice = env_int("E13_SLICE");
+    st.wg_threads = env_int("E13_WG");
+    st.wgs_per_cu = env_int("E13_WGS_PER_CU");
+    st.gate = env_int("E13_GATE");
+    e13_ctx *sc = NULL;
+    int src = e13_open(env_int("E13_DEVICE"), &st, &sc);
+    if (src) {
+      fprintf(stderr, "e13_open: %s\n", e13_strerror(src));
+      return 1;
+    }
+    uint64_t seed0 = strtoull(argv[2], NULL, 0), k0 = argc > 3 ? strtoull(argv[3], NULL, 0) : 0;
+    long long n = argc > 4 ? atoll(argv[4]) : 16;
+    e13_batch_sum sm;
+    double w0 = now_s();
+    src = e13_synth_run(sc, seed0, k0, n, &sm);
+    double wall = now_s() - w0;
+    if (src) {
+      fprintf(stderr, "synth: %s\n", e13_strerror(src));
+      return 1;
+    }
+    printf("synth seed0=0x%llx k0=%llu n=%llu digest=%016llx steps=%llu survivors=%llu dead-at-stage(0..9):",
+           (unsigned long long)seed0, (unsigned long long)k0, (unsigned long long)sm.n,
+           (unsigned long long)sm.digest, (unsigned long long)sm.steps, (unsigned long long)sm.survivors);
+    for (int j = 0; j < 10; j++) printf(" %llu", (unsigned long long)sm.dead_at_stage[j]);
+    printf("\n");
+    fprintf(stderr, "  kernel=%.2fms %.3e steps/s %.3e tables/s (wall %.2fs incl. generation and copies)\n",
+            sm.kernel_ms, (double)sm.steps / (sm.kernel_ms * 1e-3), (double)sm.n / (sm.kernel_ms * 1e-3), wall);
+    e13_close(sc);
+    return 0;
   }
   char *text = read_file(argv[2]);
   if (!text) {
@@ -236,6 +294,14 @@ int main(int argc, char **argv) {
   tu.gate = env_int("E13_GATE");
   tu.wg_threads = env_int("E13_WG");
   tu.wgs_per_cu = env_int("E13_WGS_PER_CU");
+  tu.refill_tries = env_int("E13_REFILL_TRIES");
+  tu.seed_per_wave = env_int("E13_SEED_PER_WAVE");
+  tu.sched_every = env_int("E13_SCHED_EVERY");
+  tu.donate_max = env_int("E13_DONATE_MAX");
+  tu.idle_naps = env_int("E13_IDLE_NAPS");
+  tu.mail_every = env_int("E13_MAIL_EVERY");
+  tu.trace_events = env_int("E13_TRACE");
+  tu.graph_chain = env_int("E13_GRAPH_CHAIN");
   e13_ctx *ctx = NULL;
   int rc = e13_open(env_int("E13_DEVICE"), &tu, &ctx);
   if (rc) {

@@ -84,8 +84,22 @@ typedef struct E13Problem {
   int8_t base[E13_MAXCELLS]; /* codes after setup(): move, -1 halt, -2 free */
 } E13Problem;
 
+/* Device-resident cursor of the host-loop forms' node stack, for launch chains
+   captured in a hipGraph: each iteration pops `nin` nodes from `top`, expands
+   them, and pushes the children back, with no host round trip. */
+typedef struct E13Chain {
+  uint64_t top;      /* nodes on the stack */
+  uint64_t base;     /* first node of the current batch */
+  uint64_t peak;     /* largest top seen */
+  uint32_t nin;      /* nodes in the current batch */
+  uint32_t batch;    /* most nodes per batch */
+  uint32_t error;    /* 1: the stack or the output buffer overflowed */
+  uint32_t iters;    /* iterations that found work */
+  uint64_t pad[2];
+} E13Chain;
+
 typedef struct E13ExpandArgs {
-  const E13Node *in;
+  const E13Node *in;      /* ignored when chain is set: in = stack + chain->base */
   E13Node *out;
   E13Counters *ctr;
   E13SurvRec *surv;
@@ -95,26 +109,42 @@ typedef struct E13ExpandArgs {
   uint32_t nin, out_cap, surv_cap, nslot;
   int32_t slice, cap, refill_tries, nq;
   uint32_t chunk, pad0, pad1, pad2; /* work items a wave claims per atomic */
+  E13Chain *chain;        /* set: nin and in come from the device cursor */
+  const E13Node *stack;   /* the stack the chain cursor indexes */
 } E13ExpandArgs;
 
-/* Work-pool and termination state of the persistent search (one per search). */
+/* Scheduling state of the persistent search (one per search). */
 typedef struct E13Sched {
-  uint64_t head, tail;  /* pool ring: nodes claimed / nodes reserved (monotone) */
-  int32_t busy;         /* waves holding work */
-  int32_t error;        /* nonzero: abort (1 private stack full, 2 pool wait timed out, 3 idle watchdog) */
-  int32_t donations, pool_takes;
+  uint64_t pad1[2];
+  int32_t busy;         /* waves holding work, plus deliveries not yet taken */
+  int32_t error;        /* nonzero: abort (1 private stack full, 2 incomplete delivery, 3 idle watchdog,
+                           4 LDS base) */
+  int32_t donations, pool_takes; /* nodes delivered / taken */
   int32_t exited;       /* waves that have left */
   int32_t pad0;
   uint64_t pad[3];
 } E13Sched;
+
+/* A wave's mailbox.  A donor that won the wave's bit in the hungry map writes
+   the node as 64-bit elements (delivery tag << 32 | node word), then sets
+   flag = tag.  The wave polls only its own flag; a flag that differs from the
+   last tag it took is a delivery, and the node is complete when every element
+   carries that tag (stale words from an earlier delivery carry another tag).
+   flag = E13_MAIL_DONE: the search is over.  Tags are never 0. */
+#define E13_MAIL_DONE 0xFFFFFFFFFFFFFFFFull
+typedef struct E13Mail {
+  uint64_t flag;
+  uint64_t pad[7];
+  uint64_t w[E13_NODE_WORDS];
+} E13Mail;
 
 #define E13_MAXPROB 16
 #define E13_PCNT 16       /* per-problem counters: leaves, steps, survivors, hist[0..9] */
 
 typedef struct E13SearchArgs {
   E13Node *stacks;        /* nwaves x stk_cap private stacks */
-  E13Node *pool;          /* pool_cap donated nodes */
-  uint64_t *pool_seq;     /* pool_cap sequence words: slot j holds node j when pool_seq = j + 1 */
+  E13Mail *mail;          /* one mailbox per wave */
+  uint64_t *hungry;       /* hungry map: bit w set = wave w asks for a node (hungry_words words) */
   E13Sched *sch;
   const E13Problem *pbs;  /* np problems */
   uint64_t *pcnt;         /* np x E13_PCNT */
@@ -122,10 +152,16 @@ typedef struct E13SearchArgs {
   E13SurvRec *surv;
   const uint64_t *key;
   const uint64_t *tgt;
-  int32_t np, slice, stk_cap, pool_cap;
+  int32_t np, slice, stk_cap, hungry_words;
   int32_t refill_tries, donate_every, pool_min_idle, total_waves;
-  uint32_t surv_cap, spin_limit, idle_limit, sched_every; /* sched_every: power of two, rounds between
-                                                              reads of the shared scheduler state */
+  uint32_t surv_cap, spin_limit;
+  uint32_t idle_limit;    /* idle watchdog, in ticks of the 100 MHz constant clock */
+  uint32_t sched_every;   /* rounds between reads of the hungry map while no wave is hungry */
+  const E13Node *seeds;   /* nseeds frontier nodes dealt round-robin to the waves' stacks at start */
+  uint32_t nseeds;
+  int32_t idle_naps, mail_every, trace_cap;
+  uint64_t *trace;        /* optional: trace_cap words per wave, word 0 = events logged, then
+                             (realtime << 16 | aux << 8 | event) */
 } E13SearchArgs;
 
 typedef struct E13BatchArgs {
@@ -136,7 +172,8 @@ typedef struct E13BatchArgs {
   const uint64_t *key;
   const uint64_t *tgt;
   const int32_t *dl;      /* device copy of the stage deadlines dl[0..9] */
-  int32_t n, nq, slice, pad;
+  int32_t n, nq, slice;
+  uint32_t chunk;         /* tables a wave claims per atomic (refilling form) */
 } E13BatchArgs;
 
 #endif

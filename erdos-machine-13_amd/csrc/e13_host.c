@@ -19,9 +19,14 @@
 /* Launch wrappers (e13_kernels.hip). */
 size_t e13k_lds_bytes(int wg_threads, int gate);
 int e13k_launch_expand(const E13ExpandArgs *a, int grid, int wg_threads, int gate, int refill, hipStream_t s);
-int e13k_launch_batch(const E13BatchArgs *a, int grid, int wg_threads, int gate, hipStream_t s);
+int e13k_launch_batch(const E13BatchArgs *a, int grid, int wg_threads, int gate, int refill, hipStream_t s);
 size_t e13k_search_lds_bytes(int wg_threads, int gate, int np);
 int e13k_launch_search(const E13SearchArgs *a, int grid, int wg_threads, int gate, hipStream_t s);
+int e13k_launch_chain_pop(E13Chain *ch, E13Counters *ctr, hipStream_t s);
+int e13k_launch_chain_push(E13Chain *ch, E13Node *stack, uint64_t stack_cap, const E13Node *out, uint32_t out_cap,
+                           const E13Counters *ctr, int grid, hipStream_t s);
+
+_Static_assert(sizeof(E13Node) % 16 == 0, "node records are copied as 16-byte vectors");
 
 static const long T14S[E13_NS + 1] = {0, 2, 22, 90, 268, 944, 3434, 13450, 52068, 208864}; /* e13search.c:70 */
 
@@ -47,12 +52,18 @@ struct e13_ctx {
   /* persistent search */
   E13Node *d_stacks;
   size_t stacks_cap; /* nodes */
-  E13Node *d_pool;
-  uint64_t *d_pool_seq;
-  size_t pool_cap;
+  E13Mail *d_mail;     /* one mailbox per wave */
+  uint64_t *d_hungry;  /* hungry map */
+  size_t mail_cap;     /* waves */
   E13Sched *d_sch;
   E13Problem *d_pbs;
   uint64_t *d_pcnt;
+  E13Node *d_seeds;
+  size_t seeds_cap;
+  E13Chain *d_chain; /* launch-chain cursor of the host-loop forms */
+  uint64_t *d_trace; /* wave event log of the last traced search */
+  size_t trace_words;
+  int trace_waves, trace_per_wave;
 };
 
 const char *e13_strerror(int code) {
@@ -200,6 +211,13 @@ static void tuning_defaults(e13_tuning *t) {
   t->gate = 1;
   t->refill_tries = 2;
   t->stack_nodes = 1LL << 25;
+  t->seed_per_wave = 0;
+  t->sched_every = 8;
+  t->donate_max = 4;
+  t->idle_naps = 4;
+  t->mail_every = 1;
+  t->trace_events = 0;
+  t->graph_chain = 0;
 }
 
 static void tuning_merge(e13_tuning *dst, const e13_tuning *src) {
@@ -213,6 +231,16 @@ static void tuning_merge(e13_tuning *dst, const e13_tuning *src) {
   if (src->gate > 0) dst->gate = src->gate;
   if (src->refill_tries > 0) dst->refill_tries = src->refill_tries;
   if (src->stack_nodes > 0) dst->stack_nodes = src->stack_nodes;
+  if (src->seed_per_wave > 0) dst->seed_per_wave = src->seed_per_wave;
+  if (src->seed_per_wave < 0) dst->seed_per_wave = 0; /* negative: no seeding */
+  if (src->sched_every > 0) dst->sched_every = src->sched_every;
+  if (src->donate_max > 0) dst->donate_max = src->donate_max;
+  if (src->idle_naps > 0) dst->idle_naps = src->idle_naps;
+  if (src->mail_every > 0) dst->mail_every = src->mail_every;
+  if (src->trace_events > 0) dst->trace_events = src->trace_events;
+  if (src->trace_events < 0) dst->trace_events = 0; /* negative: tracing off */
+  if (src->graph_chain > 0) dst->graph_chain = src->graph_chain;
+  if (src->graph_chain < 0) dst->graph_chain = 0; /* negative: host loop */
 }
 
 static int tuning_valid(const e13_tuning *t) {
@@ -225,6 +253,10 @@ static int tuning_valid(const e13_tuning *t) {
   if (t->gate < 1 || t->gate > 2) return 0;
   if (t->refill_tries < 1 || t->refill_tries > 64) return 0;
   if (t->stack_nodes < 1024) return 0;
+  if (t->seed_per_wave < 0 || t->seed_per_wave > 64) return 0;
+  if (t->sched_every < 1 || t->donate_max < 1 || t->idle_naps < 1 || t->mail_every < 1) return 0;
+  if (t->trace_events < 0 || t->trace_events > 1 << 16) return 0;
+  if (t->graph_chain < 0 || t->graph_chain > 4096) return 0;
   return 1;
 }
 
@@ -298,11 +330,14 @@ void e13_close(e13_ctx *c) {
   hipFree(c->d_caps);
   hipFree(c->d_leaves);
   hipFree(c->d_stacks);
-  hipFree(c->d_pool);
-  hipFree(c->d_pool_seq);
+  hipFree(c->d_mail);
+  hipFree(c->d_hungry);
   hipFree(c->d_sch);
   hipFree(c->d_pbs);
   hipFree(c->d_pcnt);
+  hipFree(c->d_seeds);
+  hipFree(c->d_trace);
+  hipFree(c->d_chain);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -402,6 +437,75 @@ static int check_problem(const e13_problem *p) {
   return 1;
 }
 
+/* The host-loop search as hipGraph launch chains: graph_chain iterations of
+   (pop a batch, expand it, push the children) captured once, then the graph is
+   launched until the device cursor finds the stack empty.  One host round
+   trip per graph instead of per launch; the results are those of the plain
+   host loop (same batches in the same order). */
+static int run_chain(e13_ctx *c, E13ExpandArgs *a, size_t batch, int grid, int wg, int refill, size_t *peak) {
+  const e13_tuning *tu = &c->tune;
+  if (!c->d_chain && hipMalloc((void **)&c->d_chain, sizeof(E13Chain)) != hipSuccess) return E13_ENOMEM;
+  E13Chain ch;
+  memset(&ch, 0, sizeof ch);
+  ch.top = 1; /* the root, already on the stack */
+  ch.peak = 1;
+  ch.batch = (uint32_t)batch;
+  HIPCHK(hipMemcpyAsync(c->d_chain, &ch, sizeof ch, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  a->chain = c->d_chain;
+  a->stack = c->d_stack;
+  a->in = NULL;
+  a->nin = 0;
+  const int push_grid = 4 * c->ncu;
+  hipGraph_t g = NULL;
+  hipGraphExec_t ge = NULL;
+  HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int lr = 0;
+  for (int i = 0; i < tu->graph_chain && lr == 0; i++) {
+    lr = e13k_launch_chain_pop(c->d_chain, c->d_ctr, c->stream);
+    if (lr == 0) lr = e13k_launch_expand(a, grid, wg, tu->gate == 2, refill, c->stream);
+    if (lr == 0)
+      lr = e13k_launch_chain_push(c->d_chain, c->d_stack, (uint64_t)c->stack_cap, c->d_out, a->out_cap, c->d_ctr,
+                                  push_grid, c->stream);
+  }
+  hipError_t ce = hipStreamEndCapture(c->stream, &g);
+  if (lr != 0 || ce != hipSuccess) {
+    fprintf(stderr, "libe13: graph capture failed: %s\n", hipGetErrorString(lr ? (hipError_t)lr : ce));
+    if (g) hipGraphDestroy(g);
+    return E13_EHIP;
+  }
+  if (hipGraphInstantiate(&ge, g, NULL, NULL, 0) != hipSuccess) {
+    hipGraphDestroy(g);
+    return E13_EHIP;
+  }
+  int rc = E13_OK;
+  for (;;) {
+    float ms = 0;
+    if (hipEventRecord(c->ev0, c->stream) != hipSuccess || hipGraphLaunch(ge, c->stream) != hipSuccess ||
+        hipEventRecord(c->ev1, c->stream) != hipSuccess ||
+        hipMemcpyAsync(&ch, c->d_chain, sizeof ch, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess || hipEventElapsedTime(&ms, c->ev0, c->ev1) != hipSuccess) {
+      rc = E13_EHIP;
+      break;
+    }
+    c->stats.kernel_ms += ms;
+    c->stats.launches += 3 * tu->graph_chain;
+    c->stats.graphs += 1;
+    if (ch.error) {
+      if (ch.error == 2) fprintf(stderr, "libe13: dynamic LDS does not start at LDS address 0\n");
+      rc = ch.error == 2 ? E13_EHIP : E13_ESTACK;
+      break;
+    }
+    if (ch.top == 0) break;
+  }
+  *peak = (size_t)ch.peak;
+  hipGraphExecDestroy(ge);
+  hipGraphDestroy(g);
+  a->chain = NULL;
+  a->stack = NULL;
+  return rc;
+}
+
 int e13_search(e13_ctx *c, const e13_problem *p, e13_counts *out, e13_survivor *surv, uint64_t surv_cap,
                uint64_t *n_surv) {
   if (!c || !out || !check_problem(p)) return E13_EINVAL;
@@ -446,6 +550,11 @@ int e13_search(e13_ctx *c, const e13_problem *p, e13_counts *out, e13_survivor *
   const int wg = fit_wg(c, tu->wg_threads, tu->gate == 2);
   if (e13k_lds_bytes(wg, tu->gate == 2) > c->lds_max) return E13_EINVAL;
   int grid_used = 0;
+  if (tu->graph_chain > 0) {
+    grid_used = refill ? c->ncu * tu->wgs_per_cu : (int)((batch * (size_t)nslot + (size_t)wg - 1) / (size_t)wg);
+    if ((rc = run_chain(c, &a, batch, grid_used, wg, refill, &peak))) return rc;
+    top = 0;
+  }
   while (top > 0) {
     size_t nin = top < batch ? top : batch;
     size_t base_i = top - nin;
@@ -560,19 +669,20 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
   const int total_waves = grid * (wg / 64);
   const int stk_cap = tu->stack_nodes >= 64 * (int64_t)total_waves ? (int)(tu->stack_nodes / total_waves) : 64;
   const size_t need_stacks = (size_t)total_waves * (size_t)stk_cap;
-  const size_t pool_cap = 1 << 16;
+  if (total_waves > 64 * 64) return E13_EINVAL; /* the hungry map is read as at most 64 words */
+  const int hungry_words = (total_waves + 63) / 64;
   int rc;
   if ((rc = ensure_buf((void **)&c->d_stacks, &c->stacks_cap, need_stacks, sizeof(E13Node)))) return rc;
-  if (c->pool_cap < pool_cap) {
-    hipFree(c->d_pool);
-    hipFree(c->d_pool_seq);
-    c->d_pool = NULL;
-    c->d_pool_seq = NULL;
-    c->pool_cap = 0;
-    if (hipMalloc((void **)&c->d_pool, pool_cap * sizeof(E13Node)) != hipSuccess ||
-        hipMalloc((void **)&c->d_pool_seq, pool_cap * sizeof(uint64_t)) != hipSuccess)
+  if (c->mail_cap < (size_t)total_waves) {
+    hipFree(c->d_mail);
+    hipFree(c->d_hungry);
+    c->d_mail = NULL;
+    c->d_hungry = NULL;
+    c->mail_cap = 0;
+    if (hipMalloc((void **)&c->d_mail, (size_t)total_waves * sizeof(E13Mail)) != hipSuccess ||
+        hipMalloc((void **)&c->d_hungry, 64 * sizeof(uint64_t)) != hipSuccess)
       return E13_ENOMEM;
-    c->pool_cap = pool_cap;
+    c->mail_cap = (size_t)total_waves;
   }
   if (!c->d_sch && hipMalloc((void **)&c->d_sch, sizeof(E13Sched)) != hipSuccess) return E13_ENOMEM;
   if (!c->d_pbs && hipMalloc((void **)&c->d_pbs, E13_MAXPROB * sizeof(E13Problem)) != hipSuccess) return E13_ENOMEM;
@@ -581,32 +691,99 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
   size_t want_surv = surv_cap > 4096 ? (size_t)surv_cap : 4096;
   if ((rc = ensure_buf((void **)&c->d_surv, &c->surv_cap, want_surv, sizeof(E13SurvRec)))) return rc;
 
-  /* problems, roots in pool slots 0..np-1, slot sequence words */
+  /* Breadth-first seeding: a few k_expand launches per problem grow a frontier
+     of about four nodes per wave; leaves met on the way are counted here.  The
+     persistent launch then starts with the frontier dealt to the waves. */
+  const size_t seed_target = (size_t)tu->seed_per_wave * (size_t)total_waves;
+  if ((rc = ensure_buf((void **)&c->d_seeds, &c->seeds_cap, 64 * (size_t)total_waves + (size_t)np, sizeof(E13Node))))
+    return rc;
+  if ((rc = ensure_buf((void **)&c->d_stack, &c->stack_cap, c->seeds_cap, sizeof(E13Node)))) return rc;
   E13Problem pbs[E13_MAXPROB];
-  E13Node roots[E13_MAXPROB];
   for (int k = 0; k < np; k++) {
     fill_problem(&ps[k], &pbs[k]);
-    root_node(&ps[k], &roots[k]);
-    roots[k].prob = k;
+    memset(&out[k], 0, sizeof out[k]);
   }
-  uint64_t *seq = (uint64_t *)malloc(pool_cap * sizeof(uint64_t));
-  if (!seq) return E13_ENOMEM;
-  for (size_t i = 0; i < pool_cap; i++) seq[i] = i < (size_t)np ? i + 1 : i;
+  HIPCHK(hipMemcpyAsync(c->d_pbs, pbs, (size_t)np * sizeof(E13Problem), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemsetAsync(c->d_ctr, 0, sizeof(E13Counters), c->stream));
+  size_t nseeds = 0;
+  uint64_t prev[3 + E13_NS + 1] = {0};
+  for (int k = 0; k < np; k++) {
+    const int nslot = 4 * ps[k].nq + 1;
+    const size_t target_k = (seed_target + (size_t)np - 1) / (size_t)np;
+    E13Node root;
+    root_node(&ps[k], &root);
+    root.prob = k;
+    HIPCHK(hipMemcpyAsync(c->d_stack, &root, sizeof root, hipMemcpyHostToDevice, c->stream));
+    size_t top = 1;
+    E13ExpandArgs ea;
+    memset(&ea, 0, sizeof ea);
+    ea.pb = c->d_pbs + k;
+    ea.nslot = (uint32_t)nslot;
+    ea.nq = ps[k].nq;
+    ea.ctr = c->d_ctr;
+    ea.surv = c->d_surv;
+    ea.key = c->d_key;
+    ea.tgt = c->d_tgt;
+    ea.surv_cap = (uint32_t)c->surv_cap;
+    ea.slice = tu->slice;
+    ea.cap = tu->cap;
+    ea.refill_tries = tu->refill_tries;
+    ea.chunk = 64;
+    for (int level = 0; level < 8 && top > 0 && top < target_k && seed_target > 0; level++) {
+      const size_t room = c->seeds_cap - nseeds;
+      if (top * (size_t)nslot > c->stack_cap || top * (size_t)nslot > room) break;
+      if ((rc = ensure_buf((void **)&c->d_out, &c->out_cap, top * (size_t)nslot, sizeof(E13Node)))) return rc;
+      ea.in = c->d_stack;
+      ea.nin = (uint32_t)top;
+      ea.out = c->d_out;
+      ea.out_cap = (uint32_t)(top * (size_t)nslot);
+      HIPCHK(hipMemsetAsync(&c->d_ctr->item, 0, 2 * sizeof(uint32_t), c->stream));
+      int lr = e13k_launch_expand(&ea, c->ncu * tu->wgs_per_cu, 256, tu->gate == 2, 1, c->stream);
+      if (lr != 0) {
+        fprintf(stderr, "libe13: expand launch failed: %s\n", hipGetErrorString((hipError_t)lr));
+        return E13_EHIP;
+      }
+      uint32_t io[4];
+      HIPCHK(hipMemcpyAsync(io, &c->d_ctr->item, sizeof io, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      c->stats.launches++;
+      if (io[3]) return io[3] & 4u ? E13_EHIP : E13_ESTACK;
+      if (io[1])
+        HIPCHK(hipMemcpyAsync(c->d_stack, c->d_out, (size_t)io[1] * sizeof(E13Node), hipMemcpyDeviceToDevice,
+                              c->stream));
+      top = io[1];
+    }
+    if (top)
+      HIPCHK(hipMemcpyAsync(c->d_seeds + nseeds, c->d_stack, top * sizeof(E13Node), hipMemcpyDeviceToDevice,
+                            c->stream));
+    nseeds += top;
+    /* this problem's share of the counters so far */
+    E13Counters h1;
+    HIPCHK(hipMemcpyAsync(&h1, c->d_ctr, sizeof h1, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    out[k].leaves += h1.leaves - prev[0];
+    out[k].steps += h1.steps - prev[1];
+    out[k].survivors += h1.survivors - prev[2];
+    for (int j = 0; j <= E13_NS; j++) out[k].dead_at_stage[j] += h1.hist[j] - prev[3 + j];
+    prev[0] = h1.leaves;
+    prev[1] = h1.steps;
+    prev[2] = h1.survivors;
+    for (int j = 0; j <= E13_NS; j++) prev[3 + j] = h1.hist[j];
+  }
   E13Sched sch;
   memset(&sch, 0, sizeof sch);
-  sch.tail = (uint64_t)np;
-  HIPCHK(hipMemcpyAsync(c->d_pbs, pbs, (size_t)np * sizeof(E13Problem), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(c->d_pool, roots, (size_t)np * sizeof(E13Node), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(c->d_pool_seq, seq, pool_cap * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream));
+  sch.busy = nseeds < (size_t)total_waves ? (int32_t)nseeds : total_waves; /* waves dealt a seed */
   HIPCHK(hipMemcpyAsync(c->d_sch, &sch, sizeof sch, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemsetAsync(c->d_mail, 0, (size_t)total_waves * sizeof(E13Mail), c->stream));
+  HIPCHK(hipMemsetAsync(c->d_hungry, 0, 64 * sizeof(uint64_t), c->stream));
   HIPCHK(hipMemsetAsync(c->d_pcnt, 0, (size_t)np * E13_PCNT * sizeof(uint64_t), c->stream));
-  HIPCHK(hipMemsetAsync(c->d_ctr, 0, sizeof(E13Counters), c->stream));
 
   E13SearchArgs a;
   memset(&a, 0, sizeof a);
   a.stacks = c->d_stacks;
-  a.pool = c->d_pool;
-  a.pool_seq = c->d_pool_seq;
+  a.mail = c->d_mail;
+  a.hungry = c->d_hungry;
+  a.hungry_words = hungry_words;
   a.sch = c->d_sch;
   a.pbs = c->d_pbs;
   a.pcnt = c->d_pcnt;
@@ -617,19 +794,31 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
   a.np = np;
   a.slice = tu->slice;
   a.stk_cap = stk_cap;
-  a.pool_cap = (int32_t)pool_cap;
   a.refill_tries = tu->refill_tries;
-  a.donate_every = 4;   /* nodes a wave may give away per scheduler refresh */
-  a.sched_every = 64;   /* rounds between scheduler reads, a power of two */
+  a.donate_every = tu->donate_max;
+  a.sched_every = (uint32_t)tu->sched_every;
+  a.idle_naps = tu->idle_naps;
+  a.mail_every = tu->mail_every;
   a.pool_min_idle = 16;
   a.total_waves = total_waves;
   a.surv_cap = (uint32_t)c->surv_cap;
-  a.spin_limit = 1u << 22;
-  a.idle_limit = 1u << 22;
+  a.spin_limit = 1u << 20; /* mailbox re-reads before a delivery counts as lost */
+  a.idle_limit = 3000000000u; /* idle watchdog: 30 s of the 100 MHz constant clock */
+  a.seeds = c->d_seeds;
+  a.nseeds = (uint32_t)nseeds;
+  c->trace_waves = c->trace_per_wave = 0;
+  if (tu->trace_events > 0) {
+    const size_t per = (size_t)tu->trace_events + 1, words = per * (size_t)total_waves;
+    if ((rc = ensure_buf((void **)&c->d_trace, &c->trace_words, words, sizeof(uint64_t)))) return rc;
+    HIPCHK(hipMemsetAsync(c->d_trace, 0, words * sizeof(uint64_t), c->stream));
+    a.trace = c->d_trace;
+    a.trace_cap = (int32_t)per;
+    c->trace_waves = total_waves;
+    c->trace_per_wave = (int)per;
+  }
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   int lr = e13k_launch_search(&a, grid, wg, tu->gate == 2, c->stream);
   if (lr != 0) {
-    free(seq);
     fprintf(stderr, "libe13: search launch failed: %s\n", hipGetErrorString((hipError_t)lr));
     return E13_EHIP;
   }
@@ -641,7 +830,6 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
   HIPCHK(hipMemcpyAsync(&h, c->d_ctr, sizeof h, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipMemcpyAsync(&sch, c->d_sch, sizeof sch, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  free(seq);
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->stats.kernel_ms += ms;
@@ -659,17 +847,17 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
   if (sch.error) {
     fprintf(stderr, "libe13: persistent search aborted (%s)\n",
             sch.error == 1   ? "a private node stack is full"
-            : sch.error == 2 ? "a pool slot was not filled in time"
+            : sch.error == 2 ? "a mailbox delivery never arrived complete"
             : sch.error == 3 ? "idle watchdog"
                              : "dynamic LDS does not start at LDS address 0");
     return E13_ESTACK;
   }
   for (int k = 0; k < np; k++) {
     const uint64_t *q = pcnt + k * E13_PCNT;
-    out[k].leaves = q[0];
-    out[k].steps = q[1];
-    out[k].survivors = q[2];
-    for (int j = 0; j <= E13_NS; j++) out[k].dead_at_stage[j] = q[3 + j];
+    out[k].leaves += q[0];
+    out[k].steps += q[1];
+    out[k].survivors += q[2];
+    for (int j = 0; j <= E13_NS; j++) out[k].dead_at_stage[j] += q[3 + j];
   }
   uint64_t ns = h.nsurv;
   if (n_surv) *n_surv = ns;
@@ -689,6 +877,18 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
   return E13_OK;
 }
 
+int e13_get_trace(e13_ctx *c, uint64_t *buf, uint64_t cap_words, int32_t *waves, int32_t *per_wave) {
+  if (!c) return E13_EINVAL;
+  if (waves) *waves = c->trace_waves;
+  if (per_wave) *per_wave = c->trace_per_wave;
+  const uint64_t words = (uint64_t)c->trace_waves * (uint64_t)c->trace_per_wave;
+  if (!buf || !words) return E13_OK;
+  if (cap_words < words) return E13_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMemcpy(buf, c->d_trace, words * sizeof(uint64_t), hipMemcpyDeviceToHost));
+  return E13_OK;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Complete tables.                                                          */
 /* ------------------------------------------------------------------------ */
@@ -705,14 +905,18 @@ static int batch_launch(e13_ctx *c, const void *tabs_dev, int nq, int n, const v
   a.nq = nq;
   a.slice = c->tune.slice;
   a.dl = c->d_dl;
+  a.chunk = 128;
   HIPCHK(hipMemsetAsync(c->d_ctr, 0, sizeof(E13Counters), c->stream));
   const int wg = fit_wg(c, c->tune.wg_threads, c->tune.gate == 2);
   if (e13k_lds_bytes(wg, c->tune.gate == 2) > c->lds_max) return E13_EINVAL;
-  int grid = c->ncu * c->tune.wgs_per_cu;
+  /* form 1: one table per lane, a grid covering the batch; otherwise a
+     persistent grid whose lanes take the next table as soon as theirs ends */
+  const int refill = c->tune.form != 1;
   long need = ((long)n + wg - 1) / wg;
+  int grid = refill ? c->ncu * c->tune.wgs_per_cu : (int)need;
   if (need < grid) grid = (int)(need > 0 ? need : 1);
   HIPCHK(hipEventRecord(c->ev0, c->stream));
-  int lr = e13k_launch_batch(&a, grid, wg, c->tune.gate == 2, c->stream);
+  int lr = e13k_launch_batch(&a, grid, wg, c->tune.gate == 2, refill, c->stream);
   if (lr != 0) {
     fprintf(stderr, "libe13: batch launch failed: %s\n", hipGetErrorString((hipError_t)lr));
     return E13_EHIP;
@@ -894,3 +1098,111 @@ void e13_synth_table(uint64_t seed0, uint64_t k, int8_t *tab28, int32_t *cap) {
   double u = (double)(splitmix(&st) >> 11) * (1.0 / 9007199254740992.0);
   *cap = (int32_t)floor(10.0 * pow(2e4, u));
 }
+
+/* ------------------------------------------------------------------------ */
+/* Synthetic stress batch.                                                   */
+/* ------------------------------------------------------------------------ */
+static uint64_t rec_mix(uint64_t x) { return splitmix(&x); }
+
+static uint64_t record_hash(uint64_t k, const e13_leaf *l) {
+  uint64_t h = rec_mix(k);
+  h = rec_mix(h ^ (uint64_t)(uint32_t)l->result);
+  h = rec_mix(h ^ (uint64_t)(uint32_t)l->stage);
+  h = rec_mix(h ^ (uint64_t)(uint32_t)l->steps);
+  h = rec_mix(h ^ (uint64_t)(uint32_t)l->st0);
+  h = rec_mix(h ^ (uint64_t)(uint32_t)l->head0);
+  h = rec_mix(h ^ (uint64_t)(uint32_t)(l->abs_ok | (l->rel_ok << 1)));
+  h = rec_mix(h ^ l->ch[0]);
+  h = rec_mix(h ^ l->ch[1]);
+  h = rec_mix(h ^ l->ch[2]);
+  return h;
+}
+
+int e13_batch_digest(const e13_leaf *lv, int64_t n, uint64_t k0, e13_batch_sum *out) {
+  if (!lv || !out || n < 0) return E13_EINVAL;
+  memset(out, 0, sizeof *out);
+  out->n = (uint64_t)n;
+  for (int64_t j = 0; j < n; j++) {
+    out->digest += record_hash(k0 + (uint64_t)j, &lv[j]);
+    out->steps += (uint64_t)lv[j].steps;
+    if (lv[j].result == E13_SURV) out->survivors++;
+    else if (lv[j].stage >= 0 && lv[j].stage <= E13_NS) out->dead_at_stage[lv[j].stage]++;
+  }
+  return E13_OK;
+}
+
+int e13_synth_run(e13_ctx *c, uint64_t seed0, uint64_t k0, int64_t n, e13_batch_sum *out) {
+  if (!c || !out || n < 1 || n > (1LL << 30)) return E13_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  uint8_t *h = (uint8_t *)malloc((size_t)n * E13_MAXCELLS);
+  int32_t *caps = (int32_t *)malloc((size_t)n * sizeof(int32_t));
+  E13LeafRec *lr = (E13LeafRec *)malloc((size_t)n * sizeof(E13LeafRec));
+  e13_leaf *lv = (e13_leaf *)malloc((size_t)n * sizeof(e13_leaf));
+  int rc = E13_OK;
+  if (!h || !caps || !lr || !lv) {
+    rc = E13_ENOMEM;
+    goto done;
+  }
+  for (int64_t j = 0; j < n; j++) {
+    int8_t t[28];
+    e13_synth_table(seed0, k0 + (uint64_t)j, t, &caps[j]);
+    uint8_t *row = h + (size_t)j * E13_MAXCELLS;
+    for (int i = 0; i < E13_MAXCELLS; i++) row[i] = i < 28 && t[i] >= 0 ? (uint8_t)t[i] : E13_C_HALT;
+  }
+  size_t cap_before = c->batch_cap;
+  if ((rc = ensure_buf(&c->d_tabs, &c->batch_cap, (size_t)n, E13_MAXCELLS))) goto done;
+  if (c->batch_cap != cap_before || !c->d_caps || !c->d_leaves) {
+    hipFree(c->d_caps);
+    hipFree(c->d_leaves);
+    c->d_caps = c->d_leaves = NULL;
+    if (hipMalloc(&c->d_caps, c->batch_cap * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc(&c->d_leaves, c->batch_cap * sizeof(E13LeafRec)) != hipSuccess) {
+      rc = E13_ENOMEM;
+      goto done;
+    }
+  }
+  if (hipMemcpyAsync(c->d_tabs, h, (size_t)n * E13_MAXCELLS, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipMemcpyAsync(c->d_caps, caps, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream) != hipSuccess) {
+    rc = E13_EHIP;
+    goto done;
+  }
+  if ((rc = batch_launch(c, c->d_tabs, 14, (int)n, c->d_caps, c->d_leaves))) goto done;
+  E13Counters hc;
+  float ms = 0;
+  if (hipMemcpyAsync(lr, c->d_leaves, (size_t)n * sizeof(E13LeafRec), hipMemcpyDeviceToHost, c->stream) !=
+          hipSuccess ||
+      hipMemcpyAsync(&hc, c->d_ctr, sizeof hc, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess || hipEventElapsedTime(&ms, c->ev0, c->ev1) != hipSuccess) {
+    rc = E13_EHIP;
+    goto done;
+  }
+  if (hc.overflow & 4u) {
+    fprintf(stderr, "libe13: dynamic LDS does not start at LDS address 0\n");
+    rc = E13_EHIP;
+    goto done;
+  }
+  c->stats.kernel_ms += ms;
+  c->stats.rounds += (int64_t)hc.rounds;
+  c->stats.lane_slots += (int64_t)hc.lane_slots;
+  c->stats.form = c->tune.form == 1 ? 1 : 2;
+  for (int64_t j = 0; j < n; j++) {
+    lv[j].result = lr[j].result;
+    lv[j].stage = lr[j].stage;
+    lv[j].steps = lr[j].steps;
+    lv[j].st0 = lr[j].st0;
+    lv[j].head0 = lr[j].head0;
+    lv[j].abs_ok = lr[j].abs_ok;
+    lv[j].rel_ok = lr[j].rel_ok;
+    lv[j].event = lr[j].pad;
+    memcpy(lv[j].ch, lr[j].ch, sizeof lv[j].ch);
+  }
+  rc = e13_batch_digest(lv, n, k0, out);
+  out->kernel_ms = ms;
+done:
+  free(h);
+  free(caps);
+  free(lr);
+  free(lv);
+  return rc;
+}
+

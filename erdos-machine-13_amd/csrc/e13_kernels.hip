@@ -288,6 +288,20 @@ __device__ __forceinline__ long long uni64(long long v) {
 // Core-clock time stamp for the per-phase cycle counters.
 __device__ __forceinline__ uint64_t stamp() { return __builtin_readcyclecounter(); }
 
+// Wave event log of the persistent search (when a.trace is set): lane 0 appends
+// (constant-rate real time << 16 | aux << 8 | event) to the wave's own region.
+enum { TR_SEEDED = 1, TR_TAKE = 2, TR_RUNNING = 3, TR_IDLE = 4, TR_EXIT = 5, TR_GIVE = 6, TR_ROUND = 7 };
+#define TRACE_EV2(ev, aux)                                                                            \
+  do {                                                                                                \
+    if (a.trace != nullptr && lane == 0) {                                                            \
+      if (tr_n + 1u < (uint32_t)a.trace_cap)                                                          \
+        a.trace[(size_t)wid * (size_t)a.trace_cap + 1u + tr_n] =                                      \
+            (__builtin_readsteadycounter() << 16) | ((uint64_t)((aux) & 0xFF) << 8) | (ev);           \
+      ++tr_n;                                                                                         \
+    }                                                                                                 \
+  } while (0)
+#define TRACE_EV(ev) TRACE_EV2(ev, 0)
+
 __device__ __forceinline__ uint32_t rank_in(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
@@ -379,7 +393,13 @@ __global__ void __launch_bounds__(256) k_expand(E13ExpandArgs a) {
   if (GATE == GATE_HASH) fill_keyb(c.sb, c.keyb, a.key);
   __syncthreads();
 
-  const uint32_t nitems = a.nin * a.nslot;
+  uint32_t nin = a.nin;
+  const E13Node *in = a.in;
+  if (a.chain != nullptr) { // launch chain: this iteration's batch is on the device
+    nin = a.chain->nin;
+    in = a.stack + a.chain->base;
+  }
+  const uint32_t nitems = nin * a.nslot;
   Lane L;
   L.dlcap = -1;
   int active = 0, t0 = 0, budget = 0, nintro = 0, front = 0, depth = 0, nbound = 0;
@@ -422,7 +442,7 @@ __global__ void __launch_bounds__(256) k_expand(E13ExpandArgs a) {
         }
         if (!active && k < nitems) {
           const uint32_t ni = k / a.nslot, slot = k - ni * a.nslot;
-          const E13Node *nd = a.in + ni;
+          const E13Node *nd = in + ni;
           const int pend = nd->pend;
           int nb = nd->budget, nin2 = nd->nintro, val = -1, ok = 1;
           if (pend < 0) {
@@ -601,7 +621,7 @@ __global__ void __launch_bounds__(256) k_expand(E13ExpandArgs a) {
 // Complete tables from the blank tape (run_full, e13search.c:428-433): the
 // mutant sweep and the synthetic stress batch.  Persistent lanes with refill.
 // ---------------------------------------------------------------------------
-template <int GATE>
+template <int GATE, int REFILL>
 __global__ void __launch_bounds__(256) k_batch(E13BatchArgs a) {
   extern __shared__ __align__(16) uint32_t smem[];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
@@ -624,42 +644,55 @@ __global__ void __launch_bounds__(256) k_batch(E13BatchArgs a) {
   uint32_t k = 0;
   uint64_t my_steps = 0, w_slots = 0, w_rounds = 0;
   bool exhausted = false;
+  uint32_t ck = 0, ce = 0; // this wave's claimed table range
+  if (!REFILL) { // plain form: one table per lane, no refill
+    ck = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+    ce = min(ck + 64u, (uint32_t)a.n);
+    exhausted = true;
+  }
   for (;;) {
-    const uint64_t need = ballot(!active);
-    if (need != 0 && !exhausted) {
-      const int cnt = __popcll(need);
-      const int leader = __ffsll((long long)need) - 1;
-      uint32_t b0 = 0;
-      if (lane == leader) b0 = atomicAdd(&a.ctr->item, (uint32_t)cnt);
-      b0 = __shfl(b0, leader);
-      if (b0 + (uint32_t)cnt >= (uint32_t)a.n) exhausted = true;
-      if (!active) {
-        const uint32_t kk = b0 + rank_in(need);
-        if (kk < (uint32_t)a.n) {
-          k = kk;
-          build_rows(c, a.tabs + (size_t)k * E13_MAXCELLS, a.nq, -1, -1, 1, nullptr);
-          for (int w = 0; w < E13_TW; ++w) lds_st(c.sb, c.tbl + w * 256, 0u);
-          L.q = E13_QOFF;
-          L.row = c.rbl;
-          L.t = 0;
-          L.nt = 1;
-          L.stage = 0;
-          L.st0 = -1;
-          L.q0 = E13_QOFF;
-          L.flags = 3;
-          L.ones = 0;
-          L.hb = 0;
-          L.ch0 = L.ch1 = L.ch2 = 0;
-          L.dlcap = a.caps ? a.caps[k] : -1;
-          L.tcap = 0x7FFFFFFF;
-          arm_stage(L, c);
-          active = 1;
-        }
+    uint64_t need = ballot(!active);
+    for (int tries = 0; need != 0 && tries < 2; ++tries) {
+      if (ck >= ce) {
+        if (exhausted) break;
+        // tables come from a wave-private range claimed `chunk` at a time:
+        // one atomic on the shared cursor per chunk, not per refill
+        uint32_t b0 = 0;
+        if (lane == 0) b0 = atomicAdd(&a.ctr->item, a.chunk);
+        b0 = (uint32_t)uni((int)b0);
+        if (b0 >= (uint32_t)a.n || b0 + a.chunk >= (uint32_t)a.n) exhausted = true;
+        ck = b0;
+        ce = min(b0 + a.chunk, (uint32_t)a.n);
+        if (ck >= ce) break;
       }
+      const uint32_t give = min((uint32_t)__popcll(need), ce - ck);
+      const uint32_t rk = rank_in(need);
+      if (!active && rk < give) {
+        k = ck + rk;
+        build_rows(c, a.tabs + (size_t)k * E13_MAXCELLS, a.nq, -1, -1, 1, nullptr);
+        for (int w = 0; w < E13_TW; ++w) lds_st(c.sb, c.tbl + w * 256, 0u);
+        L.q = E13_QOFF;
+        L.row = c.rbl;
+        L.t = 0;
+        L.nt = 1;
+        L.stage = 0;
+        L.st0 = -1;
+        L.q0 = E13_QOFF;
+        L.flags = 3;
+        L.ones = 0;
+        L.hb = 0;
+        L.ch0 = L.ch1 = L.ch2 = 0;
+        L.dlcap = a.caps ? a.caps[k] : -1;
+        L.tcap = 0x7FFFFFFF;
+        arm_stage(L, c);
+        active = 1;
+      }
+      ck += give;
+      need = ballot(!active);
     }
     const uint64_t act = ballot(active);
     if (act == 0) {
-      if (exhausted) break;
+      if (exhausted && ck >= ce) break;
       continue;
     }
     w_rounds++;
@@ -688,6 +721,44 @@ __global__ void __launch_bounds__(256) k_batch(E13BatchArgs a) {
     atomicAdd((unsigned long long *)&a.ctr->steps, (unsigned long long)s_steps);
     atomicAdd((unsigned long long *)&a.ctr->rounds, (unsigned long long)w_rounds);
     atomicAdd((unsigned long long *)&a.ctr->lane_slots, (unsigned long long)w_slots);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Launch chains of the host-loop forms.  One iteration is k_chain_pop ->
+// k_expand (chain mode) -> k_chain_push; the stack cursor stays on the device,
+// so a hipGraph of many iterations runs with no host round trip.  An
+// iteration that finds the stack empty does nothing.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(64) k_chain_pop(E13Chain *ch, E13Counters *ctr) {
+  if (threadIdx.x == 0) {
+    const uint64_t top = ch->error ? 0ull : ch->top;
+    const uint32_t nin = top < (uint64_t)ch->batch ? (uint32_t)top : ch->batch;
+    ch->nin = nin;
+    ch->base = top - nin;
+    if (nin) ch->iters += 1u;
+    ctr->item = 0u; // the expansion's work cursor and child count restart (no memset node in a graph)
+    ctr->nout = 0u;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_chain_push(E13Chain *ch, E13Node *stack, uint64_t stack_cap,
+                                                    const E13Node *out, uint32_t out_cap, const E13Counters *ctr) {
+  const uint32_t nout = ctr->nout, nin = ch->nin;
+  const uint64_t base = ch->base;
+  if (nin == 0u) return;
+  if (ctr->overflow != 0u || nout > out_cap || base + nout > stack_cap) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) ch->error = (ctr->overflow & 4u) ? 2u : 1u;
+    return;
+  }
+  const uint4 *src = (const uint4 *)out;
+  uint4 *dst = (uint4 *)(stack + base);
+  const uint64_t n16 = (uint64_t)nout * (sizeof(E13Node) / 16);
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0) { // read only by the next kernel in stream order
+    ch->top = base + nout;
+    if (base + nout > ch->peak) ch->peak = base + nout;
   }
 }
 
@@ -741,6 +812,12 @@ __device__ __forceinline__ void vm_drain() {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
   __builtin_amdgcn_s_waitcnt(0);
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// The search is over (or aborted): every wave's mailbox says so.
+__device__ void broadcast_done(const E13SearchArgs &a, int lane) {
+  for (int w = lane; w < a.total_waves; w += 64) st_agent64(&a.mail[w].flag, E13_MAIL_DONE);
+  vm_drain();
 }
 
 // Encode both row sets of the staged node: A as stored, B with unread
@@ -877,8 +954,29 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
   L.tcap = 0x7FFFFFFF;
   int active = 0, t0 = 0, budget = 0, nintro = 0, prob = 0, depth = 0, nbound = 0, nq = 0;
   long long top = 0, bot = 0, staged = -1;
-  int cur_slot = 0, busy = 0, round = 0, idle_polls = 0, hungry = 0;
-  uint64_t w_slots = 0, w_rounds = 0;
+  int cur_slot = 0, busy = 0, requested = 0, mail_in = 1, idle_polls = 0;
+  uint64_t last_tag = 0; // tag of the last delivery taken from this wave's mailbox
+  uint64_t idle_since = 0; // constant-clock time this wave went idle (0: not idle)
+  // the frontier from the breadth-first expansion: seed i goes to wave i mod total_waves
+  // (the host counted every wave that receives one as busy)
+  {
+    const int W = a.total_waves;
+    const int mine = (int)(a.nseeds / (uint32_t)W) + (wid < (int)(a.nseeds % (uint32_t)W) ? 1 : 0);
+    for (int j = 0; j < mine; ++j) {
+      const uint32_t *src = (const uint32_t *)(a.seeds + (size_t)wid + (size_t)j * (size_t)W);
+      uint32_t *dst = (uint32_t *)stk_node(a, wid, j);
+      const uint32_t v0 = src[lane];
+      const uint32_t v1 = lane < E13_NODE_WORDS - 64 ? src[64 + lane] : 0u;
+      dst[lane] = v0;
+      if (lane < E13_NODE_WORDS - 64) dst[64 + lane] = v1;
+    }
+    top = mine;
+    busy = mine > 0 ? 1 : 0;
+  }
+  uint32_t tr_n = 0;
+  if (busy) TRACE_EV(TR_SEEDED);
+  int refresh_in = (wid % 8) + 1;
+  uint64_t w_slots = 0, w_rounds = 0, w_takes = 0, w_gives = 0;
   unsigned long long w_steps = 0;
   uint64_t cyc0 = 0, cyc1 = 0, cyc2 = 0, cyc3 = 0, ts = stamp(), tn;
 
@@ -977,110 +1075,107 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
     tn = stamp();
     cyc0 += tn - ts;
     ts = tn;
-    // ---- an empty private stack: take a node from the pool (time: pool/idle) ----
-    if (top == bot) {
+    // ---- this wave's mailbox: a donor may have delivered a node (time: pool/idle) ----
+    if (requested && --mail_in <= 0) {
+      mail_in = a.mail_every;
+      uint64_t f = 0;
+      if (lane == 0) f = ld_agent64(&a.mail[wid].flag);
+      f = (uint64_t)uni64((long long)f);
+      if (f == E13_MAIL_DONE) break; // the search is over (or aborted)
+      if (f != last_tag) {
+        if (top - bot + 1 > a.stk_cap) {
+          if (lane == 0) atomicMax(&a.sch->error, 1);
+          broadcast_done(a, lane);
+          break;
+        }
+        // read until every element carries this delivery's tag
+        const uint32_t tag = (uint32_t)f;
+        const uint64_t *src = a.mail[wid].w;
+        uint64_t e0 = 0, e1 = 0;
+        uint32_t spins = 0;
+        for (;;) {
+          e0 = ld_agent64(src + lane);
+          e1 = lane < E13_NODE_WORDS - 64 ? ld_agent64(src + 64 + lane) : ((uint64_t)tag << 32);
+          const bool ok = (uint32_t)(e0 >> 32) == tag && (uint32_t)(e1 >> 32) == tag;
+          if (ballot(!ok) == 0) break;
+          if (++spins > a.spin_limit) break;
+          __builtin_amdgcn_s_sleep(2);
+        }
+        if (spins > a.spin_limit) {
+          if (lane == 0) atomicMax(&a.sch->error, 2);
+          broadcast_done(a, lane);
+          break;
+        }
+        uint32_t *dst = (uint32_t *)stk_node(a, wid, top);
+        dst[lane] = (uint32_t)e0;
+        if (lane < E13_NODE_WORDS - 64) dst[64 + lane] = (uint32_t)e1;
+        // the donor counted the delivery as busy: it becomes this wave's count,
+        // or is returned when this wave was counted already
+        if (busy && lane == 0) atomicSub(&a.sch->busy, 1);
+        last_tag = f;
+        requested = 0;
+        busy = 1;
+        idle_since = 0;
+        top += 1;
+        ++w_takes;
+        TRACE_EV(TR_TAKE);
+        continue; // refill from it at once
+      }
+    }
+    // ---- an empty private stack and idle lanes: ask for work (one bit in the hungry map) ----
+    if (!requested && top == bot) {
       const int idle = __popcll(ballot(!active));
       if (idle >= a.pool_min_idle || idle == 64) {
-        long long got = -1;
-        int inc = 0;
-        if (lane == 0) {
-          uint64_t h = __hip_atomic_load(&a.sch->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          const uint64_t t = __hip_atomic_load(&a.sch->tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          if (h < t) {
-            if (!busy) { // counted busy before the claim, so the end test never misses a node in flight
-              atomicAdd(&a.sch->busy, 1);
-              vm_drain();
-              inc = 1;
-            }
-            if (__hip_atomic_compare_exchange_strong(&a.sch->head, &h, h + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_AGENT))
-              got = (long long)h;
-          }
-        }
-        if (uni(inc)) busy = 1;
-        got = uni64(got);
-        if (got >= 0) {
-          // the donor reserved this slot and is writing it: wait for its sequence word
-          const size_t ps = (size_t)(got % a.pool_cap);
-          uint32_t spins = 0, ready = 0;
-          for (;;) {
-            if (lane == 0) ready = ld_agent64(&a.pool_seq[ps]) == (uint64_t)got + 1;
-            ready = uni((int)ready);
-            if (ready) break;
-            if (++spins > a.spin_limit) {
-              if (lane == 0) atomicMax(&a.sch->error, 2);
-              break;
-            }
-            __builtin_amdgcn_s_sleep(8);
-          }
-          if (!ready) break;
-          const uint32_t *src = (const uint32_t *)(a.pool + ps);
-          uint32_t *dst = (uint32_t *)stk_node(a, wid, top);
-          const uint32_t v0 = ld_agent(src + lane);
-          const uint32_t v1 = lane < E13_NODE_WORDS - 64 ? ld_agent(src + 64 + lane) : 0u;
-          dst[lane] = v0;
-          if (lane < E13_NODE_WORDS - 64) dst[64 + lane] = v1;
-          vm_drain();
-          if (lane == 0) { // the slot is free for the producer of node got + pool_cap
-            st_agent64(&a.pool_seq[ps], (uint64_t)got + (uint64_t)a.pool_cap);
-            atomicAdd(&a.sch->pool_takes, 1);
-          }
-          top += 1;
-          continue; // refill from it at once
-        }
+        if (lane == 0) atomicOr((unsigned long long *)&a.hungry[wid >> 6], 1ull << (wid & 63));
+        requested = 1;
+        mail_in = 1;
       }
     }
     const uint64_t act = ballot(active);
     if (act == 0) {
       if (top != bot) continue;
-      // no work here: leave the busy count, then look for the end
+      // no work here: leave the busy count; the wave that takes it to zero ends the search
       if (busy) {
-        vm_drain(); // every donation of this wave is complete before it stops counting as busy
-        if (lane == 0) atomicSub(&a.sch->busy, 1);
+        vm_drain(); // every delivery of this wave is complete before it stops counting as busy
+        int last = 0;
+        if (lane == 0) last = atomicSub(&a.sch->busy, 1) == 1;
         busy = 0;
+        TRACE_EV(TR_IDLE);
+        if (uni(last)) {
+          broadcast_done(a, lane);
+          break;
+        }
       }
-      int done = 0;
-      if (lane == 0) {
-        // busy first, then the pool: a wave that sees no busy wave and an empty
-        // pool after that can leave; one that sees a node keeps trying to claim
-        const int b = ld_agent_i(&a.sch->busy);
-        vm_drain();
-        const uint64_t h = ld_agent64(&a.sch->head);
-        const uint64_t t = ld_agent64(&a.sch->tail);
-        const int err = ld_agent_i(&a.sch->error);
-        done = ((b == 0 && h >= t) || err) ? 1 : 0;
+      const uint64_t now = __builtin_readsteadycounter();
+      if (idle_since == 0) idle_since = now;
+      if ((++idle_polls & 63) == 0) { // an abort may have overwritten the end mark of this mailbox
+        int err = 0;
+        if (lane == 0) err = ld_agent_i(&a.sch->error);
+        if (uni(err)) break;
       }
-      if (uni(done)) {
-        if (lane == 0) atomicAdd(&a.sch->exited, 1);
-        break;
-      }
-      if (++idle_polls > a.idle_limit) { // watchdog: no work for seconds while others still count busy
+      if (now - idle_since > (uint64_t)a.idle_limit) { // watchdog: no delivery and no end for this long
         if (lane == 0) atomicMax(&a.sch->error, 3);
+        broadcast_done(a, lane);
         break;
       }
-      // exponential backoff: idle waves poll the pool rarely, so busy waves are not slowed
-      const int naps = idle_polls < 7 ? 1 << (idle_polls - 1) : 64;
-      for (int z = 0; z < naps; ++z) __builtin_amdgcn_s_sleep(127);
+      for (int z = 0; z < a.idle_naps; ++z) __builtin_amdgcn_s_sleep(8);
       tn = stamp();
       cyc3 += tn - ts;
       ts = tn;
       continue;
     }
-    idle_polls = 0;
-    if (!busy) { // lanes still running with an empty stack and no pool node: still busy
-      if (lane == 0) atomicAdd(&a.sch->busy, 1);
-      busy = 1;
-    }
+    idle_since = 0;
     w_rounds++;
     w_slots += (uint64_t)__popcll(act);
-    // shared scheduler state, read every sched_every rounds (staggered over the
-    // waves so the words are not hot spots); the loads complete while the lanes step
-    const bool refresh = ((round + wid) & (int)(a.sched_every - 1)) == 0;
-    int s_busy = 0, s_exited = 0, s_err = 0;
-    if (refresh && lane == 0) {
-      s_busy = ld_agent_i(&a.sch->busy);
-      s_exited = ld_agent_i(&a.sch->exited);
-      s_err = ld_agent_i(&a.sch->error);
+    if ((w_rounds & 7u) == 0u) TRACE_EV2(TR_ROUND, __popcll(act) | (top - bot > 2 ? 0x80 : 0));
+    // shared state, read every few rounds (staggered over the waves): the abort
+    // flag and the hungry map; the loads complete while the lanes step
+    const bool refresh = --refresh_in <= 0;
+    int s_err = 0;
+    uint64_t hw = 0; // hungry-map word `lane` (maps of up to 4096 waves)
+    if (refresh) {
+      if (lane == 0) s_err = ld_agent_i(&a.sch->error);
+      if (lane < a.hungry_words && top - bot >= 2) hw = ld_agent64(&a.hungry[lane]);
     }
     tn = stamp();
     cyc3 += tn - ts;
@@ -1128,6 +1223,7 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
       const int n = __popcll(em);
       if (top - bot + n > a.stk_cap) {
         if (lane == 0) atomicMax(&a.sch->error, 1);
+        broadcast_done(a, lane);
         break;
       }
       if (staged >= bot && staged < top && lane == 0) stk_node(a, wid, staged)->next_slot = cur_slot;
@@ -1143,55 +1239,73 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
       cur_slot = 0;
     }
     if (fin) active = 0;
+    if (requested && top > bot) { // work of its own again: withdraw the request unless a donor took it
+      const uint64_t bit = 1ull << (wid & 63);
+      int withdrawn = 0;
+      if (lane == 0) withdrawn = (atomicAnd((unsigned long long *)&a.hungry[wid >> 6], ~bit) & bit) != 0;
+      if (uni(withdrawn)) requested = 0; // else a delivery is on its way: keep polling the mailbox
+    }
     tn = stamp();
     cyc2 += tn - ts;
     ts = tn;
     if (refresh) {
       if (uni(s_err)) break;
-      hungry = uni(a.total_waves - s_busy - s_exited);
-    }
-    // ---- feed the pool when waves are hungry: give away the oldest node ----
-    ++round;
-    for (int dn = 0; refresh && dn < a.donate_every && top - bot >= 2 && hungry > 0; ++dn) {
-      const int hungry_est = hungry;
-      int give = 0;
-      uint64_t t = 0;
-      if (lane == 0) {
-        const uint64_t h = ld_agent64(&a.sch->head);
-        t = ld_agent64(&a.sch->tail);
-        if ((long long)(t - h) < (long long)hungry_est && t - h < (uint64_t)a.pool_cap &&
-            ld_agent64(&a.pool_seq[t % a.pool_cap]) == t &&
-            __hip_atomic_compare_exchange_strong(&a.sch->tail, &t, t + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT))
-          give = 1;
-      }
-      give = uni(give);
-      if (give) {
-        t = (uint64_t)uni64((long long)t);
-        if (staged == bot) staged = -1;
+      // ---- deliver the oldest nodes to hungry waves ----
+      uint64_t nz = ballot(hw != 0);
+      int given = 0;
+      while (nz != 0 && given < a.donate_every && top - bot >= 2) {
+        // a word and a bit chosen by this wave's id, so donors spread over the hungry waves
+        const int rot = (wid * 7) & 63;
+        const uint64_t nzr = (nz >> rot) | (rot ? (nz << (64 - rot)) : 0ull);
+        const int wsel = (__ffsll((long long)nzr) - 1 + rot) & 63;
+        const uint64_t word = (uint64_t)__shfl((long long)hw, wsel);
+        const int brot = (wid * 13 + given) & 63;
+        const uint64_t wr = (word >> brot) | (brot ? (word << (64 - brot)) : 0ull);
+        const int bsel = (__ffsll((long long)wr) - 1 + brot) & 63;
+        const uint64_t bit = 1ull << bsel;
+        int won = 0;
+        if (lane == 0) {
+          won = (atomicAnd((unsigned long long *)&a.hungry[wsel], ~bit) & bit) != 0;
+          if (won) atomicAdd(&a.sch->busy, 1); // the delivery counts as busy until it is taken
+        }
+        won = uni(won);
+        if (lane == wsel) hw &= ~bit;
+        nz = ballot(hw != 0);
+        if (!won) continue;
+        const int dst_w = wsel * 64 + bsel;
+        if (staged == bot) { // a delivery may sit above it: its slot cursor travels with it
+          if (lane == 0) stk_node(a, wid, staged)->next_slot = cur_slot;
+          staged = -1;
+        }
         wave_mem_sync();
         const uint32_t *src = (const uint32_t *)stk_node(a, wid, bot);
-        uint32_t *dst = (uint32_t *)(a.pool + t % a.pool_cap);
+        uint64_t *dst = a.mail[dst_w].w;
+        const uint64_t tag = ((uint64_t)(wid + 1) << 19) | ((w_gives + 1) & 0x7FFFFull); // unique, never 0
         const uint32_t v0 = src[lane];
         const uint32_t v1 = lane < E13_NODE_WORDS - 64 ? src[64 + lane] : 0u;
-        st_agent(dst + lane, v0);
-        if (lane < E13_NODE_WORDS - 64) st_agent(dst + 64 + lane, v1);
-        vm_drain(); // the node is complete before its sequence word says so
-        if (lane == 0) {
-          st_agent64(&a.pool_seq[t % a.pool_cap], t + 1);
-          atomicAdd(&a.sch->donations, 1);
-        }
+        st_agent64(dst + lane, (tag << 32) | v0);
+        if (lane < E13_NODE_WORDS - 64) st_agent64(dst + 64 + lane, (tag << 32) | v1);
+        vm_drain(); // the elements are issued and complete before the flag
+        if (lane == 0) st_agent64(&a.mail[dst_w].flag, tag);
         bot += 1;
-        hungry -= 1;
-      } else {
-        break;
+        ++given;
+        ++w_gives;
+        TRACE_EV(TR_GIVE);
       }
+      refresh_in = given > 0 || nz != 0 ? 2 : (int)a.sched_every; // look again soon while waves wait
     }
     tn = stamp();
     cyc3 += tn - ts;
     ts = tn;
   }
   // ---- flush ----
+  if (lane == 0) {
+    atomicAdd(&a.sch->exited, 1);
+    atomicAdd(&a.sch->donations, (int)w_gives);
+    atomicAdd(&a.sch->pool_takes, (int)w_takes);
+  }
+  TRACE_EV(TR_EXIT);
+  if (a.trace != nullptr && lane == 0) a.trace[(size_t)wid * (size_t)a.trace_cap] = tr_n;
   cyc3 += stamp() - ts;
   if (lane == 0) {
     atomicAdd((unsigned long long *)&a.ctr->cyc[0], (unsigned long long)cyc0);
@@ -1230,6 +1344,17 @@ int e13k_launch_expand(const E13ExpandArgs *a, int grid, int wg_threads, int gat
   return (int)hipGetLastError();
 }
 
+int e13k_launch_chain_pop(E13Chain *ch, E13Counters *ctr, hipStream_t s) {
+  hipLaunchKernelGGL(k_chain_pop, dim3(1), dim3(64), 0, s, ch, ctr);
+  return (int)hipGetLastError();
+}
+
+int e13k_launch_chain_push(E13Chain *ch, E13Node *stack, uint64_t stack_cap, const E13Node *out, uint32_t out_cap,
+                           const E13Counters *ctr, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_chain_push, dim3(grid), dim3(256), 0, s, ch, stack, stack_cap, out, out_cap, ctr);
+  return (int)hipGetLastError();
+}
+
 size_t e13k_search_lds_bytes(int wg_threads, int gate, int np) {
   const int nw = wg_threads / WAVE;
   return (size_t)nw * (ROWS_BYTES + TAPE_BYTES + STG_WORDS * 4) + (gate == GATE_HASH ? KEYB_WORDS * 8 : 0) +
@@ -1243,10 +1368,15 @@ int e13k_launch_search(const E13SearchArgs *a, int grid, int wg_threads, int gat
   return (int)hipGetLastError();
 }
 
-int e13k_launch_batch(const E13BatchArgs *a, int grid, int wg_threads, int gate, hipStream_t s) {
+int e13k_launch_batch(const E13BatchArgs *a, int grid, int wg_threads, int gate, int refill, hipStream_t s) {
   const size_t lds = e13k_lds_bytes(wg_threads, gate);
-  if (gate == GATE_HASH) hipLaunchKernelGGL((k_batch<GATE_HASH>), dim3(grid), dim3(wg_threads), lds, s, *a);
-  else hipLaunchKernelGGL((k_batch<GATE_COUNT>), dim3(grid), dim3(wg_threads), lds, s, *a);
+  if (gate == GATE_HASH) {
+    if (refill) hipLaunchKernelGGL((k_batch<GATE_HASH, 1>), dim3(grid), dim3(wg_threads), lds, s, *a);
+    else hipLaunchKernelGGL((k_batch<GATE_HASH, 0>), dim3(grid), dim3(wg_threads), lds, s, *a);
+  } else {
+    if (refill) hipLaunchKernelGGL((k_batch<GATE_COUNT, 1>), dim3(grid), dim3(wg_threads), lds, s, *a);
+    else hipLaunchKernelGGL((k_batch<GATE_COUNT, 0>), dim3(grid), dim3(wg_threads), lds, s, *a);
+  }
   return (int)hipGetLastError();
 }
 

@@ -86,6 +86,17 @@ typedef struct {
   int32_t gate;        /* 1 ones-count gate, 2 reference Zobrist hash gate */
   int32_t refill_tries;
   int64_t stack_nodes; /* node stack capacity */
+  /* persistent form (3) scheduling */
+  int32_t seed_per_wave;  /* breadth-first frontier per wave before the launch; 0 = start from the roots */
+  int32_t sched_every;    /* rounds between reads of the hungry map while no wave is hungry */
+  int32_t donate_max;     /* nodes a wave may deliver per read of the hungry map */
+  int32_t idle_naps;      /* an idle wave's pause between mailbox polls, in units of s_sleep(8) */
+  int32_t mail_every;     /* rounds between mailbox polls of a wave whose lanes still run */
+  int32_t trace_events;   /* > 0: log up to this many scheduling events per wave (e13_get_trace) */
+  /* host-loop forms (1, 2) */
+  int32_t graph_chain;    /* > 0: iterations (pop, expand, push) per captured hipGraph; 0 = one launch
+                             and one host round trip per iteration */
+  int32_t pad_g;
 } e13_tuning;
 
 typedef struct {
@@ -98,6 +109,7 @@ typedef struct {
   int64_t donations, pool_takes; /* persistent form: nodes moved between waves */
   int64_t cycles[4];             /* wave clock cycles spent refilling lanes, stepping, finishing
                                     machines, and on the work pool / idle (summed over waves) */
+  int64_t graphs;                /* hipGraph launches (graph_chain > 0) */
 } e13_stats;
 
 typedef struct e13_ctx e13_ctx;
@@ -110,6 +122,15 @@ const char *e13_strerror(int code);
 int e13_set_tuning(e13_ctx *ctx, const e13_tuning *t);
 int e13_get_tuning(e13_ctx *ctx, e13_tuning *t);
 int e13_get_stats(e13_ctx *ctx, e13_stats *s);
+
+/* The wave event log of the last persistent search run with trace_events > 0:
+   per wave `per_wave` words, word 0 = events logged, then
+   (t << 16 | aux << 8 | event) with t in ticks of the GPU's constant 100 MHz
+   clock and event 1 seeded, 2 took a pool node while idle, 3 counted busy
+   again, 4 idle, 5 exit, 6 gave a node, 7 every 8th round (aux = running lanes,
+   +128 when more than two nodes wait on the wave's stack).
+   No reference counterpart (a diagnostic of this implementation's scheduler). */
+int e13_get_trace(e13_ctx *ctx, uint64_t *buf, uint64_t cap_words, int32_t *waves, int32_t *per_wave);
 
 /* Zobrist tables: KEY[0..1363], TGT[0..9] (e13search.c:437-444). */
 int e13_keys(uint64_t *key, uint64_t *tgt);
@@ -160,6 +181,27 @@ int e13_format_hash(const e13_problem *p, const e13_survivor *s, char *buf, int 
 /* Synthetic stress table k: splitmix64 from seed0 + k; 28 cells v = next() % 56;
    halt cell next() % 28; cap floor(10 * 2e4^u), u = (next() >> 11) * 2^-53. */
 void e13_synth_table(uint64_t seed0, uint64_t k, int8_t *tab28, int32_t *cap);
+
+/* Totals and an order-independent digest of a batch of complete-table runs.
+   Record hash of table k: h = mix(k), then h = mix(h ^ x) for x = result,
+   stage, steps, st0, head0, abs_ok | rel_ok << 1 (each as 32-bit two's
+   complement), ch0, ch1, ch2, where mix(x) is the splitmix64 output for state
+   x; digest = sum of h mod 2^64. */
+typedef struct {
+  uint64_t n, digest, steps, survivors;
+  uint64_t dead_at_stage[10];
+  double kernel_ms; /* HIP-event time of the batch kernel */
+} e13_batch_sum;
+
+int e13_batch_digest(const e13_leaf *leaves, int64_t n, uint64_t k0, e13_batch_sum *out);
+
+/* The synthetic batching stress test: tables k0 .. k0+n-1 of e13_synth_table,
+   each with its cap as every stage deadline, made on the host, copied to HBM,
+   and run in one batch launch (form 1: one table per lane; otherwise lanes
+   refill from a device cursor as their tables end); returns the totals and
+   digest of e13_batch_digest.  No reference counterpart: the workload is
+   defined by this build. */
+int e13_synth_run(e13_ctx *ctx, uint64_t seed0, uint64_t k0, int64_t n, e13_batch_sum *out);
 
 #ifdef __cplusplus
 }

@@ -17,6 +17,8 @@
  * Extra modes for the tests:
  *   e13oracle synth SEED0 K0 N        synthetic stress tables K0..K0+N-1: one
  *                                     line "k result stage t st0 head0 ch0 ch1 ch2"
+ *   e13oracle synthsum SEED0 K0 N     the same tables: one line with the
+ *                                     order-independent digest and totals"
  *   e13oracle hash  machine-lines <   SURVIVOR lines on stdin -> HASH lines
  *                                     (all cells fixed; '?---' read as halt)
  *
@@ -159,6 +161,18 @@ int main(int argc, char **argv) {
              sim.st0, sim.head0, (unsigned long long)sim.ch[0], (unsigned long long)sim.ch[1],
              (unsigned long long)sim.ch[2]);
     }
+    return 0;
+  }
+  if (!strcmp(mode, "synthsum")) {
+    uint64_t seed0 = strtoull(argv[2], NULL, 0), k0 = argc > 3 ? strtoull(argv[3], NULL, 0) : 0;
+    long long n = argc > 4 ? atoll(argv[4]) : 16;
+    eo_synth_sum sm;
+    eo_synth_digest(seed0, k0, n, 0, &sm);
+    printf("synth seed0=0x%llx k0=%llu n=%llu digest=%016llx steps=%llu survivors=%llu dead-at-stage(0..9):",
+           (unsigned long long)seed0, (unsigned long long)k0, (unsigned long long)sm.n,
+           (unsigned long long)sm.digest, (unsigned long long)sm.steps, (unsigned long long)sm.survivors);
+    for (int j = 0; j <= EO_NS; j++) printf(" %llu", (unsigned long long)sm.hist[j]);
+    printf("\n");
     return 0;
   }
   if (!strcmp(mode, "hash")) {

@@ -107,6 +107,17 @@ int eo_recheck(const int *tab, int nq, int st0, int head0, int abs_ok, int rel_o
    cell, log-uniform step cap in [10, 2e5]); see e13oracle_core.c. */
 void eo_synth_table(uint64_t seed0, uint64_t k, int *tab28, int32_t *cap);
 
+/* Order-independent digest and totals of the synthetic tables k0..k0+n-1, each
+   run with its cap as every stage deadline (see e13oracle_core.c for the
+   record hash). */
+typedef struct {
+  uint64_t n, digest, steps, survivors;
+  uint64_t hist[EO_NS + 1];
+} eo_synth_sum;
+uint64_t eo_synth_record_hash(uint64_t k, int result, int stage, long long t, int st0, int head0, int abs_ok,
+                              int rel_ok, const uint64_t ch[3]);
+void eo_synth_digest(uint64_t seed0, uint64_t k0, long long n, int threads, eo_synth_sum *out);
+
 #ifdef __cplusplus
 }
 #endif

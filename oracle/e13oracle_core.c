@@ -405,3 +405,63 @@ void eo_synth_table(uint64_t seed0, uint64_t k, int *tab28, int32_t *cap) {
   double u = (double)(eo_splitmix(&st) >> 11) * (1.0 / 9007199254740992.0);
   *cap = (int32_t)floor(10.0 * pow(2e4, u));
 }
+
+/* --------------------------------------------------------------------------
+   Order-independent digest of a synthetic batch (tables k0 .. k0+n-1).  Per
+   table: h = mix(k); then h = mix(h ^ x) for x = result, stage, t, st0, head0,
+   abs_ok | rel_ok << 1 (each as a 32-bit two's-complement value), ch0, ch1,
+   ch2; the digest is the sum of h mod 2^64, with mix(x) the splitmix64
+   output for state x.  Totals: steps, survivors, dead-at-stage histogram.
+   -------------------------------------------------------------------------- */
+static uint64_t sd_mix(uint64_t x) { return eo_splitmix(&x); }
+
+uint64_t eo_synth_record_hash(uint64_t k, int result, int stage, long long t, int st0, int head0, int abs_ok,
+                              int rel_ok, const uint64_t ch[3]) {
+  uint64_t h = sd_mix(k);
+  h = sd_mix(h ^ (uint64_t)(uint32_t)result);
+  h = sd_mix(h ^ (uint64_t)(uint32_t)stage);
+  h = sd_mix(h ^ (uint64_t)(uint32_t)t);
+  h = sd_mix(h ^ (uint64_t)(uint32_t)st0);
+  h = sd_mix(h ^ (uint64_t)(uint32_t)head0);
+  h = sd_mix(h ^ (uint64_t)(uint32_t)(abs_ok | (rel_ok << 1)));
+  h = sd_mix(h ^ ch[0]);
+  h = sd_mix(h ^ ch[1]);
+  h = sd_mix(h ^ ch[2]);
+  return h;
+}
+
+void eo_synth_digest(uint64_t seed0, uint64_t k0, long long n, int threads, eo_synth_sum *out) {
+  eo_synth_sum tot;
+  memset(&tot, 0, sizeof tot);
+  if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel
+  {
+    eo_synth_sum me;
+    memset(&me, 0, sizeof me);
+#pragma omp for schedule(dynamic, 256)
+    for (long long j = 0; j < n; j++) {
+      int tab[2 * EO_MAXQ];
+      int32_t cap, dl[EO_NS + 1];
+      eo_synth_table(seed0, k0 + (uint64_t)j, tab, &cap);
+      for (int s = 0; s <= EO_NS; s++) dl[s] = cap;
+      eo_sim *sim = (eo_sim *)malloc(sizeof(eo_sim));
+      int r = eo_run_full(tab, sim, dl);
+      me.digest += eo_synth_record_hash(k0 + (uint64_t)j, r, sim->stage, sim->t, sim->st0, sim->head0, sim->abs_ok,
+                                        sim->rel_ok, sim->ch);
+      me.steps += (uint64_t)sim->t;
+      if (r == EO_SURV) me.survivors++;
+      else me.hist[sim->stage]++;
+      free(sim);
+    }
+#pragma omp critical
+    {
+      tot.digest += me.digest;
+      tot.steps += me.steps;
+      tot.survivors += me.survivors;
+      for (int s = 0; s <= EO_NS; s++) tot.hist[s] += me.hist[s];
+    }
+  }
+  tot.n = (uint64_t)n;
+  *out = tot;
+}
+

@@ -139,3 +139,27 @@ def gpu_ctx(e13):
     ctx = e13.Context(0)
     yield ctx
     ctx.close()
+
+
+class OracleSim(ctypes.Structure):
+    """eo_sim of oracle/e13oracle.h."""
+    _fields_ = [("head", ctypes.c_int), ("st", ctypes.c_int), ("nt", ctypes.c_int), ("stage", ctypes.c_int),
+                ("st0", ctypes.c_int), ("head0", ctypes.c_int), ("abs_ok", ctypes.c_int),
+                ("rel_ok", ctypes.c_int), ("cs", ctypes.c_int * 3), ("t", ctypes.c_longlong),
+                ("deadline", ctypes.c_longlong), ("hb", ctypes.c_uint64), ("hc", ctypes.c_uint64),
+                ("ch", ctypes.c_uint64 * 3), ("tape", ctypes.c_uint8 * 1364)]
+
+
+def oracle_full(L, tab, dl):
+    """The oracle's run_full(): (result, stage, t, st0, head0, abs_ok, rel_ok, ch)."""
+    s = OracleSim()
+    t32 = (ctypes.c_int * 32)(*(list(tab) + [-1] * (32 - len(tab))))
+    r = L.eo_run_full(t32, ctypes.byref(s), (ctypes.c_int32 * 10)(*dl) if dl else None)
+    return (r, s.stage, s.t, s.st0, s.head0, s.abs_ok, s.rel_ok, tuple(s.ch))
+
+
+def synth_golden():
+    """{n: summary line} of the synthetic stress batch (seed 0x5EED) from the oracle."""
+    with open(os.path.join(GOLDEN, "synth", "synth_5eed.txt")) as f:
+        return {int(re.search(r" n=(\d+)", l).group(1)): l.strip() for l in f if l.startswith("synth ")}
+

@@ -8,8 +8,8 @@ import ctypes
 
 import pytest
 
-from conftest import (REF_DUMP, REF_RESULTS, XS, oracle_lib, parse_summaries, read_machine, read_text, ref_hashes,
-                      survivor_lines)
+from conftest import (REF_DUMP, REF_RESULTS, XS, oracle_full, oracle_lib, parse_summaries, read_machine, read_text,
+                      ref_hashes, survivor_lines, synth_golden)
 
 pytestmark = pytest.mark.gpu
 
@@ -72,19 +72,6 @@ def test_mutants(e13, gpu_ctx, base):
     assert got == want
 
 
-def oracle_full(L, tab, dl):
-    class Sim(ctypes.Structure):
-        _fields_ = [("head", ctypes.c_int), ("st", ctypes.c_int), ("nt", ctypes.c_int), ("stage", ctypes.c_int),
-                    ("st0", ctypes.c_int), ("head0", ctypes.c_int), ("abs_ok", ctypes.c_int),
-                    ("rel_ok", ctypes.c_int), ("cs", ctypes.c_int * 3), ("t", ctypes.c_longlong),
-                    ("deadline", ctypes.c_longlong), ("hb", ctypes.c_uint64), ("hc", ctypes.c_uint64),
-                    ("ch", ctypes.c_uint64 * 3), ("tape", ctypes.c_uint8 * 1364)]
-    s = Sim()
-    t32 = (ctypes.c_int * 32)(*(list(tab) + [-1] * (32 - len(tab))))
-    r = L.eo_run_full(t32, ctypes.byref(s), (ctypes.c_int32 * 10)(*dl) if dl else None)
-    return (r, s.stage, s.t, s.st0, s.head0, s.abs_ok, s.rel_ok, tuple(s.ch))
-
-
 @pytest.mark.parametrize("gate", [1, 2])
 def test_synthetic_batch_matches_oracle(e13, gpu_ctx, gate):
     n = 1 << 13
@@ -105,6 +92,19 @@ def test_synthetic_batch_matches_oracle(e13, gpu_ctx, gate):
         got = (lv[k].result, lv[k].stage, lv[k].steps, lv[k].st0, lv[k].head0, lv[k].abs_ok, lv[k].rel_ok,
                tuple(lv[k].ch))
         assert got == want, (k, got, want)
+
+
+@pytest.mark.parametrize("n,form,gate", [(1 << 16, 2, 2), (1 << 16, 1, 1), (1 << 22, 2, 1), (1 << 22, 1, 1)])
+def test_synthetic_stress_batch_matches_golden(e13, gpu_ctx, n, form, gate):
+    """The 2^22-table stress batch (uneven lifetimes, 10..2e5 steps): totals and
+    the order-independent digest of every table's result equal the oracle's."""
+    say(f"synth_run: {n} synthetic tables form={form} gate={gate}")
+    gpu_ctx.set_tuning(form=form, gate=gate)
+    try:
+        sm = gpu_ctx.synth_run(0x5EED, 0, n)
+    finally:
+        gpu_ctx.set_tuning(form=3, gate=1)
+    assert sm.line(0x5EED, 0) == synth_golden()[n]
 
 
 def check_against(sums, golden, keys=("leaves", "survivors", "steps", "hist")):
@@ -226,6 +226,9 @@ def test_counter_3state_symmetric(e13, gpu_ctx, base):
                                   dict(form=2, slice=200, cap=100000, batch=1 << 18),
                                   dict(form=2, wg_threads=64, wgs_per_cu=1, refill_tries=1),
                                   dict(form=1, cap=33, batch=17),
+                                  dict(form=1, cap=33, batch=17, graph_chain=8),
+                                  dict(form=2, slice=5, cap=97, batch=3, graph_chain=16),
+                                  dict(form=2, graph_chain=4),
                                   dict(form=3, slice=1), dict(form=3, slice=7, refill_tries=1),
                                   dict(form=3, slice=300, wg_threads=128, wgs_per_cu=3),
                                   dict(form=3, wg_threads=64, wgs_per_cu=1, stack_nodes=1 << 16)])
@@ -236,7 +239,7 @@ def test_results_invariant_under_tuning(e13, gpu_ctx, base, tune):
         sums, lines, hashes = search_all(e13, gpu_ctx, base, [7, 8, 12], fmax=2)
     finally:
         gpu_ctx.set_tuning(form=3, slice=32, cap=4096, batch=1 << 17, wg_threads=256, wgs_per_cu=2, refill_tries=2,
-                           stack_nodes=1 << 25)
+                           stack_nodes=1 << 25, graph_chain=-1)
     golden = parse_summaries(read_text(REF_DUMP, "delete_f2.txt"))
     check_against(sums, {X: golden[X] for X in (7, 8, 12)})
     assert lines == survivor_lines(read_text(REF_RESULTS, "1_local_edits_surv_f2.txt"))

@@ -193,6 +193,32 @@ def test_synth_tables_match_oracle(e13):
     assert 0.2 < frac < 0.3
 
 
+def test_batch_digest_matches_oracle(e13):
+    """The library's batch digest (host C) over the oracle's own per-table
+    results equals the oracle's synthsum digest of the same tables."""
+    from conftest import oracle_full, run_oracle
+    L = oracle_lib()
+    L.eo_synth_table.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int),
+                                 ctypes.POINTER(ctypes.c_int32)]
+    n = 3000
+    lv = (e13.Leaf * n)()
+    for k in range(n):
+        t = (ctypes.c_int * 32)()
+        c = ctypes.c_int32()
+        L.eo_synth_table(0x5EED, 100 + k, t, ctypes.byref(c))
+        r, stage, steps, st0, head0, ab, rel, ch = oracle_full(L, list(t)[:28], [c.value] * 10)
+        lv[k].result, lv[k].stage, lv[k].steps, lv[k].st0, lv[k].head0 = r, stage, steps, st0, head0
+        lv[k].abs_ok, lv[k].rel_ok = ab, rel
+        for i in range(3):
+            lv[k].ch[i] = ch[i]
+    mine = e13.batch_digest(lv, 100).line(0x5EED, 100)
+    want = run_oracle("synthsum", "0x5EED", 100, n)[0].strip()
+    assert mine == want
+    # a planted slip (one step count off by one) changes the digest
+    lv[7].steps += 1
+    assert e13.batch_digest(lv, 100).digest != int(re.search(r"digest=([0-9a-f]+)", want).group(1), 16)
+
+
 def test_open_without_device_fails_cleanly(e13):
     import torch
     if torch.cuda.is_available():
