@@ -86,6 +86,47 @@ def extra_workload(e13, ctx, name, problems, want_leaves, want_steps, want_surv)
             "kernel_ms": s1["kernel_ms"] - s0["kernel_ms"]}
 
 
+def synth_workload(e13, ctx, n=1 << 22):
+    """The synthetic stress batch: 2^22 seeded random tables with log-uniform
+    caps (10..2e5 steps), in the refilling form and the plain one-table-per-lane
+    form; totals and digest checked against the oracle's golden line."""
+    with open(os.path.join(ROOT, "tests", "golden", "synth", "synth_5eed.txt")) as f:
+        want = [l.strip() for l in f if f" n={n} " in l][0]
+    out = {"workload": f"synthetic stress batch, {n} tables, seed 0x5EED"}
+    for form, name in ((2, "refill"), (1, "plain")):
+        ctx.set_tuning(form=form)
+        sm = ctx.synth_run(0x5EED, 0, n)
+        out[name] = {"kernel_ms": sm.kernel_ms, "steps_per_s": sm.steps / (sm.kernel_ms * 1e-3),
+                     "tables_per_s": n / (sm.kernel_ms * 1e-3), "bit_identical_digest": sm.line(0x5EED, 0) == want}
+    ctx.set_tuning(form=3)
+    out["steps"] = sm.steps
+    out["refill_over_plain"] = out["plain"]["kernel_ms"] / out["refill"]["kernel_ms"]
+    return out
+
+
+def plain_vs_batched(e13, ctx, problem):
+    """The 3-state counter search in the plain per-leaf-launch form (form 1:
+    one machine per lane, one launch per batch, host loop) against the
+    persistent batched form (form 3), counts checked both ways."""
+    res = {"workload": "3-state counter search: plain per-leaf-launch form vs persistent form"}
+    for form in (1, 3):
+        ctx.set_tuning(form=form)
+        ctx.sync()
+        s0 = ctx.stats()
+        t0 = time.perf_counter()
+        cs, sv, ns = ctx.search_many([problem], surv_cap=4096) if form == 3 else (lambda r: ([r[0]], r[1], r[2]))(
+            ctx.search(problem, surv_cap=4096))
+        dt = time.perf_counter() - t0
+        s1 = ctx.stats()
+        ok = cs[0].leaves == 3078656272 and cs[0].steps == 319112357176 and ns == 61
+        res[f"form{form}"] = {"seconds": dt, "kernel_ms": s1["kernel_ms"] - s0["kernel_ms"],
+                              "launches": s1["launches"] - s0["launches"], "steps_per_s": cs[0].steps / dt,
+                              "bit_identical_counts": ok}
+    ctx.set_tuning(form=3)
+    res["persistent_over_plain"] = res["form1"]["seconds"] / res["form3"]["seconds"]
+    return res
+
+
 def cpu_baseline(max_s=120):
     """The reference CPU search (or the oracle port) on the same workload."""
     ref = os.path.join(ROOT, "oracle", "_ref", "e13search")
@@ -188,6 +229,8 @@ def main():
                                     3078656272, 319112357176, 61))
         f4 = [e13.setup(base, X, 4) for X in XS]
         extra.append(extra_workload(e13, ctx, "delete FMAX=4 X=0..13", f4, 5986617022, 586158995069, 57))
+        extra.append(synth_workload(e13, ctx))
+        extra.append(plain_vs_batched(e13, ctx, counter))
     line = {
         "metric": "simulated TM steps/s, local-edit search FMAX=3 (X=0..13) on erdos_14, bit-identical to the "
                   "reference",

@@ -77,13 +77,23 @@ class BatchSum(ctypes.Structure):
                 f"survivors={self.survivors} dead-at-stage(0..9): " + " ".join(str(x) for x in self.dead_at_stage))
 
 
+NFEAT = 32
+MLP_PARAMS = 6402
+
+
+class CostRec(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_float * NFEAT), ("leaves", ctypes.c_uint64), ("steps", ctypes.c_uint64),
+                ("prob", ctypes.c_int32), ("depth", ctypes.c_int32)]
+
+
 class Tuning(ctypes.Structure):
     _fields_ = [("form", ctypes.c_int32), ("slice", ctypes.c_int32), ("wg_threads", ctypes.c_int32),
                 ("wgs_per_cu", ctypes.c_int32), ("cap", ctypes.c_int32), ("batch", ctypes.c_int32),
                 ("gate", ctypes.c_int32), ("refill_tries", ctypes.c_int32), ("stack_nodes", ctypes.c_int64),
                 ("seed_per_wave", ctypes.c_int32), ("sched_every", ctypes.c_int32), ("donate_max", ctypes.c_int32),
                 ("idle_naps", ctypes.c_int32), ("mail_every", ctypes.c_int32), ("trace_events", ctypes.c_int32),
-                ("graph_chain", ctypes.c_int32), ("pad_g", ctypes.c_int32)]
+                ("graph_chain", ctypes.c_int32), ("cost_log", ctypes.c_int32),
+                ("learned", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -106,7 +116,9 @@ class Stats(ctypes.Structure):
 EXPORTS = ("e13_open", "e13_close", "e13_strerror", "e13_set_tuning", "e13_get_tuning", "e13_get_stats",
            "e13_keys", "e13_parse_machine", "e13_setup", "e13_search", "e13_search_many", "e13_simulate_batch",
            "e13_simulate_batch_device", "e13_sync", "e13_recheck", "e13_format_survivor", "e13_format_hash",
-           "e13_synth_table", "e13_get_trace", "e13_batch_digest", "e13_synth_run")
+           "e13_synth_table", "e13_get_trace", "e13_batch_digest", "e13_synth_run", "e13_get_cost_log",
+           "e13_cost_targets", "e13_mlp_init", "e13_mlp_forward", "e13_mlp_grad", "e13_mlp_train",
+           "e13_set_cost_model")
 
 _lib = None
 
@@ -157,6 +169,17 @@ def lib():
     L.e13_synth_table.restype = None
     L.e13_batch_digest.argtypes = [P(Leaf), ctypes.c_int64, ctypes.c_uint64, P(BatchSum)]
     L.e13_synth_run.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int64, P(BatchSum)]
+    FP = P(ctypes.c_float)
+    L.e13_get_cost_log.argtypes = [ctypes.c_void_p, P(CostRec), ctypes.c_int64, P(ctypes.c_int64)]
+    L.e13_cost_targets.argtypes = [ctypes.c_uint64, ctypes.c_uint64, FP]
+    L.e13_cost_targets.restype = None
+    L.e13_mlp_init.argtypes = [ctypes.c_uint64, FP]
+    L.e13_mlp_init.restype = None
+    L.e13_mlp_forward.argtypes = [ctypes.c_void_p, FP, FP, ctypes.c_int64, FP]
+    L.e13_mlp_grad.argtypes = [ctypes.c_void_p, FP, FP, FP, ctypes.c_int64, FP, P(ctypes.c_double)]
+    L.e13_mlp_train.argtypes = [ctypes.c_void_p, FP, FP, FP, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_float, ctypes.c_uint64, FP]
+    L.e13_set_cost_model.argtypes = [ctypes.c_void_p, FP]
     L.e13_get_trace.argtypes = [ctypes.c_void_p, P(ctypes.c_uint64), ctypes.c_uint64, P(ctypes.c_int32),
                                 P(ctypes.c_int32)]
     _lib = L
@@ -240,6 +263,31 @@ def batch_digest(leaves, k0=0):
     out = BatchSum()
     _check(lib().e13_batch_digest(arr, n, k0, ctypes.byref(out)), "e13_batch_digest")
     return out
+
+
+def _floats(a, n):
+    """A ctypes float array of n values from a sequence or a numpy array."""
+    try:
+        import numpy as np
+        arr = np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(-1))
+        if arr.size != n:
+            raise ValueError(f"expected {n} floats, got {arr.size}")
+        return (ctypes.c_float * n).from_buffer_copy(arr.tobytes())
+    except ImportError:
+        return (ctypes.c_float * n)(*a)
+
+
+def mlp_init(seed):
+    """Deterministic initial parameters (6402 floats)."""
+    p = (ctypes.c_float * MLP_PARAMS)()
+    lib().e13_mlp_init(seed, p)
+    return list(p)
+
+
+def cost_targets(leaves, steps):
+    t = (ctypes.c_float * 2)()
+    lib().e13_cost_targets(leaves, steps, t)
+    return list(t)
 
 
 def recheck(tab, nq, st0, head0, abs_ok, rel_ok, steps):
@@ -330,6 +378,38 @@ class Context:
         out = BatchSum()
         _check(lib().e13_synth_run(self._h, seed0, k0, n, ctypes.byref(out)), "e13_synth_run")
         return out
+
+    def cost_log(self):
+        """[CostRec] of the last persistent search run with cost_log=1."""
+        n = ctypes.c_int64()
+        _check(lib().e13_get_cost_log(self._h, None, 0, ctypes.byref(n)), "e13_get_cost_log")
+        out = (CostRec * max(n.value, 1))()
+        _check(lib().e13_get_cost_log(self._h, out, n.value, ctypes.byref(n)), "e13_get_cost_log")
+        return [out[i] for i in range(n.value)]
+
+    def mlp_forward(self, params, x, n):
+        y = (ctypes.c_float * (2 * n))()
+        _check(lib().e13_mlp_forward(self._h, _floats(params, MLP_PARAMS), _floats(x, n * NFEAT), n, y),
+               "e13_mlp_forward")
+        return list(y)
+
+    def mlp_grad(self, params, x, t, n):
+        g = (ctypes.c_float * MLP_PARAMS)()
+        loss = ctypes.c_double()
+        _check(lib().e13_mlp_grad(self._h, _floats(params, MLP_PARAMS), _floats(x, n * NFEAT), _floats(t, 2 * n), n,
+                                  g, ctypes.byref(loss)), "e13_mlp_grad")
+        return list(g), loss.value
+
+    def mlp_train(self, params, x, t, n, epochs=200, batch=256, lr=2e-3, seed=1):
+        p = _floats(params, MLP_PARAMS)
+        losses = (ctypes.c_float * epochs)()
+        _check(lib().e13_mlp_train(self._h, p, _floats(x, n * NFEAT), _floats(t, 2 * n), n, epochs, batch, lr, seed,
+                                   losses), "e13_mlp_train")
+        return list(p), list(losses)
+
+    def set_cost_model(self, params):
+        _check(lib().e13_set_cost_model(self._h, _floats(params, MLP_PARAMS) if params is not None else None),
+               "e13_set_cost_model")
 
     def sync(self):
         _check(lib().e13_sync(self._h), "e13_sync")

@@ -38,16 +38,20 @@ typedef struct E13Node {
   int32_t stage, st0, q0, flags;   /* last accepted stage, stage-1 state and head; flags: bit0 abs_ok,
                                       bit1 rel_ok, bits 2..4 cs[0..2] */
   int32_t ones, budget, nintro, pend; /* ones at q >= 61, edits left, symmetric states in use, cell to bind */
-  int32_t prob, depth, nbound, next_slot; /* problem index, binding depth, cells bound so far, first child
-                                             slot not yet handed out */
+  int32_t prob, depth, nbound, next_slot; /* problem index, binding depth, cells bound so far, and the
+                                             child slots not yet handed out: first | end << 16 (end 0 =
+                                             all slots; a split node carries part of the range) */
   uint64_t hb;                     /* block hash (hash-gate build only) */
   uint64_t ch[3];                  /* counter hashes at the accepted stages */
   uint8_t tab[E13_MAXCELLS];       /* word 24 */
-  uint32_t tape[E13_TW_PAD];       /* word 32 */
+  uint32_t tape[E13_TW];           /* word 32 */
+  uint32_t seed;                   /* word 75: the frontier node this node descends from (cost logging) */
 } E13Node;
 #define E13_NODE_WORDS 76
+#define E13_NODE_W_SLOTS 15 /* word of next_slot */
 #define E13_NODE_W_TAB 24
 #define E13_NODE_W_TAPE 32
+#define E13_NODE_W_SEED 75
 
 /* Survivor as the kernel records it. */
 typedef struct E13SurvRec {
@@ -160,6 +164,8 @@ typedef struct E13SearchArgs {
   const E13Node *seeds;   /* nseeds frontier nodes dealt round-robin to the waves' stacks at start */
   uint32_t nseeds;
   int32_t idle_naps, mail_every, trace_cap;
+  uint32_t nseed_cnt;     /* cost logging: seeds with counters (0: off) */
+  uint64_t *seed_cnt;     /* cost logging: per seed, leaves and steps of its subtree */
   uint64_t *trace;        /* optional: trace_cap words per wave, word 0 = events logged, then
                              (realtime << 16 | aux << 8 | event) */
 } E13SearchArgs;

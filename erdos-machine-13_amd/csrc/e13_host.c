@@ -16,55 +16,9 @@
 #include "e13.h"
 #include "e13_dev.h"
 
-/* Launch wrappers (e13_kernels.hip). */
-size_t e13k_lds_bytes(int wg_threads, int gate);
-int e13k_launch_expand(const E13ExpandArgs *a, int grid, int wg_threads, int gate, int refill, hipStream_t s);
-int e13k_launch_batch(const E13BatchArgs *a, int grid, int wg_threads, int gate, int refill, hipStream_t s);
-size_t e13k_search_lds_bytes(int wg_threads, int gate, int np);
-int e13k_launch_search(const E13SearchArgs *a, int grid, int wg_threads, int gate, hipStream_t s);
-int e13k_launch_chain_pop(E13Chain *ch, E13Counters *ctr, hipStream_t s);
-int e13k_launch_chain_push(E13Chain *ch, E13Node *stack, uint64_t stack_cap, const E13Node *out, uint32_t out_cap,
-                           const E13Counters *ctr, int grid, hipStream_t s);
-
-_Static_assert(sizeof(E13Node) % 16 == 0, "node records are copied as 16-byte vectors");
+#include "e13_ctx.h"
 
 static const long T14S[E13_NS + 1] = {0, 2, 22, 90, 268, 944, 3434, 13450, 52068, 208864}; /* e13search.c:70 */
-
-struct e13_ctx {
-  int device, ncu;
-  size_t lds_max; /* largest dynamic LDS a workgroup may use */
-  hipStream_t stream;
-  hipEvent_t ev0, ev1;
-  e13_tuning tune;
-  e13_stats stats;
-  E13Counters *d_ctr;
-  uint64_t *d_key, *d_tgt;
-  E13Problem *d_pb;
-  int32_t *d_dl;
-  E13Node *d_stack;
-  size_t stack_cap;
-  E13Node *d_out;
-  size_t out_cap;
-  E13SurvRec *d_surv;
-  size_t surv_cap;
-  void *d_tabs, *d_caps, *d_leaves;
-  size_t batch_cap;
-  /* persistent search */
-  E13Node *d_stacks;
-  size_t stacks_cap; /* nodes */
-  E13Mail *d_mail;     /* one mailbox per wave */
-  uint64_t *d_hungry;  /* hungry map */
-  size_t mail_cap;     /* waves */
-  E13Sched *d_sch;
-  E13Problem *d_pbs;
-  uint64_t *d_pcnt;
-  E13Node *d_seeds;
-  size_t seeds_cap;
-  E13Chain *d_chain; /* launch-chain cursor of the host-loop forms */
-  uint64_t *d_trace; /* wave event log of the last traced search */
-  size_t trace_words;
-  int trace_waves, trace_per_wave;
-};
 
 const char *e13_strerror(int code) {
   switch (code) {
@@ -81,19 +35,11 @@ const char *e13_strerror(int code) {
   }
 }
 
-#define HIPCHK(x)                                                                                                     \
-  do {                                                                                                                \
-    hipError_t e_ = (x);                                                                                              \
-    if (e_ != hipSuccess) {                                                                                           \
-      fprintf(stderr, "libe13: %s failed: %s (%s:%d)\n", #x, hipGetErrorString(e_), __FILE__, __LINE__);             \
-      return E13_EHIP;                                                                                                \
-    }                                                                                                                 \
-  } while (0)
 
 /* ------------------------------------------------------------------------ */
 /* Zobrist tables (e13search.c:98-103, 437-444).                             */
 /* ------------------------------------------------------------------------ */
-static uint64_t splitmix(uint64_t *s) {
+uint64_t e13i_splitmix(uint64_t *s) {
   uint64_t z = (*s += 0x9E3779B97F4A7C15ULL);
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
@@ -102,7 +48,7 @@ static uint64_t splitmix(uint64_t *s) {
 
 int e13_keys(uint64_t *key, uint64_t *tgt) {
   uint64_t seed = 20261028ULL;
-  for (int i = 0; i < 1364; i++) key[i] = splitmix(&seed);
+  for (int i = 0; i < 1364; i++) key[i] = e13i_splitmix(&seed);
   tgt[0] = 0;
   for (int n = 1; n <= E13_NS; n++) {
     uint64_t h = 0;
@@ -241,6 +187,10 @@ static void tuning_merge(e13_tuning *dst, const e13_tuning *src) {
   if (src->trace_events < 0) dst->trace_events = 0; /* negative: tracing off */
   if (src->graph_chain > 0) dst->graph_chain = src->graph_chain;
   if (src->graph_chain < 0) dst->graph_chain = 0; /* negative: host loop */
+  if (src->cost_log > 0) dst->cost_log = 1;
+  if (src->cost_log < 0) dst->cost_log = 0;
+  if (src->learned > 0) dst->learned = src->learned > 2 ? 2 : src->learned;
+  if (src->learned < 0) dst->learned = 0;
 }
 
 static int tuning_valid(const e13_tuning *t) {
@@ -337,6 +287,7 @@ void e13_close(e13_ctx *c) {
   hipFree(c->d_pcnt);
   hipFree(c->d_seeds);
   hipFree(c->d_trace);
+  e13i_free_cost(c);
   hipFree(c->d_chain);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
@@ -377,7 +328,7 @@ static int fit_wg(const e13_ctx *c, int wg, int gate_hash) {
   return wg;
 }
 
-static int ensure_buf(void **p, size_t *cap, size_t need, size_t elem) {
+int e13i_ensure_buf(void **p, size_t *cap, size_t need, size_t elem) {
   if (*cap >= need) return E13_OK;
   hipFree(*p);
   *p = NULL;
@@ -390,7 +341,7 @@ static int ensure_buf(void **p, size_t *cap, size_t need, size_t elem) {
 /* ------------------------------------------------------------------------ */
 /* Search.                                                                   */
 /* ------------------------------------------------------------------------ */
-static void fill_problem(const e13_problem *p, E13Problem *pb) {
+void e13i_fill_problem(const e13_problem *p, E13Problem *pb) {
   memset(pb, 0, sizeof *pb);
   pb->nq = p->nq;
   pb->nslot = 4 * p->nq + 1;
@@ -402,7 +353,7 @@ static void fill_problem(const e13_problem *p, E13Problem *pb) {
 }
 
 /* The blank-tape root (sim_init, e13search.c:123-129; init_root, e13gpu.py:276-283). */
-static void root_node(const e13_problem *p, E13Node *nd) {
+void e13i_root_node(const e13_problem *p, E13Node *nd) {
   memset(nd, 0, sizeof *nd);
   nd->q = E13_QOFF;
   nd->st = 0;
@@ -515,16 +466,16 @@ int e13_search(e13_ctx *c, const e13_problem *p, e13_counts *out, e13_survivor *
   const int nslot = 4 * p->nq + 1;
   const size_t batch = (size_t)tu->batch;
   int rc;
-  if ((rc = ensure_buf((void **)&c->d_stack, &c->stack_cap, (size_t)tu->stack_nodes, sizeof(E13Node))))
+  if ((rc = e13i_ensure_buf((void **)&c->d_stack, &c->stack_cap, (size_t)tu->stack_nodes, sizeof(E13Node))))
     return rc;
-  if ((rc = ensure_buf((void **)&c->d_out, &c->out_cap, batch * (size_t)nslot, sizeof(E13Node)))) return rc;
+  if ((rc = e13i_ensure_buf((void **)&c->d_out, &c->out_cap, batch * (size_t)nslot, sizeof(E13Node)))) return rc;
   size_t want_surv = surv_cap > 4096 ? (size_t)surv_cap : 4096;
-  if ((rc = ensure_buf((void **)&c->d_surv, &c->surv_cap, want_surv, sizeof(E13SurvRec)))) return rc;
+  if ((rc = e13i_ensure_buf((void **)&c->d_surv, &c->surv_cap, want_surv, sizeof(E13SurvRec)))) return rc;
 
   E13ExpandArgs a;
   memset(&a, 0, sizeof a);
   E13Problem pb;
-  fill_problem(p, &pb);
+  e13i_fill_problem(p, &pb);
   HIPCHK(hipMemcpyAsync(c->d_pb, &pb, sizeof pb, hipMemcpyHostToDevice, c->stream));
   a.pb = c->d_pb;
   a.nslot = (uint32_t)nslot;
@@ -542,7 +493,7 @@ int e13_search(e13_ctx *c, const e13_problem *p, e13_counts *out, e13_survivor *
   a.chunk = 256;
 
   E13Node root;
-  root_node(p, &root);
+  e13i_root_node(p, &root);
   HIPCHK(hipMemsetAsync(c->d_ctr, 0, sizeof(E13Counters), c->stream));
   HIPCHK(hipMemcpyAsync(c->d_stack, &root, sizeof root, hipMemcpyHostToDevice, c->stream));
   size_t top = 1, peak = 1;
@@ -654,6 +605,164 @@ static void finish_survivor(const e13_problem *p, const E13SurvRec *r, e13_survi
   s->read_mask = mask;
 }
 
+/* ---- the frontier and the cost model ---- */
+static int frontier_predict(e13_ctx *c, const e13_problem *ps, const E13Node *hn, size_t n, float *cost) {
+  float *x = (float *)malloc(n * E13_NFEAT * sizeof(float)), *y = (float *)malloc(n * 2 * sizeof(float));
+  int rc = E13_ENOMEM;
+  if (x && y) {
+    for (size_t i = 0; i < n; i++) e13i_node_features(&ps[hn[i].prob], &hn[i], x + i * E13_NFEAT);
+    rc = e13i_mlp_predict_dev(c, x, (int)n, y);
+    if (!rc)
+      for (size_t i = 0; i < n; i++) cost[i] = y[2 * i + 1]; /* predicted log steps of the subtree */
+  }
+  free(x);
+  free(y);
+  return rc;
+}
+
+static const float *g_sort_cost;
+static int by_cost_desc(const void *a, const void *b) {
+  const float ca = g_sort_cost[*(const int32_t *)a], cb = g_sort_cost[*(const int32_t *)b];
+  if (ca != cb) return ca > cb ? -1 : 1;
+  return *(const int32_t *)a - *(const int32_t *)b;
+}
+
+/* Nodes d[0..n) by predicted cost, costliest first (host copy in *hn_out). */
+static int frontier_sorted(e13_ctx *c, const e13_problem *ps, const E13Node *d, size_t n, E13Node **hn_out,
+                           int32_t **ord_out) {
+  E13Node *hn = (E13Node *)malloc(n * sizeof(E13Node));
+  float *cost = (float *)malloc(n * sizeof(float));
+  int32_t *ord = (int32_t *)malloc(n * sizeof(int32_t));
+  int rc = E13_ENOMEM;
+  if (hn && cost && ord) {
+    rc = E13_OK;
+    if (hipMemcpyAsync(hn, d, n * sizeof(E13Node), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+      rc = E13_EHIP;
+    if (!rc) rc = frontier_predict(c, ps, hn, n, cost);
+    if (!rc) {
+      for (size_t i = 0; i < n; i++) ord[i] = (int32_t)i;
+      g_sort_cost = cost;
+      qsort(ord, n, sizeof(int32_t), by_cost_desc);
+    }
+  }
+  free(cost);
+  if (rc) {
+    free(hn);
+    free(ord);
+    return rc;
+  }
+  *hn_out = hn;
+  *ord_out = ord;
+  return E13_OK;
+}
+
+/* Split step of the learned seeding: the costlier half of the frontier d[0..n)
+   stays in d (to be expanded), the rest goes to `seeds`; *ne = nodes kept. */
+static int learned_split(e13_ctx *c, const e13_problem *ps, E13Node *d, size_t n, E13Node *seeds, size_t *ne) {
+  E13Node *hn = NULL;
+  int32_t *ord = NULL;
+  int rc = frontier_sorted(c, ps, d, n, &hn, &ord);
+  if (rc) return rc;
+  const size_t keep = (n + 1) / 2;
+  E13Node *tmp = (E13Node *)malloc(n * sizeof(E13Node));
+  if (!tmp) rc = E13_ENOMEM;
+  if (!rc) {
+    for (size_t i = 0; i < n; i++) tmp[i] = hn[ord[i]];
+    if (hipMemcpyAsync(d, tmp, keep * sizeof(E13Node), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        (n > keep &&
+         hipMemcpyAsync(seeds, tmp + keep, (n - keep) * sizeof(E13Node), hipMemcpyHostToDevice, c->stream) !=
+             hipSuccess) ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+      rc = E13_EHIP;
+  }
+  free(tmp);
+  free(hn);
+  free(ord);
+  *ne = keep;
+  return rc;
+}
+
+/* Order step: the whole frontier costliest first, so the waves that are dealt
+   seeds round-robin each start with one of the largest subtrees. */
+static int learned_order(e13_ctx *c, const e13_problem *ps, E13Node *d, size_t n) {
+  E13Node *hn = NULL;
+  int32_t *ord = NULL;
+  int rc = frontier_sorted(c, ps, d, n, &hn, &ord);
+  if (rc) return rc;
+  E13Node *tmp = (E13Node *)malloc(n * sizeof(E13Node));
+  if (!tmp) rc = E13_ENOMEM;
+  if (!rc) {
+    for (size_t i = 0; i < n; i++) tmp[i] = hn[ord[i]];
+    if (hipMemcpyAsync(d, tmp, n * sizeof(E13Node), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+      rc = E13_EHIP;
+  }
+  free(tmp);
+  free(hn);
+  free(ord);
+  return rc;
+}
+
+/* A seeded random order of the frontier (tuning.learned = 2; a check that the
+   order of the frontier never changes the results). */
+static int random_order(e13_ctx *c, E13Node *d, size_t n) {
+  E13Node *hn = (E13Node *)malloc(n * sizeof(E13Node));
+  int rc = hn ? E13_OK : E13_ENOMEM;
+  if (!rc && (hipMemcpyAsync(hn, d, n * sizeof(E13Node), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+              hipStreamSynchronize(c->stream) != hipSuccess))
+    rc = E13_EHIP;
+  if (!rc) {
+    uint64_t st = 0x0D0E5EEDull + n;
+    for (size_t i = n - 1; i > 0; i--) {
+      const size_t j = (size_t)(e13i_splitmix(&st) % (uint64_t)(i + 1));
+      E13Node t = hn[i];
+      hn[i] = hn[j];
+      hn[j] = t;
+    }
+    if (hipMemcpyAsync(d, hn, n * sizeof(E13Node), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+      rc = E13_EHIP;
+  }
+  free(hn);
+  return rc;
+}
+
+/* The cost log of a finished search: each seed's features and subtree counts. */
+static int collect_cost_log(e13_ctx *c, const e13_problem *ps, size_t n) {
+  c->cost_n = 0;
+  if (!n) return E13_OK;
+  if ((int64_t)n > c->cost_cap) {
+    e13_cost_rec *r = (e13_cost_rec *)realloc(c->cost_log, n * sizeof(e13_cost_rec));
+    if (!r) return E13_ENOMEM;
+    c->cost_log = r;
+    c->cost_cap = (int64_t)n;
+  }
+  E13Node *hn = (E13Node *)malloc(n * sizeof(E13Node));
+  uint64_t *cnt = (uint64_t *)malloc(2 * n * sizeof(uint64_t));
+  int rc = E13_OK;
+  if (!hn || !cnt) rc = E13_ENOMEM;
+  if (!rc && (hipMemcpyAsync(hn, c->d_seeds, n * sizeof(E13Node), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+              hipMemcpyAsync(cnt, c->d_seed_cnt, 2 * n * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream) !=
+                  hipSuccess ||
+              hipStreamSynchronize(c->stream) != hipSuccess))
+    rc = E13_EHIP;
+  if (!rc) {
+    for (size_t i = 0; i < n; i++) {
+      e13_cost_rec *r = &c->cost_log[i];
+      e13i_node_features(&ps[hn[i].prob], &hn[i], r->x);
+      r->leaves = cnt[2 * i];
+      r->steps = cnt[2 * i + 1];
+      r->prob = hn[i].prob;
+      r->depth = hn[i].depth;
+    }
+    c->cost_n = (int64_t)n;
+  }
+  free(hn);
+  free(cnt);
+  return rc;
+}
+
 /* The persistent form: every problem in one launch (see k_search). */
 int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, e13_survivor *surv,
                     uint64_t surv_cap, uint64_t *n_surv) {
@@ -672,7 +781,7 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
   if (total_waves > 64 * 64) return E13_EINVAL; /* the hungry map is read as at most 64 words */
   const int hungry_words = (total_waves + 63) / 64;
   int rc;
-  if ((rc = ensure_buf((void **)&c->d_stacks, &c->stacks_cap, need_stacks, sizeof(E13Node)))) return rc;
+  if ((rc = e13i_ensure_buf((void **)&c->d_stacks, &c->stacks_cap, need_stacks, sizeof(E13Node)))) return rc;
   if (c->mail_cap < (size_t)total_waves) {
     hipFree(c->d_mail);
     hipFree(c->d_hungry);
@@ -689,18 +798,19 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
   if (!c->d_pcnt && hipMalloc((void **)&c->d_pcnt, E13_MAXPROB * E13_PCNT * sizeof(uint64_t)) != hipSuccess)
     return E13_ENOMEM;
   size_t want_surv = surv_cap > 4096 ? (size_t)surv_cap : 4096;
-  if ((rc = ensure_buf((void **)&c->d_surv, &c->surv_cap, want_surv, sizeof(E13SurvRec)))) return rc;
+  if ((rc = e13i_ensure_buf((void **)&c->d_surv, &c->surv_cap, want_surv, sizeof(E13SurvRec)))) return rc;
 
   /* Breadth-first seeding: a few k_expand launches per problem grow a frontier
      of about four nodes per wave; leaves met on the way are counted here.  The
      persistent launch then starts with the frontier dealt to the waves. */
   const size_t seed_target = (size_t)tu->seed_per_wave * (size_t)total_waves;
-  if ((rc = ensure_buf((void **)&c->d_seeds, &c->seeds_cap, 64 * (size_t)total_waves + (size_t)np, sizeof(E13Node))))
+  const int learned = tu->learned == 1 && c->have_model;
+  if ((rc = e13i_ensure_buf((void **)&c->d_seeds, &c->seeds_cap, 64 * (size_t)total_waves + (size_t)np, sizeof(E13Node))))
     return rc;
-  if ((rc = ensure_buf((void **)&c->d_stack, &c->stack_cap, c->seeds_cap, sizeof(E13Node)))) return rc;
+  if ((rc = e13i_ensure_buf((void **)&c->d_stack, &c->stack_cap, c->seeds_cap, sizeof(E13Node)))) return rc;
   E13Problem pbs[E13_MAXPROB];
   for (int k = 0; k < np; k++) {
-    fill_problem(&ps[k], &pbs[k]);
+    e13i_fill_problem(&ps[k], &pbs[k]);
     memset(&out[k], 0, sizeof out[k]);
   }
   HIPCHK(hipMemcpyAsync(c->d_pbs, pbs, (size_t)np * sizeof(E13Problem), hipMemcpyHostToDevice, c->stream));
@@ -711,7 +821,7 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
     const int nslot = 4 * ps[k].nq + 1;
     const size_t target_k = (seed_target + (size_t)np - 1) / (size_t)np;
     E13Node root;
-    root_node(&ps[k], &root);
+    e13i_root_node(&ps[k], &root);
     root.prob = k;
     HIPCHK(hipMemcpyAsync(c->d_stack, &root, sizeof root, hipMemcpyHostToDevice, c->stream));
     size_t top = 1;
@@ -729,10 +839,16 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
     ea.cap = tu->cap;
     ea.refill_tries = tu->refill_tries;
     ea.chunk = 64;
-    for (int level = 0; level < 8 && top > 0 && top < target_k && seed_target > 0; level++) {
+    for (int level = 0; level < 12 && top > 0 && top < target_k && seed_target > 0; level++) {
+      if (learned && top > 1) { /* split by predicted cost: expand the costlier half, keep the rest as seeds */
+        size_t ne = top;
+        if ((rc = learned_split(c, ps, c->d_stack, top, c->d_seeds + nseeds, &ne))) return rc;
+        nseeds += top - ne;
+        top = ne;
+      }
       const size_t room = c->seeds_cap - nseeds;
       if (top * (size_t)nslot > c->stack_cap || top * (size_t)nslot > room) break;
-      if ((rc = ensure_buf((void **)&c->d_out, &c->out_cap, top * (size_t)nslot, sizeof(E13Node)))) return rc;
+      if ((rc = e13i_ensure_buf((void **)&c->d_out, &c->out_cap, top * (size_t)nslot, sizeof(E13Node)))) return rc;
       ea.in = c->d_stack;
       ea.nin = (uint32_t)top;
       ea.out = c->d_out;
@@ -770,6 +886,16 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
     prev[2] = h1.survivors;
     for (int j = 0; j <= E13_NS; j++) prev[3 + j] = h1.hist[j];
   }
+  if (learned && nseeds > 1 && (rc = learned_order(c, ps, c->d_seeds, nseeds))) return rc;
+  if (tu->learned == 2 && nseeds > 1 && (rc = random_order(c, c->d_seeds, nseeds))) return rc;
+  const int logging = tu->cost_log == 1;
+  if (logging) { /* number the frontier and count each subtree */
+    if ((rc = e13i_ensure_buf((void **)&c->d_seed_cnt, &c->seed_cnt_cap, nseeds > 0 ? 2 * nseeds : 2,
+                              sizeof(uint64_t))))
+      return rc;
+    HIPCHK(hipMemsetAsync(c->d_seed_cnt, 0, 2 * nseeds * sizeof(uint64_t), c->stream));
+    if (e13k_launch_seed_ids(c->d_seeds, (uint32_t)nseeds, c->stream)) return E13_EHIP;
+  }
   E13Sched sch;
   memset(&sch, 0, sizeof sch);
   sch.busy = nseeds < (size_t)total_waves ? (int32_t)nseeds : total_waves; /* waves dealt a seed */
@@ -806,10 +932,14 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
   a.idle_limit = 3000000000u; /* idle watchdog: 30 s of the 100 MHz constant clock */
   a.seeds = c->d_seeds;
   a.nseeds = (uint32_t)nseeds;
+  if (logging) {
+    a.seed_cnt = c->d_seed_cnt;
+    a.nseed_cnt = (uint32_t)nseeds;
+  }
   c->trace_waves = c->trace_per_wave = 0;
   if (tu->trace_events > 0) {
     const size_t per = (size_t)tu->trace_events + 1, words = per * (size_t)total_waves;
-    if ((rc = ensure_buf((void **)&c->d_trace, &c->trace_words, words, sizeof(uint64_t)))) return rc;
+    if ((rc = e13i_ensure_buf((void **)&c->d_trace, &c->trace_words, words, sizeof(uint64_t)))) return rc;
     HIPCHK(hipMemsetAsync(c->d_trace, 0, words * sizeof(uint64_t), c->stream));
     a.trace = c->d_trace;
     a.trace_cap = (int32_t)per;
@@ -852,6 +982,7 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
                              : "dynamic LDS does not start at LDS address 0");
     return E13_ESTACK;
   }
+  if (logging && (rc = collect_cost_log(c, ps, nseeds))) return rc;
   for (int k = 0; k < np; k++) {
     const uint64_t *q = pcnt + k * E13_PCNT;
     out[k].leaves += q[0];
@@ -954,7 +1085,7 @@ int e13_simulate_batch(e13_ctx *c, const int8_t *tabs, int ncells, int n, const 
   }
   int rc;
   size_t cap_before = c->batch_cap;
-  if ((rc = ensure_buf(&c->d_tabs, &c->batch_cap, (size_t)n, E13_MAXCELLS))) {
+  if ((rc = e13i_ensure_buf(&c->d_tabs, &c->batch_cap, (size_t)n, E13_MAXCELLS))) {
     free(h);
     return rc;
   }
@@ -1093,16 +1224,16 @@ int e13_format_hash(const e13_problem *p, const e13_survivor *s, char *buf, int 
 
 void e13_synth_table(uint64_t seed0, uint64_t k, int8_t *tab28, int32_t *cap) {
   uint64_t st = seed0 + k;
-  for (int i = 0; i < 28; i++) tab28[i] = (int8_t)(splitmix(&st) % 56);
-  tab28[splitmix(&st) % 28] = E13_HALT;
-  double u = (double)(splitmix(&st) >> 11) * (1.0 / 9007199254740992.0);
+  for (int i = 0; i < 28; i++) tab28[i] = (int8_t)(e13i_splitmix(&st) % 56);
+  tab28[e13i_splitmix(&st) % 28] = E13_HALT;
+  double u = (double)(e13i_splitmix(&st) >> 11) * (1.0 / 9007199254740992.0);
   *cap = (int32_t)floor(10.0 * pow(2e4, u));
 }
 
 /* ------------------------------------------------------------------------ */
 /* Synthetic stress batch.                                                   */
 /* ------------------------------------------------------------------------ */
-static uint64_t rec_mix(uint64_t x) { return splitmix(&x); }
+static uint64_t rec_mix(uint64_t x) { return e13i_splitmix(&x); }
 
 static uint64_t record_hash(uint64_t k, const e13_leaf *l) {
   uint64_t h = rec_mix(k);
@@ -1150,7 +1281,7 @@ int e13_synth_run(e13_ctx *c, uint64_t seed0, uint64_t k0, int64_t n, e13_batch_
     for (int i = 0; i < E13_MAXCELLS; i++) row[i] = i < 28 && t[i] >= 0 ? (uint8_t)t[i] : E13_C_HALT;
   }
   size_t cap_before = c->batch_cap;
-  if ((rc = ensure_buf(&c->d_tabs, &c->batch_cap, (size_t)n, E13_MAXCELLS))) goto done;
+  if ((rc = e13i_ensure_buf(&c->d_tabs, &c->batch_cap, (size_t)n, E13_MAXCELLS))) goto done;
   if (c->batch_cap != cap_before || !c->d_caps || !c->d_leaves) {
     hipFree(c->d_caps);
     hipFree(c->d_leaves);

@@ -403,6 +403,7 @@ __global__ void __launch_bounds__(256) k_expand(E13ExpandArgs a) {
   Lane L;
   L.dlcap = -1;
   int active = 0, t0 = 0, budget = 0, nintro = 0, front = 0, depth = 0, nbound = 0;
+  uint32_t seed = 0; // frontier node the lane's machine descends from
   uint64_t my_steps = 0;
   uint32_t my_leaves = 0, my_surv = 0;
   uint64_t w_slots = 0, w_rounds = 0;
@@ -490,6 +491,7 @@ __global__ void __launch_bounds__(256) k_expand(E13ExpandArgs a) {
             budget = nb;
             nintro = nin2;
             front = nd->prob;
+            seed = nd->seed;
             depth = nd->depth + (pend >= 0);
             nbound = nd->nbound + (pend >= 0);
             build_rows(c, nd->tab, a.nq, pend, val, budget, a.pb->base);
@@ -583,7 +585,7 @@ __global__ void __launch_bounds__(256) k_expand(E13ExpandArgs a) {
             v4.x = lds_ld(c.sb, c.tbl + (4 * w4 + 0) * 256);
             v4.y = lds_ld(c.sb, c.tbl + (4 * w4 + 1) * 256);
             v4.z = lds_ld(c.sb, c.tbl + (4 * w4 + 2) * 256);
-            v4.w = (4 * w4 + 3 < E13_TW) ? lds_ld(c.sb, c.tbl + (4 * w4 + 3) * 256) : 0u;
+            v4.w = (4 * w4 + 3 < E13_TW) ? lds_ld(c.sb, c.tbl + (4 * w4 + 3) * 256) : seed;
             dst[w4] = v4;
           }
         } else {
@@ -724,6 +726,12 @@ __global__ void __launch_bounds__(256) k_batch(E13BatchArgs a) {
   }
 }
 
+// Number the frontier nodes of a persistent search (cost logging): seed i = i.
+__global__ void __launch_bounds__(256) k_seed_ids(E13Node *s, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) s[i].seed = i;
+}
+
 // ---------------------------------------------------------------------------
 // Launch chains of the host-loop forms.  One iteration is k_chain_pop ->
 // k_expand (chain mode) -> k_chain_push; the stack cursor stays on the device,
@@ -784,16 +792,14 @@ __device__ __forceinline__ void wave_mem_sync() { // this wave's global stores v
   __builtin_amdgcn_wave_barrier();
 }
 
+// A node's child slots still to hand out, as stored in E13Node.next_slot:
+// first | end << 16 (end 0 = all of the problem's slots).
+__device__ __forceinline__ int slot_range(int first, int end) { return first | (end << 16); }
+
 __device__ __forceinline__ E13Node *stk_node(const E13SearchArgs &a, int wid, long long idx) {
   return a.stacks + (size_t)wid * (size_t)a.stk_cap + (size_t)(idx % a.stk_cap);
 }
 
-__device__ __forceinline__ uint32_t ld_agent(const uint32_t *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_agent(uint32_t *p, uint32_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 __device__ __forceinline__ uint64_t ld_agent64(const uint64_t *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -855,7 +861,7 @@ __device__ void stage_node(uint32_t *stg, const uint32_t *nd, const E13Problem *
 // This lane's machine as a node record in LDS (the staging area): the same
 // words write_node stores, so the newest child needs no reload from HBM.
 __device__ __forceinline__ void stage_own(const Ctx &c, uint32_t *stg, const Lane &L, int budget, int nintro, int pend,
-                                          int prob, int depth, int nbound, int nq) {
+                                          int prob, int depth, int nbound, int nq, uint32_t seed) {
   stg[0] = (uint32_t)L.q;
   stg[1] = (uint32_t)((L.row - c.rbl) >> 8);
   stg[2] = (uint32_t)L.t;
@@ -883,12 +889,12 @@ __device__ __forceinline__ void stage_own(const Ctx &c, uint32_t *stg, const Lan
 #pragma unroll
   for (int i = 0; i < E13_MAXCELLS / 4; ++i) stg[E13_NODE_W_TAB + i] = codes_word(c, nq, i);
   for (int w = 0; w < E13_TW_PAD; ++w)
-    stg[E13_NODE_W_TAPE + w] = w < E13_TW ? lds_ld(c.sb, c.tbl + w * 256) : 0u;
+    stg[E13_NODE_W_TAPE + w] = w < E13_TW ? lds_ld(c.sb, c.tbl + w * 256) : seed;
 }
 
 // Copy this lane's machine into node record `o` (plain stores).
 __device__ __forceinline__ void write_node(const Ctx &c, E13Node *o, const Lane &L, int budget, int nintro, int pend,
-                                           int prob, int depth, int nbound, int nq) {
+                                           int prob, int depth, int nbound, int nq, uint32_t seed) {
   o->q = L.q;
   o->st = (L.row - c.rbl) >> 8;
   o->t = L.t;
@@ -918,7 +924,7 @@ __device__ __forceinline__ void write_node(const Ctx &c, E13Node *o, const Lane 
     v4.x = lds_ld(c.sb, c.tbl + (4 * w4 + 0) * 256);
     v4.y = lds_ld(c.sb, c.tbl + (4 * w4 + 1) * 256);
     v4.z = lds_ld(c.sb, c.tbl + (4 * w4 + 2) * 256);
-    v4.w = (4 * w4 + 3 < E13_TW) ? lds_ld(c.sb, c.tbl + (4 * w4 + 3) * 256) : 0u;
+    v4.w = (4 * w4 + 3 < E13_TW) ? lds_ld(c.sb, c.tbl + (4 * w4 + 3) * 256) : seed;
     dst[w4] = v4;
   }
 }
@@ -953,8 +959,10 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
   L.dlcap = -1;
   L.tcap = 0x7FFFFFFF;
   int active = 0, t0 = 0, budget = 0, nintro = 0, prob = 0, depth = 0, nbound = 0, nq = 0;
+  uint32_t seed = 0; // frontier node the lane's machine descends from
   long long top = 0, bot = 0, staged = -1;
-  int cur_slot = 0, busy = 0, requested = 0, mail_in = 1, idle_polls = 0;
+  int cur_slot = 0, cur_end = 0; // the staged node's child slots still to hand out: [cur_slot, cur_end)
+  int busy = 0, requested = 0, mail_in = 1, idle_polls = 0;
   uint64_t last_tag = 0; // tag of the last delivery taken from this wave's mailbox
   uint64_t idle_since = 0; // constant-clock time this wave went idle (0: not idle)
   // the frontier from the breadth-first expansion: seed i goes to wave i mod total_waves
@@ -987,17 +995,20 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
       if (need == 0 || top == bot) break;
       const long long tidx = top - 1;
       if (staged != tidx) {
-        if (staged >= bot && staged < top && lane == 0) stk_node(a, wid, staged)->next_slot = cur_slot;
+        if (staged >= bot && staged < top && lane == 0)
+          stk_node(a, wid, staged)->next_slot = slot_range(cur_slot, cur_end);
         wave_mem_sync();
         stage_node(stg, (const uint32_t *)stk_node(a, wid, tidx), a.pbs, lane);
         staged = tidx;
-        cur_slot = uni((int)stg[15]);
+        const uint32_t sw = (uint32_t)uni((int)stg[15]);
+        cur_slot = (int)(sw & 0xFFFFu);
+        cur_end = (int)(sw >> 16);
+        if (cur_end == 0) cur_end = a.pbs[uni((int)stg[12])].nslot;
       }
       const int sprob = uni((int)stg[12]);
       const E13Problem *pb = a.pbs + sprob;
-      const int nslot = pb->nslot;
       const int cnt = __popcll(need);
-      const int give = min(cnt, nslot - cur_slot);
+      const int give = min(cnt, cur_end - cur_slot);
       if (!active) {
         const int r = (int)rank_in(need);
         if (r < give) {
@@ -1013,6 +1024,8 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
               else if (orig == -1) { // keeping a halt: one leaf at the node's stage, 0 steps
                 atomicAdd(&pc[sprob * E13_PCNT + 0], 1ull);
                 atomicAdd(&pc[sprob * E13_PCNT + 3 + (int)stg[4]], 1ull);
+                const uint32_t sd = stg[E13_NODE_W_SEED];
+                if (sd < a.nseed_cnt) atomicAdd((unsigned long long *)&a.seed_cnt[2 * sd], 1ull);
                 ok = 0;
               } else val = orig;
             } else {
@@ -1031,6 +1044,7 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
           }
           if (ok) {
             prob = sprob;
+            seed = stg[E13_NODE_W_SEED];
             nq = pb->nq;
             c.dl = pb->dl;
             L.q = (int)stg[0];
@@ -1067,7 +1081,7 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
         }
       }
       cur_slot += give;
-      if (cur_slot >= nslot) { // every child slot handed out: pop
+      if (cur_slot >= cur_end) { // every child slot handed out: pop
         top -= 1;
         staged = -1;
       }
@@ -1167,7 +1181,7 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
     idle_since = 0;
     w_rounds++;
     w_slots += (uint64_t)__popcll(act);
-    if ((w_rounds & 7u) == 0u) TRACE_EV2(TR_ROUND, __popcll(act) | (top - bot > 2 ? 0x80 : 0));
+    if ((w_rounds & 63u) == 0u) TRACE_EV2(TR_ROUND, __popcll(act) | (top - bot > 2 ? 0x80 : 0));
     // shared state, read every few rounds (staggered over the waves): the abort
     // flag and the hungry map; the loads complete while the lanes step
     const bool refresh = --refresh_in <= 0;
@@ -1175,7 +1189,8 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
     uint64_t hw = 0; // hungry-map word `lane` (maps of up to 4096 waves)
     if (refresh) {
       if (lane == 0) s_err = ld_agent_i(&a.sch->error);
-      if (lane < a.hungry_words && top - bot >= 2) hw = ld_agent64(&a.hungry[lane]);
+      const bool can_give = top - bot >= 2 || (top - bot == 1 && staged == top - 1 && cur_end - cur_slot >= 2);
+      if (lane < a.hungry_words && can_give) hw = ld_agent64(&a.hungry[lane]);
     }
     tn = stamp();
     cyc3 += tn - ts;
@@ -1215,6 +1230,10 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
         }
       }
       atomicAdd(&pc[prob * E13_PCNT + 1], d);
+      if (seed < a.nseed_cnt) { // cost logging: this subtree's leaves and steps
+        if (ev == EV_DEAD || ev == EV_SURV) atomicAdd((unsigned long long *)&a.seed_cnt[2 * seed], 1ull);
+        atomicAdd((unsigned long long *)&a.seed_cnt[2 * seed + 1], d);
+      }
     }
     // ---- live children onto the private stack ----
     const bool emit = fin && (ev == EV_STOP || ev == EV_CAP);
@@ -1226,17 +1245,20 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
         broadcast_done(a, lane);
         break;
       }
-      if (staged >= bot && staged < top && lane == 0) stk_node(a, wid, staged)->next_slot = cur_slot;
+      if (staged >= bot && staged < top && lane == 0)
+        stk_node(a, wid, staged)->next_slot = slot_range(cur_slot, cur_end);
       if (emit) {
         const int r = (int)rank_in(em);
         write_node(c, stk_node(a, wid, top + r), L, budget, nintro, ev == EV_STOP ? evcell : -1, prob, depth, nbound,
-                   nq);
-        if (r == n - 1) stage_own(c, stg, L, budget, nintro, ev == EV_STOP ? evcell : -1, prob, depth, nbound, nq);
+                   nq, seed);
+        if (r == n - 1)
+          stage_own(c, stg, L, budget, nintro, ev == EV_STOP ? evcell : -1, prob, depth, nbound, nq, seed);
       }
       stage_rows(stg, a.pbs, lane);
       top += n;
       staged = top - 1;
       cur_slot = 0;
+      cur_end = a.pbs[uni((int)stg[12])].nslot;
     }
     if (fin) active = 0;
     if (requested && top > bot) { // work of its own again: withdraw the request unless a donor took it
@@ -1250,10 +1272,16 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
     ts = tn;
     if (refresh) {
       if (uni(s_err)) break;
-      // ---- deliver the oldest nodes to hungry waves ----
+      // ---- deliver work to hungry waves: the oldest node, or with a single
+      // node, the upper part of the staged node's child slots ----
       uint64_t nz = ballot(hw != 0);
       int given = 0;
-      while (nz != 0 && given < a.donate_every && top - bot >= 2) {
+      const int idle_now = __popcll(ballot(!active));
+      for (;;) {
+        const bool whole = top - bot >= 2;
+        const int rem = cur_end - cur_slot, keep = max(idle_now, rem / 2);
+        const bool split = !whole && top - bot == 1 && staged == top - 1 && rem >= 2 && keep < rem;
+        if (nz == 0 || given >= a.donate_every || !(whole || split)) break;
         // a word and a bit chosen by this wave's id, so donors spread over the hungry waves
         const int rot = (wid * 7) & 63;
         const uint64_t nzr = (nz >> rot) | (rot ? (nz << (64 - rot)) : 0ull);
@@ -1273,21 +1301,27 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
         nz = ballot(hw != 0);
         if (!won) continue;
         const int dst_w = wsel * 64 + bsel;
-        if (staged == bot) { // a delivery may sit above it: its slot cursor travels with it
-          if (lane == 0) stk_node(a, wid, staged)->next_slot = cur_slot;
+        uint32_t slots_word = 0xFFFFFFFFu; // the donated copy's slot range (split only)
+        if (whole && staged == bot) { // a delivery may sit above it: its slot cursor travels with it
+          if (lane == 0) stk_node(a, wid, staged)->next_slot = slot_range(cur_slot, cur_end);
           staged = -1;
         }
+        if (!whole) { // keep [cur_slot, cur_slot + keep), give the rest
+          slots_word = slot_range(cur_slot + keep, cur_end);
+          cur_end = cur_slot + keep;
+        }
         wave_mem_sync();
-        const uint32_t *src = (const uint32_t *)stk_node(a, wid, bot);
+        const uint32_t *src = (const uint32_t *)stk_node(a, wid, whole ? bot : staged);
         uint64_t *dst = a.mail[dst_w].w;
         const uint64_t tag = ((uint64_t)(wid + 1) << 19) | ((w_gives + 1) & 0x7FFFFull); // unique, never 0
-        const uint32_t v0 = src[lane];
+        uint32_t v0 = src[lane];
         const uint32_t v1 = lane < E13_NODE_WORDS - 64 ? src[64 + lane] : 0u;
+        if (!whole && lane == E13_NODE_W_SLOTS) v0 = slots_word;
         st_agent64(dst + lane, (tag << 32) | v0);
         if (lane < E13_NODE_WORDS - 64) st_agent64(dst + 64 + lane, (tag << 32) | v1);
         vm_drain(); // the elements are issued and complete before the flag
         if (lane == 0) st_agent64(&a.mail[dst_w].flag, tag);
-        bot += 1;
+        if (whole) bot += 1;
         ++given;
         ++w_gives;
         TRACE_EV(TR_GIVE);
@@ -1341,6 +1375,12 @@ int e13k_launch_expand(const E13ExpandArgs *a, int grid, int wg_threads, int gat
     if (refill) hipLaunchKernelGGL((k_expand<GATE_COUNT, 1>), dim3(grid), dim3(wg_threads), lds, s, *a);
     else hipLaunchKernelGGL((k_expand<GATE_COUNT, 0>), dim3(grid), dim3(wg_threads), lds, s, *a);
   }
+  return (int)hipGetLastError();
+}
+
+int e13k_launch_seed_ids(E13Node *seeds, uint32_t n, hipStream_t s) {
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_seed_ids, dim3((n + 255) / 256), dim3(256), 0, s, seeds, n);
   return (int)hipGetLastError();
 }
 

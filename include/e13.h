@@ -96,7 +96,10 @@ typedef struct {
   /* host-loop forms (1, 2) */
   int32_t graph_chain;    /* > 0: iterations (pop, expand, push) per captured hipGraph; 0 = one launch
                              and one host round trip per iteration */
-  int32_t pad_g;
+  /* persistent form: the frontier and the cost model (e13_set_cost_model) */
+  int32_t cost_log;       /* 1: log each frontier node's features and subtree leaves/steps (e13_get_cost_log) */
+  int32_t learned;        /* 1: split and order the frontier by the installed cost model; 2: a seeded
+                             random order (a check); 0 or negative: the static split */
 } e13_tuning;
 
 typedef struct {
@@ -127,7 +130,7 @@ int e13_get_stats(e13_ctx *ctx, e13_stats *s);
    per wave `per_wave` words, word 0 = events logged, then
    (t << 16 | aux << 8 | event) with t in ticks of the GPU's constant 100 MHz
    clock and event 1 seeded, 2 took a pool node while idle, 3 counted busy
-   again, 4 idle, 5 exit, 6 gave a node, 7 every 8th round (aux = running lanes,
+   again, 4 idle, 5 exit, 6 gave a node, 7 every 64th round (aux = running lanes,
    +128 when more than two nodes wait on the wave's stack).
    No reference counterpart (a diagnostic of this implementation's scheduler). */
 int e13_get_trace(e13_ctx *ctx, uint64_t *buf, uint64_t cap_words, int32_t *waves, int32_t *per_wave);
@@ -202,6 +205,41 @@ int e13_batch_digest(const e13_leaf *leaves, int64_t n, uint64_t k0, e13_batch_s
    digest of e13_batch_digest.  No reference counterpart: the workload is
    defined by this build. */
 int e13_synth_run(e13_ctx *ctx, uint64_t seed0, uint64_t k0, int64_t n, e13_batch_sum *out);
+
+/* ---- Subtree cost model (no reference counterpart; the nearest is the Knuth
+   estimator probe(), e13search.c:256-313, which estimates the same two
+   quantities by random descents).  A 32 -> 64 -> 64 -> 2 perceptron (ReLU,
+   fp32) maps a frontier node's features to (log1p(leaves)/16, log1p(steps)/24)
+   of its subtree.  It only orders and splits the frontier of the persistent
+   search: every subtree is still searched, so counts and survivors are the
+   same with and without it. ---- */
+#define E13_NFEAT 32
+#define E13_MLP_PARAMS 6402 /* W1[64][32] b1[64] W2[64][64] b2[64] W3[2][64] b3[2] */
+
+typedef struct {
+  float x[E13_NFEAT];     /* features of the frontier node */
+  uint64_t leaves, steps; /* of its subtree */
+  int32_t prob, depth;
+} e13_cost_rec;
+
+/* Frontier log of the last persistent search run with tuning.cost_log = 1. */
+int e13_get_cost_log(e13_ctx *ctx, e13_cost_rec *out, int64_t cap, int64_t *n);
+/* Regression targets of a subtree: t[0] = log1p(leaves)/16, t[1] = log1p(steps)/24. */
+void e13_cost_targets(uint64_t leaves, uint64_t steps, float *t2);
+/* Deterministic He-uniform initialisation from splitmix64(seed). */
+void e13_mlp_init(uint64_t seed, float *params);
+/* y[n][2] = f(x[n][32]) on the GPU. */
+int e13_mlp_forward(e13_ctx *ctx, const float *params, const float *x, int64_t n, float *y);
+/* Gradient of L = (1/n) sum (f(x) - t)^2 over all n samples (GPU), and L. */
+int e13_mlp_grad(e13_ctx *ctx, const float *params, const float *x, const float *t, int64_t n, float *grad,
+                 double *loss);
+/* Adam training on the GPU: `epochs` passes over a seeded shuffle in batches
+   of `batch`; params in/out; loss_per_epoch[epochs] (may be NULL) holds the
+   mean loss over each epoch's batches. */
+int e13_mlp_train(e13_ctx *ctx, float *params, const float *x, const float *t, int64_t n, int epochs, int batch,
+                  float lr, uint64_t seed, float *loss_per_epoch);
+/* Install (or with NULL remove) the model used when tuning.learned = 1. */
+int e13_set_cost_model(e13_ctx *ctx, const float *params);
 
 #ifdef __cplusplus
 }

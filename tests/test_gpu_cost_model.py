@@ -1,0 +1,135 @@
+"""The learned subtree cost model on the GPU: the hand-written fp32 kernels
+against the fp64 oracle, training on the search's own cost logs, what the
+trained network gains over the static split, and result invariance under
+static, learned and random frontier orders."""
+import heapq
+
+import numpy as np
+import pytest
+
+from conftest import REF_DUMP, REF_RESULTS, XS, parse_summaries, read_machine, read_text, survivor_lines
+from test_cost_model import oracle_mlp, ref_grad, dptr
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def base(e13):
+    return e13.parse_machine(read_machine())
+
+
+def random_problem(seed, n):
+    rng = np.random.default_rng(seed)
+    return (rng.normal(0, 1, (n, 32)).astype(np.float32), rng.normal(0, 0.5, (n, 2)).astype(np.float32))
+
+
+def test_gradient_kernels_match_oracle(e13, gpu_ctx):
+    L = oracle_mlp()
+    for n in (1, 33, 1000):
+        say(f"mlp_grad: n={n}")
+        x, t = random_problem(n, n)
+        P = np.array(e13.mlp_init(3), dtype=np.float32)
+        P[2048:2112] = 0.05  # some positive biases so every unit is active somewhere
+        g, loss = gpu_ctx.mlp_grad(P, x, t, n)
+        lref, gref = ref_grad(L, P.astype(np.float64), x.astype(np.float64), t.astype(np.float64))
+        g = np.array(g)
+        assert abs(loss - lref) <= 1e-5 * max(lref, 1e-9), (n, loss, lref)
+        # fp32 kernels against fp64: error relative to the gradient's scale
+        assert np.max(np.abs(g - gref)) <= 1e-4 * np.max(np.abs(gref)), (n, np.max(np.abs(g - gref)))
+
+
+def test_forward_kernel_matches_oracle(e13, gpu_ctx):
+    L = oracle_mlp()
+    x, _ = random_problem(5, 777)
+    P = np.array(e13.mlp_init(9), dtype=np.float32)
+    say("mlp_forward: n=777")
+    y = np.array(gpu_ctx.mlp_forward(P, x, 777)).reshape(777, 2)
+    yref = np.zeros((777, 2))
+    P64, x64 = P.astype(np.float64), x.astype(np.float64)
+    L.eo_mlp_forward(dptr(P64), dptr(x64), 777, dptr(yref))
+    assert np.max(np.abs(y - yref)) <= 1e-5 * (1 + np.max(np.abs(yref)))
+
+
+def cost_log(e13, ctx, base, fmax, want):
+    """A frontier cost log of the local-edit search (counts checked)."""
+    say(f"search: delete FMAX={fmax} X=0..13 with cost logging")
+    ctx.set_tuning(seed_per_wave=4, cost_log=1)
+    try:
+        cs, sv, ns = ctx.search_many([e13.setup(base, X, fmax) for X in XS], surv_cap=4096)
+        log = ctx.cost_log()
+    finally:
+        ctx.set_tuning(seed_per_wave=-1, cost_log=-1)
+    for X, c in zip(XS, cs):
+        assert (c.leaves, c.steps) == (want[X]["leaves"], want[X]["steps"]), X
+    x = np.array([list(r.x) for r in log], dtype=np.float32)
+    t = np.array([e13.cost_targets(r.leaves, r.steps) for r in log], dtype=np.float32)
+    steps = np.array([r.steps for r in log], dtype=np.float64)
+    return x, t, steps
+
+
+def makespan(costs, order, workers):
+    """List scheduling: each job in `order` goes to the least-loaded worker."""
+    h = [0.0] * workers
+    for j in order:
+        heapq.heappush(h, heapq.heappop(h) + costs[j])
+    return max(h)
+
+
+def ranks(v):
+    r = np.empty(len(v))
+    r[np.argsort(v, kind="stable")] = np.arange(len(v))
+    return r
+
+
+def test_trained_model_orders_subtrees(e13, gpu_ctx, base):
+    """Train on the FMAX=2 and FMAX=3 logs (GPU kernels), predict the held-out
+    FMAX=4 frontier: the prediction ranks subtrees, and scheduling the frontier
+    costliest-first by it shortens the makespan against the static order."""
+    f2 = parse_summaries(read_text(REF_DUMP, "delete_f2.txt"))
+    f3 = parse_summaries(read_text(REF_DUMP, "delete_f3.txt"))
+    f4 = parse_summaries(read_text(REF_DUMP, "delete_f4.txt"))
+    x2, t2, _ = cost_log(e13, gpu_ctx, base, 2, f2)
+    x3, t3, _ = cost_log(e13, gpu_ctx, base, 3, f3)
+    x4, t4, s4 = cost_log(e13, gpu_ctx, base, 4, f4)
+    x, t = np.concatenate([x2, x3]), np.concatenate([t2, t3])
+    say(f"mlp_train: {len(x)} samples")
+    P0 = e13.mlp_init(1)
+    _, l0 = gpu_ctx.mlp_grad(P0, x, t, len(x))
+    P, losses = gpu_ctx.mlp_train(P0, x, t, len(x), epochs=150, batch=256, lr=2e-3, seed=1)
+    _, l1 = gpu_ctx.mlp_grad(P, x, t, len(x))
+    say(f"loss {l0:.4f} -> {l1:.4f}")
+    assert l1 < 0.25 * l0 and losses[-1] < losses[0]
+    y = np.array(gpu_ctx.mlp_forward(P, x4, len(x4))).reshape(-1, 2)
+    rho = np.corrcoef(ranks(y[:, 1]), ranks(t4[:, 1]))[0, 1]
+    static = makespan(s4, range(len(s4)), 64)
+    learned = makespan(s4, np.argsort(-y[:, 1], kind="stable"), 64)
+    oracle = makespan(s4, np.argsort(-s4, kind="stable"), 64)
+    say(f"held-out FMAX=4: spearman {rho:.3f}, makespan static/learned {static / learned:.3f} "
+        f"(true-cost order {static / oracle:.3f})")
+    assert rho > 0.3
+    assert learned < static
+
+
+@pytest.mark.parametrize("learned", [1, 2])
+def test_frontier_order_never_changes_results(e13, gpu_ctx, base, learned):
+    """Learned split/order (1) and a random order (2) give the reference's
+    counts and survivors, as the static split does."""
+    say(f"search: delete FMAX=3 X=0..13 learned={learned}")
+    gpu_ctx.set_cost_model(e13.mlp_init(4))  # any model: the order may be arbitrary
+    gpu_ctx.set_tuning(seed_per_wave=4, learned=learned)
+    try:
+        cs, sv, ns = gpu_ctx.search_many([e13.setup(base, X, 3) for X in XS], surv_cap=4096)
+    finally:
+        gpu_ctx.set_tuning(seed_per_wave=-1, learned=-1)
+        gpu_ctx.set_cost_model(None)
+    f3 = parse_summaries(read_text(REF_DUMP, "delete_f3.txt"))
+    for X, c in zip(XS, cs):
+        d = c.as_dict()
+        assert (d["leaves"], d["steps"], d["hist"]) == (f3[X]["leaves"], f3[X]["steps"], f3[X]["hist"]), X
+    ps = {X: e13.setup(base, X, 3) for X in XS}
+    lines = sorted(e13.format_survivor(ps[XS[s.prob]], s) for s in sv)
+    assert lines == survivor_lines(read_text(REF_RESULTS, "1_local_edits_surv_f3.txt"))
