@@ -93,7 +93,7 @@ class Tuning(ctypes.Structure):
                 ("seed_per_wave", ctypes.c_int32), ("sched_every", ctypes.c_int32), ("donate_max", ctypes.c_int32),
                 ("idle_naps", ctypes.c_int32), ("mail_every", ctypes.c_int32), ("trace_events", ctypes.c_int32),
                 ("graph_chain", ctypes.c_int32), ("cost_log", ctypes.c_int32),
-                ("learned", ctypes.c_int32)]
+                ("learned", ctypes.c_int32), ("solo_steps", ctypes.c_int32), ("pad_s", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

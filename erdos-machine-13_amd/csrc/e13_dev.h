@@ -164,6 +164,7 @@ typedef struct E13SearchArgs {
   const E13Node *seeds;   /* nseeds frontier nodes dealt round-robin to the waves' stacks at start */
   uint32_t nseeds;
   int32_t idle_naps, mail_every, trace_cap;
+  int32_t solo;           /* > 0: steps per round of the scalar loop for a wave's lone running lane */
   uint32_t nseed_cnt;     /* cost logging: seeds with counters (0: off) */
   uint64_t *seed_cnt;     /* cost logging: per seed, leaves and steps of its subtree */
   uint64_t *trace;        /* optional: trace_cap words per wave, word 0 = events logged, then
@@ -180,6 +181,8 @@ typedef struct E13BatchArgs {
   const int32_t *dl;      /* device copy of the stage deadlines dl[0..9] */
   int32_t n, nq, slice;
   uint32_t chunk;         /* tables a wave claims per atomic (refilling form) */
+  int32_t solo;           /* > 0: steps per round of the scalar loop for a wave's lone running lane */
+  int32_t pad_b;
 } E13BatchArgs;
 
 #endif

@@ -164,6 +164,7 @@ static void tuning_defaults(e13_tuning *t) {
   t->mail_every = 1;
   t->trace_events = 0;
   t->graph_chain = 0;
+  t->solo_steps = 256;
 }
 
 static void tuning_merge(e13_tuning *dst, const e13_tuning *src) {
@@ -190,6 +191,8 @@ static void tuning_merge(e13_tuning *dst, const e13_tuning *src) {
   if (src->cost_log > 0) dst->cost_log = 1;
   if (src->cost_log < 0) dst->cost_log = 0;
   if (src->learned > 0) dst->learned = src->learned > 2 ? 2 : src->learned;
+  if (src->solo_steps > 0) dst->solo_steps = src->solo_steps;
+  if (src->solo_steps < 0) dst->solo_steps = 0;
   if (src->learned < 0) dst->learned = 0;
 }
 
@@ -925,6 +928,7 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
   a.sched_every = (uint32_t)tu->sched_every;
   a.idle_naps = tu->idle_naps;
   a.mail_every = tu->mail_every;
+  a.solo = tu->solo_steps;
   a.pool_min_idle = 16;
   a.total_waves = total_waves;
   a.surv_cap = (uint32_t)c->surv_cap;
@@ -1036,7 +1040,7 @@ static int batch_launch(e13_ctx *c, const void *tabs_dev, int nq, int n, const v
   a.nq = nq;
   a.slice = c->tune.slice;
   a.dl = c->d_dl;
-  a.chunk = 128;
+  a.solo = c->tune.solo_steps;
   HIPCHK(hipMemsetAsync(c->d_ctr, 0, sizeof(E13Counters), c->stream));
   const int wg = fit_wg(c, c->tune.wg_threads, c->tune.gate == 2);
   if (e13k_lds_bytes(wg, c->tune.gate == 2) > c->lds_max) return E13_EINVAL;
@@ -1046,6 +1050,9 @@ static int batch_launch(e13_ctx *c, const void *tabs_dev, int nq, int n, const v
   long need = ((long)n + wg - 1) / wg;
   int grid = refill ? c->ncu * c->tune.wgs_per_cu : (int)need;
   if (need < grid) grid = (int)(need > 0 ? need : 1);
+  /* claims of 128 tables per atomic once the batch is many times the lanes;
+     smaller batches claim one table per idle lane, so every wave gets work */
+  a.chunk = (long)n >= 16L * grid * wg ? 128u : 0u;
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   int lr = e13k_launch_batch(&a, grid, wg, c->tune.gate == 2, refill, c->stream);
   if (lr != 0) {

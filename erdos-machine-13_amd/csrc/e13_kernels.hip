@@ -276,6 +276,71 @@ __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int &
   return step_once<GATE>(L, c, evcell);
 }
 
+// run_slice for a wave with exactly one running lane (exec holds only that
+// lane): the same steps, exit test and hand-off to step_once, with every value
+// made wave-uniform so the loop runs on the scalar unit without exec
+// bookkeeping, the 16 rule rows in registers and the current tape word cached
+// (written back when the head crosses into another word).  A lone lane is
+// bound by the latency of one step, which this cuts; ones-count gate only.
+__device__ __forceinline__ int solo_uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ int run_solo(Lane &L, const Ctx &c, int nsteps, int &evcell) {
+  const int tbl = solo_uni(c.tbl), rbl = solo_uni(c.rbl);
+  int q = solo_uni(L.q);
+  int st = (solo_uni(L.row) - rbl) >> 8;
+  const int t_round = solo_uni(L.t);
+  if ((unsigned)q > (unsigned)E13_QMAX) { // a run_slice round ended on the step that left the window
+    L.t = t_round - 1;                     // that step does not count
+    return EV_DEAD;
+  }
+  const int left = solo_uni(L.tstop) - t_round;
+  const int target = solo_uni(L.target), qA = solo_uni(L.qA), qR = solo_uni(L.qR);
+  const int st0r = (solo_uni(L.row0) - rbl) >> 8;
+  const bool nt1 = solo_uni(L.nt) == 1;
+  int ones = solo_uni(L.ones);
+  uint32_t rows[16]; // vector registers, read at the uniform state index (no copies per step)
+#pragma unroll
+  for (int k = 0; k < 16; ++k) rows[k] = lds_ld(c.sb, rbl + k * 256);
+  int wi = q >> 5;
+  uint32_t word = (uint32_t)solo_uni((int)lds_ld(c.sb, (wi << 8) + tbl));
+  const int n_eff = min(nsteps, left); // the deadline/cap test is due at r == left
+  int r = 0, why = 0;                  // why: 1 stage gate or unbound/halt read, 2 left the window
+  for (; r < n_eff; ++r) {
+    const uint32_t qb = (uint32_t)q & 31u;
+    const uint32_t sym = (word >> qb) & 1u;
+    const uint32_t e = (uint32_t)solo_uni((int)rows[st & 15]) >> (sym << 4);
+    const uint32_t nx = e & 0x0F00u;
+    if ((nx == 0x0F00u) | (ones == target)) { // rare: settle it exactly in step_once
+      if ((nx == 0x0F00u) | nt1 | ((st == st0r) & ((q == qA) | (q == qR)))) {
+        why = 1;
+        break;
+      }
+    }
+    const uint32_t w = e & 1u;
+    word ^= (w ^ sym) << qb;
+    ones += q >= E13_QBLK ? (int)w - (int)sym : 0;
+    q += (int)(e & 2u) - 1;
+    if ((unsigned)q > (unsigned)E13_QMAX) { // off the window: dead, and this step does not count
+      why = 2;
+      break;
+    }
+    st = (int)(nx >> 8);
+    if ((q >> 5) != wi) { // into another tape word
+      lds_st(c.sb, (wi << 8) + tbl, word);
+      wi = q >> 5;
+      word = (uint32_t)solo_uni((int)lds_ld(c.sb, (wi << 8) + tbl));
+    }
+  }
+  lds_st(c.sb, (wi << 8) + tbl, word);
+  L.q = q;
+  L.row = rbl + st * 256;
+  L.ones = ones;
+  L.t = t_round + r;
+  if (why == 2) return EV_DEAD;
+  if (why == 1 || r >= left) return step_once<GATE_COUNT>(L, c, evcell);
+  return EV_NONE;
+}
+
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -657,14 +722,15 @@ __global__ void __launch_bounds__(256) k_batch(E13BatchArgs a) {
     for (int tries = 0; need != 0 && tries < 2; ++tries) {
       if (ck >= ce) {
         if (exhausted) break;
-        // tables come from a wave-private range claimed `chunk` at a time:
-        // one atomic on the shared cursor per chunk, not per refill
+        // tables come from a wave-private range claimed `chunk` at a time (one
+        // atomic on the shared cursor per chunk); chunk 0: as many as lanes are idle
+        const uint32_t want = a.chunk ? a.chunk : (uint32_t)__popcll(need);
         uint32_t b0 = 0;
-        if (lane == 0) b0 = atomicAdd(&a.ctr->item, a.chunk);
+        if (lane == 0) b0 = atomicAdd(&a.ctr->item, want);
         b0 = (uint32_t)uni((int)b0);
-        if (b0 >= (uint32_t)a.n || b0 + a.chunk >= (uint32_t)a.n) exhausted = true;
+        if (b0 >= (uint32_t)a.n || b0 + want >= (uint32_t)a.n) exhausted = true;
         ck = b0;
-        ce = min(b0 + a.chunk, (uint32_t)a.n);
+        ce = min(b0 + want, (uint32_t)a.n);
         if (ck >= ce) break;
       }
       const uint32_t give = min((uint32_t)__popcll(need), ce - ck);
@@ -700,7 +766,10 @@ __global__ void __launch_bounds__(256) k_batch(E13BatchArgs a) {
     w_rounds++;
     w_slots += (uint64_t)__popcll(act);
     int ev = EV_NONE, evcell = -1;
-    if (active) ev = run_slice<GATE>(L, c, a.slice, evcell);
+    // a lone running lane takes the scalar loop (longer rounds: nothing to refill meanwhile
+    // once the batch is exhausted)
+    const bool solo = GATE == GATE_COUNT && a.solo > 0 && __popcll(act) == 1;
+    if (active) ev = solo ? run_solo(L, c, exhausted ? a.solo : a.slice, evcell) : run_slice<GATE>(L, c, a.slice, evcell);
     if (active && ev != EV_NONE) {
       active = 0;
       my_steps += (uint64_t)L.t;
@@ -1196,8 +1265,10 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
     cyc3 += tn - ts;
     ts = tn;
     // ---- a round of steps (time: stepping) ----
+    // a lone running lane with nothing to hand out takes the scalar loop for a longer round
     int ev = EV_NONE, evcell = -1;
-    if (active) ev = run_slice<GATE>(L, c, a.slice, evcell);
+    const bool solo = GATE == GATE_COUNT && a.solo > 0 && __popcll(act) == 1 && top == bot;
+    if (active) ev = solo ? run_solo(L, c, a.solo, evcell) : run_slice<GATE>(L, c, a.slice, evcell);
     tn = stamp();
     cyc1 += tn - ts;
     ts = tn;

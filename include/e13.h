@@ -100,6 +100,10 @@ typedef struct {
   int32_t cost_log;       /* 1: log each frontier node's features and subtree leaves/steps (e13_get_cost_log) */
   int32_t learned;        /* 1: split and order the frontier by the installed cost model; 2: a seeded
                              random order (a check); 0 or negative: the static split */
+  /* tape placement: a wave whose only running lane has nothing to refill keeps that lane's
+     rule rows and current tape word in registers and steps it on the scalar unit */
+  int32_t solo_steps;     /* > 0: steps per such round (registers); negative: off (the LDS loop only) */
+  int32_t pad_s;
 } e13_tuning;
 
 typedef struct {

@@ -230,6 +230,8 @@ def test_counter_3state_symmetric(e13, gpu_ctx, base):
                                   dict(form=2, slice=5, cap=97, batch=3, graph_chain=16),
                                   dict(form=2, graph_chain=4),
                                   dict(form=3, slice=1), dict(form=3, slice=7, refill_tries=1),
+                                  dict(form=3, solo_steps=1), dict(form=3, solo_steps=-1),
+                                  dict(form=2, solo_steps=3),
                                   dict(form=3, slice=300, wg_threads=128, wgs_per_cu=3),
                                   dict(form=3, wg_threads=64, wgs_per_cu=1, stack_nodes=1 << 16)])
 def test_results_invariant_under_tuning(e13, gpu_ctx, base, tune):
@@ -239,7 +241,7 @@ def test_results_invariant_under_tuning(e13, gpu_ctx, base, tune):
         sums, lines, hashes = search_all(e13, gpu_ctx, base, [7, 8, 12], fmax=2)
     finally:
         gpu_ctx.set_tuning(form=3, slice=32, cap=4096, batch=1 << 17, wg_threads=256, wgs_per_cu=2, refill_tries=2,
-                           stack_nodes=1 << 25, graph_chain=-1)
+                           stack_nodes=1 << 25, graph_chain=-1, solo_steps=256)
     golden = parse_summaries(read_text(REF_DUMP, "delete_f2.txt"))
     check_against(sums, {X: golden[X] for X in (7, 8, 12)})
     assert lines == survivor_lines(read_text(REF_RESULTS, "1_local_edits_surv_f2.txt"))
