@@ -86,6 +86,11 @@ class CostRec(ctypes.Structure):
                 ("prob", ctypes.c_int32), ("depth", ctypes.c_int32)]
 
 
+class TuneReport(ctypes.Structure):
+    _fields_ = [("trials", ctypes.c_int32), ("pad", ctypes.c_int32), ("first_ms", ctypes.c_double),
+                ("best_ms", ctypes.c_double)]
+
+
 class Tuning(ctypes.Structure):
     _fields_ = [("form", ctypes.c_int32), ("slice", ctypes.c_int32), ("wg_threads", ctypes.c_int32),
                 ("wgs_per_cu", ctypes.c_int32), ("cap", ctypes.c_int32), ("batch", ctypes.c_int32),
@@ -118,7 +123,7 @@ EXPORTS = ("e13_open", "e13_close", "e13_strerror", "e13_set_tuning", "e13_get_t
            "e13_simulate_batch_device", "e13_sync", "e13_recheck", "e13_format_survivor", "e13_format_hash",
            "e13_synth_table", "e13_get_trace", "e13_batch_digest", "e13_synth_run", "e13_get_cost_log",
            "e13_cost_targets", "e13_mlp_init", "e13_mlp_forward", "e13_mlp_grad", "e13_mlp_train",
-           "e13_set_cost_model")
+           "e13_set_cost_model", "e13_autotune", "e13_save_tuning", "e13_load_tuning", "e13_tuning_file")
 
 _lib = None
 
@@ -180,6 +185,10 @@ def lib():
     L.e13_mlp_train.argtypes = [ctypes.c_void_p, FP, FP, FP, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
                                 ctypes.c_float, ctypes.c_uint64, FP]
     L.e13_set_cost_model.argtypes = [ctypes.c_void_p, FP]
+    L.e13_autotune.argtypes = [ctypes.c_void_p, P(Problem), ctypes.c_int, ctypes.c_int, P(Tuning), P(TuneReport)]
+    L.e13_save_tuning.argtypes = [P(Tuning), ctypes.c_char_p]
+    L.e13_load_tuning.argtypes = [ctypes.c_char_p, P(Tuning)]
+    L.e13_tuning_file.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
     L.e13_get_trace.argtypes = [ctypes.c_void_p, P(ctypes.c_uint64), ctypes.c_uint64, P(ctypes.c_int32),
                                 P(ctypes.c_int32)]
     _lib = L
@@ -288,6 +297,17 @@ def cost_targets(leaves, steps):
     t = (ctypes.c_float * 2)()
     lib().e13_cost_targets(leaves, steps, t)
     return list(t)
+
+
+def save_tuning(tuning, path):
+    t = Tuning(**{k: v for k, v in tuning.items() if k in dict(Tuning._fields_)})
+    _check(lib().e13_save_tuning(ctypes.byref(t), path.encode()), "e13_save_tuning")
+
+
+def load_tuning(path):
+    t = Tuning()
+    _check(lib().e13_load_tuning(path.encode(), ctypes.byref(t)), "e13_load_tuning")
+    return t.as_dict()
 
 
 def recheck(tab, nq, st0, head0, abs_ok, rel_ok, steps):
@@ -410,6 +430,21 @@ class Context:
     def set_cost_model(self, params):
         _check(lib().e13_set_cost_model(self._h, _floats(params, MLP_PARAMS) if params is not None else None),
                "e13_set_cost_model")
+
+    def autotune(self, problems, reps=1):
+        """Measure settings on a persistent search of `problems`; installs and returns
+        (tuning dict, report)."""
+        arr = (Problem * len(problems))(*problems)
+        t = Tuning()
+        r = TuneReport()
+        _check(lib().e13_autotune(self._h, arr, len(problems), reps, ctypes.byref(t), ctypes.byref(r)),
+               "e13_autotune")
+        return t.as_dict(), {"trials": r.trials, "first_ms": r.first_ms, "best_ms": r.best_ms}
+
+    def tuning_file(self):
+        buf = ctypes.create_string_buffer(1024)
+        _check(lib().e13_tuning_file(self._h, buf, 1024), "e13_tuning_file")
+        return buf.value.decode()
 
     def sync(self):
         _check(lib().e13_sync(self._h), "e13_sync")

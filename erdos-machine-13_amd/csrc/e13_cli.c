@@ -6,6 +6,8 @@
  *   e13search_gpu free    machine.txt X CELL...
  *   e13search_gpu freesym machine.txt X LO HI CELL...
  *   e13search_gpu synth   SEED0 K0 N      (the synthetic stress batch; one summary line)
+ *   e13search_gpu tune    machine.txt FMAX X...   (autotune on that search; saves the settings
+ *                                                  to $E13_TUNING_OUT or the device's tuning file)
  *
  * Argument meaning, stdout/stderr split and line formats follow
  * src/e13search.c main() (:435-472), report() (:184-200) and the summary at
@@ -355,6 +357,30 @@ int main(int argc, char **argv) {
       run_many(ctx, base, nq, argc - 4, argv + 4, fmax);
     else
       for (int i = 4; i < argc; i++) run_one(ctx, base, nq, atoi(argv[i]), fmax, NULL, -1, -1);
+  } else if (!strcmp(mode, "tune")) { /* tune machine.txt FMAX X...: autotune on that delete search */
+    if (argc < 5) return 1;
+    int fmax = atoi(argv[3]), np = 0;
+    e13_problem ps[16];
+    for (int i = 4; i < argc && np < 16; i++) {
+      e13_problem p;
+      if (e13_setup(base, nq, atoi(argv[i]), fmax, NULL, -1, -1, &p) == E13_OK) ps[np++] = p;
+    }
+    e13_tuning best;
+    e13_tune_report rep;
+    rc = np ? e13_autotune(ctx, ps, np, 2, &best, &rep) : E13_EINVAL;
+    if (rc) {
+      fprintf(stderr, "autotune: %s\n", e13_strerror(rc));
+      return 1;
+    }
+    char path[1024];
+    const char *out = getenv("E13_TUNING_OUT");
+    if (out && *out) snprintf(path, sizeof path, "%s", out);
+    else if (e13_tuning_file(ctx, path, sizeof path)) path[0] = 0;
+    fprintf(stderr, "autotune: %d trials, %.1f ms -> %.1f ms; wgs_per_cu=%d slice=%d refill_tries=%d solo_steps=%d "
+                    "seed_per_wave=%d donate_max=%d sched_every=%d wg_threads=%d idle_naps=%d\n",
+            rep.trials, rep.first_ms, rep.best_ms, best.wgs_per_cu, best.slice, best.refill_tries, best.solo_steps,
+            best.seed_per_wave, best.donate_max, best.sched_every, best.wg_threads, best.idle_naps);
+    if (path[0] && e13_save_tuning(&best, path) == E13_OK) fprintf(stderr, "autotune: saved to %s\n", path);
   } else if (!strcmp(mode, "free") || !strcmp(mode, "freesym")) {
     if (argc < 4) return 1;
     uint8_t fb[32] = {0};

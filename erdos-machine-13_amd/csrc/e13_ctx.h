@@ -33,6 +33,7 @@ _Static_assert(sizeof(E13Node) % 16 == 0, "node records are copied as 16-byte ve
 
 struct e13_ctx {
   int device, ncu;
+  char arch[256]; /* gcnArchName */
   size_t lds_max; /* largest dynamic LDS a workgroup may use */
   hipStream_t stream;
   hipEvent_t ev0, ev1;
@@ -96,5 +97,7 @@ E13_INTERNAL void e13i_fill_problem(const e13_problem *p, E13Problem *pb);
 E13_INTERNAL void e13i_node_features(const e13_problem *p, const E13Node *nd, float *f);
 E13_INTERNAL int e13i_mlp_predict_dev(e13_ctx *c, const float *x_host, int n, float *y_host);
 E13_INTERNAL void e13i_free_cost(e13_ctx *c);
+/* stored per-device tuning (e13_tune.c) */
+E13_INTERNAL int e13i_tuning_path(const char *arch, int ncu, char *buf, int len);
 
 #endif

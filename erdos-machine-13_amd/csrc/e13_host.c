@@ -221,12 +221,6 @@ int e13_open(int device, const e13_tuning *t, e13_ctx **out) {
   if (device < 0 || device >= ndev) return E13_EINVAL;
   e13_ctx *c = (e13_ctx *)calloc(1, sizeof *c);
   if (!c) return E13_ENOMEM;
-  tuning_defaults(&c->tune);
-  tuning_merge(&c->tune, t);
-  if (!tuning_valid(&c->tune)) {
-    free(c);
-    return E13_EINVAL;
-  }
   c->device = device;
   if (hipSetDevice(device) != hipSuccess) {
     free(c);
@@ -239,6 +233,21 @@ int e13_open(int device, const e13_tuning *t, e13_ctx **out) {
   }
   c->ncu = prop.multiProcessorCount;
   c->lds_max = prop.sharedMemPerBlock;
+  snprintf(c->arch, sizeof c->arch, "%s", prop.gcnArchName);
+  /* defaults, then this device's stored (autotuned) settings, then the caller's */
+  tuning_defaults(&c->tune);
+  char path[1024];
+  e13_tuning stored;
+  if (e13i_tuning_path(c->arch, c->ncu, path, sizeof path) && e13_load_tuning(path, &stored) == E13_OK) {
+    e13_tuning tmp = c->tune;
+    tuning_merge(&tmp, &stored);
+    if (tuning_valid(&tmp)) c->tune = tmp;
+  }
+  tuning_merge(&c->tune, t);
+  if (!tuning_valid(&c->tune)) {
+    free(c);
+    return E13_EINVAL;
+  }
   if (hipStreamCreate(&c->stream) != hipSuccess || hipEventCreate(&c->ev0) != hipSuccess ||
       hipEventCreate(&c->ev1) != hipSuccess) {
     free(c);

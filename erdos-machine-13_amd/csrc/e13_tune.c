@@ -1,0 +1,194 @@
+/* e13_tune.c -- the autotuner: picks launch and schedule settings for this
+ * device by measuring them on a real workload, and stores them per device.
+ *
+ * No reference counterpart (the reference adapts only its GPU batch size,
+ * src/e13gpu.py:427-430, and takes E13_SLICE from the environment, :346).
+ *
+ * Coordinate descent over the knobs that move the persistent search: resident
+ * waves per CU (with one lane per leaf, the occupancy knob: a step is a serial
+ * dependency chain, so a leaf has no work for a second lane), the refill
+ * granularity (steps per round, refill passes), the tape placement of a lone
+ * running lane (LDS loop only, or the register loop with its round length),
+ * and the work-queue depth (frontier per wave, nodes per delivery pass, rounds
+ * between hungry-map reads).  Every trial's counts must equal the first
+ * trial's; the settings change the schedule, never the result.
+ */
+#define _GNU_SOURCE 1 /* dladdr */
+#include <dlfcn.h>
+#include <libgen.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "e13_ctx.h"
+
+#define NKNOB 9
+static const char *knob_name[NKNOB] = {"wgs_per_cu", "slice",       "refill_tries", "solo_steps", "seed_per_wave",
+                                       "donate_max", "sched_every", "wg_threads",   "idle_naps"};
+
+static int32_t *knob(e13_tuning *t, int k) {
+  switch (k) {
+  case 0: return &t->wgs_per_cu;
+  case 1: return &t->slice;
+  case 2: return &t->refill_tries;
+  case 3: return &t->solo_steps;
+  case 4: return &t->seed_per_wave;
+  case 5: return &t->donate_max;
+  case 6: return &t->sched_every;
+  case 7: return &t->wg_threads;
+  default: return &t->idle_naps;
+  }
+}
+
+/* Candidate values (-1: off / none, as tuning_merge reads it). */
+static const int32_t cand[NKNOB][5] = {
+    {1, 2, 3, 4, 0},      {16, 32, 64, 128, 0}, {1, 2, 4, 0, 0},  {-1, 64, 256, 1024, 0}, {-1, 2, 8, 0, 0},
+    {2, 4, 8, 16, 0},     {2, 8, 32, 0, 0},     {128, 256, 0, 0, 0}, {1, 4, 16, 0, 0}};
+
+static int same_counts(const e13_counts *a, const e13_counts *b, int np) {
+  for (int k = 0; k < np; k++) {
+    if (a[k].leaves != b[k].leaves || a[k].steps != b[k].steps || a[k].survivors != b[k].survivors) return 0;
+    for (int j = 0; j <= E13_NS; j++)
+      if (a[k].dead_at_stage[j] != b[k].dead_at_stage[j]) return 0;
+  }
+  return 1;
+}
+
+/* Kernel time of the workload under tuning t (best of reps), checked against ref. */
+static int trial(e13_ctx *c, const e13_tuning *t, const e13_problem *ps, int np, int reps, e13_counts *ref,
+                 int *have_ref, double *ms) {
+  e13_tuning t2 = *t; /* "off" is 0 in a read-back tuning, -1 when set */
+  if (t2.solo_steps == 0) t2.solo_steps = -1;
+  if (t2.seed_per_wave == 0) t2.seed_per_wave = -1;
+  int rc = e13_set_tuning(c, &t2);
+  if (rc) return rc;
+  e13_counts cnt[E13_MAXPROB];
+  *ms = 1e300;
+  for (int r = 0; r < reps; r++) {
+    e13_stats s0, s1;
+    e13_get_stats(c, &s0);
+    uint64_t ns = 0;
+    if ((rc = e13_search_many(c, ps, np, cnt, NULL, 0, &ns))) return rc;
+    e13_get_stats(c, &s1);
+    const double k = s1.kernel_ms - s0.kernel_ms;
+    if (k < *ms) *ms = k;
+    if (!*have_ref) {
+      memcpy(ref, cnt, (size_t)np * sizeof *cnt);
+      *have_ref = 1;
+    } else if (!same_counts(ref, cnt, np)) {
+      fprintf(stderr, "libe13: autotune: counts changed under a tuning setting\n");
+      return E13_EINVAL;
+    }
+  }
+  return E13_OK;
+}
+
+int e13_autotune(e13_ctx *c, const e13_problem *ps, int np, int reps, e13_tuning *best, e13_tune_report *rep) {
+  if (!c || !ps || np < 1 || np > E13_MAXPROB || reps < 1) return E13_EINVAL;
+  e13_tuning cur, start;
+  e13_get_tuning(c, &start);
+  cur = start;
+  cur.form = 3;
+  e13_counts ref[E13_MAXPROB];
+  int have_ref = 0, trials = 0, rc;
+  double best_ms, ms;
+  if ((rc = trial(c, &cur, ps, np, reps, ref, &have_ref, &best_ms))) goto fail;
+  trials++;
+  const double first_ms = best_ms;
+  for (int pass = 0; pass < 2; pass++) {
+    int changed = 0;
+    for (int k = 0; k < NKNOB; k++) {
+      const int32_t keep = *knob(&cur, k);
+      int32_t pick = keep;
+      for (int i = 0; i < 5 && cand[k][i] != 0; i++) {
+        const int32_t v = cand[k][i];
+        /* the current value (as stored: off is 0 after merging) is already measured */
+        if (v == keep || (v < 0 && keep == 0)) continue;
+        e13_tuning t = cur;
+        *knob(&t, k) = v;
+        if ((rc = trial(c, &t, ps, np, reps, ref, &have_ref, &ms))) goto fail;
+        trials++;
+        if (ms < 0.98 * best_ms) { /* a clear gain only: timing noise does not move the choice */
+          best_ms = ms;
+          pick = v;
+        }
+      }
+      if (pick != keep) changed = 1;
+      *knob(&cur, k) = pick;
+    }
+    if (!changed) break;
+  }
+  if (cur.solo_steps == 0) cur.solo_steps = -1;
+  if (cur.seed_per_wave == 0) cur.seed_per_wave = -1;
+  if ((rc = e13_set_tuning(c, &cur))) goto fail;
+  e13_get_tuning(c, &cur);
+  if (best) *best = cur;
+  if (rep) {
+    memset(rep, 0, sizeof *rep);
+    rep->trials = trials;
+    rep->first_ms = first_ms;
+    rep->best_ms = best_ms;
+  }
+  return E13_OK;
+fail:
+  e13_set_tuning(c, &start);
+  return rc;
+}
+
+int e13_save_tuning(const e13_tuning *t, const char *path) {
+  if (!t || !path) return E13_EINVAL;
+  FILE *f = fopen(path, "w");
+  if (!f) return E13_EINVAL;
+  e13_tuning tmp = *t;
+  fprintf(f, "# e13 tuning (autotuned): name value\n");
+  for (int k = 0; k < NKNOB; k++) {
+    int32_t v = *knob(&tmp, k);
+    if (k == 3 || k == 4) v = v > 0 ? v : -1; /* off is written as -1, which tuning_merge reads as off */
+    fprintf(f, "%s %d\n", knob_name[k], (int)v);
+  }
+  fprintf(f, "gate %d\n", (int)t->gate);
+  fclose(f);
+  return E13_OK;
+}
+
+int e13_load_tuning(const char *path, e13_tuning *t) {
+  if (!path || !t) return E13_EINVAL;
+  FILE *f = fopen(path, "r");
+  if (!f) return E13_EINVAL;
+  memset(t, 0, sizeof *t);
+  char name[64];
+  int v;
+  char line[256];
+  while (fgets(line, sizeof line, f)) {
+    if (line[0] == '#' || sscanf(line, "%63s %d", name, &v) != 2) continue;
+    for (int k = 0; k < NKNOB; k++)
+      if (!strcmp(name, knob_name[k])) *knob(t, k) = v;
+    if (!strcmp(name, "gate")) t->gate = v;
+  }
+  fclose(f);
+  return E13_OK;
+}
+
+/* Where this device's stored tuning lives: $E13_TUNING_FILE, else
+   <library dir>/../tuning/<arch>_<cus>cu.txt. */
+int e13i_tuning_path(const char *arch, int ncu, char *buf, int len) {
+  const char *env = getenv("E13_TUNING_FILE");
+  if (env && *env) {
+    snprintf(buf, (size_t)len, "%s", env);
+    return 1;
+  }
+  Dl_info info;
+  if (!dladdr((void *)&e13_autotune, &info) || !info.dli_fname) return 0;
+  char tmp[1024];
+  snprintf(tmp, sizeof tmp, "%s", info.dli_fname);
+  char a[64];
+  snprintf(a, sizeof a, "%s", arch);
+  char *colon = strchr(a, ':');
+  if (colon) *colon = 0;
+  snprintf(buf, (size_t)len, "%s/../tuning/%s_%dcu.txt", dirname(tmp), a, ncu);
+  return 1;
+}
+
+int e13_tuning_file(e13_ctx *c, char *buf, int len) {
+  if (!c || !buf || len < 2) return E13_EINVAL;
+  return e13i_tuning_path(c->arch, c->ncu, buf, len) ? E13_OK : E13_EINVAL;
+}

@@ -210,6 +210,26 @@ int e13_batch_digest(const e13_leaf *leaves, int64_t n, uint64_t k0, e13_batch_s
    defined by this build. */
 int e13_synth_run(e13_ctx *ctx, uint64_t seed0, uint64_t k0, int64_t n, e13_batch_sum *out);
 
+/* ---- Autotuner (no reference counterpart; the reference adapts only its GPU
+   batch size, e13gpu.py:427-430).  Coordinate descent over resident waves per
+   CU, steps per round, refill passes, the lone-lane tape placement (LDS loop or
+   register loop), frontier per wave, nodes per delivery pass, hungry-map read
+   interval, workgroup size and idle poll interval, each trial a persistent
+   search of ps[0..np) (best of reps).  Every trial's counts must equal the
+   first's (E13_EINVAL otherwise).  The chosen settings are installed in ctx.
+   e13_open loads a device's stored settings from $E13_TUNING_FILE or from
+   <library dir>/../tuning/<arch>_<CUs>cu.txt (e13_tuning_file) when present;
+   fields the caller sets override them. ---- */
+typedef struct {
+  int32_t trials, pad;
+  double first_ms, best_ms; /* kernel time with the starting and the chosen settings */
+} e13_tune_report;
+
+int e13_autotune(e13_ctx *ctx, const e13_problem *ps, int np, int reps, e13_tuning *best, e13_tune_report *rep);
+int e13_save_tuning(const e13_tuning *t, const char *path);
+int e13_load_tuning(const char *path, e13_tuning *t);
+int e13_tuning_file(e13_ctx *ctx, char *buf, int len);
+
 /* ---- Subtree cost model (no reference counterpart; the nearest is the Knuth
    estimator probe(), e13search.c:256-313, which estimates the same two
    quantities by random descents).  A 32 -> 64 -> 64 -> 2 perceptron (ReLU,

@@ -245,3 +245,25 @@ def test_results_invariant_under_tuning(e13, gpu_ctx, base, tune):
     golden = parse_summaries(read_text(REF_DUMP, "delete_f2.txt"))
     check_against(sums, {X: golden[X] for X in (7, 8, 12)})
     assert lines == survivor_lines(read_text(REF_RESULTS, "1_local_edits_surv_f2.txt"))
+
+
+def test_autotune_keeps_results(e13, gpu_ctx, base):
+    """The autotuner's trials all give the same counts (it checks each against
+    the first) and the settings it installs still give the reference's."""
+    say("autotune: delete FMAX=2 X=0..13")
+    before = gpu_ctx.tuning()
+    ps = [e13.setup(base, X, 2) for X in XS]
+    try:
+        best, rep = gpu_ctx.autotune(ps, reps=1)
+        assert rep["trials"] >= 10 and rep["best_ms"] <= rep["first_ms"] * 1.0001
+        cs, sv, ns = gpu_ctx.search_many(ps, surv_cap=4096)
+    finally:
+        restore = {k: v for k, v in before.items() if k not in ("pad_s",)}
+        restore["solo_steps"] = restore["solo_steps"] or -1
+        restore["seed_per_wave"] = restore["seed_per_wave"] or -1
+        gpu_ctx.set_tuning(**restore)
+    golden = parse_summaries(read_text(REF_DUMP, "delete_f2.txt"))
+    for X, c in zip(XS, cs):
+        d = c.as_dict()
+        assert (d["leaves"], d["steps"], d["hist"]) == (golden[X]["leaves"], golden[X]["steps"], golden[X]["hist"])
+

@@ -219,6 +219,19 @@ def test_batch_digest_matches_oracle(e13):
     assert e13.batch_digest(lv, 100).digest != int(re.search(r"digest=([0-9a-f]+)", want).group(1), 16)
 
 
+def test_tuning_file_round_trip(e13, tmp_path):
+    """Stored per-device settings: written and read back field by field; off
+    (0 in a read-back tuning) is stored as -1, which a merge reads as off."""
+    t = {"wgs_per_cu": 3, "slice": 64, "refill_tries": 4, "solo_steps": 0, "seed_per_wave": 8, "donate_max": 16,
+         "sched_every": 2, "wg_threads": 128, "idle_naps": 1, "gate": 1}
+    path = str(tmp_path / "gfx950_256cu.txt")
+    e13.save_tuning(t, path)
+    back = e13.load_tuning(path)
+    for k, v in t.items():
+        assert back[k] == (-1 if k == "solo_steps" and v == 0 else v), k
+    assert back["form"] == 0 and back["cap"] == 0  # fields not stored stay unset
+
+
 def test_open_without_device_fails_cleanly(e13):
     import torch
     if torch.cuda.is_available():
