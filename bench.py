@@ -7,17 +7,22 @@ dead-at-stage histograms are compared with the reference's (the golden
 fixture tests/golden/ref_dump/delete_f3.txt); a mismatch aborts the run.
 
 value = simulated TM steps per second over the K timed steps (the whole
-search: every X in one persistent launch, plus the survivor re-check).
-Leaves/s and the CPU baseline are reported beside it, and the two larger
-searches the build targets -- the symmetric 3-state counter search
-(3,078,656,272 leaves) and the FMAX=4 local-edit search (5,986,617,022
-leaves) -- are run once each with their counts checked.
+search: every X in one persistent launch, plus the survivor re-check), with
+the device's stored autotuned settings (erdos-machine-13_amd/tuning/).
+Leaves/s and the CPU baseline are reported beside it.  Once each, with counts
+checked: the symmetric 3-state counter search (3,078,656,272 leaves) and the
+FMAX=4 search (5,986,617,022 leaves); the 2^22-table synthetic stress batch
+(refilling vs plain form, digest checked); the counter search in the plain
+per-leaf-launch form vs the persistent form; and FMAX=4 in the plain form, with
+the static split and with the split learned by the cost model (trained here,
+on the GPU, from the FMAX=2 and FMAX=3 searches' cost logs).
 
-roofline: the dominant kernel is the expansion kernel.  Its algorithmic work
-is 13 integer lane-ops per simulated step; achieved = steps x 13 / kernel
-time, with the kernel time taken from HIP events on the library's own stream
-around every launch of the timed region.  peak = the gfx950 VALU int32 issue
-rate, 256 CU x 64 lanes/clk x 2.4 GHz = 3.93e13 op/s.
+roofline: the dominant kernel is the persistent search kernel.  Its
+algorithmic work is 13 integer lane-ops and 8 LDS bytes per simulated step;
+achieved = steps x 13 / kernel time, with the kernel time taken from HIP events
+on the library's own stream around every launch of the timed region.
+peak = the gfx950 VALU int32 issue rate, 256 CU x 64 lanes/clk x 2.4 GHz =
+3.93e13 op/s (LDS: 256 CU x 128 B/clk x 2.4 GHz).
 
 cpu_baseline: the reference program compiled from its own source
 (oracle/_ref/e13search, kind "reference"), or the oracle's restatement when
@@ -40,6 +45,8 @@ FMAX = 3
 XS = list(range(14))
 OPS_PER_STEP = 13
 VALU_PEAK = 256 * 64 * 2.4e9  # int32 lane-ops/s
+LDS_BYTES_PER_STEP = 8        # one tape word read and written back
+LDS_PEAK = 256 * 128 * 2.4e9  # B/s
 
 
 def golden():
@@ -127,6 +134,69 @@ def plain_vs_batched(e13, ctx, problem):
     return res
 
 
+def learned_vs_static(e13, ctx, base):
+    """The FMAX=4 search (5,986,617,022 leaves) three ways: the plain per-leaf-launch
+    form (form 1, host loop), the persistent form with the static split of a
+    breadth-first frontier, and the same with the frontier split and ordered by
+    the cost model trained on the GPU (hand-written kernels) from the FMAX=2 and
+    FMAX=3 searches' own cost logs.  Counts checked every time."""
+    import numpy as np
+    want = (5986617022, 586158995069, 57)
+
+    def run(problems, **tune):
+        ctx.set_tuning(**tune)
+        ctx.sync()
+        s0 = ctx.stats()
+        t0 = time.perf_counter()
+        if tune.get("form", 3) == 3:
+            cs, sv, ns = ctx.search_many(problems, surv_cap=4096)
+        else:
+            cs, ns = [], 0
+            for p in problems:
+                c, sv, n1 = ctx.search(p, surv_cap=4096)
+                cs.append(c)
+                ns += n1
+        dt = time.perf_counter() - t0
+        s1 = ctx.stats()
+        ok = (sum(c.leaves for c in cs), sum(c.steps for c in cs), ns) == want
+        return {"seconds": dt, "kernel_ms": s1["kernel_ms"] - s0["kernel_ms"], "launches": s1["launches"] - s0["launches"],
+                "steps_per_s": want[1] / dt, "bit_identical_counts": ok}
+
+    def log(fmax):
+        ctx.set_tuning(seed_per_wave=4, cost_log=1, learned=-1)
+        ctx.search_many([e13.setup(base, X, fmax) for X in XS], surv_cap=4096)
+        ctx.set_tuning(cost_log=-1)
+        recs = ctx.cost_log()
+        return (np.array([list(r.x) for r in recs], dtype=np.float32),
+                np.array([e13.cost_targets(r.leaves, r.steps) for r in recs], dtype=np.float32))
+
+    f4 = [e13.setup(base, X, 4) for X in XS]
+    before = ctx.tuning()
+    out = {"workload": "delete FMAX=4 X=0..13: plain per-leaf-launch form vs persistent static split vs learned split"}
+    try:
+        x2, t2 = log(2)
+        x3, t3 = log(3)
+        x, t = np.concatenate([x2, x3]), np.concatenate([t2, t3])
+        t0 = time.perf_counter()
+        params, losses = ctx.mlp_train(e13.mlp_init(1), x, t, len(x), epochs=150, batch=256, lr=2e-3, seed=1)
+        out["training"] = {"samples": len(x), "epochs": 150, "seconds": time.perf_counter() - t0,
+                           "loss_first_epoch": losses[0], "loss_last_epoch": losses[-1]}
+        ctx.set_cost_model(params)
+        out["static_split"] = run(f4, seed_per_wave=4, learned=-1)
+        out["learned_split"] = run(f4, seed_per_wave=4, learned=1)
+        out["plain_form1"] = run(f4, form=1, seed_per_wave=-1, learned=-1)
+    finally:
+        restore = {k: v for k, v in before.items() if k != "pad_s"}
+        restore["solo_steps"] = restore["solo_steps"] or -1
+        restore["seed_per_wave"] = restore["seed_per_wave"] or -1
+        restore["learned"] = restore["learned"] or -1
+        ctx.set_tuning(**restore)
+        ctx.set_cost_model(None)
+    out["learned_over_plain"] = out["plain_form1"]["seconds"] / out["learned_split"]["seconds"]
+    out["learned_over_static"] = out["static_split"]["seconds"] / out["learned_split"]["seconds"]
+    return out
+
+
 def cpu_baseline(max_s=120):
     """The reference CPU search (or the oracle port) on the same workload."""
     ref = os.path.join(ROOT, "oracle", "_ref", "e13search")
@@ -205,6 +275,9 @@ def main():
         problems = [(13, e13.setup(base, 13, free_cells="D0 G0 K0 K1 L1 M1".split(), sym=(10, 13)))]
         want = {13: (3078656272, 61, 319112357176, [0, 2957671647, 119843176, 1115937, 22970, 2236, 1, 0, 244, 0])}
     ctx = e13.Context(0)
+    tuning = ctx.tuning()
+    tuning_file = os.path.relpath(ctx.tuning_file(), ROOT)
+    tuning_file = tuning_file if os.path.exists(os.path.join(ROOT, tuning_file)) else None
     for _ in range(args.warmup):
         one_pass(e13, ctx, problems, want)
     ctx.sync()
@@ -231,6 +304,7 @@ def main():
         extra.append(extra_workload(e13, ctx, "delete FMAX=4 X=0..13", f4, 5986617022, 586158995069, 57))
         extra.append(synth_workload(e13, ctx))
         extra.append(plain_vs_batched(e13, ctx, counter))
+        extra.append(learned_vs_static(e13, ctx, base))
     line = {
         "metric": "simulated TM steps/s, local-edit search FMAX=3 (X=0..13) on erdos_14, bit-identical to the "
                   "reference",
@@ -250,11 +324,17 @@ def main():
                    if args.workload == "fmax3" else
                    "freesym X=13 10..13 D0 G0 K0 K1 L1 M1 (3,078,656,272 leaves, 319,112,357,176 steps)",
                    "form": s1["form"], "slice": s1["slice"], "wg_threads": s1["wg_threads"], "grid": s1["grid"],
-                   "gate": "ones-count" if s1["gate"] == 1 else "zobrist-hash"},
+                   "gate": "ones-count" if s1["gate"] == 1 else "zobrist-hash",
+                   "tuning": {k: v for k, v in tuning.items() if k in ("wgs_per_cu", "slice", "refill_tries", "solo_steps",
+                                                                       "seed_per_wave", "donate_max", "sched_every",
+                                                                       "idle_naps")},
+                   "tuning_file": tuning_file},
         "leaves_per_s": leaves / dt,
         "bit_identical": True,
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_PEAK / 1e12, "unit": "TOP/s",
                      "frac": achieved / VALU_PEAK, "traffic": traffic_from_profiles(),
+                     "lds": {"achieved_GBps": steps * LDS_BYTES_PER_STEP / kernel_s / 1e9,
+                             "peak_GBps": LDS_PEAK / 1e9, "frac": steps * LDS_BYTES_PER_STEP / kernel_s / LDS_PEAK},
                      "kernel": "k_search" if s1["form"] == 3 else "k_expand", "launches": launches,
                      "kernel_ms_per_launch": kernel_s * 1e3 / max(launches, 1),
                      "steps_per_s_kernel": steps / kernel_s,
