@@ -186,11 +186,7 @@ def learned_vs_static(e13, ctx, base):
         out["learned_split"] = run(f4, seed_per_wave=4, learned=1)
         out["plain_form1"] = run(f4, form=1, seed_per_wave=-1, learned=-1)
     finally:
-        restore = {k: v for k, v in before.items() if k != "pad_s"}
-        restore["solo_steps"] = restore["solo_steps"] or -1
-        restore["seed_per_wave"] = restore["seed_per_wave"] or -1
-        restore["learned"] = restore["learned"] or -1
-        ctx.set_tuning(**restore)
+        ctx.restore_tuning(before)
         ctx.set_cost_model(None)
     out["learned_over_plain"] = out["plain_form1"]["seconds"] / out["learned_split"]["seconds"]
     out["learned_over_static"] = out["static_split"]["seconds"] / out["learned_split"]["seconds"]

@@ -98,7 +98,7 @@ class Tuning(ctypes.Structure):
                 ("seed_per_wave", ctypes.c_int32), ("sched_every", ctypes.c_int32), ("donate_max", ctypes.c_int32),
                 ("idle_naps", ctypes.c_int32), ("mail_every", ctypes.c_int32), ("trace_events", ctypes.c_int32),
                 ("graph_chain", ctypes.c_int32), ("cost_log", ctypes.c_int32),
-                ("learned", ctypes.c_int32), ("solo_steps", ctypes.c_int32), ("pad_s", ctypes.c_int32)]
+                ("learned", ctypes.c_int32), ("solo_steps", ctypes.c_int32), ("evict_after", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -348,6 +348,17 @@ class Context:
 
     def set_tuning(self, **kw):
         _check(lib().e13_set_tuning(self._h, ctypes.byref(Tuning(**kw))), "e13_set_tuning")
+
+    # fields where 0 reads back as "off" but a set of 0 means "keep"
+    _OFF_FIELDS = ("seed_per_wave", "trace_events", "graph_chain", "cost_log", "learned", "solo_steps", "evict_after")
+
+    def restore_tuning(self, t):
+        """Set every field of a tuning dict read back by tuning() (off fields included)."""
+        kw = {k: v for k, v in t.items() if k in dict(Tuning._fields_)}
+        for k in self._OFF_FIELDS:
+            if kw.get(k, 1) == 0:
+                kw[k] = -1
+        self.set_tuning(**kw)
 
     def tuning(self):
         t = Tuning()

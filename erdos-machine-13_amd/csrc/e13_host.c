@@ -165,6 +165,7 @@ static void tuning_defaults(e13_tuning *t) {
   t->trace_events = 0;
   t->graph_chain = 0;
   t->solo_steps = 256;
+  t->evict_after = 0;
 }
 
 static void tuning_merge(e13_tuning *dst, const e13_tuning *src) {
@@ -193,6 +194,8 @@ static void tuning_merge(e13_tuning *dst, const e13_tuning *src) {
   if (src->learned > 0) dst->learned = src->learned > 2 ? 2 : src->learned;
   if (src->solo_steps > 0) dst->solo_steps = src->solo_steps;
   if (src->solo_steps < 0) dst->solo_steps = 0;
+  if (src->evict_after > 0) dst->evict_after = src->evict_after;
+  if (src->evict_after < 0) dst->evict_after = 0;
   if (src->learned < 0) dst->learned = 0;
 }
 
@@ -938,6 +941,7 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
   a.idle_naps = tu->idle_naps;
   a.mail_every = tu->mail_every;
   a.solo = tu->solo_steps;
+  a.evict_after = tu->evict_after;
   a.pool_min_idle = 16;
   a.total_waves = total_waves;
   a.surv_cap = (uint32_t)c->surv_cap;

@@ -284,8 +284,43 @@ __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int &
 // bound by the latency of one step, which this cuts; ones-count gate only.
 __device__ __forceinline__ int solo_uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-__device__ __forceinline__ int run_solo(Lane &L, const Ctx &c, int nsteps, int &evcell) {
+// Block macro-steps of the lone lane: the run of the machine inside one
+// aligned 4-cell block of tape (a nibble) until the head leaves it is a pure
+// function of (state, nibble, entry cell).  It is memoised in the rule-row
+// columns of the wave's idle lanes (free while one lane runs alone): entry
+// word = gen << 16 | new nibble << 12 | exit right << 11 | exit state << 7 |
+// steps (1..127; 0: the run reads a halt or an unbound cell, or stays in the
+// block for 127 steps).  gen (>= 0x8000, one per call) tells this call's
+// entries from older ones and from rule rows (whose halves stay below 0x4000).
+#define SOLO_MEMO_GEN0 0x8000u
+
+// The next memo generation, advanced by the whole wave before a lone-lane round
+// (every lane keeps the same count); when the 15-bit range is used up, the idle
+// lanes clear their rule-row columns and the count starts again.
+__device__ __forceinline__ void solo_next_gen(uint32_t &gen, bool active, const Ctx &c, uint64_t act, uint64_t mach,
+                                              uint64_t &last) {
+  // the memo stays valid while the same lane runs the same machine (a lane
+  // counts the machines it takes, 58 bits: no wrap)
+  const int sl = __ffsll((long long)act) - 1;
+  const uint64_t m_lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)mach, sl);
+  const uint64_t m_hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(mach >> 32), sl);
+  const uint64_t who = ((uint64_t)sl << 58) | ((m_hi << 32 | m_lo) & ((1ull << 58) - 1));
+  const uint64_t prev = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(last >> 32)) << 32) |
+                        (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)last);
+  if (who == prev) return;
+  last = who;
+  gen = (uint32_t)__builtin_amdgcn_readfirstlane((int)gen) + 1u;
+  if (gen > 0xFFFFu) {
+    gen = SOLO_MEMO_GEN0 + 1u;
+    if (!active)
+      for (int k = 0; k < 16; ++k) lds_st(c.sb, c.rbl + k * 256, 0u);
+  }
+}
+
+__device__ __forceinline__ int run_solo(Lane &L, const Ctx &c, int nsteps, int &evcell, const uint32_t g) {
   const int tbl = solo_uni(c.tbl), rbl = solo_uni(c.rbl);
+  const int me = solo_uni((int)(threadIdx.x & 63u));
+  const int rbase = rbl - me * 4; // the wave's rule-row area: word k of lane j at rbase + k*256 + j*4
   int q = solo_uni(L.q);
   int st = (solo_uni(L.row) - rbl) >> 8;
   const int t_round = solo_uni(L.t);
@@ -293,27 +328,148 @@ __device__ __forceinline__ int run_solo(Lane &L, const Ctx &c, int nsteps, int &
     L.t = t_round - 1;                     // that step does not count
     return EV_DEAD;
   }
-  const int left = solo_uni(L.tstop) - t_round;
-  const int target = solo_uni(L.target), qA = solo_uni(L.qA), qR = solo_uni(L.qR);
-  const int st0r = (solo_uni(L.row0) - rbl) >> 8;
-  const bool nt1 = solo_uni(L.nt) == 1;
+  int left = solo_uni(L.tstop) - t_round;
+  int target = solo_uni(L.target), qA = solo_uni(L.qA), qR = solo_uni(L.qR);
+  int st0r = (solo_uni(L.row0) - rbl) >> 8;
+  bool nt1 = solo_uni(L.nt) == 1;
   int ones = solo_uni(L.ones);
   uint32_t rows[16]; // vector registers, read at the uniform state index (no copies per step)
 #pragma unroll
   for (int k = 0; k < 16; ++k) rows[k] = lds_ld(c.sb, rbl + k * 256);
+  // sweep rules: (state, symbol) pairs whose rule rewrites the symbol read and
+  // keeps the state; bit 2*state + symbol
+  uint32_t sweep = 0;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const uint32_t pr = (uint32_t)solo_uni((int)rows[k]);
+    const uint32_t e0 = pr & 0xFFFFu, e1 = pr >> 16;
+    sweep |= (((e0 & 0x0F00u) == ((uint32_t)k << 8)) & ((e0 & 1u) == 0u)) ? 1u << (2 * k) : 0u;
+    sweep |= (((e1 & 0x0F00u) == ((uint32_t)k << 8)) & ((e1 & 1u) == 1u)) ? 1u << (2 * k + 1) : 0u;
+  }
   int wi = q >> 5;
   uint32_t word = (uint32_t)solo_uni((int)lds_ld(c.sb, (wi << 8) + tbl));
-  const int n_eff = min(nsteps, left); // the deadline/cap test is due at r == left
-  int r = 0, why = 0;                  // why: 1 stage gate or unbound/halt read, 2 left the window
+  int n_eff = min(nsteps, left); // the deadline/cap test is due at r == left
+  int r = 0, why = 0;            // why = 2: left the window
   for (; r < n_eff; ++r) {
     const uint32_t qb = (uint32_t)q & 31u;
     const uint32_t sym = (word >> qb) & 1u;
     const uint32_t e = (uint32_t)solo_uni((int)rows[st & 15]) >> (sym << 4);
     const uint32_t nx = e & 0x0F00u;
-    if ((nx == 0x0F00u) | (ones == target)) { // rare: settle it exactly in step_once
+    if ((sweep >> (2 * st + (int)sym)) & 1u) {
+      // a run of equal cells under a rule that rewrites them and keeps the state:
+      // each cell of the run is one step that changes nothing but the head, and
+      // with the ones count off its target no gate can fire inside it.  Take the
+      // run up to the end of this tape word, the window edge and the deadline.
+      const uint32_t diff = word ^ (0u - sym); // bits that differ from sym
+      int k;
+      if (e & 2u) { // right: cells qb, qb+1, ...
+        const uint32_t up = diff >> qb;
+        k = up ? __builtin_ctz(up) : 32 - (int)qb;
+        k = min(k, E13_QMAX - q);
+      } else { // left: cells qb, qb-1, ...
+        const uint32_t dn = diff << (31u - qb);
+        k = dn ? __builtin_clz(dn) : (int)qb + 1;
+        k = min(k, q);
+      }
+      k = min(k, n_eff - r);
+      if (ones == target) { // the gate can fire only in the stage-1 state at qA or qR (or anywhere at nt 1)
+        if (nt1 | ((st == st0r) & ((q == qA) | (q == qR)))) k = 0;
+        else if (st == st0r) { // stop on the first gate cell ahead; the next iteration tests it
+          if (e & 2u) {
+            if (qA > q) k = min(k, qA - q);
+            if (qR > q) k = min(k, qR - q);
+          } else {
+            if (qA < q) k = min(k, q - qA);
+            if (qR < q) k = min(k, q - qR);
+          }
+        }
+      }
+      if (k > 0) {
+        q += (e & 2u) ? k : -k;
+        r += k - 1; // the loop's ++r counts the last cell
+        if ((q >> 5) != wi) {
+          lds_st(c.sb, (wi << 8) + tbl, word);
+          wi = q >> 5;
+          word = (uint32_t)solo_uni((int)lds_ld(c.sb, (wi << 8) + tbl));
+        }
+        continue;
+      }
+    }
+    // a block macro-step: away from the window edges and the block boundary at
+    // q 61 (ones are counted for q >= 61), with the ones count more than 4 from
+    // its target (a 4-cell run moves it by at most 4, so no gate can fire)
+    const int qn = q & ~3;
+    // (inside the block the head visits only its 4 cells: with neither gate cell
+    // among them, no gate can fire whatever the ones count)
+    const bool far = (ones - target > 4) | (target - ones > 4);
+    const bool no_gate_cell = !nt1 & ((qA >> 2) != (qn >> 2)) & ((qR >> 2) != (qn >> 2));
+    if ((qn >= 4) & (qn + 4 <= E13_QMAX) & (qn != 60) & (far | no_gate_cell)) {
+      const uint32_t nb = qb & ~3u, nib = (word >> nb) & 15u, pos = qb & 3u;
+      const int col = (int)((nib << 2) | pos);
+      if (col != me) {
+        const int ma = rbase + st * 256 + col * 4;
+        uint32_t ent = (uint32_t)solo_uni((int)lds_ld(c.sb, ma));
+        if ((ent >> 16) != g) { // not yet known this call: run the block once
+          uint32_t v = nib, s2 = (uint32_t)st;
+          int p = (int)pos, k2 = 0, right = 0, ok = 0;
+          while (k2 < 127) {
+            const uint32_t b = (v >> p) & 1u;
+            const uint32_t e2 = ((uint32_t)solo_uni((int)rows[s2 & 15u]) >> (b << 4)) & 0xFFFFu;
+            if ((e2 & 0x0F00u) == 0x0F00u) break; // halt or unbound read: not a plain run
+            v = (v & ~(1u << p)) | ((e2 & 1u) << p);
+            p += (int)(e2 & 2u) - 1;
+            s2 = e2 >> 8;
+            ++k2;
+            if ((unsigned)p > 3u) {
+              right = p > 3;
+              ok = 1;
+              break;
+            }
+          }
+          ent = (g << 16) | (v << 12) | ((uint32_t)right << 11) | (s2 << 7) | (uint32_t)(ok ? k2 : 0);
+          lds_st(c.sb, ma, ent);
+        }
+        const int k3 = (int)(ent & 127u);
+        if ((k3 > 0) & (k3 <= n_eff - r)) {
+          const uint32_t nv = (ent >> 12) & 15u;
+          word = (word & ~(15u << nb)) | (nv << nb);
+          ones += qn >= E13_QBLK ? __builtin_popcount(nv) - __builtin_popcount(nib) : 0;
+          q = (ent & 0x800u) ? qn + 4 : qn - 1;
+          st = (int)((ent >> 7) & 15u);
+          r += k3 - 1; // the loop's ++r counts the last step
+          if ((q >> 5) != wi) {
+            lds_st(c.sb, (wi << 8) + tbl, word);
+            wi = q >> 5;
+            word = (uint32_t)solo_uni((int)lds_ld(c.sb, (wi << 8) + tbl));
+          }
+          continue;
+        }
+      }
+    }
+    if ((nx == 0x0F00u) | (ones == target)) { // rare: settle it exactly in step_once, in place
       if ((nx == 0x0F00u) | nt1 | ((st == st0r) & ((q == qA) | (q == qR)))) {
-        why = 1;
-        break;
+        lds_st(c.sb, (wi << 8) + tbl, word);
+        L.q = q;
+        L.row = rbl + st * 256;
+        L.ones = ones;
+        L.t = t_round + r;
+        const int ev1 = step_once<GATE_COUNT>(L, c, evcell);
+        if (ev1 != EV_NONE) return ev1; // the machine ended or stopped; L is current
+        // a stage start may have been accepted: the gate and deadline registers moved
+        q = solo_uni(L.q);
+        st = (solo_uni(L.row) - rbl) >> 8;
+        ones = solo_uni(L.ones);
+        r = solo_uni(L.t) - t_round - 1; // the loop's ++r counts the step step_once made
+        target = solo_uni(L.target);
+        qA = solo_uni(L.qA);
+        qR = solo_uni(L.qR);
+        st0r = (solo_uni(L.row0) - rbl) >> 8;
+        nt1 = solo_uni(L.nt) == 1;
+        left = solo_uni(L.tstop) - t_round;
+        n_eff = min(nsteps, left);
+        wi = q >> 5;
+        word = (uint32_t)solo_uni((int)lds_ld(c.sb, (wi << 8) + tbl));
+        continue;
       }
     }
     const uint32_t w = e & 1u;
@@ -337,7 +493,7 @@ __device__ __forceinline__ int run_solo(Lane &L, const Ctx &c, int nsteps, int &
   L.ones = ones;
   L.t = t_round + r;
   if (why == 2) return EV_DEAD;
-  if (why == 1 || r >= left) return step_once<GATE_COUNT>(L, c, evcell);
+  if (r >= left) return step_once<GATE_COUNT>(L, c, evcell); // the deadline or cap is due
   return EV_NONE;
 }
 
@@ -710,6 +866,10 @@ __global__ void __launch_bounds__(256) k_batch(E13BatchArgs a) {
   int active = 0;
   uint32_t k = 0;
   uint64_t my_steps = 0, w_slots = 0, w_rounds = 0;
+  // rule rows start cleared: the lone-lane memo reads idle lanes' columns
+  for (int r0 = 0; r0 < 16; ++r0) lds_st(c.sb, c.rbl + r0 * 256, 0u);
+  uint32_t solo_gen = SOLO_MEMO_GEN0;
+  uint64_t solo_last = ~0ull, mach = 0;
   bool exhausted = false;
   uint32_t ck = 0, ce = 0; // this wave's claimed table range
   if (!REFILL) { // plain form: one table per lane, no refill
@@ -754,6 +914,7 @@ __global__ void __launch_bounds__(256) k_batch(E13BatchArgs a) {
         L.tcap = 0x7FFFFFFF;
         arm_stage(L, c);
         active = 1;
+        ++mach; // a new machine in this lane
       }
       ck += give;
       need = ballot(!active);
@@ -769,7 +930,9 @@ __global__ void __launch_bounds__(256) k_batch(E13BatchArgs a) {
     // a lone running lane takes the scalar loop (longer rounds: nothing to refill meanwhile
     // once the batch is exhausted)
     const bool solo = GATE == GATE_COUNT && a.solo > 0 && __popcll(act) == 1;
-    if (active) ev = solo ? run_solo(L, c, exhausted ? a.solo : a.slice, evcell) : run_slice<GATE>(L, c, a.slice, evcell);
+    if (solo) solo_next_gen(solo_gen, active, c, act, mach, solo_last);
+    if (active)
+      ev = solo ? run_solo(L, c, exhausted ? a.solo : a.slice, evcell, solo_gen) : run_slice<GATE>(L, c, a.slice, evcell);
     if (active && ev != EV_NONE) {
       active = 0;
       my_steps += (uint64_t)L.t;
@@ -1029,6 +1192,10 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
   L.tcap = 0x7FFFFFFF;
   int active = 0, t0 = 0, budget = 0, nintro = 0, prob = 0, depth = 0, nbound = 0, nq = 0;
   uint32_t seed = 0; // frontier node the lane's machine descends from
+  // rule rows start cleared: the lone-lane memo reads idle lanes' columns
+  for (int r0 = 0; r0 < 16; ++r0) lds_st(c.sb, c.rbl + r0 * 256, 0u);
+  uint32_t solo_gen = SOLO_MEMO_GEN0;
+  uint64_t solo_last = ~0ull, mach = 0;
   long long top = 0, bot = 0, staged = -1;
   int cur_slot = 0, cur_end = 0; // the staged node's child slots still to hand out: [cur_slot, cur_end)
   int busy = 0, requested = 0, mail_in = 1, idle_polls = 0;
@@ -1146,6 +1313,7 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
             }
             for (int w = 0; w < E13_TW; ++w) lds_st(c.sb, c.tbl + w * 256, stg[E13_NODE_W_TAPE + w]);
             active = 1;
+            ++mach; // a new machine in this lane
           }
         }
       }
@@ -1206,9 +1374,11 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
       }
     }
     // ---- an empty private stack and idle lanes: ask for work (one bit in the hungry map) ----
+    // (not while its only running lane is a long runner: that machine keeps the scalar loop)
     if (!requested && top == bot) {
       const int idle = __popcll(ballot(!active));
-      if (idle >= a.pool_min_idle || idle == 64) {
+      const bool hold = a.evict_after > 0 && idle == 63 && ballot(active && L.t >= a.evict_after) != 0;
+      if ((idle >= a.pool_min_idle || idle == 64) && !hold) {
         if (lane == 0) atomicOr((unsigned long long *)&a.hungry[wid >> 6], 1ull << (wid & 63));
         requested = 1;
         mail_in = 1;
@@ -1258,7 +1428,8 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
     uint64_t hw = 0; // hungry-map word `lane` (maps of up to 4096 waves)
     if (refresh) {
       if (lane == 0) s_err = ld_agent_i(&a.sch->error);
-      const bool can_give = top - bot >= 2 || (top - bot == 1 && staged == top - 1 && cur_end - cur_slot >= 2);
+      const bool can_give = top - bot >= 2 || (top - bot == 1 && staged == top - 1 && cur_end - cur_slot >= 2) ||
+                            (a.evict_after > 0 && top == bot && __popcll(act) >= 2);
       if (lane < a.hungry_words && can_give) hw = ld_agent64(&a.hungry[lane]);
     }
     tn = stamp();
@@ -1268,11 +1439,25 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
     // a lone running lane with nothing to hand out takes the scalar loop for a longer round
     int ev = EV_NONE, evcell = -1;
     const bool solo = GATE == GATE_COUNT && a.solo > 0 && __popcll(act) == 1 && top == bot;
-    if (active) ev = solo ? run_solo(L, c, a.solo, evcell) : run_slice<GATE>(L, c, a.slice, evcell);
+    if (solo) solo_next_gen(solo_gen, active, c, act, mach, solo_last);
+    if (active) ev = solo ? run_solo(L, c, a.solo, evcell, solo_gen) : run_slice<GATE>(L, c, a.slice, evcell);
     tn = stamp();
     cyc1 += tn - ts;
     ts = tn;
     // ---- finished machines ----
+    // ---- long runners to hungry waves: with nothing on its stack to give, a wave
+    // pauses machines that are far along (evict_after steps from the blank tape:
+    // past the early stage bounds they tend to run to the late deadlines) as
+    // resumable nodes, as at a launch cap, so the donation below hands them out
+    // and each runs alone, in the scalar loop ----
+    if (refresh && a.evict_after > 0 && top == bot) {
+      const uint64_t running = ballot(active && ev == EV_NONE);
+      if (__popcll(running) >= 2 && ballot(hw != 0) != 0) {
+        const uint64_t cand = ballot(active && ev == EV_NONE && L.t >= a.evict_after);
+        const int n_ev = min(min(__popcll(cand), a.donate_every), __popcll(running) - 1);
+        if (active && ev == EV_NONE && ((cand >> lane) & 1ull) && (int)rank_in(cand) < n_ev) ev = EV_CAP;
+      }
+    }
     const bool fin = active && ev != EV_NONE;
     if (fin) {
       const unsigned long long d = (unsigned long long)(L.t - t0);
@@ -1349,7 +1534,8 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
       int given = 0;
       const int idle_now = __popcll(ballot(!active));
       for (;;) {
-        const bool whole = top - bot >= 2;
+        // a whole node: keep one unless lanes still run (they keep the wave busy)
+        const bool whole = top - bot >= 2 || (top - bot == 1 && ballot(active) != 0 && a.evict_after > 0);
         const int rem = cur_end - cur_slot, keep = max(idle_now, rem / 2);
         const bool split = !whole && top - bot == 1 && staged == top - 1 && rem >= 2 && keep < rem;
         if (nz == 0 || given >= a.donate_every || !(whole || split)) break;

@@ -103,7 +103,9 @@ typedef struct {
   /* tape placement: a wave whose only running lane has nothing to refill keeps that lane's
      rule rows and current tape word in registers and steps it on the scalar unit */
   int32_t solo_steps;     /* > 0: steps per such round (registers); negative: off (the LDS loop only) */
-  int32_t pad_s;
+  int32_t evict_after;    /* > 0: while waves are hungry, a wave with nothing to give hands over machines
+                             this many steps from the blank tape, and a wave whose only running machine is
+                             one keeps it alone (register loop); negative: off */
 } e13_tuning;
 
 typedef struct {

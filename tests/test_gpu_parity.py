@@ -231,17 +231,18 @@ def test_counter_3state_symmetric(e13, gpu_ctx, base):
                                   dict(form=2, graph_chain=4),
                                   dict(form=3, slice=1), dict(form=3, slice=7, refill_tries=1),
                                   dict(form=3, solo_steps=1), dict(form=3, solo_steps=-1),
+                                  dict(form=3, evict_after=1, solo_steps=7), dict(form=3, evict_after=-1),
                                   dict(form=2, solo_steps=3),
                                   dict(form=3, slice=300, wg_threads=128, wgs_per_cu=3),
                                   dict(form=3, wg_threads=64, wgs_per_cu=1, stack_nodes=1 << 16)])
 def test_results_invariant_under_tuning(e13, gpu_ctx, base, tune):
     say(f"search: delete FMAX=2 X=7,8,12 tuning={tune}")
+    before = gpu_ctx.tuning()
     gpu_ctx.set_tuning(**tune)
     try:
         sums, lines, hashes = search_all(e13, gpu_ctx, base, [7, 8, 12], fmax=2)
     finally:
-        gpu_ctx.set_tuning(form=3, slice=32, cap=4096, batch=1 << 17, wg_threads=256, wgs_per_cu=2, refill_tries=2,
-                           stack_nodes=1 << 25, graph_chain=-1, solo_steps=256)
+        gpu_ctx.restore_tuning(before)
     golden = parse_summaries(read_text(REF_DUMP, "delete_f2.txt"))
     check_against(sums, {X: golden[X] for X in (7, 8, 12)})
     assert lines == survivor_lines(read_text(REF_RESULTS, "1_local_edits_surv_f2.txt"))
@@ -258,10 +259,7 @@ def test_autotune_keeps_results(e13, gpu_ctx, base):
         assert rep["trials"] >= 10 and rep["best_ms"] <= rep["first_ms"] * 1.0001
         cs, sv, ns = gpu_ctx.search_many(ps, surv_cap=4096)
     finally:
-        restore = {k: v for k, v in before.items() if k not in ("pad_s",)}
-        restore["solo_steps"] = restore["solo_steps"] or -1
-        restore["seed_per_wave"] = restore["seed_per_wave"] or -1
-        gpu_ctx.set_tuning(**restore)
+        gpu_ctx.restore_tuning(before)
     golden = parse_summaries(read_text(REF_DUMP, "delete_f2.txt"))
     for X, c in zip(XS, cs):
         d = c.as_dict()
