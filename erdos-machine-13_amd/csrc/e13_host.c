@@ -164,7 +164,7 @@ static void tuning_defaults(e13_tuning *t) {
   t->mail_every = 1;
   t->trace_events = 0;
   t->graph_chain = 0;
-  t->solo_steps = 256;
+  t->solo_steps = 4096;
   t->evict_after = 0;
 }
 

@@ -276,76 +276,78 @@ __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int &
   return step_once<GATE>(L, c, evcell);
 }
 
-// run_slice for a wave with exactly one running lane (exec holds only that
-// lane): the same steps, exit test and hand-off to step_once, with every value
-// made wave-uniform so the loop runs on the scalar unit without exec
-// bookkeeping, the 16 rule rows in registers and the current tape word cached
-// (written back when the head crosses into another word).  A lone lane is
-// bound by the latency of one step, which this cuts; ones-count gate only.
+// run_slice for a wave with exactly one running lane, sl: the same steps,
+// exit test and hand-off to step_once, run by the whole wave in wave-uniform
+// control flow (every lane active) so that every value of the loop is uniform
+// and it runs on the scalar unit without exec bookkeeping.  Lane sl's state is
+// read once (v_readlane), its 16 rule rows sit in lanes 0..15 of one vector
+// register and are read at the state index (v_readlane with a scalar lane
+// select), the current tape word is cached (written back when the head
+// crosses into another word), and the rare steps are settled by step_once on
+// lane sl alone.  A lone lane is bound by the latency of one step, which this
+// cuts; ones-count gate only.
 __device__ __forceinline__ int solo_uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int lane_of(int v, int sl) { return __builtin_amdgcn_readlane(v, sl); }
 
 // Block macro-steps of the lone lane: the run of the machine inside one
-// aligned 4-cell block of tape (a nibble) until the head leaves it is a pure
-// function of (state, nibble, entry cell).  It is memoised in the rule-row
-// columns of the wave's idle lanes (free while one lane runs alone): entry
-// word = gen << 16 | new nibble << 12 | exit right << 11 | exit state << 7 |
-// steps (1..127; 0: the run reads a halt or an unbound cell, or stays in the
-// block for 127 steps).  gen (>= 0x8000, one per call) tells this call's
-// entries from older ones and from rule rows (whose halves stay below 0x4000).
+// aligned 8-cell block of tape (a byte), up to the step that leaves it or for
+// at most SOLO_STAY steps, is a pure function of (state, byte, entry cell).
+// It is memoised in a tagged table of 1024 slots hashed from that key, kept in
+// tape words 0..31 of the wave's idle lanes (free while one lane runs alone):
+// slot s has its tag (gen << 16 | key) in word s >> 6 and its value in word
+// 16 + (s >> 6) of lane s & 63 (lane sl's slots move to lane (s & 63) ^ 32).
+// value = new byte | state after << 8 | exit right << 12 | code << 16, where
+// code 1..SOLO_STAY: the run leaves the block after that many steps, on the
+// side the right bit names; code SOLO_STAY+1+p: it is still inside after
+// SOLO_STAY steps, at block cell p (a machine caught in a loop inside a few
+// cells); code 0: the run reads a halt or an unbound cell.
 #define SOLO_MEMO_GEN0 0x8000u
+#define SOLO_MEMO_ROWS 16
+#define SOLO_STAY 247
 
-// The next memo generation, advanced by the whole wave before a lone-lane round
-// (every lane keeps the same count); when the 15-bit range is used up, the idle
-// lanes clear their rule-row columns and the count starts again.
-__device__ __forceinline__ void solo_next_gen(uint32_t &gen, bool active, const Ctx &c, uint64_t act, uint64_t mach,
+// The memo generation, advanced by the whole wave before a lone-lane round
+// when the lone lane or the wave's fill count changed since the last one (a
+// refill rewrites idle lanes' tape words, where the memo lives); the idle
+// lanes then clear their tag words.  gen stays in 0x8001..0xFFFF, so a
+// cleared tag (0) never matches.
+__device__ __forceinline__ void solo_next_gen(uint32_t &gen, bool active, const Ctx &c, uint64_t act, uint64_t fills,
                                               uint64_t &last) {
-  // the memo stays valid while the same lane runs the same machine (a lane
-  // counts the machines it takes, 58 bits: no wrap)
   const int sl = __ffsll((long long)act) - 1;
-  const uint64_t m_lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)mach, sl);
-  const uint64_t m_hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(mach >> 32), sl);
-  const uint64_t who = ((uint64_t)sl << 58) | ((m_hi << 32 | m_lo) & ((1ull << 58) - 1));
-  const uint64_t prev = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(last >> 32)) << 32) |
-                        (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)last);
-  if (who == prev) return;
+  const uint64_t who = (fills << 6) | (uint64_t)sl;
+  if (who == last) return;
   last = who;
-  gen = (uint32_t)__builtin_amdgcn_readfirstlane((int)gen) + 1u;
-  if (gen > 0xFFFFu) {
-    gen = SOLO_MEMO_GEN0 + 1u;
-    if (!active)
-      for (int k = 0; k < 16; ++k) lds_st(c.sb, c.rbl + k * 256, 0u);
-  }
+  gen += 1u;
+  if (gen > 0xFFFFu) gen = SOLO_MEMO_GEN0 + 1u;
+  if (!active)
+    for (int w = 0; w < SOLO_MEMO_ROWS; ++w) lds_st(c.sb, c.tbl + w * 256, 0u);
 }
 
-__device__ __forceinline__ int run_solo(Lane &L, const Ctx &c, int nsteps, int &evcell, const uint32_t g) {
-  const int tbl = solo_uni(c.tbl), rbl = solo_uni(c.rbl);
-  const int me = solo_uni((int)(threadIdx.x & 63u));
-  const int rbase = rbl - me * 4; // the wave's rule-row area: word k of lane j at rbase + k*256 + j*4
-  int q = solo_uni(L.q);
-  int st = (solo_uni(L.row) - rbl) >> 8;
-  const int t_round = solo_uni(L.t);
+__device__ __forceinline__ int run_solo(Lane &L, const Ctx &c, int nsteps_in, int &evcell, const uint32_t g_in,
+                                        const int sl) {
+  const int lane = (int)(threadIdx.x & 63u);
+  const int nsteps = solo_uni(nsteps_in);
+  const uint32_t g = (uint32_t)solo_uni((int)g_in);
+  const int tbl = lane_of(c.tbl, sl), rbl = lane_of(c.rbl, sl);
+  const int tbase = tbl - sl * 4; // the wave's tape area: word w of lane j at tbase + w*256 + j*4
+  int q = lane_of(L.q, sl);
+  int st = (lane_of(L.row, sl) - rbl) >> 8;
+  const int t_round = lane_of(L.t, sl);
   if ((unsigned)q > (unsigned)E13_QMAX) { // a run_slice round ended on the step that left the window
-    L.t = t_round - 1;                     // that step does not count
+    if (lane == sl) L.t = t_round - 1;    // that step does not count
     return EV_DEAD;
   }
-  int left = solo_uni(L.tstop) - t_round;
-  int target = solo_uni(L.target), qA = solo_uni(L.qA), qR = solo_uni(L.qR);
-  int st0r = (solo_uni(L.row0) - rbl) >> 8;
-  bool nt1 = solo_uni(L.nt) == 1;
-  int ones = solo_uni(L.ones);
-  uint32_t rows[16]; // vector registers, read at the uniform state index (no copies per step)
-#pragma unroll
-  for (int k = 0; k < 16; ++k) rows[k] = lds_ld(c.sb, rbl + k * 256);
+  int left = lane_of(L.tstop, sl) - t_round;
+  int target = lane_of(L.target, sl), qA = lane_of(L.qA, sl), qR = lane_of(L.qR, sl);
+  int st0r = (lane_of(L.row0, sl) - rbl) >> 8;
+  bool nt1 = lane_of(L.nt, sl) == 1;
+  int ones = lane_of(L.ones, sl);
+  // rule row k of lane sl in lane k (k < 16; lanes 16..63 hold copies)
+  const uint32_t rowv = lds_ld(c.sb, rbl + (lane & 15) * 256);
   // sweep rules: (state, symbol) pairs whose rule rewrites the symbol read and
-  // keeps the state; bit 2*state + symbol
-  uint32_t sweep = 0;
-#pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    const uint32_t pr = (uint32_t)solo_uni((int)rows[k]);
-    const uint32_t e0 = pr & 0xFFFFu, e1 = pr >> 16;
-    sweep |= (((e0 & 0x0F00u) == ((uint32_t)k << 8)) & ((e0 & 1u) == 0u)) ? 1u << (2 * k) : 0u;
-    sweep |= (((e1 & 0x0F00u) == ((uint32_t)k << 8)) & ((e1 & 1u) == 1u)) ? 1u << (2 * k + 1) : 0u;
-  }
+  // keeps the state; bit `state` of sw0 (symbol 0) or sw1 (symbol 1)
+  const uint32_t kv = (uint32_t)(lane & 15) << 8, e0v = rowv & 0xFFFFu, e1v = rowv >> 16;
+  const uint32_t sw0 = (uint32_t)__ballot((lane < 16) & ((e0v & 0x0F00u) == kv) & ((e0v & 1u) == 0u));
+  const uint32_t sw1 = (uint32_t)__ballot((lane < 16) & ((e1v & 0x0F00u) == kv) & ((e1v & 1u) == 1u));
   int wi = q >> 5;
   uint32_t word = (uint32_t)solo_uni((int)lds_ld(c.sb, (wi << 8) + tbl));
   int n_eff = min(nsteps, left); // the deadline/cap test is due at r == left
@@ -353,9 +355,9 @@ __device__ __forceinline__ int run_solo(Lane &L, const Ctx &c, int nsteps, int &
   for (; r < n_eff; ++r) {
     const uint32_t qb = (uint32_t)q & 31u;
     const uint32_t sym = (word >> qb) & 1u;
-    const uint32_t e = (uint32_t)solo_uni((int)rows[st & 15]) >> (sym << 4);
+    const uint32_t e = (uint32_t)lane_of((int)rowv, st) >> (sym << 4);
     const uint32_t nx = e & 0x0F00u;
-    if ((sweep >> (2 * st + (int)sym)) & 1u) {
+    if ((((sym ? sw1 : sw0) >> st) & 1u) != 0u) {
       // a run of equal cells under a rule that rewrites them and keeps the state:
       // each cell of the run is one step that changes nothing but the head, and
       // with the ones count off its target no gate can fire inside it.  Take the
@@ -395,77 +397,88 @@ __device__ __forceinline__ int run_solo(Lane &L, const Ctx &c, int nsteps, int &
         continue;
       }
     }
-    // a block macro-step: away from the window edges and the block boundary at
-    // q 61 (ones are counted for q >= 61), with the ones count more than 4 from
-    // its target (a 4-cell run moves it by at most 4, so no gate can fire)
-    const int qn = q & ~3;
-    // (inside the block the head visits only its 4 cells: with neither gate cell
-    // among them, no gate can fire whatever the ones count)
-    const bool far = (ones - target > 4) | (target - ones > 4);
-    const bool no_gate_cell = !nt1 & ((qA >> 2) != (qn >> 2)) & ((qR >> 2) != (qn >> 2));
-    if ((qn >= 4) & (qn + 4 <= E13_QMAX) & (qn != 60) & (far | no_gate_cell)) {
-      const uint32_t nb = qb & ~3u, nib = (word >> nb) & 15u, pos = qb & 3u;
-      const int col = (int)((nib << 2) | pos);
-      if (col != me) {
-        const int ma = rbase + st * 256 + col * 4;
-        uint32_t ent = (uint32_t)solo_uni((int)lds_ld(c.sb, ma));
-        if ((ent >> 16) != g) { // not yet known this call: run the block once
-          uint32_t v = nib, s2 = (uint32_t)st;
-          int p = (int)pos, k2 = 0, right = 0, ok = 0;
-          while (k2 < 127) {
-            const uint32_t b = (v >> p) & 1u;
-            const uint32_t e2 = ((uint32_t)solo_uni((int)rows[s2 & 15u]) >> (b << 4)) & 0xFFFFu;
-            if ((e2 & 0x0F00u) == 0x0F00u) break; // halt or unbound read: not a plain run
-            v = (v & ~(1u << p)) | ((e2 & 1u) << p);
-            p += (int)(e2 & 2u) - 1;
-            s2 = e2 >> 8;
-            ++k2;
-            if ((unsigned)p > 3u) {
-              right = p > 3;
-              ok = 1;
-              break;
-            }
+    // a block macro-step: the byte of tape around the head, away from the
+    // window edges (a step off the window is not counted), with the ones count
+    // more than 8 from its target (an 8-cell run moves it by at most 8, so no
+    // gate can fire) or neither gate cell inside the block (the head visits
+    // only its 8 cells: no gate can fire whatever the ones count)
+    const int qn = q & ~7;
+    const bool far = (ones - target > 8) | (target - ones > 8);
+    const bool no_gate_cell = !nt1 & ((qA >> 3) != (qn >> 3)) & ((qR >> 3) != (qn >> 3));
+    if ((qn >= 8) & (qn + 8 <= E13_QMAX) & (far | no_gate_cell)) {
+      const uint32_t nb = qb & ~7u, byt = (word >> nb) & 255u, pos = qb & 7u;
+      const uint32_t key = ((uint32_t)st << 11) | (byt << 3) | pos;
+      const uint32_t slot = (key * 0x9E3779B1u) >> 22;
+      int col = (int)(slot & 63u);
+      if (col == sl) col ^= 32;
+      const int ta = tbase + (int)(slot >> 6) * 256 + col * 4, va = ta + SOLO_MEMO_ROWS * 256;
+      const uint32_t tag = (g << 16) | key;
+      const uint32_t t_ld = lds_ld(c.sb, ta), v_ld = lds_ld(c.sb, va);
+      uint32_t val = (uint32_t)solo_uni((int)v_ld);
+      if ((uint32_t)solo_uni((int)t_ld) != tag) { // not yet known: run the block once
+        uint32_t v = byt, s2 = (uint32_t)st;
+        int p = (int)pos, k2 = 0, right = 0, code = 0;
+        while (k2 < SOLO_STAY) {
+          const uint32_t b = (v >> p) & 1u;
+          const uint32_t e2 = ((uint32_t)lane_of((int)rowv, (int)s2) >> (b << 4)) & 0xFFFFu;
+          if ((e2 & 0x0F00u) == 0x0F00u) break; // halt or unbound read: not a plain run
+          v = (v & ~(1u << p)) | ((e2 & 1u) << p);
+          p += (int)(e2 & 2u) - 1;
+          s2 = e2 >> 8;
+          ++k2;
+          if ((unsigned)p > 7u) {
+            right = p > 7;
+            code = k2;
+            break;
           }
-          ent = (g << 16) | (v << 12) | ((uint32_t)right << 11) | (s2 << 7) | (uint32_t)(ok ? k2 : 0);
-          lds_st(c.sb, ma, ent);
         }
-        const int k3 = (int)(ent & 127u);
-        if ((k3 > 0) & (k3 <= n_eff - r)) {
-          const uint32_t nv = (ent >> 12) & 15u;
-          word = (word & ~(15u << nb)) | (nv << nb);
-          ones += qn >= E13_QBLK ? __builtin_popcount(nv) - __builtin_popcount(nib) : 0;
-          q = (ent & 0x800u) ? qn + 4 : qn - 1;
-          st = (int)((ent >> 7) & 15u);
-          r += k3 - 1; // the loop's ++r counts the last step
-          if ((q >> 5) != wi) {
-            lds_st(c.sb, (wi << 8) + tbl, word);
-            wi = q >> 5;
-            word = (uint32_t)solo_uni((int)lds_ld(c.sb, (wi << 8) + tbl));
-          }
-          continue;
+        if (k2 == SOLO_STAY && code == 0) code = SOLO_STAY + 1 + p; // still inside the block
+        val = v | (s2 << 8) | ((uint32_t)right << 12) | ((uint32_t)code << 16);
+        lds_st(c.sb, ta, tag);
+        lds_st(c.sb, va, val);
+      }
+      const int code = (int)(val >> 16), k3 = min(code, SOLO_STAY);
+      if ((code > 0) & (k3 <= n_eff - r)) {
+        const uint32_t nv = val & 255u;
+        word = (word & ~(255u << nb)) | (nv << nb);
+        // ones are counted for q >= 61: all of a block above, the upper cells of the block at q 56
+        const uint32_t cm = qn >= E13_QBLK ? 255u : (qn + 8 > E13_QBLK ? (255u << (E13_QBLK - qn)) & 255u : 0u);
+        ones += __builtin_popcount(nv & cm) - __builtin_popcount(byt & cm);
+        q = code > SOLO_STAY ? qn + (code - SOLO_STAY - 1) : ((val >> 12) & 1u) ? qn + 8 : qn - 1;
+        st = (int)((val >> 8) & 15u);
+        r += k3 - 1; // the loop's ++r counts the last step
+        if ((q >> 5) != wi) {
+          lds_st(c.sb, (wi << 8) + tbl, word);
+          wi = q >> 5;
+          word = (uint32_t)solo_uni((int)lds_ld(c.sb, (wi << 8) + tbl));
         }
+        continue;
       }
     }
-    if ((nx == 0x0F00u) | (ones == target)) { // rare: settle it exactly in step_once, in place
+    if ((nx == 0x0F00u) | (ones == target)) { // rare: settle it exactly in step_once, on lane sl
       if ((nx == 0x0F00u) | nt1 | ((st == st0r) & ((q == qA) | (q == qR)))) {
         lds_st(c.sb, (wi << 8) + tbl, word);
-        L.q = q;
-        L.row = rbl + st * 256;
-        L.ones = ones;
-        L.t = t_round + r;
-        const int ev1 = step_once<GATE_COUNT>(L, c, evcell);
-        if (ev1 != EV_NONE) return ev1; // the machine ended or stopped; L is current
+        int ev1 = EV_NONE;
+        if (lane == sl) {
+          L.q = q;
+          L.row = rbl + st * 256;
+          L.ones = ones;
+          L.t = t_round + r;
+          ev1 = step_once<GATE_COUNT>(L, c, evcell);
+        }
+        ev1 = lane_of(ev1, sl);
+        if (ev1 != EV_NONE) return ev1; // the machine ended or stopped; lane sl's state is current
         // a stage start may have been accepted: the gate and deadline registers moved
-        q = solo_uni(L.q);
-        st = (solo_uni(L.row) - rbl) >> 8;
-        ones = solo_uni(L.ones);
-        r = solo_uni(L.t) - t_round - 1; // the loop's ++r counts the step step_once made
-        target = solo_uni(L.target);
-        qA = solo_uni(L.qA);
-        qR = solo_uni(L.qR);
-        st0r = (solo_uni(L.row0) - rbl) >> 8;
-        nt1 = solo_uni(L.nt) == 1;
-        left = solo_uni(L.tstop) - t_round;
+        q = lane_of(L.q, sl);
+        st = (lane_of(L.row, sl) - rbl) >> 8;
+        ones = lane_of(L.ones, sl);
+        r = lane_of(L.t, sl) - t_round - 1; // the loop's ++r counts the step step_once made
+        target = lane_of(L.target, sl);
+        qA = lane_of(L.qA, sl);
+        qR = lane_of(L.qR, sl);
+        st0r = (lane_of(L.row0, sl) - rbl) >> 8;
+        nt1 = lane_of(L.nt, sl) == 1;
+        left = lane_of(L.tstop, sl) - t_round;
         n_eff = min(nsteps, left);
         wi = q >> 5;
         word = (uint32_t)solo_uni((int)lds_ld(c.sb, (wi << 8) + tbl));
@@ -488,13 +501,16 @@ __device__ __forceinline__ int run_solo(Lane &L, const Ctx &c, int nsteps, int &
     }
   }
   lds_st(c.sb, (wi << 8) + tbl, word);
-  L.q = q;
-  L.row = rbl + st * 256;
-  L.ones = ones;
-  L.t = t_round + r;
-  if (why == 2) return EV_DEAD;
-  if (r >= left) return step_once<GATE_COUNT>(L, c, evcell); // the deadline or cap is due
-  return EV_NONE;
+  int ev = EV_NONE;
+  if (lane == sl) {
+    L.q = q;
+    L.row = rbl + st * 256;
+    L.ones = ones;
+    L.t = t_round + r;
+    if (why == 2) ev = EV_DEAD;
+    else if (r >= left) ev = step_once<GATE_COUNT>(L, c, evcell); // the deadline or cap is due
+  }
+  return lane_of(ev, sl);
 }
 
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
@@ -866,10 +882,8 @@ __global__ void __launch_bounds__(256) k_batch(E13BatchArgs a) {
   int active = 0;
   uint32_t k = 0;
   uint64_t my_steps = 0, w_slots = 0, w_rounds = 0;
-  // rule rows start cleared: the lone-lane memo reads idle lanes' columns
-  for (int r0 = 0; r0 < 16; ++r0) lds_st(c.sb, c.rbl + r0 * 256, 0u);
   uint32_t solo_gen = SOLO_MEMO_GEN0;
-  uint64_t solo_last = ~0ull, mach = 0;
+  uint64_t solo_last = ~0ull, mach = 0; // mach: lane slots this wave has handed out (wave-uniform)
   bool exhausted = false;
   uint32_t ck = 0, ce = 0; // this wave's claimed table range
   if (!REFILL) { // plain form: one table per lane, no refill
@@ -914,9 +928,9 @@ __global__ void __launch_bounds__(256) k_batch(E13BatchArgs a) {
         L.tcap = 0x7FFFFFFF;
         arm_stage(L, c);
         active = 1;
-        ++mach; // a new machine in this lane
       }
       ck += give;
+      mach += give;
       need = ballot(!active);
     }
     const uint64_t act = ballot(active);
@@ -930,9 +944,13 @@ __global__ void __launch_bounds__(256) k_batch(E13BatchArgs a) {
     // a lone running lane takes the scalar loop (longer rounds: nothing to refill meanwhile
     // once the batch is exhausted)
     const bool solo = GATE == GATE_COUNT && a.solo > 0 && __popcll(act) == 1;
-    if (solo) solo_next_gen(solo_gen, active, c, act, mach, solo_last);
-    if (active)
-      ev = solo ? run_solo(L, c, exhausted ? a.solo : a.slice, evcell, solo_gen) : run_slice<GATE>(L, c, a.slice, evcell);
+    if (solo) { // the whole wave runs the lone lane's loop
+      solo_next_gen(solo_gen, active, c, act, mach, solo_last);
+      ev = run_solo(L, c, exhausted ? a.solo : a.slice, evcell, solo_gen, __ffsll((long long)act) - 1);
+      if (!active) ev = EV_NONE;
+    } else if (active) {
+      ev = run_slice<GATE>(L, c, a.slice, evcell);
+    }
     if (active && ev != EV_NONE) {
       active = 0;
       my_steps += (uint64_t)L.t;
@@ -1192,10 +1210,8 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
   L.tcap = 0x7FFFFFFF;
   int active = 0, t0 = 0, budget = 0, nintro = 0, prob = 0, depth = 0, nbound = 0, nq = 0;
   uint32_t seed = 0; // frontier node the lane's machine descends from
-  // rule rows start cleared: the lone-lane memo reads idle lanes' columns
-  for (int r0 = 0; r0 < 16; ++r0) lds_st(c.sb, c.rbl + r0 * 256, 0u);
   uint32_t solo_gen = SOLO_MEMO_GEN0;
-  uint64_t solo_last = ~0ull, mach = 0;
+  uint64_t solo_last = ~0ull, mach = 0; // mach: lane slots this wave has handed out (wave-uniform)
   long long top = 0, bot = 0, staged = -1;
   int cur_slot = 0, cur_end = 0; // the staged node's child slots still to hand out: [cur_slot, cur_end)
   int busy = 0, requested = 0, mail_in = 1, idle_polls = 0;
@@ -1313,11 +1329,11 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
             }
             for (int w = 0; w < E13_TW; ++w) lds_st(c.sb, c.tbl + w * 256, stg[E13_NODE_W_TAPE + w]);
             active = 1;
-            ++mach; // a new machine in this lane
           }
         }
       }
       cur_slot += give;
+      mach += (uint64_t)give;
       if (cur_slot >= cur_end) { // every child slot handed out: pop
         top -= 1;
         staged = -1;
@@ -1439,8 +1455,13 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
     // a lone running lane with nothing to hand out takes the scalar loop for a longer round
     int ev = EV_NONE, evcell = -1;
     const bool solo = GATE == GATE_COUNT && a.solo > 0 && __popcll(act) == 1 && top == bot;
-    if (solo) solo_next_gen(solo_gen, active, c, act, mach, solo_last);
-    if (active) ev = solo ? run_solo(L, c, a.solo, evcell, solo_gen) : run_slice<GATE>(L, c, a.slice, evcell);
+    if (solo) { // the whole wave runs the lone lane's loop
+      solo_next_gen(solo_gen, active, c, act, mach, solo_last);
+      ev = run_solo(L, c, a.solo, evcell, solo_gen, __ffsll((long long)act) - 1);
+      if (!active) ev = EV_NONE;
+    } else if (active) {
+      ev = run_slice<GATE>(L, c, a.slice, evcell);
+    }
     tn = stamp();
     cyc1 += tn - ts;
     ts = tn;

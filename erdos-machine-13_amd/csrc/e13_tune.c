@@ -41,7 +41,7 @@ static int32_t *knob(e13_tuning *t, int k) {
 
 /* Candidate values (-1: off / none, as tuning_merge reads it). */
 static const int32_t cand[NKNOB][5] = {
-    {1, 2, 3, 4, 0},      {16, 32, 64, 128, 0}, {1, 2, 4, 0, 0},  {-1, 64, 256, 1024, 0}, {-1, 2, 8, 0, 0},
+    {1, 2, 3, 4, 0},      {16, 32, 64, 128, 0}, {1, 2, 4, 0, 0},  {-1, 1024, 4096, 16384, 0}, {-1, 2, 8, 0, 0},
     {2, 4, 8, 16, 0},     {2, 8, 32, 0, 0},     {128, 256, 0, 0, 0}, {1, 4, 16, 0, 0}};
 
 static int same_counts(const e13_counts *a, const e13_counts *b, int np) {
