@@ -108,11 +108,11 @@ class Stats(ctypes.Structure):
     _fields_ = [("kernel_ms", ctypes.c_double), ("launches", ctypes.c_int64), ("rounds", ctypes.c_int64),
                 ("lane_slots", ctypes.c_int64), ("peak_stack", ctypes.c_int64), ("slice", ctypes.c_int32),
                 ("wg_threads", ctypes.c_int32), ("grid", ctypes.c_int32), ("gate", ctypes.c_int32),
-                ("form", ctypes.c_int32), ("pad", ctypes.c_int32), ("donations", ctypes.c_int64),
+                ("form", ctypes.c_int32), ("profile", ctypes.c_int32), ("donations", ctypes.c_int64),
                 ("pool_takes", ctypes.c_int64), ("cycles", ctypes.c_int64 * 4), ("graphs", ctypes.c_int64)]
 
     def as_dict(self):
-        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("pad", "cycles")}
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "cycles"}
         d["cycles"] = list(self.cycles)
         return d
 
@@ -123,7 +123,8 @@ EXPORTS = ("e13_open", "e13_close", "e13_strerror", "e13_set_tuning", "e13_get_t
            "e13_simulate_batch_device", "e13_sync", "e13_recheck", "e13_format_survivor", "e13_format_hash",
            "e13_synth_table", "e13_get_trace", "e13_batch_digest", "e13_synth_run", "e13_get_cost_log",
            "e13_cost_targets", "e13_mlp_init", "e13_mlp_forward", "e13_mlp_grad", "e13_mlp_train",
-           "e13_set_cost_model", "e13_autotune", "e13_save_tuning", "e13_load_tuning", "e13_tuning_file")
+           "e13_set_cost_model", "e13_autotune", "e13_save_tuning", "e13_load_tuning", "e13_tuning_file",
+           "e13_tuning_file_small", "e13_search_profile")
 
 _lib = None
 
@@ -189,6 +190,8 @@ def lib():
     L.e13_save_tuning.argtypes = [P(Tuning), ctypes.c_char_p]
     L.e13_load_tuning.argtypes = [ctypes.c_char_p, P(Tuning)]
     L.e13_tuning_file.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
+    L.e13_tuning_file_small.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
+    L.e13_search_profile.argtypes = [ctypes.c_void_p, P(Problem), ctypes.c_int, P(Tuning)]
     L.e13_get_trace.argtypes = [ctypes.c_void_p, P(ctypes.c_uint64), ctypes.c_uint64, P(ctypes.c_int32),
                                 P(ctypes.c_int32)]
     _lib = L
@@ -452,10 +455,22 @@ class Context:
                "e13_autotune")
         return t.as_dict(), {"trials": r.trials, "first_ms": r.first_ms, "best_ms": r.best_ms}
 
-    def tuning_file(self):
+    def tuning_file(self, small=False):
+        """Path of this device's stored settings (small=True: the small-search profile)."""
         buf = ctypes.create_string_buffer(1024)
-        _check(lib().e13_tuning_file(self._h, buf, 1024), "e13_tuning_file")
+        fn = lib().e13_tuning_file_small if small else lib().e13_tuning_file
+        _check(fn(self._h, buf, 1024), "e13_tuning_file")
         return buf.value.decode()
+
+    def search_profile(self, problems):
+        """(profile, settings) a persistent search of these problems runs with:
+        profile 0 = the main settings, 1 = the small-search profile."""
+        n = len(problems)
+        ps = (Problem * n)(*problems)
+        t = Tuning()
+        prof = lib().e13_search_profile(self._h, ps, n, ctypes.byref(t))
+        _check(min(prof, 0), "e13_search_profile")
+        return prof, t.as_dict()
 
     def sync(self):
         _check(lib().e13_sync(self._h), "e13_sync")

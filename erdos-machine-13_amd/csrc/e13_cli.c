@@ -7,7 +7,8 @@
  *   e13search_gpu freesym machine.txt X LO HI CELL...
  *   e13search_gpu synth   SEED0 K0 N      (the synthetic stress batch; one summary line)
  *   e13search_gpu tune    machine.txt FMAX X...   (autotune on that search; saves the settings
- *                                                  to $E13_TUNING_OUT or the device's tuning file)
+ *   e13search_gpu tune    machine.txt freesym X LO HI CELL...   to $E13_TUNING_OUT or to the
+ *                                       device's tuning file of the profile the search runs with)
  *
  * Argument meaning, stdout/stderr split and line formats follow
  * src/e13search.c main() (:435-472), report() (:184-200) and the summary at
@@ -153,11 +154,12 @@ static int run_one(e13_ctx *ctx, const int32_t *base, int nq_base, int X, int fm
   for (int j = 0; j <= E13_NS; j++) fprintf(stderr, " %llu", (unsigned long long)cnt.dead_at_stage[j]);
   fprintf(stderr,
           "\n  launches=%lld peak stack=%lld nodes kernel=%.1fms %.2e steps/s rounds=%lld lane-slots=%lld "
-          "donations=%lld pool-takes=%lld\nX=%d digest=%016llx\n",
+          "donations=%lld pool-takes=%lld settings=%s\nX=%d digest=%016llx\n",
           (long long)(st1.launches - st0.launches), (long long)st1.peak_stack, st1.kernel_ms - st0.kernel_ms,
           wall > 0 ? (double)cnt.steps / wall : 0.0, (long long)(st1.rounds - st0.rounds),
           (long long)(st1.lane_slots - st0.lane_slots), (long long)(st1.donations - st0.donations),
-          (long long)(st1.pool_takes - st0.pool_takes), X, (unsigned long long)digest);
+          (long long)(st1.pool_takes - st0.pool_takes), st1.profile == E13_PROFILE_SMALL ? "small-search" : "main", X,
+          (unsigned long long)digest);
   print_cycles(&st0, &st1);
   dump_trace(ctx);
   if (ns > cap) fprintf(stderr, "WARNING: %llu survivors, only %llu printed\n", (unsigned long long)ns,
@@ -225,9 +227,9 @@ static int run_many(e13_ctx *ctx, const int32_t *base, int nq_base, int nx, char
     fprintf(stderr, "\nX=%d digest=%016llx\n", xv[k], (unsigned long long)digest[k]);
     tot += cnt[k].steps;
   }
-  fprintf(stderr, "  one launch for %d problems: kernel=%.1fms %.2e steps/s, rounds=%lld lane-slots=%lld "
+  fprintf(stderr, "  one launch for %d problems (%s settings): kernel=%.1fms %.2e steps/s, rounds=%lld lane-slots=%lld "
                   "donations=%lld pool-takes=%lld\n",
-          np, st1.kernel_ms - st0.kernel_ms, wall > 0 ? (double)tot / wall : 0.0, (long long)(st1.rounds - st0.rounds),
+          np, st1.profile == E13_PROFILE_SMALL ? "small-search" : "main", st1.kernel_ms - st0.kernel_ms, wall > 0 ? (double)tot / wall : 0.0, (long long)(st1.rounds - st0.rounds),
           (long long)(st1.lane_slots - st0.lane_slots), (long long)(st1.donations - st0.donations),
           (long long)(st1.pool_takes - st0.pool_takes));
   print_cycles(&st0, &st1);
@@ -358,14 +360,31 @@ int main(int argc, char **argv) {
       run_many(ctx, base, nq, argc - 4, argv + 4, fmax);
     else
       for (int i = 4; i < argc; i++) run_one(ctx, base, nq, atoi(argv[i]), fmax, NULL, -1, -1);
-  } else if (!strcmp(mode, "tune")) { /* tune machine.txt FMAX X...: autotune on that delete search */
+  } else if (!strcmp(mode, "tune")) { /* tune machine.txt FMAX X... | tune machine.txt freesym X LO HI CELL... */
     if (argc < 5) return 1;
-    int fmax = atoi(argv[3]), np = 0;
+    int np = 0;
     e13_problem ps[16];
-    for (int i = 4; i < argc && np < 16; i++) {
-      e13_problem p;
-      if (e13_setup(base, nq, atoi(argv[i]), fmax, NULL, -1, -1, &p) == E13_OK) ps[np++] = p;
+    if (!strcmp(argv[3], "freesym")) { /* autotune on that free-cell search */
+      if (argc < 8) return 1;
+      uint8_t fb[32] = {0};
+      for (int i = 7; i < argc; i++) {
+        int q = argv[i][0] - 'A', s = argv[i][1] - '0';
+        if (q < 0 || q >= nq || (s != 0 && s != 1)) {
+          fprintf(stderr, "bad cell %s\n", argv[i]);
+          return 1;
+        }
+        fb[2 * q + s] = 1;
+      }
+      if (e13_setup(base, nq, atoi(argv[4]), 0, fb, atoi(argv[5]), atoi(argv[6]), &ps[0]) == E13_OK) np = 1;
+    } else { /* autotune on that delete search */
+      int fmax = atoi(argv[3]);
+      for (int i = 4; i < argc && np < 16; i++) {
+        e13_problem p;
+        if (e13_setup(base, nq, atoi(argv[i]), fmax, NULL, -1, -1, &p) == E13_OK) ps[np++] = p;
+      }
     }
+    /* the stored profile these problems run with is the one the result replaces */
+    const int profile = np ? e13_search_profile(ctx, ps, np, NULL) : E13_PROFILE_MAIN;
     e13_tuning best;
     e13_tune_report rep;
     rc = np ? e13_autotune(ctx, ps, np, 2, &best, &rep) : E13_EINVAL;
@@ -376,11 +395,14 @@ int main(int argc, char **argv) {
     char path[1024];
     const char *out = getenv("E13_TUNING_OUT");
     if (out && *out) snprintf(path, sizeof path, "%s", out);
-    else if (e13_tuning_file(ctx, path, sizeof path)) path[0] = 0;
-    fprintf(stderr, "autotune: %d trials, %.1f ms -> %.1f ms; wgs_per_cu=%d slice=%d refill_tries=%d solo_steps=%d "
-                    "seed_per_wave=%d donate_max=%d sched_every=%d wg_threads=%d idle_naps=%d\n",
-            rep.trials, rep.first_ms, rep.best_ms, best.wgs_per_cu, best.slice, best.refill_tries, best.solo_steps,
-            best.seed_per_wave, best.donate_max, best.sched_every, best.wg_threads, best.idle_naps);
+    else if ((profile == E13_PROFILE_SMALL ? e13_tuning_file_small : e13_tuning_file)(ctx, path, sizeof path))
+      path[0] = 0;
+    fprintf(stderr, "autotune (%s profile): %d trials, %.1f ms -> %.1f ms; wgs_per_cu=%d slice=%d refill_tries=%d "
+                    "solo_steps=%d seed_per_wave=%d donate_max=%d sched_every=%d wg_threads=%d idle_naps=%d "
+                    "evict_after=%d\n",
+            profile == E13_PROFILE_SMALL ? "small-search" : "main", rep.trials, rep.first_ms, rep.best_ms,
+            best.wgs_per_cu, best.slice, best.refill_tries, best.solo_steps, best.seed_per_wave, best.donate_max,
+            best.sched_every, best.wg_threads, best.idle_naps, best.evict_after);
     if (path[0] && e13_save_tuning(&best, path) == E13_OK) fprintf(stderr, "autotune: saved to %s\n", path);
   } else if (!strcmp(mode, "free") || !strcmp(mode, "freesym")) {
     if (argc < 4) return 1;

@@ -23,6 +23,7 @@ size_t e13k_lds_bytes(int wg_threads, int gate);
 int e13k_launch_expand(const E13ExpandArgs *a, int grid, int wg_threads, int gate, int refill, hipStream_t s);
 int e13k_launch_batch(const E13BatchArgs *a, int grid, int wg_threads, int gate, int refill, hipStream_t s);
 size_t e13k_search_lds_bytes(int wg_threads, int gate, int np);
+int e13k_search_max_wgs(int wg_threads, int gate, int np);
 int e13k_launch_search(const E13SearchArgs *a, int grid, int wg_threads, int gate, hipStream_t s);
 int e13k_launch_chain_pop(E13Chain *ch, E13Counters *ctr, hipStream_t s);
 int e13k_launch_chain_push(E13Chain *ch, E13Node *stack, uint64_t stack_cap, const E13Node *out, uint32_t out_cap,
@@ -38,6 +39,8 @@ struct e13_ctx {
   hipStream_t stream;
   hipEvent_t ev0, ev1;
   e13_tuning tune;
+  e13_tuning tune_small; /* settings for small searches (valid when has_small) */
+  int has_small, auto_profile; /* auto_profile: no e13_set_tuning since e13_open */
   e13_stats stats;
   E13Counters *d_ctr;
   uint64_t *d_key, *d_tgt;
@@ -98,6 +101,6 @@ E13_INTERNAL void e13i_node_features(const e13_problem *p, const E13Node *nd, fl
 E13_INTERNAL int e13i_mlp_predict_dev(e13_ctx *c, const float *x_host, int n, float *y_host);
 E13_INTERNAL void e13i_free_cost(e13_ctx *c);
 /* stored per-device tuning (e13_tune.c) */
-E13_INTERNAL int e13i_tuning_path(const char *arch, int ncu, char *buf, int len);
+E13_INTERNAL int e13i_tuning_path(const char *arch, int ncu, int small, char *buf, int len);
 
 #endif

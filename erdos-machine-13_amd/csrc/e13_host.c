@@ -158,14 +158,14 @@ static void tuning_defaults(e13_tuning *t) {
   t->refill_tries = 2;
   t->stack_nodes = 1LL << 25;
   t->seed_per_wave = 0;
-  t->sched_every = 8;
+  t->sched_every = 32;
   t->donate_max = 4;
   t->idle_naps = 4;
   t->mail_every = 1;
   t->trace_events = 0;
   t->graph_chain = 0;
   t->solo_steps = 4096;
-  t->evict_after = 0;
+  t->evict_after = 16384;
 }
 
 static void tuning_merge(e13_tuning *dst, const e13_tuning *src) {
@@ -237,17 +237,29 @@ int e13_open(int device, const e13_tuning *t, e13_ctx **out) {
   c->ncu = prop.multiProcessorCount;
   c->lds_max = prop.sharedMemPerBlock;
   snprintf(c->arch, sizeof c->arch, "%s", prop.gcnArchName);
-  /* defaults, then this device's stored (autotuned) settings, then the caller's */
+  /* defaults, then this device's stored (autotuned) settings, then the caller's;
+     the small-search profile likewise, over the stored main settings */
   tuning_defaults(&c->tune);
   char path[1024];
   e13_tuning stored;
-  if (e13i_tuning_path(c->arch, c->ncu, path, sizeof path) && e13_load_tuning(path, &stored) == E13_OK) {
+  if (e13i_tuning_path(c->arch, c->ncu, 0, path, sizeof path) && e13_load_tuning(path, &stored) == E13_OK) {
     e13_tuning tmp = c->tune;
     tuning_merge(&tmp, &stored);
     if (tuning_valid(&tmp)) c->tune = tmp;
   }
+  c->tune_small = c->tune;
+  if (e13i_tuning_path(c->arch, c->ncu, 1, path, sizeof path) && e13_load_tuning(path, &stored) == E13_OK) {
+    e13_tuning tmp = c->tune_small;
+    tuning_merge(&tmp, &stored);
+    if (tuning_valid(&tmp)) {
+      c->tune_small = tmp;
+      c->has_small = 1;
+    }
+  }
+  c->auto_profile = 1;
   tuning_merge(&c->tune, t);
-  if (!tuning_valid(&c->tune)) {
+  tuning_merge(&c->tune_small, t);
+  if (!tuning_valid(&c->tune) || !tuning_valid(&c->tune_small)) {
     free(c);
     return E13_EINVAL;
   }
@@ -316,7 +328,26 @@ int e13_set_tuning(e13_ctx *c, const e13_tuning *t) {
   tuning_merge(&n, t);
   if (!tuning_valid(&n)) return E13_EINVAL;
   c->tune = n;
+  c->auto_profile = 0; /* installed settings hold for every search from now on */
   return E13_OK;
+}
+
+/* Edit depth of a problem: the cells a search binds with a free choice of
+   values (E13_FORCED: free cells and rules into the deleted state) plus the
+   edits it may make to the others. */
+static int edit_depth(const e13_problem *p) {
+  int d = p->budget;
+  for (int i = 0; i < 2 * p->nq && i < E13_MAXCELLS; i++) d += p->tab[i] == E13_FORCED;
+  return d;
+}
+
+int e13_search_profile(e13_ctx *c, const e13_problem *ps, int np, e13_tuning *t) {
+  if (!c || !ps || np < 1) return E13_EINVAL;
+  int small = c->has_small && c->auto_profile;
+  for (int k = 0; k < np && small; k++)
+    if (edit_depth(&ps[k]) > E13_SMALL_DEPTH) small = 0;
+  if (t) *t = small ? c->tune_small : c->tune;
+  return small ? E13_PROFILE_SMALL : E13_PROFILE_MAIN;
 }
 
 int e13_get_tuning(e13_ctx *c, e13_tuning *t) {
@@ -785,11 +816,18 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
   for (int k = 0; k < np; k++)
     if (!check_problem(&ps[k])) return E13_EINVAL;
   HIPCHK(hipSetDevice(c->device));
-  const e13_tuning *tu = &c->tune;
+  const int profile = e13_search_profile(c, ps, np, NULL);
+  const e13_tuning *tu = profile == E13_PROFILE_SMALL ? &c->tune_small : &c->tune;
+  c->stats.profile = profile;
   int wg = tu->wg_threads;
   while (wg > 64 && e13k_search_lds_bytes(wg, tu->gate == 2, np) > c->lds_max) wg -= 64;
   if (e13k_search_lds_bytes(wg, tu->gate == 2, np) > c->lds_max) return E13_EINVAL;
-  const int grid = c->ncu * tu->wgs_per_cu;
+  /* every workgroup of the persistent kernel must be resident at once: a wave
+     that could start only after others exit would hold its frontier share and
+     its place in the busy count while they wait for it */
+  const int fit = e13k_search_max_wgs(wg, tu->gate == 2, np);
+  if (fit < 1) return E13_EINVAL;
+  const int grid = c->ncu * (tu->wgs_per_cu < fit ? tu->wgs_per_cu : fit);
   const int total_waves = grid * (wg / 64);
   const int stk_cap = tu->stack_nodes >= 64 * (int64_t)total_waves ? (int)(tu->stack_nodes / total_waves) : 64;
   const size_t need_stacks = (size_t)total_waves * (size_t)stk_cap;

@@ -1679,6 +1679,18 @@ size_t e13k_search_lds_bytes(int wg_threads, int gate, int np) {
          (size_t)np * E13_PCNT * 8;
 }
 
+// Workgroups of k_search that fit on one CU at once (registers, LDS, wave
+// slots); -1 when the runtime cannot tell.
+int e13k_search_max_wgs(int wg_threads, int gate, int np) {
+  const size_t lds = e13k_search_lds_bytes(wg_threads, gate, np);
+  int n = 0;
+  const hipError_t e =
+      gate == GATE_HASH
+          ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_search<GATE_HASH>, wg_threads, lds)
+          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_search<GATE_COUNT>, wg_threads, lds);
+  return e == hipSuccess ? n : -1;
+}
+
 int e13k_launch_search(const E13SearchArgs *a, int grid, int wg_threads, int gate, hipStream_t s) {
   const size_t lds = e13k_search_lds_bytes(wg_threads, gate, a->np);
   if (gate == GATE_HASH) hipLaunchKernelGGL((k_search<GATE_HASH>), dim3(grid), dim3(wg_threads), lds, s, *a);

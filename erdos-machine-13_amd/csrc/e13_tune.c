@@ -9,8 +9,9 @@
  * dependency chain, so a leaf has no work for a second lane), the refill
  * granularity (steps per round, refill passes), the tape placement of a lone
  * running lane (LDS loop only, or the register loop with its round length),
- * and the work-queue depth (frontier per wave, nodes per delivery pass, rounds
- * between hungry-map reads).  Every trial's counts must equal the first
+ * the work-queue depth (frontier per wave, nodes per delivery pass, rounds
+ * between hungry-map reads) and the step count after which a machine is handed
+ * to an idle wave to run alone.  Every trial's counts must equal the first
  * trial's; the settings change the schedule, never the result.
  */
 #define _GNU_SOURCE 1 /* dladdr */
@@ -21,9 +22,9 @@
 
 #include "e13_ctx.h"
 
-#define NKNOB 9
+#define NKNOB 10
 static const char *knob_name[NKNOB] = {"wgs_per_cu", "slice",       "refill_tries", "solo_steps", "seed_per_wave",
-                                       "donate_max", "sched_every", "wg_threads",   "idle_naps"};
+                                       "donate_max", "sched_every", "wg_threads",   "idle_naps",  "evict_after"};
 
 static int32_t *knob(e13_tuning *t, int k) {
   switch (k) {
@@ -35,14 +36,15 @@ static int32_t *knob(e13_tuning *t, int k) {
   case 5: return &t->donate_max;
   case 6: return &t->sched_every;
   case 7: return &t->wg_threads;
-  default: return &t->idle_naps;
+  case 8: return &t->idle_naps;
+  default: return &t->evict_after;
   }
 }
 
 /* Candidate values (-1: off / none, as tuning_merge reads it). */
 static const int32_t cand[NKNOB][5] = {
     {1, 2, 3, 4, 0},      {16, 32, 64, 128, 0}, {1, 2, 4, 0, 0},  {-1, 1024, 4096, 16384, 0}, {-1, 2, 8, 0, 0},
-    {2, 4, 8, 16, 0},     {2, 8, 32, 0, 0},     {128, 256, 0, 0, 0}, {1, 4, 16, 0, 0}};
+    {2, 4, 8, 16, 0},     {8, 32, 128, 0, 0},   {128, 256, 0, 0, 0}, {1, 4, 16, 0, 0},        {-1, 4096, 16384, 65536, 0}};
 
 static int same_counts(const e13_counts *a, const e13_counts *b, int np) {
   for (int k = 0; k < np; k++) {
@@ -59,6 +61,7 @@ static int trial(e13_ctx *c, const e13_tuning *t, const e13_problem *ps, int np,
   e13_tuning t2 = *t; /* "off" is 0 in a read-back tuning, -1 when set */
   if (t2.solo_steps == 0) t2.solo_steps = -1;
   if (t2.seed_per_wave == 0) t2.seed_per_wave = -1;
+  if (t2.evict_after == 0) t2.evict_after = -1;
   int rc = e13_set_tuning(c, &t2);
   if (rc) return rc;
   e13_counts cnt[E13_MAXPROB];
@@ -84,15 +87,18 @@ static int trial(e13_ctx *c, const e13_tuning *t, const e13_problem *ps, int np,
 
 int e13_autotune(e13_ctx *c, const e13_problem *ps, int np, int reps, e13_tuning *best, e13_tune_report *rep) {
   if (!c || !ps || np < 1 || np > E13_MAXPROB || reps < 1) return E13_EINVAL;
-  e13_tuning cur, start;
-  e13_get_tuning(c, &start);
+  e13_tuning cur, start, installed;
+  e13_get_tuning(c, &installed);
+  e13_search_profile(c, ps, np, &start); /* the profile these problems would run with */
   cur = start;
   cur.form = 3;
   e13_counts ref[E13_MAXPROB];
   int have_ref = 0, trials = 0, rc;
   double best_ms, ms;
+  const int log = getenv("E13_TUNE_LOG") && *getenv("E13_TUNE_LOG") != '0'; /* a line per trial on stderr */
   if ((rc = trial(c, &cur, ps, np, reps, ref, &have_ref, &best_ms))) goto fail;
   trials++;
+  if (log) fprintf(stderr, "autotune: start -> %.1f ms\n", best_ms);
   const double first_ms = best_ms;
   for (int pass = 0; pass < 2; pass++) {
     int changed = 0;
@@ -105,8 +111,10 @@ int e13_autotune(e13_ctx *c, const e13_problem *ps, int np, int reps, e13_tuning
         if (v == keep || (v < 0 && keep == 0)) continue;
         e13_tuning t = cur;
         *knob(&t, k) = v;
+        if (log) fprintf(stderr, "autotune: %s=%d ...", knob_name[k], (int)v);
         if ((rc = trial(c, &t, ps, np, reps, ref, &have_ref, &ms))) goto fail;
         trials++;
+        if (log) fprintf(stderr, " %.1f ms\n", ms);
         if (ms < 0.98 * best_ms) { /* a clear gain only: timing noise does not move the choice */
           best_ms = ms;
           pick = v;
@@ -119,6 +127,7 @@ int e13_autotune(e13_ctx *c, const e13_problem *ps, int np, int reps, e13_tuning
   }
   if (cur.solo_steps == 0) cur.solo_steps = -1;
   if (cur.seed_per_wave == 0) cur.seed_per_wave = -1;
+  if (cur.evict_after == 0) cur.evict_after = -1;
   if ((rc = e13_set_tuning(c, &cur))) goto fail;
   e13_get_tuning(c, &cur);
   if (best) *best = cur;
@@ -130,7 +139,7 @@ int e13_autotune(e13_ctx *c, const e13_problem *ps, int np, int reps, e13_tuning
   }
   return E13_OK;
 fail:
-  e13_set_tuning(c, &start);
+  e13_set_tuning(c, &installed);
   return rc;
 }
 
@@ -142,7 +151,7 @@ int e13_save_tuning(const e13_tuning *t, const char *path) {
   fprintf(f, "# e13 tuning (autotuned): name value\n");
   for (int k = 0; k < NKNOB; k++) {
     int32_t v = *knob(&tmp, k);
-    if (k == 3 || k == 4) v = v > 0 ? v : -1; /* off is written as -1, which tuning_merge reads as off */
+    if (k == 3 || k == 4 || k == 9) v = v > 0 ? v : -1; /* off is written as -1, which tuning_merge reads as off */
     fprintf(f, "%s %d\n", knob_name[k], (int)v);
   }
   fprintf(f, "gate %d\n", (int)t->gate);
@@ -169,11 +178,25 @@ int e13_load_tuning(const char *path, e13_tuning *t) {
 }
 
 /* Where this device's stored tuning lives: $E13_TUNING_FILE, else
-   <library dir>/../tuning/<arch>_<cus>cu.txt. */
-int e13i_tuning_path(const char *arch, int ncu, char *buf, int len) {
+   <library dir>/../tuning/<arch>_<cus>cu.txt; the small-search profile at the
+   same path with _small before the extension. */
+static void small_name(char *buf, int len) {
+  char tmp[1024];
+  snprintf(tmp, sizeof tmp, "%s", buf);
+  char *dot = strrchr(tmp, '.'), *slash = strrchr(tmp, '/');
+  if (dot && (!slash || dot > slash)) {
+    *dot = 0;
+    snprintf(buf, (size_t)len, "%s_small.%s", tmp, dot + 1);
+  } else {
+    snprintf(buf, (size_t)len, "%s_small", tmp);
+  }
+}
+
+int e13i_tuning_path(const char *arch, int ncu, int small, char *buf, int len) {
   const char *env = getenv("E13_TUNING_FILE");
   if (env && *env) {
     snprintf(buf, (size_t)len, "%s", env);
+    if (small) small_name(buf, len);
     return 1;
   }
   Dl_info info;
@@ -185,10 +208,16 @@ int e13i_tuning_path(const char *arch, int ncu, char *buf, int len) {
   char *colon = strchr(a, ':');
   if (colon) *colon = 0;
   snprintf(buf, (size_t)len, "%s/../tuning/%s_%dcu.txt", dirname(tmp), a, ncu);
+  if (small) small_name(buf, len);
   return 1;
 }
 
 int e13_tuning_file(e13_ctx *c, char *buf, int len) {
   if (!c || !buf || len < 2) return E13_EINVAL;
-  return e13i_tuning_path(c->arch, c->ncu, buf, len) ? E13_OK : E13_EINVAL;
+  return e13i_tuning_path(c->arch, c->ncu, 0, buf, len) ? E13_OK : E13_EINVAL;
+}
+
+int e13_tuning_file_small(e13_ctx *c, char *buf, int len) {
+  if (!c || !buf || len < 2) return E13_EINVAL;
+  return e13i_tuning_path(c->arch, c->ncu, 1, buf, len) ? E13_OK : E13_EINVAL;
 }

@@ -114,7 +114,8 @@ typedef struct {
   int64_t rounds;        /* wave rounds */
   int64_t lane_slots;    /* active lanes summed over rounds */
   int64_t peak_stack;
-  int32_t slice, wg_threads, grid, gate, form, pad;
+  int32_t slice, wg_threads, grid, gate, form;
+  int32_t profile;       /* settings profile of the last persistent search (E13_PROFILE_*) */
   int64_t donations, pool_takes; /* persistent form: nodes moved between waves */
   int64_t cycles[4];             /* wave clock cycles spent refilling lanes, stepping, finishing
                                     machines, and on the work pool / idle (summed over waves) */
@@ -216,12 +217,20 @@ int e13_synth_run(e13_ctx *ctx, uint64_t seed0, uint64_t k0, int64_t n, e13_batc
    batch size, e13gpu.py:427-430).  Coordinate descent over resident waves per
    CU, steps per round, refill passes, the lone-lane tape placement (LDS loop or
    register loop), frontier per wave, nodes per delivery pass, hungry-map read
-   interval, workgroup size and idle poll interval, each trial a persistent
-   search of ps[0..np) (best of reps).  Every trial's counts must equal the
-   first's (E13_EINVAL otherwise).  The chosen settings are installed in ctx.
+   interval, workgroup size, idle poll interval and the step count after which
+   a machine goes to an idle wave to run alone, each trial a persistent search
+   of ps[0..np) (best of reps), starting from the profile those problems use.
+   Every trial's counts must equal the first's (E13_EINVAL otherwise).  The
+   chosen settings are installed in ctx.
    e13_open loads a device's stored settings from $E13_TUNING_FILE or from
-   <library dir>/../tuning/<arch>_<CUs>cu.txt (e13_tuning_file) when present;
-   fields the caller sets override them. ---- */
+   <library dir>/../tuning/<arch>_<CUs>cu.txt (e13_tuning_file) when present,
+   and a second profile for small searches from the same path with _small
+   before the extension (e13_tuning_file_small); fields the caller sets
+   override both.  A small search -- every problem of edit depth (budget plus
+   the E13_FORCED cells: free cells and rules into the deleted state) at most
+   E13_SMALL_DEPTH, about 1e8 leaves or fewer, which is bound by its longest
+   machines rather than by throughput -- runs with the small profile unless
+   settings were installed with e13_set_tuning since e13_open. ---- */
 typedef struct {
   int32_t trials, pad;
   double first_ms, best_ms; /* kernel time with the starting and the chosen settings */
@@ -231,6 +240,13 @@ int e13_autotune(e13_ctx *ctx, const e13_problem *ps, int np, int reps, e13_tuni
 int e13_save_tuning(const e13_tuning *t, const char *path);
 int e13_load_tuning(const char *path, e13_tuning *t);
 int e13_tuning_file(e13_ctx *ctx, char *buf, int len);
+int e13_tuning_file_small(e13_ctx *ctx, char *buf, int len);
+
+#define E13_SMALL_DEPTH 3
+enum { E13_PROFILE_MAIN = 0, E13_PROFILE_SMALL = 1 };
+/* The settings profile a persistent search of ps[0..np) runs with (E13_PROFILE_MAIN or
+   E13_PROFILE_SMALL); t (optional) receives its settings. */
+int e13_search_profile(e13_ctx *ctx, const e13_problem *ps, int np, e13_tuning *t);
 
 /* ---- Subtree cost model (no reference counterpart; the nearest is the Knuth
    estimator probe(), e13search.c:256-313, which estimates the same two

@@ -265,3 +265,36 @@ def test_autotune_keeps_results(e13, gpu_ctx, base):
         d = c.as_dict()
         assert (d["leaves"], d["steps"], d["hist"]) == (golden[X]["leaves"], golden[X]["steps"], golden[X]["hist"])
 
+
+
+def test_search_profiles(e13, base):
+    """A context picks the stored small-search profile for searches of edit
+    depth <= 3 and the main settings for deeper ones; settings installed with
+    set_tuning then hold for every search.  Counts are the reference's under
+    either profile."""
+    import os
+    say("profiles: delete FMAX=3 X=0,1 (small) and freesym X=13 (main) on a fresh context")
+    small = [e13.setup(base, X, 3) for X in (0, 1)]
+    large = [e13.setup(base, 13, 0, "D0 G0 K0 K1 L1 M1".split(), (10, 13))]
+    with e13.Context(0) as ctx:
+        path = ctx.tuning_file(small=True)
+        assert path.endswith("_small.txt") and path.replace("_small.txt", ".txt") == ctx.tuning_file()
+        has_small = os.path.exists(path)
+        prof, t = ctx.search_profile(small)
+        assert prof == (1 if has_small else 0)
+        if has_small:
+            stored = e13.load_tuning(path)
+            for k, v in stored.items():
+                if v > 0:
+                    assert t[k] == v, k
+        assert ctx.search_profile(large)[0] == 0
+        cs, _, _ = ctx.search_many(small)
+        assert ctx.stats()["profile"] == prof
+        golden = parse_summaries(read_text(REF_DUMP, "delete_f3.txt"))
+        for X, c in zip((0, 1), cs):
+            d = c.as_dict()
+            assert (d["leaves"], d["steps"], d["hist"]) == (golden[X]["leaves"], golden[X]["steps"], golden[X]["hist"])
+        ctx.set_tuning(slice=ctx.tuning()["slice"])
+        assert ctx.search_profile(small)[0] == 0
+        ctx.search_many(small)
+        assert ctx.stats()["profile"] == 0
