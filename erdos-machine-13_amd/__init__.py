@@ -109,7 +109,8 @@ class Stats(ctypes.Structure):
                 ("lane_slots", ctypes.c_int64), ("peak_stack", ctypes.c_int64), ("slice", ctypes.c_int32),
                 ("wg_threads", ctypes.c_int32), ("grid", ctypes.c_int32), ("gate", ctypes.c_int32),
                 ("form", ctypes.c_int32), ("profile", ctypes.c_int32), ("donations", ctypes.c_int64),
-                ("pool_takes", ctypes.c_int64), ("cycles", ctypes.c_int64 * 4), ("graphs", ctypes.c_int64)]
+                ("pool_takes", ctypes.c_int64), ("cycles", ctypes.c_int64 * 4), ("graphs", ctypes.c_int64),
+                ("search_ms", ctypes.c_double), ("search_launches", ctypes.c_int64), ("search_steps", ctypes.c_int64)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k != "cycles"}

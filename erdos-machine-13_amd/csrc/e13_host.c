@@ -1038,8 +1038,11 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
     return E13_ESTACK;
   }
   if (logging && (rc = collect_cost_log(c, ps, nseeds))) return rc;
+  c->stats.search_ms += ms;
+  c->stats.search_launches++;
   for (int k = 0; k < np; k++) {
     const uint64_t *q = pcnt + k * E13_PCNT;
+    c->stats.search_steps += (int64_t)q[1];
     out[k].leaves += q[0];
     out[k].steps += q[1];
     out[k].survivors += q[2];

@@ -120,6 +120,9 @@ typedef struct {
   int64_t cycles[4];             /* wave clock cycles spent refilling lanes, stepping, finishing
                                     machines, and on the work pool / idle (summed over waves) */
   int64_t graphs;                /* hipGraph launches (graph_chain > 0) */
+  double search_ms;              /* the persistent kernel alone (k_search): HIP-event time, */
+  int64_t search_launches;       /* launches, */
+  int64_t search_steps;          /* and the steps simulated inside it */
 } e13_stats;
 
 typedef struct e13_ctx e13_ctx;
@@ -233,7 +236,7 @@ int e13_synth_run(e13_ctx *ctx, uint64_t seed0, uint64_t k0, int64_t n, e13_batc
    settings were installed with e13_set_tuning since e13_open. ---- */
 typedef struct {
   int32_t trials, pad;
-  double first_ms, best_ms; /* kernel time with the starting and the chosen settings */
+  double first_ms, best_ms; /* wall time of the search call with the starting and the chosen settings */
 } e13_tune_report;
 
 int e13_autotune(e13_ctx *ctx, const e13_problem *ps, int np, int reps, e13_tuning *best, e13_tune_report *rep);
