@@ -249,3 +249,26 @@ def test_product_path_has_no_oracle_import():
             if fn.endswith((".py", ".c", ".h", ".hip", "Makefile")):
                 text = read_text(dirpath, fn)
                 assert "oracle" not in text.lower() or fn == "__init__.py" and "oracle" not in text, fn
+
+
+def test_lone_lane_loop_accelerations_match_plain_stepping(tmp_path):
+    """The register loop's sweeps and byte-block memo, restated in C
+    (tests/solo_loop_check.c), give exactly what plain stepping gives on random
+    machines, tapes, heads and gate registers; each of three planted slips
+    (ones mask of the block at q 56..63, gate margin 4 instead of 8, a stay
+    position off by one) is caught."""
+    import subprocess
+    src = os.path.join(ROOT, "tests", "solo_loop_check.c")
+    for plant in (0, 1, 2, 3):
+        exe = str(tmp_path / f"solo_loop_check_{plant}")
+        subprocess.run(["gcc", "-O2", f"-DPLANT={plant}", "-o", exe, src], check=True)
+        for seed in ((7, 11) if plant == 0 else (7,)):
+            r = subprocess.run([exe, str(seed), "100000"], capture_output=True, text=True, timeout=120)
+            m = re.search(r"runs=(\d+) mismatches=(\d+) stays=(\d+)", r.stdout)
+            assert m, r.stdout
+            runs, bad, stays = map(int, m.groups())
+            assert runs == 100000 and stays > 10000
+            if plant == 0:
+                assert bad == 0 and r.returncode == 0, r.stdout
+            else:
+                assert bad > 0 and r.returncode == 1, (plant, r.stdout)
