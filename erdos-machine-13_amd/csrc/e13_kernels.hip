@@ -228,24 +228,24 @@ __device__ __forceinline__ int step_once(Lane &L, const Ctx &c, int &evcell) {
 // gate may fire (ones count and stage-1 state/head prefilter, or the block
 // hash), the deadline/cap is due, or the cell read is unbound or a halt.  Any
 // of them ends the loop and step_once settles the step exactly; everything
-// else runs branch-free.  Within a round every running lane takes one step
-// per iteration, so t is the round's start count plus the (wave-uniform)
-// iteration index and needs no per-lane increment.
+// else runs branch-free.  A lane that ran the whole round has t = its start
+// count + slice; one that left early has fewer.
 template <int GATE>
 __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int &evcell) {
-  const int t_round = L.t;
-  const int left = L.tstop - t_round; // iterations before the deadline/cap test is due
+  const int t_end = L.t + slice;
   const bool nt1 = L.nt == 1;
-  int r = 0;
-  for (; r < slice; ++r) {
-    const uint32_t qb = (uint32_t)L.q & 31u;
+  int t = L.t;
+  // tape word address (q >> 5) * 256 + tbl as one shift-add and one and-or
+  const uint32_t tbl_lo = (uint32_t)c.tbl & 0xFFu, tbl_hi = (uint32_t)c.tbl - tbl_lo;
+  for (int r = 0; r < slice; ++r) {
     // q may be one step outside the window here (q = -1 or 1362, caught by the
-    // exit test below): the arithmetic shift keeps that read inside the LDS
-    // allocation (words -1..42 of this lane's tape area)
-    const int wa = (__builtin_amdgcn_sbfe(L.q, 5u, 27u) << 8) + c.tbl;
+    // exit test below): the address is then word -1 or 42, still inside this
+    // lane's tape area
+    const int wa = (int)(((((uint32_t)L.q << 3) + tbl_hi) & 0xFFFFFF00u) | tbl_lo);
     uint32_t word = lds_ld(c.sb, wa);
     const uint32_t pair = lds_ld(c.sb, L.row);
-    const uint32_t sym = __builtin_amdgcn_ubfe(word, qb, 1u);
+    // bit extract and shift use the low 5 bits of q (the hardware masks them)
+    const uint32_t sym = __builtin_amdgcn_ubfe(word, (uint32_t)L.q, 1u);
     const uint32_t e = pair >> (sym << 4);
     const uint32_t nx = e & 0x0F00u;
     bool gate;
@@ -254,9 +254,9 @@ __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int &
     else
       gate = L.hb == L.tgt;
     const bool out = (unsigned)L.q > (unsigned)E13_QMAX;
-    if (out | gate | (r >= left) | (nx == 0x0F00u)) break;
+    if (out | gate | (t >= L.tstop) | (nx == 0x0F00u)) break;
     const uint32_t w = e & 1u;
-    word ^= (w ^ sym) << qb;
+    word ^= (w ^ sym) << ((uint32_t)L.q & 31u);
     lds_st(c.sb, wa, word);
     if (GATE == GATE_COUNT) {
       L.ones += L.q >= E13_QBLK ? (int)w - (int)sym : 0;
@@ -266,9 +266,10 @@ __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int &
     }
     L.q += (int)(e & 2u) - 1;
     L.row = (int)(nx | (uint32_t)c.rbl);
+    ++t;
   }
-  L.t = t_round + r;
-  if (r == slice) return EV_NONE;
+  L.t = t;
+  if (t == t_end) return EV_NONE;
   if ((unsigned)L.q > (unsigned)E13_QMAX) { // the last step left the window: it does not count
     L.t -= 1;
     return EV_DEAD;
