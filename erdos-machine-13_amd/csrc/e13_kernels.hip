@@ -55,8 +55,12 @@ __device__ __forceinline__ uint32_t lds_base(const void *p) {
   return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void *)p;
 }
 
-__device__ __forceinline__ uint32_t enc_entry(int c) {
-  if (c < 60) return (uint32_t)(c & 3) | ((uint32_t)(c >> 2) << 8);
+// The 16-bit rule entry of the cell read on symbol `sym`: write | dir << 1 |
+// (write - sym) << 2 (two-bit signed: the ones-count change of the step, whose
+// low bit is the tape flip) | next << 8; or 0x0F00 | kind << 12 for a halt,
+// an unread editable cell or a free cell.
+__device__ __forceinline__ uint32_t enc_entry(int c, uint32_t sym) {
+  if (c < 60) return (uint32_t)(c & 3) | ((((uint32_t)(c & 1) - sym) & 3u) << 2) | ((uint32_t)(c >> 2) << 8);
   uint32_t kind = c == E13_C_HALT ? 1u : (c == E13_C_EDIT ? 2u : 3u);
   return 0x0F00u | (kind << 12);
 }
@@ -233,11 +237,12 @@ __device__ __forceinline__ int step_once(Lane &L, const Ctx &c, int &evcell) {
 template <int GATE>
 __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int &evcell) {
   const int t_end = L.t + slice;
+  const int t_lim = min(L.tstop, t_end); // the round's end or the deadline/cap test, whichever comes first
   const bool nt1 = L.nt == 1;
   int t = L.t;
   // tape word address (q >> 5) * 256 + tbl as one shift-add and one and-or
   const uint32_t tbl_lo = (uint32_t)c.tbl & 0xFFu, tbl_hi = (uint32_t)c.tbl - tbl_lo;
-  for (int r = 0; r < slice; ++r) {
+  for (;;) {
     // q may be one step outside the window here (q = -1 or 1362, caught by the
     // exit test below): the address is then word -1 or 42, still inside this
     // lane's tape area
@@ -254,22 +259,22 @@ __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int &
     else
       gate = L.hb == L.tgt;
     const bool out = (unsigned)L.q > (unsigned)E13_QMAX;
-    if (out | gate | (t >= L.tstop) | (nx == 0x0F00u)) break;
-    const uint32_t w = e & 1u;
-    word ^= (w ^ sym) << ((uint32_t)L.q & 31u);
+    if (out | gate | (t >= t_lim) | (nx == 0x0F00u)) break;
+    const uint32_t flip = __builtin_amdgcn_ubfe(e, 2u, 1u); // the write changes the cell
+    word ^= flip << ((uint32_t)L.q & 31u);
     lds_st(c.sb, wa, word);
     if (GATE == GATE_COUNT) {
-      L.ones += L.q >= E13_QBLK ? (int)w - (int)sym : 0;
+      L.ones += L.q >= E13_QBLK ? __builtin_amdgcn_sbfe((int)e, 2u, 2u) : 0; // write - read
     } else {
       const uint64_t kb = *(const uint64_t *)(c.sb + c.keyb + L.q * 8);
-      L.hb ^= kb & (0ull - (uint64_t)(w ^ sym));
+      L.hb ^= kb & (0ull - (uint64_t)flip);
     }
     L.q += (int)(e & 2u) - 1;
     L.row = (int)(nx | (uint32_t)c.rbl);
     ++t;
   }
   L.t = t;
-  if (t == t_end) return EV_NONE;
+  if (t == t_end) return EV_NONE; // a full round (a deadline due exactly now is tested next round)
   if ((unsigned)L.q > (unsigned)E13_QMAX) { // the last step left the window: it does not count
     L.t -= 1;
     return EV_DEAD;
@@ -577,7 +582,7 @@ __device__ __forceinline__ void build_rows(const Ctx &c, const uint8_t *tab, int
       if (c0 == E13_C_EDIT) c0 = base[2 * r] < 0 ? E13_C_HALT : base[2 * r];
       if (c1 == E13_C_EDIT) c1 = base[2 * r + 1] < 0 ? E13_C_HALT : base[2 * r + 1];
     }
-    lds_st(c.sb, c.rbl + r * 256, enc_entry(c0) | (enc_entry(c1) << 16));
+    lds_st(c.sb, c.rbl + r * 256, enc_entry(c0, 0u) | (enc_entry(c1, 1u) << 16));
   }
 }
 
@@ -1089,10 +1094,10 @@ __device__ void stage_rows(uint32_t *stg, const E13Problem *pbs, int lane) {
     if (lane < nq) {
       const uint8_t *tab = (const uint8_t *)(stg + E13_NODE_W_TAB);
       int c0 = tab[2 * lane], c1 = tab[2 * lane + 1];
-      A = enc_entry(c0) | (enc_entry(c1) << 16);
+      A = enc_entry(c0, 0u) | (enc_entry(c1, 1u) << 16);
       if (c0 == E13_C_EDIT) c0 = pb->base[2 * lane] < 0 ? E13_C_HALT : pb->base[2 * lane];
       if (c1 == E13_C_EDIT) c1 = pb->base[2 * lane + 1] < 0 ? E13_C_HALT : pb->base[2 * lane + 1];
-      B = enc_entry(c0) | (enc_entry(c1) << 16);
+      B = enc_entry(c0, 0u) | (enc_entry(c1, 1u) << 16);
     }
     stg[76 + lane] = A;
     stg[92 + lane] = B;
@@ -1323,7 +1328,7 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
             for (int r2 = 0; r2 < nq; ++r2) {
               uint32_t pr = stg[rs + r2];
               if (pend >> 1 == r2) {
-                const uint32_t e = enc_entry(val);
+                const uint32_t e = enc_entry(val, (uint32_t)pend & 1u);
                 pr = (pend & 1) ? ((pr & 0xFFFFu) | (e << 16)) : ((pr & 0xFFFF0000u) | e);
               }
               lds_st(c.sb, c.rbl + r2 * 256, pr);
