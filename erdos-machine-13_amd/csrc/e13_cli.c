@@ -387,7 +387,7 @@ int main(int argc, char **argv) {
     const int profile = np ? e13_search_profile(ctx, ps, np, NULL) : E13_PROFILE_MAIN;
     e13_tuning best;
     e13_tune_report rep;
-    rc = np ? e13_autotune(ctx, ps, np, 2, &best, &rep) : E13_EINVAL;
+    rc = np ? e13_autotune(ctx, ps, np, 3, &best, &rep) : E13_EINVAL;
     if (rc) {
       fprintf(stderr, "autotune: %s\n", e13_strerror(rc));
       return 1;

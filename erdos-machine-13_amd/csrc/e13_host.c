@@ -11,7 +11,10 @@
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <pthread.h>
 #include <string.h>
+#include <time.h>
+#include <unistd.h>
 
 #include "e13.h"
 #include "e13_dev.h"
@@ -424,6 +427,46 @@ void e13i_root_node(const e13_problem *p, E13Node *nd) {
 
 static void finish_survivor(const e13_problem *p, const E13SurvRec *r, e13_survivor *s, int prob);
 
+/* Survivor records to survivor lines, each re-checked by the plain simulation:
+   independent machines of up to ~835k steps each, so on host threads. */
+typedef struct {
+  const e13_problem *ps;
+  int np;
+  const E13SurvRec *rec;
+  e13_survivor *surv;
+  uint64_t n, first, stride;
+} finish_job;
+
+static void *finish_worker(void *arg) {
+  const finish_job *j = (const finish_job *)arg;
+  for (uint64_t k = j->first; k < j->n; k += j->stride) {
+    int pr = j->rec[k].prob;
+    if (pr < 0 || pr >= j->np) pr = 0;
+    finish_survivor(&j->ps[pr], &j->rec[k], &j->surv[k], pr);
+  }
+  return NULL;
+}
+
+static void finish_all(const e13_problem *ps, int np, const E13SurvRec *rec, e13_survivor *surv, uint64_t n) {
+  long cpus = sysconf(_SC_NPROCESSORS_ONLN);
+  uint64_t nt = cpus > 1 ? (uint64_t)cpus : 1;
+  if (nt > 16) nt = 16;
+  if (nt > n) nt = n;
+  pthread_t th[16];
+  finish_job jobs[16];
+  uint64_t started = 0;
+  for (uint64_t t = 0; t < nt; t++) {
+    jobs[t] = (finish_job){ps, np, rec, surv, n, t, nt};
+    if (t > 0 && pthread_create(&th[t], NULL, finish_worker, &jobs[t]) == 0) started |= 1ull << t;
+  }
+  finish_worker(&jobs[0]);
+  for (uint64_t t = 1; t < nt; t++) {
+    if (started >> t & 1) pthread_join(th[t], NULL);
+    else finish_worker(&jobs[t]); /* no thread: do its share here */
+  }
+}
+
+
 static int check_problem(const e13_problem *p) {
   if (!p || p->nq < 1 || p->nq > E13_MAXNQ || p->budget < 0) return 0;
   for (int i = 0; i < 2 * p->nq; i++) {
@@ -619,7 +662,7 @@ int e13_search(e13_ctx *c, const e13_problem *p, e13_counts *out, e13_survivor *
     E13SurvRec *rec = (E13SurvRec *)malloc(nk * sizeof(E13SurvRec));
     if (!rec) return E13_ENOMEM;
     HIPCHK(hipMemcpy(rec, c->d_surv, nk * sizeof(E13SurvRec), hipMemcpyDeviceToHost));
-    for (uint64_t k = 0; k < nk; k++) finish_survivor(p, &rec[k], &surv[k], 0);
+    finish_all(p, 1, rec, surv, nk);
     free(rec);
   }
   return E13_OK;
@@ -809,12 +852,21 @@ static int collect_cost_log(e13_ctx *c, const e13_problem *ps, size_t n) {
   return rc;
 }
 
+static double host_ms(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec * 1e3 + (double)ts.tv_nsec * 1e-6;
+}
+
 /* The persistent form: every problem in one launch (see k_search). */
 int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, e13_survivor *surv,
                     uint64_t surv_cap, uint64_t *n_surv) {
   if (!c || !ps || !out || np < 1 || np > E13_MAXPROB) return E13_EINVAL;
   for (int k = 0; k < np; k++)
     if (!check_problem(&ps[k])) return E13_EINVAL;
+  const int timing = getenv("E13_TIMING") && *getenv("E13_TIMING") != '0'; /* host phase times on stderr */
+  double tm[5] = {0, 0, 0, 0, 0};
+  tm[0] = timing ? host_ms() : 0;
   HIPCHK(hipSetDevice(c->device));
   const int profile = e13_search_profile(c, ps, np, NULL);
   const e13_tuning *tu = profile == E13_PROFILE_SMALL ? &c->tune_small : &c->tune;
@@ -907,14 +959,18 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
       ea.out = c->d_out;
       ea.out_cap = (uint32_t)(top * (size_t)nslot);
       HIPCHK(hipMemsetAsync(&c->d_ctr->item, 0, 2 * sizeof(uint32_t), c->stream));
+      HIPCHK(hipEventRecord(c->ev0, c->stream));
       int lr = e13k_launch_expand(&ea, c->ncu * tu->wgs_per_cu, 256, tu->gate == 2, 1, c->stream);
       if (lr != 0) {
         fprintf(stderr, "libe13: expand launch failed: %s\n", hipGetErrorString((hipError_t)lr));
         return E13_EHIP;
       }
+      HIPCHK(hipEventRecord(c->ev1, c->stream));
       uint32_t io[4];
       HIPCHK(hipMemcpyAsync(io, &c->d_ctr->item, sizeof io, hipMemcpyDeviceToHost, c->stream));
       HIPCHK(hipStreamSynchronize(c->stream));
+      float ems = 0;
+      if (hipEventElapsedTime(&ems, c->ev0, c->ev1) == hipSuccess) c->stats.kernel_ms += ems;
       c->stats.launches++;
       if (io[3]) return io[3] & 4u ? E13_EHIP : E13_ESTACK;
       if (io[1])
@@ -1001,6 +1057,10 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
     c->trace_waves = total_waves;
     c->trace_per_wave = (int)per;
   }
+  if (timing) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    tm[1] = host_ms();
+  }
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   int lr = e13k_launch_search(&a, grid, wg, tu->gate == 2, c->stream);
   if (lr != 0) {
@@ -1015,6 +1075,7 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
   HIPCHK(hipMemcpyAsync(&h, c->d_ctr, sizeof h, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipMemcpyAsync(&sch, c->d_sch, sizeof sch, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  if (timing) tm[2] = host_ms();
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->stats.kernel_ms += ms;
@@ -1056,12 +1117,13 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
     E13SurvRec *rec = (E13SurvRec *)malloc(nk * sizeof(E13SurvRec));
     if (!rec) return E13_ENOMEM;
     HIPCHK(hipMemcpy(rec, c->d_surv, nk * sizeof(E13SurvRec), hipMemcpyDeviceToHost));
-    for (uint64_t k = 0; k < nk; k++) {
-      int pr = rec[k].prob;
-      if (pr < 0 || pr >= np) pr = 0;
-      finish_survivor(&ps[pr], &rec[k], &surv[k], pr);
-    }
+    finish_all(ps, np, rec, surv, nk);
     free(rec);
+  }
+  if (timing) {
+    tm[3] = host_ms();
+    fprintf(stderr, "libe13: search phases: frontier %.2f ms, persistent kernel %.2f ms (events %.2f), results %.2f ms\n",
+            tm[1] - tm[0], tm[2] - tm[1], ms, tm[3] - tm[2]);
   }
   return E13_OK;
 }

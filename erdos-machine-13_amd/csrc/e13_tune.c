@@ -62,9 +62,10 @@ static double now_ms(void) {
   return (double)ts.tv_sec * 1e3 + (double)ts.tv_nsec * 1e-6;
 }
 
-/* Wall time of the search call under tuning t (best of reps): the persistent
-   kernel, the frontier launches before it and their host round trips; counts
-   checked against ref. */
+/* Wall time of the search call under tuning t (the median of reps: the best
+   of a few runs would favour settings whose times spread widely): the
+   persistent kernel, the frontier launches before it and their host round
+   trips; counts checked against ref. */
 static int trial(e13_ctx *c, const e13_tuning *t, const e13_problem *ps, int np, int reps, e13_counts *ref,
                  int *have_ref, double *ms) {
   e13_tuning t2 = *t; /* "off" is 0 in a read-back tuning, -1 when set */
@@ -74,13 +75,13 @@ static int trial(e13_ctx *c, const e13_tuning *t, const e13_problem *ps, int np,
   int rc = e13_set_tuning(c, &t2);
   if (rc) return rc;
   e13_counts cnt[E13_MAXPROB];
-  *ms = 1e300;
+  double times[64];
+  if (reps > 64) reps = 64;
   for (int r = 0; r < reps; r++) {
     uint64_t ns = 0;
     const double t0 = now_ms();
     if ((rc = e13_search_many(c, ps, np, cnt, NULL, 0, &ns))) return rc;
-    const double k = now_ms() - t0;
-    if (k < *ms) *ms = k;
+    times[r] = now_ms() - t0;
     if (!*have_ref) {
       memcpy(ref, cnt, (size_t)np * sizeof *cnt);
       *have_ref = 1;
@@ -89,6 +90,13 @@ static int trial(e13_ctx *c, const e13_tuning *t, const e13_problem *ps, int np,
       return E13_EINVAL;
     }
   }
+  for (int i = 1; i < reps; i++) /* insertion sort, then the median */
+    for (int j = i; j > 0 && times[j] < times[j - 1]; j--) {
+      const double x = times[j];
+      times[j] = times[j - 1];
+      times[j - 1] = x;
+    }
+  *ms = reps % 2 ? times[reps / 2] : 0.5 * (times[reps / 2 - 1] + times[reps / 2]);
   return E13_OK;
 }
 

@@ -222,7 +222,7 @@ int e13_synth_run(e13_ctx *ctx, uint64_t seed0, uint64_t k0, int64_t n, e13_batc
    register loop), frontier per wave, nodes per delivery pass, hungry-map read
    interval, workgroup size, idle poll interval and the step count after which
    a machine goes to an idle wave to run alone, each trial a persistent search
-   of ps[0..np) (best of reps), starting from the profile those problems use.
+   of ps[0..np) (the median of reps), starting from the profile those problems use.
    Every trial's counts must equal the first's (E13_EINVAL otherwise).  The
    chosen settings are installed in ctx.
    e13_open loads a device's stored settings from $E13_TUNING_FILE or from
