@@ -1325,15 +1325,32 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
             depth = (int)stg[13] + (pend >= 0);
             nbound = (int)stg[14] + (pend >= 0);
             const int rs = (nb == 0 && (int)stg[9] > 0) ? 92 : 76; // edits used up: unread cells keep their value
-            for (int r2 = 0; r2 < nq; ++r2) {
-              uint32_t pr = stg[rs + r2];
+            // rows and tape from the stage into this lane's columns: each chunk
+            // of stage words is read into registers before any is written, so
+            // the reads are not serialised behind the writes (the compiler
+            // cannot tell the stage from the lane columns)
+            uint32_t rw[16];
+#pragma unroll
+            for (int r2 = 0; r2 < 16; ++r2) rw[r2] = stg[rs + r2];
+#pragma unroll
+            for (int r2 = 0; r2 < 16; ++r2) {
+              uint32_t pr = rw[r2];
               if (pend >> 1 == r2) {
                 const uint32_t e = enc_entry(val, (uint32_t)pend & 1u);
                 pr = (pend & 1) ? ((pr & 0xFFFFu) | (e << 16)) : ((pr & 0xFFFF0000u) | e);
               }
-              lds_st(c.sb, c.rbl + r2 * 256, pr);
+              if (r2 < nq) lds_st(c.sb, c.rbl + r2 * 256, pr);
             }
-            for (int w = 0; w < E13_TW; ++w) lds_st(c.sb, c.tbl + w * 256, stg[E13_NODE_W_TAPE + w]);
+#pragma unroll
+            for (int w0 = 0; w0 < E13_TW; w0 += 11) {
+              uint32_t tw[11];
+#pragma unroll
+              for (int j = 0; j < 11; ++j)
+                if (w0 + j < E13_TW) tw[j] = stg[E13_NODE_W_TAPE + w0 + j];
+#pragma unroll
+              for (int j = 0; j < 11; ++j)
+                if (w0 + j < E13_TW) lds_st(c.sb, c.tbl + (w0 + j) * 256, tw[j]);
+            }
             active = 1;
           }
         }
