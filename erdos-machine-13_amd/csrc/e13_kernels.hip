@@ -1142,10 +1142,24 @@ __device__ __forceinline__ void stage_own(const Ctx &c, uint32_t *stg, const Lan
   stg[21] = (uint32_t)(L.ch1 >> 32);
   stg[22] = (uint32_t)L.ch2;
   stg[23] = (uint32_t)(L.ch2 >> 32);
+  // the lane's rows and tape are read into registers before the stage is
+  // written (the compiler cannot tell them apart and would serialise each
+  // read behind the previous write)
+  uint32_t cw[E13_MAXCELLS / 4];
 #pragma unroll
-  for (int i = 0; i < E13_MAXCELLS / 4; ++i) stg[E13_NODE_W_TAB + i] = codes_word(c, nq, i);
-  for (int w = 0; w < E13_TW_PAD; ++w)
-    stg[E13_NODE_W_TAPE + w] = w < E13_TW ? lds_ld(c.sb, c.tbl + w * 256) : seed;
+  for (int i = 0; i < E13_MAXCELLS / 4; ++i) cw[i] = codes_word(c, nq, i);
+#pragma unroll
+  for (int i = 0; i < E13_MAXCELLS / 4; ++i) stg[E13_NODE_W_TAB + i] = cw[i];
+#pragma unroll
+  for (int w0 = 0; w0 < E13_TW_PAD; w0 += 11) {
+    uint32_t tw[11];
+#pragma unroll
+    for (int j = 0; j < 11; ++j)
+      if (w0 + j < E13_TW_PAD) tw[j] = w0 + j < E13_TW ? lds_ld(c.sb, c.tbl + (w0 + j) * 256) : seed;
+#pragma unroll
+    for (int j = 0; j < 11; ++j)
+      if (w0 + j < E13_TW_PAD) stg[E13_NODE_W_TAPE + w0 + j] = tw[j];
+  }
 }
 
 // Copy this lane's machine into node record `o` (plain stores).
