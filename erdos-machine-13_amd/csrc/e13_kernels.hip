@@ -296,43 +296,37 @@ __device__ __forceinline__ int solo_uni(int v) { return __builtin_amdgcn_readfir
 __device__ __forceinline__ int lane_of(int v, int sl) { return __builtin_amdgcn_readlane(v, sl); }
 
 // Block macro-steps of the lone lane: the run of the machine inside one
-// aligned 8-cell block of tape (a byte), up to the step that leaves it or for
-// at most SOLO_STAY steps, is a pure function of (state, byte, entry cell).
-// It is memoised in a tagged table of 1024 slots hashed from that key, kept in
-// tape words 0..31 of the wave's idle lanes (free while one lane runs alone):
-// slot s has its tag (gen << 16 | key) in word s >> 6 and its value in word
-// 16 + (s >> 6) of lane s & 63 (lane sl's slots move to lane (s & 63) ^ 32).
-// value = new byte | state after << 8 | exit right << 12 | code << 16, where
-// code 1..SOLO_STAY: the run leaves the block after that many steps, on the
-// side the right bit names; code SOLO_STAY+1+p: it is still inside after
-// SOLO_STAY steps, at block cell p (a machine caught in a loop inside a few
-// cells); code 0: the run reads a halt or an unbound cell.
-#define SOLO_MEMO_GEN0 0x8000u
+// aligned 16-cell block of tape, up to the step that leaves it or for at most
+// SOLO_STAY steps, is a pure function of (state, block contents, entry cell).
+// It is memoised in a tagged table of 1024 slots hashed from that key (24
+// bits), kept in tape words 0..31 of the wave's idle lanes (free while one
+// lane runs alone): slot s has its tag (1 << 31 | key) in word s >> 6 and its
+// value in word 16 + (s >> 6) of lane s & 63 (lane sl's slots move to lane
+// (s & 63) ^ 32).  value = new contents | state after << 16 | exit right << 20
+// | code << 21, where code 1..SOLO_STAY: the run leaves the block after that
+// many steps, on the side the right bit names; code SOLO_STAY+1+p: it is still
+// inside after SOLO_STAY steps, at block cell p (a machine caught in a loop
+// inside a few cells); code 0: the run reads a halt or an unbound cell.
 #define SOLO_MEMO_ROWS 16
-#define SOLO_STAY 247
+#define SOLO_STAY 495
 
-// The memo generation, advanced by the whole wave before a lone-lane round
-// when the lone lane or the wave's fill count changed since the last one (a
-// refill rewrites idle lanes' tape words, where the memo lives); the idle
-// lanes then clear their tag words.  gen stays in 0x8001..0xFFFF, so a
-// cleared tag (0) never matches.
-__device__ __forceinline__ void solo_next_gen(uint32_t &gen, bool active, const Ctx &c, uint64_t act, uint64_t fills,
-                                              uint64_t &last) {
+// Run by the whole wave before a lone-lane round: when the lone lane or the
+// wave's fill count changed since the last such round (a refill rewrites idle
+// lanes' tape words, where the memo lives), the idle lanes clear their tag
+// words, so every tag present belongs to the machine now running alone.
+__device__ __forceinline__ void solo_memo_reset(bool active, const Ctx &c, uint64_t act, uint64_t fills,
+                                                uint64_t &last) {
   const int sl = __ffsll((long long)act) - 1;
   const uint64_t who = (fills << 6) | (uint64_t)sl;
   if (who == last) return;
   last = who;
-  gen += 1u;
-  if (gen > 0xFFFFu) gen = SOLO_MEMO_GEN0 + 1u;
   if (!active)
     for (int w = 0; w < SOLO_MEMO_ROWS; ++w) lds_st(c.sb, c.tbl + w * 256, 0u);
 }
 
-__device__ __forceinline__ int run_solo(Lane &L, const Ctx &c, int nsteps_in, int &evcell, const uint32_t g_in,
-                                        const int sl) {
+__device__ __forceinline__ int run_solo(Lane &L, const Ctx &c, int nsteps_in, int &evcell, const int sl) {
   const int lane = (int)(threadIdx.x & 63u);
   const int nsteps = solo_uni(nsteps_in);
-  const uint32_t g = (uint32_t)solo_uni((int)g_in);
   const int tbl = lane_of(c.tbl, sl), rbl = lane_of(c.rbl, sl);
   const int tbase = tbl - sl * 4; // the wave's tape area: word w of lane j at tbase + w*256 + j*4
   int q = lane_of(L.q, sl);
@@ -403,22 +397,22 @@ __device__ __forceinline__ int run_solo(Lane &L, const Ctx &c, int nsteps_in, in
         continue;
       }
     }
-    // a block macro-step: the byte of tape around the head, away from the
+    // a block macro-step: the 16 cells of tape around the head, away from the
     // window edges (a step off the window is not counted), with the ones count
-    // more than 8 from its target (an 8-cell run moves it by at most 8, so no
-    // gate can fire) or neither gate cell inside the block (the head visits
-    // only its 8 cells: no gate can fire whatever the ones count)
-    const int qn = q & ~7;
-    const bool far = (ones - target > 8) | (target - ones > 8);
-    const bool no_gate_cell = !nt1 & ((qA >> 3) != (qn >> 3)) & ((qR >> 3) != (qn >> 3));
-    if ((qn >= 8) & (qn + 8 <= E13_QMAX) & (far | no_gate_cell)) {
-      const uint32_t nb = qb & ~7u, byt = (word >> nb) & 255u, pos = qb & 7u;
-      const uint32_t key = ((uint32_t)st << 11) | (byt << 3) | pos;
+    // more than 16 from its target (a 16-cell run moves it by at most 16, so
+    // no gate can fire) or neither gate cell inside the block (the head visits
+    // only its 16 cells: no gate can fire whatever the ones count)
+    const int qn = q & ~15;
+    const bool far = (ones - target > 16) | (target - ones > 16);
+    const bool no_gate_cell = !nt1 & ((qA >> 4) != (qn >> 4)) & ((qR >> 4) != (qn >> 4));
+    if ((qn >= 16) & (qn + 16 <= E13_QMAX) & (far | no_gate_cell)) {
+      const uint32_t nb = qb & ~15u, byt = (word >> nb) & 0xFFFFu, pos = qb & 15u;
+      const uint32_t key = ((uint32_t)st << 20) | (byt << 4) | pos;
       const uint32_t slot = (key * 0x9E3779B1u) >> 22;
       int col = (int)(slot & 63u);
       if (col == sl) col ^= 32;
       const int ta = tbase + (int)(slot >> 6) * 256 + col * 4, va = ta + SOLO_MEMO_ROWS * 256;
-      const uint32_t tag = (g << 16) | key;
+      const uint32_t tag = 0x80000000u | key;
       const uint32_t t_ld = lds_ld(c.sb, ta), v_ld = lds_ld(c.sb, va);
       uint32_t val = (uint32_t)solo_uni((int)v_ld);
       if ((uint32_t)solo_uni((int)t_ld) != tag) { // not yet known: run the block once
@@ -432,26 +426,27 @@ __device__ __forceinline__ int run_solo(Lane &L, const Ctx &c, int nsteps_in, in
           p += (int)(e2 & 2u) - 1;
           s2 = e2 >> 8;
           ++k2;
-          if ((unsigned)p > 7u) {
-            right = p > 7;
+          if ((unsigned)p > 15u) {
+            right = p > 15;
             code = k2;
             break;
           }
         }
         if (k2 == SOLO_STAY && code == 0) code = SOLO_STAY + 1 + p; // still inside the block
-        val = v | (s2 << 8) | ((uint32_t)right << 12) | ((uint32_t)code << 16);
+        val = v | (s2 << 16) | ((uint32_t)right << 20) | ((uint32_t)code << 21);
         lds_st(c.sb, ta, tag);
         lds_st(c.sb, va, val);
       }
-      const int code = (int)(val >> 16), k3 = min(code, SOLO_STAY);
+      const int code = (int)(val >> 21), k3 = min(code, SOLO_STAY);
       if ((code > 0) & (k3 <= n_eff - r)) {
-        const uint32_t nv = val & 255u;
-        word = (word & ~(255u << nb)) | (nv << nb);
-        // ones are counted for q >= 61: all of a block above, the upper cells of the block at q 56
-        const uint32_t cm = qn >= E13_QBLK ? 255u : (qn + 8 > E13_QBLK ? (255u << (E13_QBLK - qn)) & 255u : 0u);
+        const uint32_t nv = val & 0xFFFFu;
+        word = (word & ~(0xFFFFu << nb)) | (nv << nb);
+        // ones are counted for q >= 61: all of a block above, the upper cells of the block at q 48
+        const uint32_t cm =
+            qn >= E13_QBLK ? 0xFFFFu : (qn + 16 > E13_QBLK ? (0xFFFFu << (E13_QBLK - qn)) & 0xFFFFu : 0u);
         ones += __builtin_popcount(nv & cm) - __builtin_popcount(byt & cm);
-        q = code > SOLO_STAY ? qn + (code - SOLO_STAY - 1) : ((val >> 12) & 1u) ? qn + 8 : qn - 1;
-        st = (int)((val >> 8) & 15u);
+        q = code > SOLO_STAY ? qn + (code - SOLO_STAY - 1) : ((val >> 20) & 1u) ? qn + 16 : qn - 1;
+        st = (int)((val >> 16) & 15u);
         r += k3 - 1; // the loop's ++r counts the last step
         if ((q >> 5) != wi) {
           lds_st(c.sb, (wi << 8) + tbl, word);
@@ -888,7 +883,6 @@ __global__ void __launch_bounds__(256) k_batch(E13BatchArgs a) {
   int active = 0;
   uint32_t k = 0;
   uint64_t my_steps = 0, w_slots = 0, w_rounds = 0;
-  uint32_t solo_gen = SOLO_MEMO_GEN0;
   uint64_t solo_last = ~0ull, mach = 0; // mach: lane slots this wave has handed out (wave-uniform)
   bool exhausted = false;
   uint32_t ck = 0, ce = 0; // this wave's claimed table range
@@ -951,8 +945,8 @@ __global__ void __launch_bounds__(256) k_batch(E13BatchArgs a) {
     // once the batch is exhausted)
     const bool solo = GATE == GATE_COUNT && a.solo > 0 && __popcll(act) == 1;
     if (solo) { // the whole wave runs the lone lane's loop
-      solo_next_gen(solo_gen, active, c, act, mach, solo_last);
-      ev = run_solo(L, c, exhausted ? a.solo : a.slice, evcell, solo_gen, __ffsll((long long)act) - 1);
+      solo_memo_reset(active, c, act, mach, solo_last);
+      ev = run_solo(L, c, exhausted ? a.solo : a.slice, evcell, __ffsll((long long)act) - 1);
       if (!active) ev = EV_NONE;
     } else if (active) {
       ev = run_slice<GATE>(L, c, a.slice, evcell);
@@ -1230,7 +1224,6 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
   L.tcap = 0x7FFFFFFF;
   int active = 0, t0 = 0, budget = 0, nintro = 0, prob = 0, depth = 0, nbound = 0, nq = 0;
   uint32_t seed = 0; // frontier node the lane's machine descends from
-  uint32_t solo_gen = SOLO_MEMO_GEN0;
   uint64_t solo_last = ~0ull, mach = 0; // mach: lane slots this wave has handed out (wave-uniform)
   long long top = 0, bot = 0, staged = -1;
   int cur_slot = 0, cur_end = 0; // the staged node's child slots still to hand out: [cur_slot, cur_end)
@@ -1493,8 +1486,8 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
     int ev = EV_NONE, evcell = -1;
     const bool solo = GATE == GATE_COUNT && a.solo > 0 && __popcll(act) == 1 && top == bot;
     if (solo) { // the whole wave runs the lone lane's loop
-      solo_next_gen(solo_gen, active, c, act, mach, solo_last);
-      ev = run_solo(L, c, a.solo, evcell, solo_gen, __ffsll((long long)act) - 1);
+      solo_memo_reset(active, c, act, mach, solo_last);
+      ev = run_solo(L, c, a.solo, evcell, __ffsll((long long)act) - 1);
       if (!active) ev = EV_NONE;
     } else if (active) {
       ev = run_slice<GATE>(L, c, a.slice, evcell);

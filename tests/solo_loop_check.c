@@ -3,8 +3,8 @@
  * The register loop of the HIP kernels (run_solo in
  * erdos-machine-13_amd/csrc/e13_kernels.hip) advances a lone machine by
  * sweeps (runs of equal cells under a rule that rewrites the symbol it reads
- * and keeps the state) and by memoised byte-block macro-steps (the run inside
- * an aligned 8-cell block, to the step that leaves it or for at most STAY
+ * and keeps the state) and by memoised block macro-steps (the run inside an
+ * aligned 16-cell block, to the step that leaves it or for at most STAY
  * steps).  Both must give exactly what plain stepping gives: the same steps,
  * the same exit (a stage-gate candidate or an exceptional rule before the
  * step, or the head leaving the window, which is not counted), the same tape,
@@ -17,7 +17,7 @@
  *
  * Usage: solo_loop_check SEED RUNS; prints "runs=N mismatches=M stays=S" and
  * exits 1 on a mismatch.  Built with -DPLANT=1 (ones mask of the boundary
- * block), -DPLANT=2 (gate margin 4 instead of 8) or -DPLANT=3 (stay position
+ * block), -DPLANT=2 (gate margin 8 instead of 16) or -DPLANT=3 (stay position
  * off by one) it carries a deliberate slip, which the check must catch.
  */
 #include <stdint.h>
@@ -30,7 +30,7 @@
 #endif
 #define QMAX 1361 /* last live head position (q = cell + 63) */
 #define QBLK 61   /* first q of the block region (cells >= -2), where ones are counted */
-#define STAY 247  /* longest memoised in-block run */
+#define STAY 495  /* longest memoised in-block run */
 
 static uint64_t splitmix(uint64_t *s) {
   uint64_t z = (*s += 0x9E3779B97F4A7C15ULL);
@@ -77,12 +77,11 @@ static int plain(Machine *m, int n, int *why) {
 
 /* The memo: 1024 slots of (tag, value) in 16 x 64 words each, column `me`
    (the running lane's own) remapped to me ^ 32, cleared per call. */
-static uint32_t mtag[16][64], mval[16][64], gen = 0x8000u;
+static uint32_t mtag[16][64], mval[16][64];
 static long stays;
 
 /* The kernel's loop (scalar restatement): up to n steps. */
 static int fast(Machine *m, int n, int *why, int me) {
-  if (++gen > 0xFFFFu) gen = 0x8001u;
   for (int w = 0; w < 16; w++)
     for (int l = 0; l < 64; l++)
       if (l != me) mtag[w][l] = 0;
@@ -135,18 +134,18 @@ static int fast(Machine *m, int n, int *why, int me) {
         continue;
       }
     }
-    const int qn = q & ~7;
-    const int margin = PLANT == 2 ? 4 : 8;
+    const int qn = q & ~15;
+    const int margin = PLANT == 2 ? 8 : 16;
     const int far = (ones - target > margin) | (target - ones > margin);
-    const int no_gate_cell = !nt1 & ((qA >> 3) != (qn >> 3)) & ((qR >> 3) != (qn >> 3));
-    if ((qn >= 8) & (qn + 8 <= QMAX) & (far | no_gate_cell)) { /* a byte-block macro-step */
-      const uint32_t nb = qb & ~7u, byt = (word >> nb) & 255u, pos = qb & 7u;
-      const uint32_t key = ((uint32_t)st << 11) | (byt << 3) | pos;
+    const int no_gate_cell = !nt1 & ((qA >> 4) != (qn >> 4)) & ((qR >> 4) != (qn >> 4));
+    if ((qn >= 16) & (qn + 16 <= QMAX) & (far | no_gate_cell)) { /* a block macro-step */
+      const uint32_t nb = qb & ~15u, byt = (word >> nb) & 0xFFFFu, pos = qb & 15u;
+      const uint32_t key = ((uint32_t)st << 20) | (byt << 4) | pos;
       const uint32_t slot = (key * 0x9E3779B1u) >> 22;
       int col = (int)(slot & 63u);
       if (col == me) col ^= 32;
       const int ws = (int)(slot >> 6);
-      const uint32_t tag = (gen << 16) | key;
+      const uint32_t tag = 0x80000000u | key;
       uint32_t val;
       if (mtag[ws][col] != tag) {
         uint32_t v = byt, s2 = (uint32_t)st;
@@ -158,29 +157,30 @@ static int fast(Machine *m, int n, int *why, int me) {
           p += (int)(e2 & 2u) - 1;
           s2 = e2 >> 8;
           ++k2;
-          if ((unsigned)p > 7u) {
-            right = p > 7;
+          if ((unsigned)p > 15u) {
+            right = p > 15;
             code = k2;
             break;
           }
         }
         if (k2 == STAY && code == 0) code = STAY + 1 + p;
-        val = v | (s2 << 8) | ((uint32_t)right << 12) | ((uint32_t)code << 16);
+        val = v | (s2 << 16) | ((uint32_t)right << 20) | ((uint32_t)code << 21);
         mtag[ws][col] = tag;
         mval[ws][col] = val;
       } else {
         val = mval[ws][col];
       }
-      const int code = (int)(val >> 16), k3 = code < STAY ? code : STAY;
+      const int code = (int)(val >> 21), k3 = code < STAY ? code : STAY;
       if ((code > 0) & (k3 <= n - r)) {
-        const uint32_t nv = val & 255u;
-        word = (word & ~(255u << nb)) | (nv << nb);
-        const uint32_t cm = PLANT == 1 ? (qn + 8 > QBLK ? 255u : 0u)
-                                       : (qn >= QBLK ? 255u : (qn + 8 > QBLK ? (255u << (QBLK - qn)) & 255u : 0u));
+        const uint32_t nv = val & 0xFFFFu;
+        word = (word & ~(0xFFFFu << nb)) | (nv << nb);
+        const uint32_t cm = PLANT == 1 ? (qn + 16 > QBLK ? 0xFFFFu : 0u)
+                                       : (qn >= QBLK ? 0xFFFFu
+                                                     : (qn + 16 > QBLK ? (0xFFFFu << (QBLK - qn)) & 0xFFFFu : 0u));
         ones += __builtin_popcount(nv & cm) - __builtin_popcount(byt & cm);
         if (code > STAY) ++stays;
-        q = code > STAY ? qn + (code - STAY - 1) + (PLANT == 3) : ((val >> 12) & 1u) ? qn + 8 : qn - 1;
-        st = (int)((val >> 8) & 15u);
+        q = code > STAY ? qn + (code - STAY - 1) + (PLANT == 3) : ((val >> 20) & 1u) ? qn + 16 : qn - 1;
+        st = (int)((val >> 16) & 15u);
         r += k3 - 1;
         if ((q >> 5) != wi) {
           m->tape[wi] = word;

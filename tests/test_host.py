@@ -252,10 +252,10 @@ def test_product_path_has_no_oracle_import():
 
 
 def test_lone_lane_loop_accelerations_match_plain_stepping(tmp_path):
-    """The register loop's sweeps and byte-block memo, restated in C
+    """The register loop's sweeps and 16-cell block memo, restated in C
     (tests/solo_loop_check.c), give exactly what plain stepping gives on random
     machines, tapes, heads and gate registers; each of three planted slips
-    (ones mask of the block at q 56..63, gate margin 4 instead of 8, a stay
+    (ones mask of the block at q 48..63, gate margin 8 instead of 16, a stay
     position off by one) is caught."""
     import subprocess
     src = os.path.join(ROOT, "tests", "solo_loop_check.c")
