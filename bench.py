@@ -88,10 +88,15 @@ def extra_workload(e13, ctx, name, problems, want_leaves, want_steps, want_surv)
     steps = sum(c.steps for c in cs)
     ok = leaves == want_leaves and ns == want_surv and (want_steps is None or steps == want_steps)
     ok = ok and all(s.recheck_ok == 1 for s in sv)
+    ks = (s1["search_ms"] - s0["search_ms"]) / 1e3
+    ksteps = s1["search_steps"] - s0["search_steps"]
     return {"workload": name, "leaves": leaves, "steps": steps, "survivors": ns, "bit_identical_counts": ok,
             "seconds": dt, "steps_per_s": steps / dt, "leaves_per_s": leaves / dt,
             "kernel_ms": s1["kernel_ms"] - s0["kernel_ms"],
-            "settings_profile": "small-search" if s1["profile"] == 1 else "main"}
+            "settings_profile": "small-search" if s1["profile"] == 1 else "main",
+            "k_search": {"ms": ks * 1e3, "steps": ksteps, "steps_per_s": ksteps / ks,
+                         "valu_frac": ksteps * OPS_PER_STEP / ks / VALU_PEAK,
+                         "lds_frac": ksteps * LDS_BYTES_PER_STEP / ks / LDS_PEAK}}
 
 
 def synth_workload(e13, ctx, n=1 << 22):
