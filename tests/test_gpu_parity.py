@@ -234,6 +234,9 @@ def test_counter_3state_symmetric(e13, gpu_ctx, base):
                                   dict(form=3, evict_after=1, solo_steps=7), dict(form=3, evict_after=-1),
                                   dict(form=2, solo_steps=3),
                                   dict(form=3, slice=300, wg_threads=128, wgs_per_cu=3),
+                                  # more workgroups per CU than can be resident, with a frontier
+                                  # dealt to every wave: the grid is clamped to the resident ones
+                                  dict(form=3, wgs_per_cu=4, seed_per_wave=8),
                                   dict(form=3, wg_threads=64, wgs_per_cu=1, stack_nodes=1 << 16)])
 def test_results_invariant_under_tuning(e13, gpu_ctx, base, tune):
     say(f"search: delete FMAX=2 X=7,8,12 tuning={tune}")
