@@ -96,7 +96,10 @@ def extra_workload(e13, ctx, name, problems, want_leaves, want_steps, want_surv)
             "settings_profile": "small-search" if s1["profile"] == 1 else "main",
             "k_search": {"ms": ks * 1e3, "steps": ksteps, "steps_per_s": ksteps / ks,
                          "valu_frac": ksteps * OPS_PER_STEP / ks / VALU_PEAK,
-                         "lds_frac": ksteps * LDS_BYTES_PER_STEP / ks / LDS_PEAK}}
+                         "lds_frac": ksteps * LDS_BYTES_PER_STEP / ks / LDS_PEAK,
+                         "waves_per_simd": s1["grid"] * s1["wg_threads"] / 64 / (256 * 4),
+                         "active_lanes_per_round": (s1["lane_slots"] - s0["lane_slots"]) / max(s1["rounds"] - s0["rounds"], 1),
+                         "steps_per_lane_round": ksteps / max(s1["lane_slots"] - s0["lane_slots"], 1)}}
 
 
 def synth_workload(e13, ctx, n=1 << 22):
@@ -348,6 +351,11 @@ def main():
                      "steps_per_launch": ksteps / max(launches, 1),
                      "steps_per_s_kernel": ksteps / kernel_s,
                      "frontier_kernels_ms_per_pass": frontier_ms / args.steps,
+                     # occupancy and divergence: resident waves per SIMD (256 CUs x 4 SIMDs),
+                     # running lanes per wave round (of 64), steps per lane per round
+                     "waves_per_simd": s1["grid"] * s1["wg_threads"] / 64 / (256 * 4),
+                     "active_lanes_per_round": (s1["lane_slots"] - s0["lane_slots"]) / max(s1["rounds"] - s0["rounds"], 1),
+                     "steps_per_lane_round": ksteps / max(s1["lane_slots"] - s0["lane_slots"], 1),
                      "wave_cycle_split": {"refill": cyc[0] / max(sum(cyc), 1), "steps": cyc[1] / max(sum(cyc), 1),
                                           "finish": cyc[2] / max(sum(cyc), 1), "pool_idle": cyc[3] / max(sum(cyc), 1)}},
         "north_star_workloads": extra,
