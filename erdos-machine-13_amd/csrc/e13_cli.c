@@ -20,8 +20,10 @@
  * Environment: E13_FORM (1 plain, 2 batched), E13_SLICE, E13_CAP, E13_BATCH,
  * E13_GATE (1 count, 2 hash), E13_WG, E13_WGS_PER_CU, E13_DEVICE,
  * E13_REFILL_TRIES, E13_SEED_PER_WAVE, E13_SCHED_EVERY, E13_DONATE_MAX,
- * E13_IDLE_NAPS, E13_MAIL_EVERY, E13_GRAPH_CHAIN, E13_SOLO_STEPS, E13_EVICT_AFTER (negative: off); E13_TRACE=N with E13_TRACE_FILE=path writes
- * the persistent search's wave event log (N events per wave) to path.
+ * E13_IDLE_NAPS, E13_MAIL_EVERY, E13_GRAPH_CHAIN, E13_SOLO_STEPS, E13_EVICT_AFTER (negative: off);
+ * E13_TUNING_FILE (stored settings; <name>_small.<ext> beside it for small searches); E13_TRACE=N with
+ * E13_TRACE_FILE=path writes the persistent search's wave event log (N events per wave) to path; the
+ * library prints its host phase times with E13_TIMING=1 and a line per autotuner trial with E13_TUNE_LOG=1.
  */
 #include <stdio.h>
 #include <stdlib.h>
