@@ -8,6 +8,9 @@
  *   e13oracle delete  machine.txt FMAX X...
  *   e13oracle free    machine.txt X CELL...
  *   e13oracle freesym machine.txt X LO HI CELL...
+ *   e13oracle estimate machine.txt PROBES X FMAX LO HI [CELL...]
+ *                     (probe i draws from its own stream, see e13oracle_core.c;
+ *                     seed from E13_SEED)
  *
  * Additions that the reference does not print: after every SURVIVOR line a
  *   HASH X=.. ch0=%016llx ch1=.. ch2=..
@@ -226,6 +229,42 @@ int main(int argc, char **argv) {
     }
     for (int i = a0; i < argc; i++) fb[2 * (argv[i][0] - 'A') + (argv[i][1] - '0')] = 1;
     run_search(argv[2], X, 0, 1, fb, lo, hi, split);
+  } else if (!strcmp(mode, "estimate")) {
+    /* e13search.c:473-510: estimate machine PROBES X FMAX LO HI [CELL...];
+       LO < 0: no symmetry; cells: free mode.  One stream per probe. */
+    if (argc < 8) {
+      fprintf(stderr, "usage: e13oracle estimate machine.txt PROBES X FMAX LO HI [CELL...]\n");
+      return 1;
+    }
+    eo_problem p;
+    memset(&p, 0, sizeof p);
+    p.nq_base = nq;
+    memcpy(p.base, base, sizeof p.base);
+    long long probes = atoll(argv[3]);
+    p.X = atoi(argv[4]);
+    p.fmax = atoi(argv[5]);
+    p.sym_lo = p.sym_hi = -1;
+    if (atoi(argv[6]) >= 0) eo_set_sym(&p, atoi(argv[6]), atoi(argv[7]));
+    for (int i = 8; i < argc; i++) p.freebase[2 * (argv[i][0] - 'A') + (argv[i][1] - '0')] = 1;
+    p.freemode = argc > 8;
+    int rc = eo_setup(&p);
+    if (rc == -2 || rc == -3) {
+      fprintf(stderr, rc == -2 ? "symmetric state is named by a fixed cell or has one\n"
+                               : "start state in the symmetric states\n");
+      return 1;
+    }
+    if (rc == -1) {
+      fprintf(stderr, "X=%d forced > fmax\n", p.X);
+      return 1;
+    }
+    const char *seed_env = getenv("E13_SEED");
+    uint64_t seed = seed_env ? strtoull(seed_env, NULL, 10) : 0;
+    eo_estimate_sum es;
+    double w0 = omp_get_wtime();
+    eo_estimate(&p, seed, probes, 0, NULL, &es);
+    printf("estimate X=%d budget=%d probes=%lld wall=%.1fs\n", p.X, p.budget, probes, omp_get_wtime() - w0);
+    const char *name[4] = {"leaves", "nodes", "steps", "survivors"};
+    for (int j = 0; j < 4; j++) printf("  %-9s %.3e  (+- %.1e standard error)\n", name[j], es.mean[j], es.se[j]);
   } else {
     fprintf(stderr, "unknown mode %s\n", mode);
     return 1;

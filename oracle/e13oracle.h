@@ -118,6 +118,18 @@ uint64_t eo_synth_record_hash(uint64_t k, int result, int stage, long long t, in
                               int rel_ok, const uint64_t ch[3]);
 void eo_synth_digest(uint64_t seed0, uint64_t k0, long long n, int threads, eo_synth_sum *out);
 
+/* Probe estimator (probe(), e13search.c:256-313; estimate mode, :473-510).
+   Probe p draws from its own stream, state eo_probe_stream(seed, p); the
+   per-probe sums are leaves, nodes, steps, survivors (weighted). */
+uint64_t eo_probe_stream(uint64_t seed, uint64_t p);
+void eo_probe(const eo_problem *p, uint64_t *rng, double out[4]);
+typedef struct {
+  long long probes;
+  double sum[4], sq[4], mean[4], se[4]; /* leaves, nodes, steps, survivors */
+} eo_estimate_sum;
+int eo_estimate(const eo_problem *p, uint64_t seed, long long probes, int threads, double *per,
+                eo_estimate_sum *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -465,3 +465,134 @@ void eo_synth_digest(uint64_t seed0, uint64_t k0, long long n, int threads, eo_s
   *out = tot;
 }
 
+
+/* --------------------------------------------------------------------------
+   Probe estimator: probe() (e13search.c:256-313) and the estimate mode
+   (e13search.c:473-510).
+   -------------------------------------------------------------------------- */
+
+/* Probe p's own splitmix64 stream, started at a hashed point.  The reference
+   starts one stream per OpenMP thread, at splitmix(0x5EED0000 + thread +
+   (seed << 20)) (e13search.c:487-491), and a thread's probes share it, so a
+   probe's draws depend on the schedule; one stream per probe makes every
+   probe's result a function of (problem, seed, p) alone. */
+uint64_t eo_probe_stream(uint64_t seed, uint64_t p) {
+  uint64_t s = 0x5EED0000ULL + (seed << 40) + p;
+  return eo_splitmix(&s);
+}
+
+typedef struct {
+  eo_sim s, kids[4 * EO_MAXQ + 1];
+} eo_probe_ws;
+
+/* probe(): e13search.c:256-313.  One random root-to-leaf path; every child
+   of a node on the path is run and counted with the path weight w, the path
+   continues through a uniformly drawn live child and w grows by the number
+   of live children.  Adds to out[0..3] = leaves, nodes, steps, survivors in
+   the reference's order of additions, so the doubles agree bit for bit. */
+static void probe_ws(const eo_problem *p, uint64_t *rng, double *out, eo_probe_ws *ws) {
+  int tab[2 * EO_MAXQ], nintro = p->sym_n0, budget = p->budget;
+  uint8_t fixed[2 * EO_MAXQ] = {0};
+  double w = 1;
+  eo_sim *s = &ws->s;
+  memcpy(tab, p->tab, sizeof tab);
+  eo_sim_init(s, NULL);
+  int i = eo_run(s, tab, fixed, NULL);
+  out[1] += 1;
+  out[2] += (double)s->t;
+  if (i < 0) {
+    out[0] += 1;
+    if (i == EO_SURV) out[3] += 1;
+    return;
+  }
+  for (;;) {
+    int orig = tab[i], vals[4 * EO_MAXQ + 1], nbs[4 * EO_MAXQ + 1], nis[4 * EO_MAXQ + 1], k = 0;
+    if (orig != EO_FREE) { /* the keep child first (e13search.c:279) */
+      vals[k] = orig;
+      nbs[k] = budget;
+      nis[k] = nintro;
+      k++;
+    }
+    if (orig == EO_FREE || budget > 0) {
+      for (int v = 0; v < 4 * p->nq; v++) {
+        int ni = sym_next(p, v, nintro);
+        if (v == orig || ni < 0) continue;
+        vals[k] = v;
+        nbs[k] = orig == EO_FREE ? budget : budget - 1;
+        nis[k] = ni;
+        k++;
+      }
+    }
+    int live[4 * EO_MAXQ + 1], nlive = 0;
+    fixed[i] = 1;
+    for (int c = 0; c < k; c++) {
+      eo_sim *ch = &ws->kids[c];
+      *ch = *s;
+      tab[i] = vals[c];
+      long long t0 = ch->t;
+      int r = eo_run(ch, tab, fixed, NULL);
+      out[1] += w;
+      out[2] += w * (double)(ch->t - t0);
+      if (r < 0) {
+        out[0] += w;
+        if (r == EO_SURV) out[3] += w;
+      } else {
+        live[nlive++] = c;
+      }
+    }
+    if (!nlive) return;
+    int c = live[eo_splitmix(rng) % (uint64_t)nlive];
+    w *= nlive;
+    tab[i] = vals[c];
+    budget = nbs[c];
+    nintro = nis[c];
+    *s = ws->kids[c];
+    i = eo_run(s, tab, fixed, NULL); /* stops again at once at the same unbound cell */
+  }
+}
+
+void eo_probe(const eo_problem *p, uint64_t *rng, double out[4]) {
+  eo_probe_ws *ws = (eo_probe_ws *)malloc(sizeof(eo_probe_ws));
+  probe_ws(p, rng, out, ws);
+  free(ws);
+}
+
+/* The estimate mode (e13search.c:473-510) with one stream per probe: probes
+   0..n-1 on `threads` OpenMP threads (<= 0: the default), the per-probe sums
+   into per[4*k..4*k+3] when per is given, then the totals added in probe order
+   and the mean and standard error as the reference prints them
+   (e13search.c:503-506). */
+int eo_estimate(const eo_problem *p, uint64_t seed, long long probes, int threads, double *per,
+                eo_estimate_sum *out) {
+  memset(out, 0, sizeof *out);
+  if (p->budget < 0 || probes <= 0) return -1;
+  double *buf = per ? per : (double *)malloc(sizeof(double) * 4 * (size_t)probes);
+  if (!buf) return -1;
+  if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel
+  {
+    eo_probe_ws *ws = (eo_probe_ws *)malloc(sizeof(eo_probe_ws));
+    long long k;
+#pragma omp for schedule(dynamic, 256)
+    for (k = 0; k < probes; k++) {
+      double o[4] = {0, 0, 0, 0};
+      uint64_t rng = eo_probe_stream(seed, (uint64_t)k);
+      probe_ws(p, &rng, o, ws);
+      memcpy(buf + 4 * k, o, sizeof o);
+    }
+    free(ws);
+  }
+  out->probes = probes;
+  for (long long k = 0; k < probes; k++)
+    for (int j = 0; j < 4; j++) {
+      out->sum[j] += buf[4 * k + j];
+      out->sq[j] += buf[4 * k + j] * buf[4 * k + j];
+    }
+  for (int j = 0; j < 4; j++) {
+    double m = out->sum[j] / (double)probes, var = out->sq[j] / (double)probes - m * m;
+    out->mean[j] = m;
+    out->se[j] = var > 0 ? sqrt(var / (double)probes) : 0.0;
+  }
+  if (!per) free(buf);
+  return 0;
+}

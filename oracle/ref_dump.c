@@ -11,6 +11,13 @@
  *   e13ref_dump hash < lines    for each SURVIVOR line, the counter hashes
  *                               ch[0..2] left by run() on that table with
  *                               every cell fixed ('?---' read as halt)
+ *   e13ref_dump probe machine X FMAX LO HI SEED N [CELL...]
+ *                               the per-probe sums (leaves, nodes, steps,
+ *                               survivors; hex of the doubles) of the
+ *                               reference's own probe() for probes 0..N-1 of
+ *                               the estimate problem (argv as in its estimate
+ *                               mode), probe i drawing from the splitmix state
+ *                               splitmix(0x5EED0000 + (SEED << 40) + i)
  */
 #define main e13ref_main
 #include REF_SRC
@@ -44,7 +51,7 @@ static int parse_tab(const char *tb, int *tab) {
 int main(int argc, char **argv) {
   ref_init();
   if (argc < 2) {
-    fprintf(stderr, "usage: e13ref_dump keys|hash ...\n");
+    fprintf(stderr, "usage: e13ref_dump keys|hash|probe ...\n");
     return 1;
   }
   if (!strcmp(argv[1], "keys")) {
@@ -68,6 +75,32 @@ int main(int argc, char **argv) {
              (unsigned long long)s->ch[1], (unsigned long long)s->ch[2], r, s->t);
     }
     free(s);
+    return 0;
+  }
+  if (!strcmp(argv[1], "probe") && argc >= 9) {
+    load(argv[2]);
+    int X = atoi(argv[3]), fmax = atoi(argv[4]), tab[2 * MAXQ];
+    unsigned long long seed = strtoull(argv[7], NULL, 10);
+    long n = atol(argv[8]);
+    unsigned char freebase[2 * MAXQ] = {0};
+    if (atoi(argv[5]) >= 0) set_sym(X, atoi(argv[5]), atoi(argv[6]));
+    for (int i = 9; i < argc; i++) freebase[2 * (argv[i][0] - 'A') + (argv[i][1] - '0')] = 1;
+    int budget = setup(X, fmax, argc > 9 ? freebase : NULL, tab);
+    if (budget < 0) return 1;
+    Ctx *x = (Ctx *)calloc(1, sizeof(Ctx));
+    x->nq = NQ - (X >= 0);
+    printf("probe X=%d budget=%d seed=%llu n=%ld\n", X, budget, seed, n);
+    for (long i = 0; i < n; i++) {
+      uint64_t st = 0x5EED0000ULL + ((uint64_t)seed << 40) + (uint64_t)i;
+      uint64_t rng = splitmix(&st);
+      double o[4] = {0, 0, 0, 0};
+      probe(x, tab, budget, &rng, o);
+      uint64_t b[4];
+      memcpy(b, o, sizeof b);
+      printf("%ld %016llx %016llx %016llx %016llx\n", i, (unsigned long long)b[0], (unsigned long long)b[1],
+             (unsigned long long)b[2], (unsigned long long)b[3]);
+    }
+    free(x);
     return 0;
   }
   (void)e13ref_main;

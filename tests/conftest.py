@@ -163,3 +163,90 @@ def synth_golden():
     with open(os.path.join(GOLDEN, "synth", "synth_5eed.txt")) as f:
         return {int(re.search(r" n=(\d+)", l).group(1)): l.strip() for l in f if l.startswith("synth ")}
 
+
+class EoProblem(ctypes.Structure):
+    _fields_ = [("nq_base", ctypes.c_int), ("base", ctypes.c_int * 32), ("X", ctypes.c_int), ("fmax", ctypes.c_int),
+                ("freemode", ctypes.c_int), ("freebase", ctypes.c_uint8 * 32), ("sym_lo", ctypes.c_int),
+                ("sym_hi", ctypes.c_int), ("nq", ctypes.c_int), ("budget", ctypes.c_int), ("sym_n0", ctypes.c_int),
+                ("tab", ctypes.c_int * 32)]
+
+
+def oracle_setup(base, X, fmax, free=None, sym=None):
+    L = oracle_lib()
+    p = EoProblem()
+    p.nq_base = len(base) // 2
+    for i, v in enumerate(base):
+        p.base[i] = v
+    p.X, p.fmax = X, fmax
+    p.sym_lo = p.sym_hi = -1
+    if free is not None:
+        p.freemode = 1
+        for c in free:
+            p.freebase[2 * (ord(c[0]) - 65) + int(c[1])] = 1
+    if sym:
+        L.eo_set_sym(ctypes.byref(p), sym[0], sym[1])
+    rc = L.eo_setup(ctypes.byref(p))
+    return rc, p
+
+
+def probe_fixture(name):
+    """A per-probe dump of the reference's probe(): (args, rows) with args the
+    estimate problem (machine, X, FMAX, LO, HI, SEED, N, cells) and rows the
+    four sums of each probe as the bit patterns of the doubles."""
+    lines = read_text(REF_DUMP, f"probe_{name}.txt").splitlines()
+    a = lines[0].split()[1:]
+    args = dict(machine=a[0], X=int(a[1]), fmax=int(a[2]), lo=int(a[3]), hi=int(a[4]), seed=int(a[5]), n=int(a[6]),
+                cells=a[7:])
+    rows = [tuple(int(v, 16) for v in l.split()[1:5]) for l in lines[2:]]
+    assert len(rows) == args["n"]
+    return args, rows
+
+
+PROBE_CASES = ["f2_x8", "f3_x0", "f1_nodel", "free_m1n0", "counter3", "v1_0ld"]
+
+
+def oracle_probe_problem(args):
+    """eo_problem of an estimate problem (free mode when cells are given)."""
+    base = []
+    for cell in read_text(MACHINES, args["machine"]).split():
+        base.append(-1 if cell.startswith("-") else (int(cell[2:]) << 2) | ((cell[1] == "R") << 1) | int(cell[0]))
+    sym = (args["lo"], args["hi"]) if args["lo"] >= 0 else None
+    return oracle_setup(base, args["X"], args["fmax"], args["cells"] or None, sym)
+
+
+def oracle_probes(L, p, seed, n):
+    """Per-probe sums of eo_probe on streams eo_probe_stream(seed, 0..n-1), as bit patterns."""
+    L.eo_probe_stream.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+    L.eo_probe_stream.restype = ctypes.c_uint64
+    out = []
+    for i in range(n):
+        rng = ctypes.c_uint64(L.eo_probe_stream(seed, i))
+        o = (ctypes.c_double * 4)()
+        L.eo_probe(ctypes.byref(p), ctypes.byref(rng), o)
+        out.append(tuple(ctypes.c_uint64.from_buffer_copy(ctypes.c_double(v)).value for v in o))
+    return out
+
+
+def v1c0_exact_rows():
+    """Rows of 4_division_6/v1/v1c0_est.txt whose standard errors are 0 (the
+    value of C0 leaves a one-leaf tree, so the estimate is exact): (value,
+    machine path, leaves, steps), the machine being c0_0LD.txt with C0 set to
+    the value (the file's own definition of its rows)."""
+    import tempfile
+    base = read_text(MACHINES, "c0_0LD.txt").split("\n")
+    d = os.path.join(tempfile.gettempdir(), "e13_c0_machines")
+    os.makedirs(d, exist_ok=True)
+    rows = []
+    for line in read_text(REF_RESULTS, "4_division_6_v1_v1c0_est.txt").splitlines():
+        f = line.split()
+        if not line.startswith("c0_") or len(f) < 7 or f[3] != "0.0e+00" or f[6] != "0.0e+00":
+            continue
+        v = f[0][3:]
+        cell = f"{v[0]}{v[1]}{ord(v[2]) - 65}"
+        rows_m = list(base)
+        rows_m[2] = cell + " " + rows_m[2].split()[1]
+        path = os.path.join(d, f"c0_{v}.txt")
+        with open(path, "w") as fh:
+            fh.write("\n".join(rows_m))
+        rows.append((v, path, float(f[2]), float(f[5])))
+    return rows

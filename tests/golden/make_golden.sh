@@ -8,7 +8,8 @@
 #                 (oracle/_ref/): per-X summaries with step totals for the
 #                 local-edit searches, the Zobrist KEY/TGT tables, and the
 #                 counter hashes ch[0..2] its run() leaves on every fixture
-#                 survivor table.
+#                 survivor table, and the per-probe sums of its probe()
+#                 estimator (probe_*.txt) on six estimate problems.
 #
 # Needs /root/reference (this container only) and `make -C oracle ref`.
 # FMAX=4 takes over an hour on 8 cores (62 min here): pass --fmax4 to include it.
@@ -19,7 +20,7 @@ ROOT=$(cd "$HERE/../.." && pwd)
 BIN=$ROOT/oracle/_ref
 mkdir -p "$HERE/ref_results" "$HERE/ref_dump" "$HERE/machines"
 
-cp "$REF/machines/erdos_14.txt" "$REF/machines/split_15.txt" "$HERE/machines/"
+cp "$REF/machines/erdos_14.txt" "$REF/machines/split_15.txt" "$REF/machines/c0_0LD.txt" "$HERE/machines/"
 for f in 1_local_edits/log_f2.txt 1_local_edits/log_f3.txt 1_local_edits/log_f4.txt \
          1_local_edits/surv_f2.txt 1_local_edits/surv_f3.txt 1_local_edits/surv_f4.txt \
          2_counter_3/log_mod.txt 2_counter_3/log_modsym.txt 2_counter_3/g_modsym.log \
@@ -28,7 +29,8 @@ for f in 1_local_edits/log_f2.txt 1_local_edits/log_f3.txt 1_local_edits/log_f4.
          4_division_6/v4/log_v4.txt 4_division_6/v4/g_v4.log \
          4_division_6/v3/log_v3.txt 4_division_6/v3/g_v3.log \
          validation/base_erdos14.txt validation/mutants.txt \
-         validation/split_test.txt validation/split_test_free.txt; do
+         validation/split_test.txt validation/split_test_free.txt \
+         4_division_6/v1/v1c0_est.txt; do
   # fixtures keep a .txt suffix whatever the source suffix
   cp "$REF/results/$f" "$HERE/ref_results/$(echo "$f" | tr '/' '_' | sed 's/\.log$/.txt/')"
 done
@@ -51,4 +53,16 @@ cat "$REF"/results/1_local_edits/surv_f{2,3,4}.txt "$REF"/results/2_counter_3/su
 grep SURVIVOR "$REF/results/validation/split_test.txt" >> "$HERE/ref_dump/_surv_in.txt"
 "$BIN/e13ref_dump" hash < "$HERE/ref_dump/_surv_in.txt" > "$HERE/ref_dump/survivor_hashes.txt"
 rm -f "$HERE/ref_dump/_surv_in.txt"
+# per-probe sums of the reference's probe() (estimate mode), one stream per
+# probe; the first line records the arguments: machine X FMAX LO HI SEED N CELL...
+probe_dump() {
+  local name=$1; shift
+  { echo "args $*"; "$BIN/e13ref_dump" probe "$HERE/machines/$1" "${@:2}"; } > "$HERE/ref_dump/probe_$name.txt"
+}
+probe_dump f2_x8 erdos_14.txt 8 2 -1 -1 0 600
+probe_dump f3_x0 erdos_14.txt 0 3 -1 -1 1 400
+probe_dump f1_nodel erdos_14.txt -1 1 -1 -1 2 400
+probe_dump free_m1n0 erdos_14.txt -1 0 -1 -1 3 600 M1 N0
+probe_dump counter3 erdos_14.txt 13 0 10 13 4 600 D0 G0 K0 K1 L1 M1
+probe_dump v1_0ld c0_0LD.txt 9 0 3 9 5 600 D0 D1 E0 E1 F0 F1 G0 G1 H0 H1 I0 I1
 echo "golden fixtures written under $HERE"

@@ -7,8 +7,8 @@ import re
 
 import pytest
 
-from conftest import (REF_DUMP, REF_RESULTS, ROOT, XS, load_pkg, oracle_lib, parse_summaries, read_machine,
-                      read_text)
+from conftest import (REF_DUMP, REF_RESULTS, ROOT, XS, load_pkg, oracle_lib, oracle_setup, parse_summaries,
+                      read_machine, read_text)
 
 
 @pytest.fixture(scope="module")
@@ -46,31 +46,6 @@ def test_parse_machine(e13):
     assert base[27] == (12 << 2) | 0 | 0     # 0LM
     with pytest.raises(e13.E13Error):
         e13.parse_machine("1L1 2R0\n")
-
-
-class EoProblem(ctypes.Structure):
-    _fields_ = [("nq_base", ctypes.c_int), ("base", ctypes.c_int * 32), ("X", ctypes.c_int), ("fmax", ctypes.c_int),
-                ("freemode", ctypes.c_int), ("freebase", ctypes.c_uint8 * 32), ("sym_lo", ctypes.c_int),
-                ("sym_hi", ctypes.c_int), ("nq", ctypes.c_int), ("budget", ctypes.c_int), ("sym_n0", ctypes.c_int),
-                ("tab", ctypes.c_int * 32)]
-
-
-def oracle_setup(base, X, fmax, free=None, sym=None):
-    L = oracle_lib()
-    p = EoProblem()
-    p.nq_base = len(base) // 2
-    for i, v in enumerate(base):
-        p.base[i] = v
-    p.X, p.fmax = X, fmax
-    p.sym_lo = p.sym_hi = -1
-    if free is not None:
-        p.freemode = 1
-        for c in free:
-            p.freebase[2 * (ord(c[0]) - 65) + int(c[1])] = 1
-    if sym:
-        L.eo_set_sym(ctypes.byref(p), sym[0], sym[1])
-    rc = L.eo_setup(ctypes.byref(p))
-    return rc, p
 
 
 @pytest.mark.parametrize("fmax", [2, 3, 4])

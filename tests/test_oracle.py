@@ -17,14 +17,21 @@ Pins, by oracle function:
   eo_recheck                every fixture survivor re-checks, with the '?' marks
                             of its line equal to the unread cells.
   eo_synth_table            the brute-force simulator on the same tables.
+  eo_probe / eo_estimate    per-probe sums of the reference's own probe()
+                            (ref_dump/probe_*.txt, one stream per probe), the
+                            exact rows of 4_division_6/v1/v1c0_est.txt (values
+                            of C0 whose tree is one leaf), and the exact search
+                            of a small free-mode tree, which the mean of many
+                            probes must estimate without bias.
 """
 import ctypes
 import random
 
 import pytest
 
-from conftest import (MACHINES, REF_DUMP, REF_RESULTS, XS, hash_lines_by_survivor, machine_path, oracle_lib,
-                      parse_summaries, read_text, ref_hashes, run_oracle, survivor_lines)
+from conftest import (MACHINES, PROBE_CASES, REF_DUMP, REF_RESULTS, XS, hash_lines_by_survivor, machine_path,
+                      oracle_lib, oracle_probe_problem, oracle_probes, parse_summaries, probe_fixture, read_text,
+                      ref_hashes, run_oracle, survivor_lines, v1c0_exact_rows)
 
 T14S = (0, 2, 22, 90, 268, 944, 3434, 13450, 52068, 208864)
 
@@ -268,3 +275,40 @@ def test_planted_mistake_is_caught(mistake):
             continue
         differ += brute_run(tab, dl, mistake) != r
     assert differ > 0
+
+
+@pytest.mark.parametrize("name", PROBE_CASES)
+def test_probe_matches_reference_probe(name):
+    """eo_probe gives the same four doubles, bit for bit, as the reference's own
+    probe() on the same stream, probe by probe."""
+    args, rows = probe_fixture(name)
+    rc, p = oracle_probe_problem(args)
+    assert rc == 0
+    got = oracle_probes(oracle_lib(), p, args["seed"], args["n"])
+    assert got == rows
+    assert len(set(rows)) > 5  # the probes take different paths
+
+
+def test_estimate_exact_rows_of_v1c0():
+    """Values of C0 whose whole tree is one leaf: every probe sees the same leaf,
+    so the estimate is exact and equals the reference's printed row."""
+    rows = v1c0_exact_rows()
+    assert len(rows) >= 20
+    for v, path, leaves, steps in rows[:8]:
+        out, _ = run_oracle("estimate", path, "200", "9", "0", "3", "9",
+                            *"D0 D1 E0 E1 F0 F1 G0 G1 H0 H1 I0 I1".split())
+        got = {l.split()[0]: float(l.split()[1]) for l in out.splitlines()[1:]}
+        assert got["leaves"] == leaves and got["steps"] == steps, v
+
+
+def test_estimate_mean_is_unbiased_on_a_small_tree():
+    """The small free-mode tree (erdos_14, cells M1 N0 free): the mean over many
+    probes lands within 5 standard errors of the exact search's leaves, steps
+    and survivors."""
+    _, err = run_oracle("free", machine_path(), "-1", "M1", "N0")
+    exact = parse_summaries(err)[-1]
+    out, _ = run_oracle("estimate", machine_path(), "200000", "-1", "0", "-1", "-1", "M1", "N0")
+    est = {l.split()[0]: (float(l.split()[1]), float(l.split()[3])) for l in out.splitlines()[1:]}
+    for key, ex in (("leaves", exact["leaves"]), ("steps", exact["steps"]), ("survivors", exact["survivors"])):
+        m, se = est[key]
+        assert se > 0 and abs(m - ex) < 5 * se, (key, m, se, ex)
