@@ -119,13 +119,30 @@ class Stats(ctypes.Structure):
 
 
 
+class EstimateSum(ctypes.Structure):
+    """e13_estimate_sum: per-quantity sums, squares, means and standard errors
+    (leaves, nodes, steps, survivors), plus the device's own figures."""
+    _fields_ = [("probes", ctypes.c_int64), ("sum", ctypes.c_double * 4), ("sq", ctypes.c_double * 4),
+                ("mean", ctypes.c_double * 4), ("se", ctypes.c_double * 4), ("sim_steps", ctypes.c_uint64),
+                ("kernel_ms", ctypes.c_double), ("launches", ctypes.c_int64), ("rounds", ctypes.c_int64),
+                ("lane_slots", ctypes.c_int64)]
+
+    NAMES = ("leaves", "nodes", "steps", "survivors")
+
+    def as_dict(self):
+        d = {n: (self.mean[j], self.se[j]) for j, n in enumerate(self.NAMES)}
+        d.update(probes=self.probes, sim_steps=self.sim_steps, kernel_ms=self.kernel_ms, launches=self.launches,
+                 rounds=self.rounds, lane_slots=self.lane_slots)
+        return d
+
+
 EXPORTS = ("e13_open", "e13_close", "e13_strerror", "e13_set_tuning", "e13_get_tuning", "e13_get_stats",
            "e13_keys", "e13_parse_machine", "e13_setup", "e13_search", "e13_search_many", "e13_simulate_batch",
            "e13_simulate_batch_device", "e13_sync", "e13_recheck", "e13_format_survivor", "e13_format_hash",
            "e13_synth_table", "e13_get_trace", "e13_batch_digest", "e13_synth_run", "e13_get_cost_log",
            "e13_cost_targets", "e13_mlp_init", "e13_mlp_forward", "e13_mlp_grad", "e13_mlp_train",
            "e13_set_cost_model", "e13_autotune", "e13_save_tuning", "e13_load_tuning", "e13_tuning_file",
-           "e13_tuning_file_small", "e13_search_profile")
+           "e13_tuning_file_small", "e13_search_profile", "e13_estimate")
 
 _lib = None
 
@@ -193,6 +210,8 @@ def lib():
     L.e13_tuning_file.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
     L.e13_tuning_file_small.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
     L.e13_search_profile.argtypes = [ctypes.c_void_p, P(Problem), ctypes.c_int, P(Tuning)]
+    L.e13_estimate.argtypes = [ctypes.c_void_p, P(Problem), ctypes.c_uint64, ctypes.c_int64, P(ctypes.c_double),
+                               P(EstimateSum)]
     L.e13_get_trace.argtypes = [ctypes.c_void_p, P(ctypes.c_uint64), ctypes.c_uint64, P(ctypes.c_int32),
                                 P(ctypes.c_int32)]
     _lib = L
@@ -472,6 +491,16 @@ class Context:
         prof = lib().e13_search_profile(self._h, ps, n, ctypes.byref(t))
         _check(min(prof, 0), "e13_search_profile")
         return prof, t.as_dict()
+
+    def estimate(self, p, probes, seed=0, per_probe=False):
+        """The probe estimator (the reference's estimate mode): EstimateSum, and
+        with per_probe=True also the list of each probe's four sums as doubles."""
+        es = EstimateSum()
+        buf = (ctypes.c_double * (4 * probes))() if per_probe else None
+        _check(lib().e13_estimate(self._h, ctypes.byref(p), seed, probes, buf, ctypes.byref(es)), "e13_estimate")
+        if per_probe:
+            return es, [tuple(buf[4 * i:4 * i + 4]) for i in range(probes)]
+        return es
 
     def sync(self):
         _check(lib().e13_sync(self._h), "e13_sync")

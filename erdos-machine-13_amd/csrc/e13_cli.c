@@ -5,6 +5,9 @@
  *   e13search_gpu delete  machine.txt FMAX X...
  *   e13search_gpu free    machine.txt X CELL...
  *   e13search_gpu freesym machine.txt X LO HI CELL...
+ *   e13search_gpu estimate machine.txt PROBES X FMAX LO HI [CELL...]   (probe estimator; LO < 0: no
+ *                                       symmetry; cells: free mode; probe i draws from its own stream of
+ *                                       seed $E13_SEED, default 0)
  *   e13search_gpu synth   SEED0 K0 N      (the synthetic stress batch; one summary line)
  *   e13search_gpu tune    machine.txt FMAX X...   (autotune on that search; saves the settings
  *   e13search_gpu tune    machine.txt freesym X LO HI CELL...   to $E13_TUNING_OUT or to the
@@ -406,6 +409,49 @@ int main(int argc, char **argv) {
             best.wgs_per_cu, best.slice, best.refill_tries, best.solo_steps, best.seed_per_wave, best.donate_max,
             best.sched_every, best.wg_threads, best.idle_naps, best.evict_after);
     if (path[0] && e13_save_tuning(&best, path) == E13_OK) fprintf(stderr, "autotune: saved to %s\n", path);
+  } else if (!strcmp(mode, "estimate")) { /* estimate machine.txt PROBES X FMAX LO HI [CELL...] (e13search.c:473-510) */
+    if (argc < 8) {
+      fprintf(stderr, "usage: e13search_gpu estimate machine.txt PROBES X FMAX LO HI [CELL...]\n");
+      return 1;
+    }
+    const long long probes = atoll(argv[3]);
+    const int X = atoi(argv[4]), fmax = atoi(argv[5]), lo = atoi(argv[6]), hi = atoi(argv[7]);
+    uint8_t fb[32] = {0};
+    for (int i = 8; i < argc; i++) {
+      int q = argv[i][0] - 'A', s = argv[i][1] - '0';
+      if (q < 0 || q >= nq || (s != 0 && s != 1)) {
+        fprintf(stderr, "bad cell %s\n", argv[i]);
+        return 1;
+      }
+      fb[2 * q + s] = 1;
+    }
+    e13_problem p;
+    rc = e13_setup(base, nq, X, fmax, argc > 8 ? fb : NULL, lo, lo >= 0 ? hi : -1, &p);
+    if (rc == E13_EFORCED) {
+      fprintf(stderr, "X=%d forced=%d > fmax\n", X, p.nforced);
+      return 1;
+    }
+    if (rc) {
+      fprintf(stderr, "setup: %s\n", e13_strerror(rc));
+      return 1;
+    }
+    const char *seed_env = getenv("E13_SEED");
+    const uint64_t seed = seed_env ? strtoull(seed_env, NULL, 10) : 0;
+    e13_estimate_sum es;
+    const double w0 = now_s();
+    rc = probes > 0 ? e13_estimate(ctx, &p, seed, probes, NULL, &es) : E13_EINVAL;
+    if (rc) {
+      fprintf(stderr, "estimate: %s\n", e13_strerror(rc));
+      return 1;
+    }
+    printf("estimate X=%d budget=%d probes=%lld wall=%.1fs\n", X, p.budget, probes, now_s() - w0);
+    const char *name[4] = {"leaves", "nodes", "steps", "survivors"};
+    for (int j = 0; j < 4; j++) printf("  %-9s %.3e  (+- %.1e standard error)\n", name[j], es.mean[j], es.se[j]);
+    fprintf(stderr, "  gpu: kernel=%.1fms %.3e probes/s %.3e steps/s (%llu steps), launches=%lld "
+                    "lanes running per round=%.1f\n",
+            es.kernel_ms, es.kernel_ms > 0 ? 1e3 * (double)probes / es.kernel_ms : 0.0,
+            es.kernel_ms > 0 ? 1e3 * (double)es.sim_steps / es.kernel_ms : 0.0, (unsigned long long)es.sim_steps,
+            (long long)es.launches, es.rounds ? (double)es.lane_slots / (double)es.rounds : 0.0);
   } else if (!strcmp(mode, "free") || !strcmp(mode, "freesym")) {
     if (argc < 4) return 1;
     uint8_t fb[32] = {0};

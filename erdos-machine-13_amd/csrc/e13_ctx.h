@@ -29,6 +29,9 @@ int e13k_launch_chain_pop(E13Chain *ch, E13Counters *ctr, hipStream_t s);
 int e13k_launch_chain_push(E13Chain *ch, E13Node *stack, uint64_t stack_cap, const E13Node *out, uint32_t out_cap,
                            const E13Counters *ctr, int grid, hipStream_t s);
 
+int e13k_probe_max_wgs(int wg_threads, int gate);
+int e13k_launch_probe(const E13ProbeArgs *a, int grid, int wg_threads, int gate, hipStream_t s);
+
 _Static_assert(sizeof(E13Node) % 16 == 0, "node records are copied as 16-byte vectors");
 
 
@@ -80,6 +83,11 @@ struct e13_ctx {
   size_t seed_cnt_cap;
   e13_cost_rec *cost_log; /* per frontier node of the last logged search */
   int64_t cost_n, cost_cap;
+  /* probe estimator (e13_estimate.c) */
+  double *d_probe_out;
+  size_t probe_cap;      /* probes the output buffer holds */
+  uint32_t *d_probe_save;
+  size_t probe_save_cap; /* waves the save area holds */
 };
 
 /* The MLP kernels (e13_mlp.hip). */
@@ -100,6 +108,7 @@ E13_INTERNAL void e13i_fill_problem(const e13_problem *p, E13Problem *pb);
 E13_INTERNAL void e13i_node_features(const e13_problem *p, const E13Node *nd, float *f);
 E13_INTERNAL int e13i_mlp_predict_dev(e13_ctx *c, const float *x_host, int n, float *y_host);
 E13_INTERNAL void e13i_free_cost(e13_ctx *c);
+E13_INTERNAL void e13i_free_estimate(e13_ctx *c);
 /* stored per-device tuning (e13_tune.c) */
 E13_INTERNAL int e13i_tuning_path(const char *arch, int ncu, int small, char *buf, int len);
 

@@ -186,4 +186,19 @@ typedef struct E13BatchArgs {
   int32_t pad_b;
 } E13BatchArgs;
 
+/* Probe estimator (probe(), e13search.c:256-313): one wave walks one probe. */
+typedef struct E13ProbeArgs {
+  const E13Problem *pb;   /* the problem (base codes, symmetry) */
+  const uint64_t *key;
+  const uint64_t *tgt;
+  double *out;            /* n x 4: leaves, nodes, steps, survivors of each probe (weighted) */
+  E13Counters *ctr;       /* item = next probe of this launch; steps, rounds, lane slots */
+  uint32_t *save;         /* per wave 64 x E13_TW tape words: finished children kept while a lone lane runs */
+  uint64_t seed;          /* probe p0 + i draws from the stream of (seed, p0 + i) */
+  uint64_t p0;
+  uint32_t n;             /* probes in this launch */
+  int32_t nq, slice, budget, nintro;
+  int32_t solo;           /* > 0: steps per round of the scalar loop for a wave's lone running lane */
+} E13ProbeArgs;
+
 #endif

@@ -318,6 +318,7 @@ void e13_close(e13_ctx *c) {
   hipFree(c->d_seeds);
   hipFree(c->d_trace);
   e13i_free_cost(c);
+  e13i_free_estimate(c);
   hipFree(c->d_chain);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);

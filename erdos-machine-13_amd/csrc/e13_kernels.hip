@@ -1665,6 +1665,250 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Probe estimator (probe(), e13search.c:256-313; estimate mode, :473-510).
+// One wave walks one probe at a time.  At each node of the path lane s runs
+// child slot s (slot 0 keeps the cell's base value, slot 1+v binds it to move
+// v, as in the expansion) to its next stop; the wave adds w per child to the
+// probe's sums in slot order -- the reference's child order, so the doubles
+// agree bit for bit -- then draws a live child from the probe's own splitmix
+// stream and continues from it with w *= (live children).  Every cell starts
+// unread (the reference's probe starts with fixed[] all zero), so the walk
+// stops at the first read of every cell, one-child cells included.  The node
+// on the path is the chosen lane's registers and LDS tape; the table is a
+// per-wave copy in LDS.  A wave whose last running child is alone runs it in
+// the scalar loop, whose block memo lives in idle lanes' tape words 0..31:
+// the finished live children's words 0..31 are kept in global memory first.
+// ---------------------------------------------------------------------------
+namespace {
+
+__device__ __forceinline__ uint64_t probe_mix(uint64_t &s) { // splitmix64, as e13search.c:98-103
+  s += 0x9E3779B97F4A7C15ull;
+  uint64_t z = s;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// Probe p's stream: its own splitmix64 state, started at a hashed point.
+__device__ __forceinline__ uint64_t probe_stream(uint64_t seed, uint64_t p) {
+  uint64_t s = 0x5EED0000ull + (seed << 40) + p;
+  return probe_mix(s);
+}
+
+__device__ __forceinline__ uint64_t rl64(uint64_t v, int l) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+struct ProbeSums {
+  double leaves, nodes, steps, surv;
+};
+
+// The children of one node in slot order (e13search.c:289-300): nodes += w,
+// steps += w * dt, leaves += w for an ended child, survivors += w for a
+// survivor.  No contraction: every product and sum rounds as the reference's.
+__device__ __forceinline__ void probe_add(ProbeSums &S, double w, uint64_t exist, uint64_t ended,
+                                                    uint64_t surv, int dt) {
+#pragma clang fp contract(off)
+  while (exist) {
+    const int s = __builtin_ctzll(exist);
+    exist &= exist - 1;
+    const int d = __builtin_amdgcn_readlane(dt, s);
+    S.nodes = S.nodes + w;
+    S.steps = S.steps + w * (double)d;
+    if ((ended >> s) & 1ull) {
+      S.leaves = S.leaves + w;
+      if ((surv >> s) & 1ull) S.surv = S.surv + w;
+    }
+  }
+}
+
+} // namespace
+
+template <int GATE>
+__global__ void __launch_bounds__(256) k_probe(E13ProbeArgs a) {
+  extern __shared__ __align__(16) uint32_t smem[];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  Ctx c;
+  c.sb = (char *)smem;
+  c.rbl = wave * ROWS_BYTES + lane * 4;
+  c.tbl = nwaves * ROWS_BYTES + wave * TAPE_BYTES + lane * 4;
+  c.keyb = nwaves * (ROWS_BYTES + TAPE_BYTES);
+  c.key = a.key;
+  c.tgt = a.tgt;
+  c.dl = a.pb->dl;
+  if (lds_base(smem) != 0u) { // the absolute LDS addressing needs the dynamic area at address 0
+    if (threadIdx.x == 0) atomicOr(&a.ctr->overflow, 4u);
+    return;
+  }
+  // the path's table: 32 codes per wave in the 64 bytes after the tapes and block keys
+  uint8_t *ptab = (uint8_t *)(c.sb + c.keyb + (GATE == GATE_HASH ? KEYB_WORDS * 8 : 0)) + wave * 64;
+  if (GATE == GATE_HASH) fill_keyb(c.sb, c.keyb, a.key);
+  __syncthreads();
+  const int wid = blockIdx.x * nwaves + wave;
+  const int tbase = c.tbl - lane * 4; // word w of lane j at tbase + w*256 + j*4
+  uint32_t *save = a.save + (size_t)wid * (64 * 32);
+  const int nq = a.nq, nslot = 4 * nq + 1;
+  Lane L;
+  L.dlcap = -1;
+  L.tcap = 0x7FFFFFFF;
+  uint64_t my_steps = 0, w_slots = 0, w_rounds = 0, fills = 0, solo_last = ~0ull;
+  for (;;) {
+    uint32_t p = 0;
+    if (lane == 0) p = atomicAdd(&a.ctr->item, 1u);
+    p = (uint32_t)uni((int)p);
+    if (p >= a.n) break;
+    uint64_t rng = probe_stream(a.seed, a.p0 + p);
+    // the root: the blank tape, stopped before the read of cell 0 at t = 0
+    // (e13search.c:268-271: nodes 1, steps 0)
+    if (lane < E13_MAXCELLS) ptab[lane] = lane < 2 * nq && a.pb->base[lane] != -2 ? E13_C_EDIT : E13_C_FREE;
+    int n_q = E13_QOFF, n_st = 0, n_t = 0, n_nt = 1, n_stage = 0, n_st0 = -1, n_q0 = E13_QOFF, n_flags = 3;
+    int n_ones = 0;
+    uint64_t n_hb = 0, n_ch0 = 0, n_ch1 = 0, n_ch2 = 0;
+    int pend = 0, budget = a.budget, nintro = a.nintro, src = -1;
+    uint64_t src_saved = 0; // 1: the source lane's words 0..31 are in save[]
+    ProbeSums S;
+    S.leaves = 0.0;
+    S.nodes = 1.0;
+    S.steps = 0.0;
+    S.surv = 0.0;
+    double w = 1.0;
+    for (;;) {
+      wave_lds_sync();
+      // ---- the node's children, lane = slot (e13search.c:278-287) ----
+      const int orig = a.pb->base[pend];
+      int ok = 0, val = -1, nb = budget, ni = nintro;
+      bool halt_keep = false;
+      if (lane < nslot) {
+        if (lane == 0) {
+          if (orig != -2) {
+            ok = 1;
+            val = orig;
+            halt_keep = orig == -1; // keeping a halt: dies on the read, 0 steps (e13search.c:164-166)
+          }
+        } else {
+          const int v = lane - 1;
+          if ((orig == -2 || budget > 0) && v != orig) {
+            const int n2 = sym_next(v, nintro, a.pb->sym_lo, a.pb->sym_hi);
+            if (n2 >= 0) {
+              ok = 1;
+              val = v;
+              ni = n2;
+              if (orig != -2) nb = budget - 1;
+            }
+          }
+        }
+      }
+      const bool run = ok && !halt_keep;
+      if (run) {
+        for (int wd = 0; wd < E13_TW; ++wd) {
+          uint32_t v = 0u;
+          if (src >= 0) v = (wd < 32 && src_saved) ? save[wd * 64 + src] : lds_ld(c.sb, tbase + wd * 256 + src * 4);
+          lds_st(c.sb, c.tbl + wd * 256, v);
+        }
+        build_rows(c, ptab, nq, pend, val, 1, nullptr);
+        L.q = n_q;
+        L.row = c.rbl + n_st * 256;
+        L.t = n_t;
+        L.nt = n_nt;
+        L.stage = n_stage;
+        L.st0 = n_st0;
+        L.q0 = n_q0;
+        L.flags = n_flags;
+        L.ones = n_ones;
+        L.hb = n_hb;
+        L.ch0 = n_ch0;
+        L.ch1 = n_ch1;
+        L.ch2 = n_ch2;
+        arm_stage(L, c);
+      }
+      fills++;
+      int active = run ? 1 : 0, res = halt_keep ? EV_DEAD : EV_NONE, evc = -1;
+      bool saved = false;
+      uint64_t savedm = 0; // lanes whose words 0..31 went to save[]
+      // ---- run every child to its next stop ----
+      for (;;) {
+        const uint64_t act = ballot(active);
+        if (act == 0) break;
+        w_rounds++;
+        w_slots += (uint64_t)__popcll(act);
+        int ev = EV_NONE, evcell = -1;
+        const bool solo = GATE == GATE_COUNT && a.solo > 0 && __popcll(act) == 1;
+        if (solo) { // the whole wave runs the lone lane's loop
+          if (!saved) { // the memo is about to take idle lanes' words 0..31
+            saved = true;
+            savedm = ballot(res == EV_STOP);
+            if (res == EV_STOP)
+              for (int wd = 0; wd < 32; ++wd) save[wd * 64 + lane] = lds_ld(c.sb, c.tbl + wd * 256);
+            wave_mem_sync();
+          }
+          solo_memo_reset(active, c, act, fills, solo_last);
+          ev = run_solo(L, c, a.solo, evcell, __ffsll((long long)act) - 1);
+          if (!active) ev = EV_NONE;
+        } else if (active) {
+          ev = run_slice<GATE>(L, c, a.slice, evcell);
+        }
+        if (active && ev != EV_NONE) {
+          active = 0;
+          res = ev;
+          evc = evcell;
+        }
+      }
+      const int dt = run ? L.t - n_t : 0;
+      my_steps += (uint64_t)dt;
+      const uint64_t exist = ballot(ok), ended = ballot(res == EV_DEAD || res == EV_SURV);
+      const uint64_t surv = ballot(res == EV_SURV), live = ballot(ok && res == EV_STOP);
+      probe_add(S, w, exist, ended, surv, dt);
+      if (live == 0) break; // e13search.c:303
+      // ---- draw the child the path goes on with (e13search.c:304-311) ----
+      const uint32_t nlive = (uint32_t)__popcll(live);
+      const uint32_t r = (uint32_t)(probe_mix(rng) % (uint64_t)nlive);
+      uint64_t m = live;
+      for (uint32_t k = 0; k < r; ++k) m &= m - 1;
+      const int ch = __builtin_ctzll(m);
+      w = w * (double)nlive;
+      n_q = __builtin_amdgcn_readlane(L.q, ch);
+      n_st = __builtin_amdgcn_readlane((L.row - c.rbl) >> 8, ch);
+      n_t = __builtin_amdgcn_readlane(L.t, ch);
+      n_nt = __builtin_amdgcn_readlane(L.nt, ch);
+      n_stage = __builtin_amdgcn_readlane(L.stage, ch);
+      n_st0 = __builtin_amdgcn_readlane(L.st0, ch);
+      n_q0 = __builtin_amdgcn_readlane(L.q0, ch);
+      n_flags = __builtin_amdgcn_readlane(L.flags, ch);
+      n_ones = __builtin_amdgcn_readlane(L.ones, ch);
+      n_hb = rl64(L.hb, ch);
+      n_ch0 = rl64(L.ch0, ch);
+      n_ch1 = rl64(L.ch1, ch);
+      n_ch2 = rl64(L.ch2, ch);
+      budget = __builtin_amdgcn_readlane(nb, ch);
+      nintro = __builtin_amdgcn_readlane(ni, ch);
+      const int nval = __builtin_amdgcn_readlane(val, ch);
+      const int npend = __builtin_amdgcn_readlane(evc, ch);
+      // a child that finished before the scalar loop started has its words 0..31 in save[];
+      // the lone lane of that loop kept its own
+      src_saved = (savedm >> ch) & 1ull;
+      if (lane == 0) ptab[pend] = (uint8_t)nval;
+      pend = npend;
+      src = ch;
+    }
+    if (lane == 0) {
+      double *o = a.out + 4 * (size_t)p;
+      o[0] = S.leaves;
+      o[1] = S.nodes;
+      o[2] = S.steps;
+      o[3] = S.surv;
+    }
+  }
+  const uint64_t s_steps = wave_sum64(my_steps);
+  if (lane == 0) {
+    atomicAdd((unsigned long long *)&a.ctr->steps, (unsigned long long)s_steps);
+    atomicAdd((unsigned long long *)&a.ctr->rounds, (unsigned long long)w_rounds);
+    atomicAdd((unsigned long long *)&a.ctr->lane_slots, (unsigned long long)w_slots);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Thin C ABI over the launches.  Shapes are checked by the caller (host C).
 // ---------------------------------------------------------------------------
 extern "C" {
@@ -1737,6 +1981,24 @@ int e13k_launch_batch(const E13BatchArgs *a, int grid, int wg_threads, int gate,
     if (refill) hipLaunchKernelGGL((k_batch<GATE_COUNT, 1>), dim3(grid), dim3(wg_threads), lds, s, *a);
     else hipLaunchKernelGGL((k_batch<GATE_COUNT, 0>), dim3(grid), dim3(wg_threads), lds, s, *a);
   }
+  return (int)hipGetLastError();
+}
+
+// The probe kernel uses the expansion's LDS layout (rows, tapes, block keys,
+// 64 bytes per wave for the path's table).
+int e13k_probe_max_wgs(int wg_threads, int gate) {
+  const size_t lds = e13k_lds_bytes(wg_threads, gate);
+  int n = 0;
+  const hipError_t e = gate == GATE_HASH
+                           ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_probe<GATE_HASH>, wg_threads, lds)
+                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_probe<GATE_COUNT>, wg_threads, lds);
+  return e == hipSuccess ? n : -1;
+}
+
+int e13k_launch_probe(const E13ProbeArgs *a, int grid, int wg_threads, int gate, hipStream_t s) {
+  const size_t lds = e13k_lds_bytes(wg_threads, gate);
+  if (gate == GATE_HASH) hipLaunchKernelGGL((k_probe<GATE_HASH>), dim3(grid), dim3(wg_threads), lds, s, *a);
+  else hipLaunchKernelGGL((k_probe<GATE_COUNT>), dim3(grid), dim3(wg_threads), lds, s, *a);
   return (int)hipGetLastError();
 }
 

@@ -180,6 +180,28 @@ int e13_simulate_batch(e13_ctx *ctx, const int8_t *tabs, int ncells, int n, cons
 int e13_simulate_batch_device(e13_ctx *ctx, const void *tabs_dev, int nq, int n, const void *caps_dev, void *out_dev);
 int e13_sync(e13_ctx *ctx);
 
+/* Probe estimator: replaces the estimate mode of the reference
+   (e13search.c:473-510) and its probe() (:256-313), Knuth's estimator with
+   every child of a path node run: each probe walks from the root through
+   uniformly drawn live children, adding w per child (w = product of the live
+   counts above it) to leaves (ended children), nodes, steps (w x the child's
+   steps) and survivors.  Probe i (0 <= i < probes) draws from its own
+   splitmix64 stream, state splitmix(0x5EED0000 + (seed << 40) + i), so its
+   sums depend on (problem, seed, i) alone; the reference shares one stream per
+   OpenMP thread (e13search.c:487-491), which ties its draws to the schedule.
+   The per-probe sums go to per_probe[4*i .. 4*i+3] when it is given; the totals
+   are added in probe order, and mean/se follow e13search.c:503-506. */
+typedef struct {
+  int64_t probes;
+  double sum[4], sq[4];   /* leaves, nodes, steps, survivors: sums of the per-probe values and squares */
+  double mean[4], se[4];  /* the estimates and their standard errors */
+  uint64_t sim_steps;     /* machine steps simulated on the device (unweighted) */
+  double kernel_ms;       /* HIP-event time of the probe kernels */
+  int64_t launches, rounds, lane_slots;
+} e13_estimate_sum;
+int e13_estimate(e13_ctx *ctx, const e13_problem *p, uint64_t seed, int64_t probes, double *per_probe,
+                 e13_estimate_sum *out);
+
 /* Plain, unhashed re-check of a survivor over stages 1..9 with the search's
    deadlines and window (after recheck.c:57-92): 1 consistent, 0 not.  The
    cells read before the stage-9 start go to *read_mask. */

@@ -1,0 +1,132 @@
+"""GPU probe estimator (k_probe through e13_estimate) against the reference's
+own probe() and the oracle.
+
+Every probe's four sums (leaves, nodes, steps, survivors, each a double built
+from products of live-child counts) must equal the reference's bit for bit:
+the dumps of the reference's probe() under tests/golden/ref_dump/probe_*.txt
+pin the first few hundred probes of six problems, the oracle (itself pinned by
+those dumps) covers tens of thousands more, and the rows of
+results/4_division_6/v1/v1c0_est.txt whose trees are one leaf are exact.
+Each test prints its case before touching the GPU.
+"""
+import ctypes
+import struct
+
+import pytest
+
+from conftest import (MACHINES, PROBE_CASES, oracle_lib, oracle_probe_problem, probe_fixture, read_text,
+                      v1c0_exact_rows)
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu] {msg}", flush=True)
+
+
+def bits(t):
+    return tuple(struct.unpack("<Q", struct.pack("<d", v))[0] for v in t)
+
+
+class EoEstimateSum(ctypes.Structure):
+    _fields_ = [("probes", ctypes.c_longlong), ("sum", ctypes.c_double * 4), ("sq", ctypes.c_double * 4),
+                ("mean", ctypes.c_double * 4), ("se", ctypes.c_double * 4)]
+
+
+def oracle_estimate(args, seed, n):
+    """Per-probe sums (as bit patterns) and totals of the oracle's estimate."""
+    L = oracle_lib()
+    rc, p = oracle_probe_problem(args)
+    assert rc == 0
+    per = (ctypes.c_double * (4 * n))()
+    es = EoEstimateSum()
+    L.eo_estimate.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_longlong, ctypes.c_int,
+                              ctypes.POINTER(ctypes.c_double), ctypes.POINTER(EoEstimateSum)]
+    assert L.eo_estimate(ctypes.byref(p), seed, n, 8, per, ctypes.byref(es)) == 0
+    return [bits(per[4 * i:4 * i + 4]) for i in range(n)], es
+
+
+def gpu_problem(e13, args):
+    base = e13.parse_machine(read_text(MACHINES, args["machine"]))
+    sym = (args["lo"], args["hi"]) if args["lo"] >= 0 else None
+    return e13.setup(base, args["X"], args["fmax"], args["cells"] or None, sym)
+
+
+@pytest.fixture(scope="module")
+def ectx(e13):
+    """A context of its own: these tests change settings."""
+    ctx = e13.Context(0)
+    yield ctx
+    ctx.close()
+
+
+@pytest.mark.parametrize("name", PROBE_CASES)
+def test_probes_match_reference_probe(e13, ectx, name):
+    args, rows = probe_fixture(name)
+    say(f"estimate {name}: {args['n']} probes, seed {args['seed']}")
+    p = gpu_problem(e13, args)
+    es, per = ectx.estimate(p, args["n"], seed=args["seed"], per_probe=True)
+    got = [bits(t) for t in per]
+    bad = [i for i in range(len(rows)) if got[i] != rows[i]]
+    assert not bad, (name, bad[:5])
+    assert es.probes == args["n"] and es.sim_steps > 0
+
+
+@pytest.mark.parametrize("name", ["f2_x8", "counter3", "v1_0ld"])
+def test_probes_match_reference_probe_hash_gate(e13, ectx, name):
+    args, rows = probe_fixture(name)
+    say(f"estimate {name} (hash gate): {args['n']} probes")
+    old = ectx.tuning()
+    ectx.set_tuning(gate=2)
+    try:
+        _, per = ectx.estimate(gpu_problem(e13, args), args["n"], seed=args["seed"], per_probe=True)
+    finally:
+        ectx.restore_tuning(old)
+    assert [bits(t) for t in per] == rows
+
+
+@pytest.mark.parametrize("name,seed,n", [("counter3", 11, 20000), ("v1_0ld", 12, 20000), ("f3_x0", 13, 4000),
+                                         ("free_m1n0", 14, 20000), ("f1_nodel", 15, 8000)])
+def test_probes_match_oracle(e13, ectx, name, seed, n):
+    args, _ = probe_fixture(name)
+    want, oes = oracle_estimate(args, seed, n)
+    say(f"estimate {name}: {n} probes, seed {seed}, against the oracle")
+    es, per = ectx.estimate(gpu_problem(e13, args), n, seed=seed, per_probe=True)
+    got = [bits(t) for t in per]
+    bad = [i for i in range(n) if got[i] != want[i]]
+    assert not bad, (name, bad[:5])
+    # totals in probe order, mean and standard error as the reference prints them
+    for j in range(4):
+        assert bits([es.sum[j], es.sq[j], es.mean[j], es.se[j]]) == bits([oes.sum[j], oes.sq[j], oes.mean[j],
+                                                                           oes.se[j]])
+
+
+@pytest.mark.parametrize("settings", [dict(solo_steps=-1), dict(slice=16), dict(slice=256, solo_steps=64),
+                                      dict(wg_threads=64)])
+def test_probes_do_not_depend_on_settings(e13, ectx, settings):
+    """Scalar loop on or off, round length, workgroup size: the same probes."""
+    args, _ = probe_fixture("counter3")
+    p = gpu_problem(e13, args)
+    say(f"estimate counter3: 3000 probes with {settings}")
+    _, ref = ectx.estimate(p, 3000, seed=21, per_probe=True)
+    old = ectx.tuning()
+    ectx.set_tuning(**settings)
+    try:
+        _, per = ectx.estimate(p, 3000, seed=21, per_probe=True)
+    finally:
+        ectx.restore_tuning(old)
+    assert [bits(t) for t in per] == [bits(t) for t in ref]
+
+
+def test_v1c0_one_leaf_rows_are_exact(e13, ectx):
+    """results/4_division_6/v1/v1c0_est.txt: values of C0 whose tree is one
+    leaf give the same estimate from every probe."""
+    rows = v1c0_exact_rows()
+    cells = "D0 D1 E0 E1 F0 F1 G0 G1 H0 H1 I0 I1".split()
+    say(f"estimate: {len(rows)} one-leaf values of C0")
+    for v, path, leaves, steps in rows:
+        with open(path) as fh:
+            base = e13.parse_machine(fh.read())
+        p = e13.setup(base, 9, 0, cells, (3, 9))
+        es = ectx.estimate(p, 256, seed=1)
+        assert (es.mean[0], es.mean[2], es.se[0], es.se[2]) == (leaves, steps, 0.0, 0.0), v
