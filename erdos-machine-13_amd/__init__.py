@@ -29,6 +29,7 @@ ERRORS = {
     0: "ok", -1: "invalid argument", -2: "HIP runtime error", -3: "out of memory",
     -4: "forced cells exceed fmax", -5: "symmetric state is named by a fixed cell or has one",
     -6: "start state in the symmetric states", -7: "node stack overflow", -8: "no HIP device",
+    -9: "stopped cleanly; the checkpoint is saved", -10: "checkpoint file unreadable, damaged or for another search",
 }
 
 
@@ -142,7 +143,7 @@ EXPORTS = ("e13_open", "e13_close", "e13_strerror", "e13_set_tuning", "e13_get_t
            "e13_synth_table", "e13_get_trace", "e13_batch_digest", "e13_synth_run", "e13_get_cost_log",
            "e13_cost_targets", "e13_mlp_init", "e13_mlp_forward", "e13_mlp_grad", "e13_mlp_train",
            "e13_set_cost_model", "e13_autotune", "e13_save_tuning", "e13_load_tuning", "e13_tuning_file",
-           "e13_tuning_file_small", "e13_search_profile", "e13_estimate")
+           "e13_tuning_file_small", "e13_search_profile", "e13_estimate", "e13_set_checkpoint")
 
 _lib = None
 
@@ -210,6 +211,7 @@ def lib():
     L.e13_tuning_file.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
     L.e13_tuning_file_small.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
     L.e13_search_profile.argtypes = [ctypes.c_void_p, P(Problem), ctypes.c_int, P(Tuning)]
+    L.e13_set_checkpoint.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_double, ctypes.c_int64]
     L.e13_estimate.argtypes = [ctypes.c_void_p, P(Problem), ctypes.c_uint64, ctypes.c_int64, P(ctypes.c_double),
                                P(EstimateSum)]
     L.e13_get_trace.argtypes = [ctypes.c_void_p, P(ctypes.c_uint64), ctypes.c_uint64, P(ctypes.c_int32),
@@ -491,6 +493,12 @@ class Context:
         prof = lib().e13_search_profile(self._h, ps, n, ctypes.byref(t))
         _check(min(prof, 0), "e13_search_profile")
         return prof, t.as_dict()
+
+    def set_checkpoint(self, path, every_sec=0.0, max_launches=0):
+        """Checkpoints of the host-loop search (path None: off); a stop raises
+        E13Error with code -9 (E13_ESTOPPED) after the save."""
+        _check(lib().e13_set_checkpoint(self._h, path.encode() if path else None, every_sec, max_launches),
+               "e13_set_checkpoint")
 
     def estimate(self, p, probes, seed=0, per_probe=False):
         """The probe estimator (the reference's estimate mode): EstimateSum, and

@@ -88,6 +88,11 @@ struct e13_ctx {
   size_t probe_cap;      /* probes the output buffer holds */
   uint32_t *d_probe_save;
   size_t probe_save_cap; /* waves the save area holds */
+  /* checkpoints of the host-loop search (e13_ckpt.c) */
+  char ckpt_path[1024];  /* empty: off */
+  double ckpt_every_s;
+  int64_t ckpt_max_launches;
+  int ckpt_slot;         /* 0: the next save goes to <path>.a, 1: to <path>.b */
 };
 
 /* The MLP kernels (e13_mlp.hip). */
@@ -109,6 +114,10 @@ E13_INTERNAL void e13i_node_features(const e13_problem *p, const E13Node *nd, fl
 E13_INTERNAL int e13i_mlp_predict_dev(e13_ctx *c, const float *x_host, int n, float *y_host);
 E13_INTERNAL void e13i_free_cost(e13_ctx *c);
 E13_INTERNAL void e13i_free_estimate(e13_ctx *c);
+E13_INTERNAL int e13i_ckpt_save(e13_ctx *c, const e13_problem *p, size_t top, size_t peak, int64_t launches,
+                                double wall_ms);
+E13_INTERNAL int e13i_ckpt_load(e13_ctx *c, const e13_problem *p, size_t *top, size_t *peak, int64_t *launches,
+                                double *wall_ms);
 /* stored per-device tuning (e13_tune.c) */
 E13_INTERNAL int e13i_tuning_path(const char *arch, int ncu, int small, char *buf, int len);
 

@@ -21,6 +21,7 @@
 
 #include "e13_ctx.h"
 
+static double host_ms(void);
 static const long T14S[E13_NS + 1] = {0, 2, 22, 90, 268, 944, 3434, 13450, 52068, 208864}; /* e13search.c:70 */
 
 const char *e13_strerror(int code) {
@@ -34,6 +35,8 @@ const char *e13_strerror(int code) {
   case E13_ESYMSTART: return "start state in the symmetric states";
   case E13_ESTACK: return "node stack overflow";
   case E13_ENODEV: return "no HIP device";
+  case E13_ESTOPPED: return "stopped cleanly; the checkpoint is saved";
+  case E13_ECKPT: return "checkpoint file unreadable, damaged or for another search";
   default: return "unknown error";
   }
 }
@@ -550,9 +553,11 @@ static int run_chain(e13_ctx *c, E13ExpandArgs *a, size_t batch, int grid, int w
 int e13_search(e13_ctx *c, const e13_problem *p, e13_counts *out, e13_survivor *surv, uint64_t surv_cap,
                uint64_t *n_surv) {
   if (!c || !out || !check_problem(p)) return E13_EINVAL;
-  if (c->tune.form == 3) return e13_search_many(c, p, 1, out, surv, surv_cap, n_surv);
+  const int ckpt = c->ckpt_path[0] != 0; /* checkpoints need the host loop: form 3 runs as form 2 */
+  if (c->tune.form == 3 && !ckpt) return e13_search_many(c, p, 1, out, surv, surv_cap, n_surv);
   HIPCHK(hipSetDevice(c->device));
   const e13_tuning *tu = &c->tune;
+  const int form = tu->form == 1 ? 1 : 2;
   const int nslot = 4 * p->nq + 1;
   const size_t batch = (size_t)tu->batch;
   int rc;
@@ -587,16 +592,45 @@ int e13_search(e13_ctx *c, const e13_problem *p, e13_counts *out, e13_survivor *
   HIPCHK(hipMemsetAsync(c->d_ctr, 0, sizeof(E13Counters), c->stream));
   HIPCHK(hipMemcpyAsync(c->d_stack, &root, sizeof root, hipMemcpyHostToDevice, c->stream));
   size_t top = 1, peak = 1;
-  const int refill = tu->form == 2;
+  const int refill = form == 2;
   const int wg = fit_wg(c, tu->wg_threads, tu->gate == 2);
   if (e13k_lds_bytes(wg, tu->gate == 2) > c->lds_max) return E13_EINVAL;
   int grid_used = 0;
-  if (tu->graph_chain > 0) {
+  /* checkpoint, clean stop and resume (e13gpu.py:349-409) */
+  int64_t launches = 0, call_launches = 0;
+  double prior_ms = 0, t_start = host_ms(), last_ckpt = t_start, last_stop_check = 0;
+  char stop_path[1100];
+  stop_path[0] = 0;
+  if (ckpt) {
+    snprintf(stop_path, sizeof stop_path, "%s.stop", c->ckpt_path);
+    if ((rc = e13i_ckpt_load(c, p, &top, &peak, &launches, &prior_ms)) < 0) return rc;
+    if (rc == 1 && getenv("E13_TIMING"))
+      fprintf(stderr, "libe13: resumed from %s: stack=%zu launches=%lld\n", c->ckpt_path, top, (long long)launches);
+    remove(stop_path); /* left over from the last clean stop */
+  }
+  if (tu->graph_chain > 0 && !ckpt) {
     grid_used = refill ? c->ncu * tu->wgs_per_cu : (int)((batch * (size_t)nslot + (size_t)wg - 1) / (size_t)wg);
     if ((rc = run_chain(c, &a, batch, grid_used, wg, refill, &peak))) return rc;
     top = 0;
   }
   while (top > 0) {
+    if (ckpt) {
+      const double now = host_ms();
+      int stop = c->ckpt_max_launches > 0 && call_launches >= c->ckpt_max_launches;
+      if (now - last_stop_check > 1000.0) { /* the stop file, at most once a second */
+        last_stop_check = now;
+        stop |= access(stop_path, F_OK) == 0;
+      }
+      if (stop || now - last_ckpt > 1000.0 * c->ckpt_every_s) {
+        if ((rc = e13i_ckpt_save(c, p, top, peak, launches, prior_ms + now - t_start))) return rc;
+        last_ckpt = host_ms();
+        if (stop) {
+          remove(stop_path);
+          c->stats.peak_stack = (int64_t)peak;
+          return E13_ESTOPPED;
+        }
+      }
+    }
     size_t nin = top < batch ? top : batch;
     size_t base_i = top - nin;
     a.in = c->d_stack + base_i;
@@ -625,6 +659,8 @@ int e13_search(e13_ctx *c, const e13_problem *p, e13_counts *out, e13_survivor *
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->stats.kernel_ms += ms;
     c->stats.launches++;
+    launches++;
+    call_launches++;
     uint32_t nout = io[1];
     if (io[3] & 4u) {
       fprintf(stderr, "libe13: dynamic LDS does not start at LDS address 0\n");
@@ -653,7 +689,7 @@ int e13_search(e13_ctx *c, const e13_problem *p, e13_counts *out, e13_survivor *
   c->stats.wg_threads = wg;
   c->stats.grid = grid_used;
   c->stats.gate = tu->gate;
-  c->stats.form = tu->form;
+  c->stats.form = form;
 
   uint64_t ns = h.nsurv;
   if (n_surv) *n_surv = ns;

@@ -31,6 +31,8 @@ extern "C" {
 #define E13_ESYMSTART (-6)  /* start state in the symmetric states (e13search.c:349) */
 #define E13_ESTACK (-7)     /* node stack capacity exceeded (e13gpu.py:420-421) */
 #define E13_ENODEV (-8)     /* no HIP device */
+#define E13_ESTOPPED (-9)   /* a clean stop: the checkpoint is saved (e13gpu.py:398-409, exit code 3) */
+#define E13_ECKPT (-10)     /* checkpoint unreadable, damaged or written for another search (e13gpu.py:360-361) */
 
 /* Result codes of a run (e13search.c:64-65). */
 #define E13_DEAD (-1)
@@ -179,6 +181,22 @@ int e13_simulate_batch(e13_ctx *ctx, const int8_t *tabs, int ncells, int n, cons
    on the context stream; returns after the launch is queued. */
 int e13_simulate_batch_device(e13_ctx *ctx, const void *tabs_dev, int nq, int n, const void *caps_dev, void *out_dev);
 int e13_sync(e13_ctx *ctx);
+
+/* Checkpoints of the host-loop search: replace the GPU driver's E13_CKPT /
+   E13_CKPT_SEC / <file>.stop handling (e13gpu.py:19-23, 349-409).  With a
+   path set, e13_search runs the host loop (form 1 as set, otherwise the
+   batched form 2: the persistent form has no point between launches where the
+   state is all in the stack) and, between launches, saves the stack, counters
+   and survivor records to <path>.a / <path>.b in turn, the pointer file <path>
+   naming the last complete one: every every_sec seconds (<= 0: 1800), when
+   the file <path>.stop appears (checked at most once a second; the file is
+   removed), or after max_launches launches of this call (> 0; 0 = no limit).
+   A stop returns E13_ESTOPPED after the save.  A search started with a
+   checkpoint present resumes from it (E13_ECKPT if it belongs to another
+   problem or is damaged); results are those of an uninterrupted run.  A stop
+   file left from an earlier clean stop is removed at the start.  path NULL or
+   "": off. */
+int e13_set_checkpoint(e13_ctx *ctx, const char *path, double every_sec, int64_t max_launches);
 
 /* Probe estimator: replaces the estimate mode of the reference
    (e13search.c:473-510) and its probe() (:256-313), Knuth's estimator with
