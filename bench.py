@@ -15,7 +15,9 @@ FMAX=4 search (5,986,617,022 leaves); the 2^22-table synthetic stress batch
 (refilling vs plain form, digest checked); the counter search in the plain
 per-leaf-launch form vs the persistent form; and FMAX=4 in the plain form, with
 the static split and with the split learned by the cost model (trained here,
-on the GPU, from the FMAX=2 and FMAX=3 searches' cost logs).
+on the GPU, from the FMAX=2 and FMAX=3 searches' cost logs); and the probe
+estimator (the reference's estimate mode) on the counter search, its first
+probes matched bit for bit against the dump of the reference's probe().
 
 roofline: the dominant kernel is the persistent search kernel.  Its
 algorithmic work is 13 integer lane-ops and 8 LDS bytes per simulated step;
@@ -202,6 +204,55 @@ def learned_vs_static(e13, ctx, base):
     return out
 
 
+def estimator_workload(e13, ctx, base, probes=1 << 20):
+    """The probe estimator (estimate mode) on the 3-state counter search: the
+    committed dump of the reference's own probe() is matched bit for bit
+    first, then 2^20 probes are timed (kernel time from HIP events)."""
+    p = e13.setup(base, 13, free_cells="D0 G0 K0 K1 L1 M1".split(), sym=(10, 13))
+    with open(os.path.join(ROOT, "tests", "golden", "ref_dump", "probe_counter3.txt")) as f:
+        lines = f.read().splitlines()
+    seed, n = int(lines[0].split()[6]), int(lines[0].split()[7])
+    rows = [tuple(int(v, 16) for v in l.split()[1:5]) for l in lines[2:2 + n]]
+    import struct
+    _, per = ctx.estimate(p, n, seed=seed, per_probe=True)
+    got = [tuple(struct.unpack("<Q", struct.pack("<d", v))[0] for v in t) for t in per]
+    if got != rows:
+        raise SystemExit("estimator: per-probe sums differ from the reference's probe()")
+    ctx.estimate(p, 4096, seed=1)  # warm
+    t0 = time.perf_counter()
+    es = ctx.estimate(p, probes, seed=0)
+    wall = time.perf_counter() - t0
+    ks = es.kernel_ms / 1e3
+    return {"workload": f"estimate (probe estimator) of freesym X=13 10..13 D0 G0 K0 K1 L1 M1, {probes} probes",
+            "kernel": "k_probe", "bit_identical_to_reference_probe": True, "probes": probes,
+            "seconds": wall, "kernel_ms": es.kernel_ms, "probes_per_s": probes / ks,
+            "steps_per_s": es.sim_steps / ks, "steps": es.sim_steps,
+            "valu_frac": es.sim_steps * OPS_PER_STEP / ks / VALU_PEAK,
+            "active_lanes_per_round": es.lane_slots / max(es.rounds, 1),
+            "estimate": {"leaves": [es.mean[0], es.se[0]], "steps": [es.mean[2], es.se[2]],
+                         "exact_leaves": 3078656272, "exact_steps": 319112357176}}
+
+
+def cpu_estimate_baseline(probes=400000, max_s=120):
+    """The reference program's own estimate mode on the same problem, all host cores."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "e13search")
+    port = os.path.join(ROOT, "oracle", "_build", "e13oracle")
+    exe, kind = (ref, "reference") if os.path.exists(ref) else (port, "port")
+    if not os.path.exists(exe):
+        return None
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    t0 = time.time()
+    try:
+        subprocess.run([exe, "estimate", MACHINE, str(probes), "13", "0", "10", "13"] + "D0 G0 K0 K1 L1 M1".split(),
+                       capture_output=True, text=True, env=env, timeout=max_s, check=True)
+    except (subprocess.TimeoutExpired, subprocess.CalledProcessError):
+        return None
+    wall = time.time() - t0
+    return {"value": probes / wall, "unit": "probes/s", "cores": threads, "kind": kind,
+            "sample": f"{probes} probes of the same estimate problem in {wall:.1f} s wall"}
+
+
 def cpu_baseline(max_s=120):
     """The reference CPU search (or the oracle port) on the same workload."""
     ref = os.path.join(ROOT, "oracle", "_ref", "e13search")
@@ -315,6 +366,10 @@ def main():
         extra.append(synth_workload(e13, ctx))
         extra.append(plain_vs_batched(e13, ctx, counter))
         extra.append(learned_vs_static(e13, ctx, base))
+        est = estimator_workload(e13, ctx, base)
+        if not args.no_cpu:
+            est["cpu"] = cpu_estimate_baseline()
+        extra.append(est)
     line = {
         "metric": "simulated TM steps/s, local-edit search FMAX=3 (X=0..13) on erdos_14, bit-identical to the "
                   "reference",
