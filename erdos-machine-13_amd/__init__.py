@@ -125,6 +125,7 @@ class EstimateSum(ctypes.Structure):
     (leaves, nodes, steps, survivors), plus the device's own figures."""
     _fields_ = [("probes", ctypes.c_int64), ("sum", ctypes.c_double * 4), ("sq", ctypes.c_double * 4),
                 ("mean", ctypes.c_double * 4), ("se", ctypes.c_double * 4), ("sim_steps", ctypes.c_uint64),
+                ("rerun_steps", ctypes.c_uint64),
                 ("kernel_ms", ctypes.c_double), ("launches", ctypes.c_int64), ("rounds", ctypes.c_int64),
                 ("lane_slots", ctypes.c_int64)]
 
@@ -132,7 +133,8 @@ class EstimateSum(ctypes.Structure):
 
     def as_dict(self):
         d = {n: (self.mean[j], self.se[j]) for j, n in enumerate(self.NAMES)}
-        d.update(probes=self.probes, sim_steps=self.sim_steps, kernel_ms=self.kernel_ms, launches=self.launches,
+        d.update(probes=self.probes, sim_steps=self.sim_steps, rerun_steps=self.rerun_steps,
+                 kernel_ms=self.kernel_ms, launches=self.launches,
                  rounds=self.rounds, lane_slots=self.lane_slots)
         return d
 

@@ -29,6 +29,7 @@ int e13k_launch_chain_pop(E13Chain *ch, E13Counters *ctr, hipStream_t s);
 int e13k_launch_chain_push(E13Chain *ch, E13Node *stack, uint64_t stack_cap, const E13Node *out, uint32_t out_cap,
                            const E13Counters *ctr, int grid, hipStream_t s);
 
+size_t e13k_probe_lds_bytes(int wg_threads, int gate);
 int e13k_probe_max_wgs(int wg_threads, int gate);
 int e13k_launch_probe(const E13ProbeArgs *a, int grid, int wg_threads, int gate, hipStream_t s);
 
@@ -86,8 +87,6 @@ struct e13_ctx {
   /* probe estimator (e13_estimate.c) */
   double *d_probe_out;
   size_t probe_cap;      /* probes the output buffer holds */
-  uint32_t *d_probe_save;
-  size_t probe_save_cap; /* waves the save area holds */
   /* checkpoints of the host-loop search (e13_ckpt.c) */
   char ckpt_path[1024];  /* empty: off */
   double ckpt_every_s;

@@ -192,8 +192,8 @@ typedef struct E13ProbeArgs {
   const uint64_t *key;
   const uint64_t *tgt;
   double *out;            /* n x 4: leaves, nodes, steps, survivors of each probe (weighted) */
-  E13Counters *ctr;       /* item = next probe of this launch; steps, rounds, lane slots */
-  uint32_t *save;         /* per wave 64 x E13_TW tape words: finished children kept while a lone lane runs */
+  E13Counters *ctr;       /* item = next probe of this launch; steps (children), cyc[0] = steps of the
+                             re-runs that make the next node of a path, rounds, lane slots */
   uint64_t seed;          /* probe p0 + i draws from the stream of (seed, p0 + i) */
   uint64_t p0;
   uint32_t n;             /* probes in this launch */

@@ -18,10 +18,8 @@
 
 void e13i_free_estimate(e13_ctx *c) {
   hipFree(c->d_probe_out);
-  hipFree(c->d_probe_save);
   c->d_probe_out = NULL;
-  c->d_probe_save = NULL;
-  c->probe_cap = c->probe_save_cap = 0;
+  c->probe_cap = 0;
 }
 
 int e13_estimate(e13_ctx *c, const e13_problem *p, uint64_t seed, int64_t probes, double *per_probe,
@@ -33,16 +31,13 @@ int e13_estimate(e13_ctx *c, const e13_problem *p, uint64_t seed, int64_t probes
   HIPCHK(hipSetDevice(c->device));
   const int gate = c->tune.gate == 2;
   const int wg = c->tune.wg_threads >= 64 && c->tune.wg_threads <= 256 ? c->tune.wg_threads / 64 * 64 : 256;
-  if (e13k_lds_bytes(wg, gate) > c->lds_max) return E13_EINVAL;
+  if (e13k_probe_lds_bytes(wg, gate) > c->lds_max) return E13_EINVAL;
   /* every workgroup resident at once: waves take probes from one cursor */
   int per_cu = e13k_probe_max_wgs(wg, gate);
   if (per_cu < 1) per_cu = 1;
   const int grid = c->ncu * per_cu;
-  const size_t waves = (size_t)grid * (size_t)(wg / 64);
   const size_t chunk = probes < PROBES_PER_LAUNCH ? (size_t)probes : PROBES_PER_LAUNCH;
   int rc = e13i_ensure_buf((void **)&c->d_probe_out, &c->probe_cap, chunk, 4 * sizeof(double));
-  if (rc) return rc;
-  rc = e13i_ensure_buf((void **)&c->d_probe_save, &c->probe_save_cap, waves, 64 * 32 * sizeof(uint32_t));
   if (rc) return rc;
   double *host = per_probe ? NULL : (double *)malloc(chunk * 4 * sizeof(double));
   if (!per_probe && !host) return E13_ENOMEM;
@@ -56,7 +51,6 @@ int e13_estimate(e13_ctx *c, const e13_problem *p, uint64_t seed, int64_t probes
   a.tgt = c->d_tgt;
   a.out = c->d_probe_out;
   a.ctr = c->d_ctr;
-  a.save = c->d_probe_save;
   a.seed = seed;
   a.nq = p->nq;
   a.slice = c->tune.slice;
@@ -91,12 +85,14 @@ int e13_estimate(e13_ctx *c, const e13_problem *p, uint64_t seed, int64_t probes
       rc = E13_EHIP;
       break;
     }
-    if (ctr.overflow & 4u) { /* the kernel found its LDS area away from address 0 */
+    if (ctr.overflow) { /* 4: LDS area away from address 0; 8, 16: a probe's bookkeeping went wrong */
+      fprintf(stderr, "libe13: probe kernel error flags %u\n", ctr.overflow);
       rc = E13_EHIP;
       break;
     }
     out->kernel_ms += ms;
     out->sim_steps += ctr.steps;
+    out->rerun_steps += ctr.cyc[0];
     out->rounds += (int64_t)ctr.rounds;
     out->lane_slots += (int64_t)ctr.lane_slots;
     out->launches++;

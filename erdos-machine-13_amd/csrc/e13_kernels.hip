@@ -1666,18 +1666,19 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
 
 // ---------------------------------------------------------------------------
 // Probe estimator (probe(), e13search.c:256-313; estimate mode, :473-510).
-// One wave walks one probe at a time.  At each node of the path lane s runs
-// child slot s (slot 0 keeps the cell's base value, slot 1+v binds it to move
-// v, as in the expansion) to its next stop; the wave adds w per child to the
-// probe's sums in slot order -- the reference's child order, so the doubles
-// agree bit for bit -- then draws a live child from the probe's own splitmix
-// stream and continues from it with w *= (live children).  Every cell starts
-// unread (the reference's probe starts with fixed[] all zero), so the walk
-// stops at the first read of every cell, one-child cells included.  The node
-// on the path is the chosen lane's registers and LDS tape; the table is a
-// per-wave copy in LDS.  A wave whose last running child is alone runs it in
-// the scalar loop, whose block memo lives in idle lanes' tape words 0..31:
-// the finished live children's words 0..31 are kept in global memory first.
+// A probe is one random root-to-leaf path: at each node on it every child is
+// run to its next stop and counted with the path weight w, then the path goes
+// on through a uniformly drawn live child and w grows by the number of live
+// children.  A wave keeps PK probes in flight, each in a slot of LDS holding
+// the node on its path (tape, state, table), its children's results and the
+// probe's sums; idle lanes take child runs from any slot, so the long child of
+// one level does not hold the wave.  When a level's children are all in, the
+// wave adds them to the sums in child order (slot 0 keeps the base value,
+// slot 1+v binds move v: the reference's child order, so its additions and
+// their rounding are the reference's), draws the next child from the probe's
+// own stream, and one lane re-runs that child from the node to make the next
+// node.  Every cell starts unread (the reference's probe starts with fixed[]
+// all zero), so the path also stops at the first read of one-child cells.
 // ---------------------------------------------------------------------------
 namespace {
 
@@ -1695,33 +1696,158 @@ __device__ __forceinline__ uint64_t probe_stream(uint64_t seed, uint64_t p) {
   return probe_mix(s);
 }
 
-__device__ __forceinline__ uint64_t rl64(uint64_t v, int l) {
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
-  return ((uint64_t)hi << 32) | lo;
+#define PK 4          // probes in flight per wave
+// a slot, in words from its base
+#define PS_TAPE 0     // the node's tape (43 words)
+#define PS_REG 44     // the node's state: q st t nt stage st0 q0 flags ones hb(2) ch0(2) ch1(2) ch2(2)
+#define PS_BUDGET 61  //   edits left, symmetric states in use, the cell to bind
+#define PS_NINTRO 62
+#define PS_PEND 63
+#define PS_TAB 64     // the path's table, 32 codes (8 words)
+#define PS_DT 72      // steps of each child of the node (64 words)
+#define PS_LIST 136   // the children to run, in child order (64 bytes)
+#define PS_EXIST 152  // existing children (2 words), ended children (2), survivors (2)
+#define PS_ENDED 154
+#define PS_SURV 156
+#define PS_PH 158     // phase
+#define PS_P 159      // probe index
+#define PS_NEXT 160   // children handed out
+#define PS_COUNT 161  // children to run
+#define PS_LEFT 162   // children still running
+#define PS_CHOSEN 163 // the drawn child
+#define PS_RNG 164    // the probe's stream (2 words)
+#define PS_W 166      // the path weight (double)
+#define PS_SUM 168    // leaves, nodes, steps, survivors (doubles)
+#define PS_WORDS 176
+enum { PH_EMPTY = 0, PH_LEVEL = 1, PH_CHILDREN = 2, PH_DRAWN = 3, PH_ADVANCE = 4 };
+#define ADV_CHILD 64  // a lane's task is the drawn child's re-run
+
+__device__ __forceinline__ uint64_t lds_ld64(const Ctx &c, int off) {
+  return (uint64_t)lds_ld(c.sb, off) | ((uint64_t)lds_ld(c.sb, off + 4) << 32);
+}
+__device__ __forceinline__ void lds_st64(const Ctx &c, int off, uint64_t v) {
+  lds_st(c.sb, off, (uint32_t)v);
+  lds_st(c.sb, off + 4, (uint32_t)(v >> 32));
+}
+__device__ __forceinline__ uint64_t uni64(uint64_t v) {
+  return (uint64_t)(uint32_t)uni((int)(uint32_t)v) | ((uint64_t)(uint32_t)uni((int)(uint32_t)(v >> 32)) << 32);
 }
 
-struct ProbeSums {
-  double leaves, nodes, steps, surv;
-};
-
-// The children of one node in slot order (e13search.c:289-300): nodes += w,
+// The children of one node in child order (e13search.c:289-300): nodes += w,
 // steps += w * dt, leaves += w for an ended child, survivors += w for a
 // survivor.  No contraction: every product and sum rounds as the reference's.
-__device__ __forceinline__ void probe_add(ProbeSums &S, double w, uint64_t exist, uint64_t ended,
-                                                    uint64_t surv, int dt) {
+__device__ __forceinline__ void probe_level_add(const Ctx &c, int sb, uint64_t exist, uint64_t ended,
+                                                uint64_t surv) {
 #pragma clang fp contract(off)
+  const double w = __builtin_bit_cast(double, lds_ld64(c, sb + PS_W * 4));
+  double leaves = __builtin_bit_cast(double, lds_ld64(c, sb + (PS_SUM + 0) * 4));
+  double nodes = __builtin_bit_cast(double, lds_ld64(c, sb + (PS_SUM + 2) * 4));
+  double steps = __builtin_bit_cast(double, lds_ld64(c, sb + (PS_SUM + 4) * 4));
+  double sv = __builtin_bit_cast(double, lds_ld64(c, sb + (PS_SUM + 6) * 4));
   while (exist) {
     const int s = __builtin_ctzll(exist);
     exist &= exist - 1;
-    const int d = __builtin_amdgcn_readlane(dt, s);
-    S.nodes = S.nodes + w;
-    S.steps = S.steps + w * (double)d;
+    const int d = (int)lds_ld(c.sb, sb + (PS_DT + s) * 4);
+    nodes = nodes + w;
+    steps = steps + w * (double)d;
     if ((ended >> s) & 1ull) {
-      S.leaves = S.leaves + w;
-      if ((surv >> s) & 1ull) S.surv = S.surv + w;
+      leaves = leaves + w;
+      if ((surv >> s) & 1ull) sv = sv + w;
     }
   }
+  if ((threadIdx.x & 63u) == 0) {
+    lds_st64(c, sb + (PS_SUM + 0) * 4, __builtin_bit_cast(uint64_t, leaves));
+    lds_st64(c, sb + (PS_SUM + 2) * 4, __builtin_bit_cast(uint64_t, nodes));
+    lds_st64(c, sb + (PS_SUM + 4) * 4, __builtin_bit_cast(uint64_t, steps));
+    lds_st64(c, sb + (PS_SUM + 6) * 4, __builtin_bit_cast(uint64_t, sv));
+  }
+}
+
+// Child ch of a node whose pending cell has base code orig (e13search.c:278-287):
+// 1 if it exists, with its value, edits left and symmetric states in use.
+__device__ __forceinline__ int probe_child(int ch, int orig, int budget, int nintro, const E13Problem *pb, int &val,
+                                           int &nb, int &ni) {
+  val = -1;
+  nb = budget;
+  ni = nintro;
+  if (ch == 0) {
+    if (orig == -2) return 0;
+    val = orig;
+    return 1;
+  }
+  const int v = ch - 1;
+  if (!(orig == -2 || budget > 0) || v == orig) return 0;
+  const int n2 = sym_next(v, nintro, pb->sym_lo, pb->sym_hi);
+  if (n2 < 0) return 0;
+  val = v;
+  ni = n2;
+  if (orig != -2) nb = budget - 1;
+  return 1;
+}
+
+// The sums of a level whose only child is child ch (nodes += w, steps += w *
+// dt; leaves and survivors when it ended), written back; e13search.c:289-300.
+__device__ __forceinline__ void probe_single_add(const Ctx &c, int sb, int dt, bool ended, bool surv) {
+#pragma clang fp contract(off)
+  const double w = __builtin_bit_cast(double, lds_ld64(c, sb + PS_W * 4));
+  const double nodes = __builtin_bit_cast(double, lds_ld64(c, sb + (PS_SUM + 2) * 4)) + w;
+  const double steps = __builtin_bit_cast(double, lds_ld64(c, sb + (PS_SUM + 4) * 4)) + w * (double)dt;
+  lds_st64(c, sb + (PS_SUM + 2) * 4, __builtin_bit_cast(uint64_t, nodes));
+  lds_st64(c, sb + (PS_SUM + 4) * 4, __builtin_bit_cast(uint64_t, steps));
+  if (ended) {
+    const double leaves = __builtin_bit_cast(double, lds_ld64(c, sb + (PS_SUM + 0) * 4)) + w;
+    lds_st64(c, sb + (PS_SUM + 0) * 4, __builtin_bit_cast(uint64_t, leaves));
+    if (surv) {
+      const double sv = __builtin_bit_cast(double, lds_ld64(c, sb + (PS_SUM + 6) * 4)) + w;
+      lds_st64(c, sb + (PS_SUM + 6) * 4, __builtin_bit_cast(uint64_t, sv));
+    }
+  }
+}
+
+// The lane's machine (the path's child that was bound to tval at the pending
+// cell) stopped before reading cell evcell: it is the node of the probe's next
+// level.  When that level's only child runs on from it (the cell keeps a move:
+// no edits left and not a free cell), the lane binds the cell in its own rows
+// and goes on as that child (returns true); otherwise it writes the node to the
+// slot for the wave to expand.
+__device__ __forceinline__ bool probe_next_node(const Ctx &c, const Lane &L, uint8_t *sbytes, int sb, int evcell,
+                                                int &tval, int tnb, int tni, const int8_t *base) {
+  const int pend = (int)lds_ld(c.sb, sb + PS_PEND * 4);
+  sbytes[sb + PS_TAB * 4 + pend] = (uint8_t)tval;
+  lds_st(c.sb, sb + PS_PEND * 4, (uint32_t)evcell);
+  lds_st(c.sb, sb + PS_BUDGET * 4, (uint32_t)tnb);
+  lds_st(c.sb, sb + PS_NINTRO * 4, (uint32_t)tni);
+  const int orig = base[evcell];
+  if (orig >= 0 && tnb == 0) { // one child, the keep child (e13search.c:279-282)
+    const int ra = c.rbl + (evcell >> 1) * 256;
+    const uint32_t e = enc_entry(orig, (uint32_t)(evcell & 1));
+    const uint32_t row = lds_ld(c.sb, ra);
+    lds_st(c.sb, ra, (evcell & 1) ? ((row & 0xFFFFu) | (e << 16)) : ((row & 0xFFFF0000u) | e));
+    tval = orig;
+    lds_st64(c, sb + PS_EXIST * 4, 1ull);
+    lds_st(c.sb, sb + PS_NEXT * 4, 1u);
+    lds_st(c.sb, sb + PS_COUNT * 4, 1u);
+    lds_st(c.sb, sb + PS_LEFT * 4, 1u);
+    lds_st(c.sb, sb + PS_PH * 4, PH_CHILDREN);
+    return true;
+  }
+  for (int wd = 0; wd < E13_TW; ++wd) lds_st(c.sb, sb + (PS_TAPE + wd) * 4, lds_ld(c.sb, c.tbl + wd * 256));
+  const int r0 = sb + PS_REG * 4;
+  lds_st(c.sb, r0, (uint32_t)L.q);
+  lds_st(c.sb, r0 + 4, (uint32_t)((L.row - c.rbl) >> 8));
+  lds_st(c.sb, r0 + 8, (uint32_t)L.t);
+  lds_st(c.sb, r0 + 12, (uint32_t)L.nt);
+  lds_st(c.sb, r0 + 16, (uint32_t)L.stage);
+  lds_st(c.sb, r0 + 20, (uint32_t)L.st0);
+  lds_st(c.sb, r0 + 24, (uint32_t)L.q0);
+  lds_st(c.sb, r0 + 28, (uint32_t)L.flags);
+  lds_st(c.sb, r0 + 32, (uint32_t)L.ones);
+  lds_st64(c, r0 + 36, L.hb);
+  lds_st64(c, r0 + 44, L.ch0);
+  lds_st64(c, r0 + 52, L.ch1);
+  lds_st64(c, r0 + 60, L.ch2);
+  lds_st(c.sb, sb + PS_PH * 4, PH_LEVEL);
+  return false;
 }
 
 } // namespace
@@ -1742,167 +1868,262 @@ __global__ void __launch_bounds__(256) k_probe(E13ProbeArgs a) {
     if (threadIdx.x == 0) atomicOr(&a.ctr->overflow, 4u);
     return;
   }
-  // the path's table: 32 codes per wave in the 64 bytes after the tapes and block keys
-  uint8_t *ptab = (uint8_t *)(c.sb + c.keyb + (GATE == GATE_HASH ? KEYB_WORDS * 8 : 0)) + wave * 64;
   if (GATE == GATE_HASH) fill_keyb(c.sb, c.keyb, a.key);
+  // this wave's PK slots, after the tapes, the block keys and 64 bytes per wave
+  const int slots = c.keyb + (GATE == GATE_HASH ? KEYB_WORDS * 8 : 0) + nwaves * 64 + wave * (PK * PS_WORDS * 4);
+  for (int i = lane; i < PK * PS_WORDS; i += 64) lds_st(c.sb, slots + i * 4, 0u); // every slot empty
   __syncthreads();
-  const int wid = blockIdx.x * nwaves + wave;
-  const int tbase = c.tbl - lane * 4; // word w of lane j at tbase + w*256 + j*4
-  uint32_t *save = a.save + (size_t)wid * (64 * 32);
+  uint8_t *sbytes = (uint8_t *)smem;
   const int nq = a.nq, nslot = 4 * nq + 1;
+  const int8_t *base = a.pb->base;
   Lane L;
   L.dlcap = -1;
   L.tcap = 0x7FFFFFFF;
-  uint64_t my_steps = 0, w_slots = 0, w_rounds = 0, fills = 0, solo_last = ~0ull;
+  int active = 0, tslot = 0, tchild = 0, t0 = 0, tval = 0, tnb = 0, tni = 0;
+  bool exhausted = false;
+  uint64_t my_steps = 0, adv_steps = 0, w_slots = 0, w_rounds = 0, fills = 0, solo_last = ~0ull;
   for (;;) {
-    uint32_t p = 0;
-    if (lane == 0) p = atomicAdd(&a.ctr->item, 1u);
-    p = (uint32_t)uni((int)p);
-    if (p >= a.n) break;
-    uint64_t rng = probe_stream(a.seed, a.p0 + p);
-    // the root: the blank tape, stopped before the read of cell 0 at t = 0
-    // (e13search.c:268-271: nodes 1, steps 0)
-    if (lane < E13_MAXCELLS) ptab[lane] = lane < 2 * nq && a.pb->base[lane] != -2 ? E13_C_EDIT : E13_C_FREE;
-    int n_q = E13_QOFF, n_st = 0, n_t = 0, n_nt = 1, n_stage = 0, n_st0 = -1, n_q0 = E13_QOFF, n_flags = 3;
-    int n_ones = 0;
-    uint64_t n_hb = 0, n_ch0 = 0, n_ch1 = 0, n_ch2 = 0;
-    int pend = 0, budget = a.budget, nintro = a.nintro, src = -1;
-    uint64_t src_saved = 0; // 1: the source lane's words 0..31 are in save[]
-    ProbeSums S;
-    S.leaves = 0.0;
-    S.nodes = 1.0;
-    S.steps = 0.0;
-    S.surv = 0.0;
-    double w = 1.0;
-    for (;;) {
-      wave_lds_sync();
-      // ---- the node's children, lane = slot (e13search.c:278-287) ----
-      const int orig = a.pb->base[pend];
-      int ok = 0, val = -1, nb = budget, ni = nintro;
-      bool halt_keep = false;
-      if (lane < nslot) {
-        if (lane == 0) {
-          if (orig != -2) {
-            ok = 1;
-            val = orig;
-            halt_keep = orig == -1; // keeping a halt: dies on the read, 0 steps (e13search.c:164-166)
-          }
+    bool progress = false;
+    int busy = 0;
+    // ---- the slots' bookkeeping (wave-uniform) ----
+    for (int k = 0; k < PK; ++k) {
+      const int sb = slots + k * (PS_WORDS * 4);
+      int ph = uni((int)lds_ld(c.sb, sb + PS_PH * 4));
+      if (ph == PH_EMPTY && !exhausted) {
+        uint32_t p = 0;
+        if (lane == 0) p = atomicAdd(&a.ctr->item, 1u);
+        p = (uint32_t)uni((int)p);
+        if (p >= a.n) {
+          exhausted = true;
         } else {
-          const int v = lane - 1;
-          if ((orig == -2 || budget > 0) && v != orig) {
-            const int n2 = sym_next(v, nintro, a.pb->sym_lo, a.pb->sym_hi);
-            if (n2 >= 0) {
-              ok = 1;
-              val = v;
-              ni = n2;
-              if (orig != -2) nb = budget - 1;
+          // the root: the blank tape stopped before the read of cell 0 at t = 0
+          // (e13search.c:268-271: nodes 1, steps 0)
+          if (lane < E13_TW) lds_st(c.sb, sb + (PS_TAPE + lane) * 4, 0u);
+          if (lane < 20) {
+            int v = 0;
+            if (lane == 0 || lane == 6) v = E13_QOFF; // q, q0
+            if (lane == 3) v = 1;                      // nt
+            if (lane == 5) v = -1;                     // st0
+            if (lane == 7) v = 3;                      // abs_ok, rel_ok
+            if (lane == 17) v = a.budget;
+            if (lane == 18) v = a.nintro;
+            lds_st(c.sb, sb + (PS_REG + lane) * 4, (uint32_t)v);
+          }
+          if (lane < E13_MAXCELLS)
+            sbytes[sb + PS_TAB * 4 + lane] = lane < 2 * nq && base[lane] != -2 ? E13_C_EDIT : E13_C_FREE;
+          if (lane == 0) {
+            lds_st(c.sb, sb + PS_P * 4, p);
+            lds_st64(c, sb + PS_RNG * 4, probe_stream(a.seed, a.p0 + p));
+            lds_st64(c, sb + PS_W * 4, __builtin_bit_cast(uint64_t, 1.0));
+            lds_st64(c, sb + (PS_SUM + 0) * 4, 0ull);
+            lds_st64(c, sb + (PS_SUM + 2) * 4, __builtin_bit_cast(uint64_t, 1.0));
+            lds_st64(c, sb + (PS_SUM + 4) * 4, 0ull);
+            lds_st64(c, sb + (PS_SUM + 6) * 4, 0ull);
+          }
+          ph = PH_LEVEL;
+          progress = true;
+          wave_lds_sync();
+        }
+      }
+      if (ph == PH_LEVEL) { // the node's children, lane = child (e13search.c:278-287)
+        const int pend = uni((int)lds_ld(c.sb, sb + PS_PEND * 4));
+        const int budget = uni((int)lds_ld(c.sb, sb + PS_BUDGET * 4));
+        const int nintro = uni((int)lds_ld(c.sb, sb + PS_NINTRO * 4));
+        const int orig = base[pend];
+        int val, nb, ni;
+        const bool ok = lane < nslot && probe_child(lane, orig, budget, nintro, a.pb, val, nb, ni);
+        const bool hk = ok && lane == 0 && orig == -1; // keeping a halt: dies on the read, 0 steps (:164-166)
+        const uint64_t exist = ballot(ok), run = ballot(ok && !hk);
+        if (ok && !hk) sbytes[sb + PS_LIST * 4 + (int)rank_in(run)] = (uint8_t)lane;
+        if (lane == 0) {
+          lds_st(c.sb, sb + PS_DT * 4, 0u);
+          lds_st64(c, sb + PS_EXIST * 4, exist);
+          lds_st64(c, sb + PS_ENDED * 4, hk ? 1ull : 0ull);
+          lds_st64(c, sb + PS_SURV * 4, 0ull);
+          lds_st(c.sb, sb + PS_NEXT * 4, 0u);
+          lds_st(c.sb, sb + PS_COUNT * 4, (uint32_t)__popcll(run));
+          lds_st(c.sb, sb + PS_LEFT * 4, (uint32_t)__popcll(run));
+        }
+        ph = PH_CHILDREN;
+        progress = true;
+        wave_lds_sync();
+      }
+      if (ph == PH_CHILDREN) {
+        const int next = uni((int)lds_ld(c.sb, sb + PS_NEXT * 4));
+        const int count = uni((int)lds_ld(c.sb, sb + PS_COUNT * 4));
+        const int left = uni((int)lds_ld(c.sb, sb + PS_LEFT * 4));
+        if (next == count && left == 0) { // every child is in
+          const uint64_t exist = uni64(lds_ld64(c, sb + PS_EXIST * 4));
+          const uint64_t ended = uni64(lds_ld64(c, sb + PS_ENDED * 4));
+          const uint64_t surv = uni64(lds_ld64(c, sb + PS_SURV * 4));
+          probe_level_add(c, sb, exist, ended, surv);
+          wave_lds_sync();
+          const uint64_t live = exist & ~ended;
+          if (live == 0) { // e13search.c:303
+            if (lane < 4) {
+              const uint32_t p = lds_ld(c.sb, sb + PS_P * 4);
+              a.out[4 * (size_t)p + lane] = __builtin_bit_cast(double, lds_ld64(c, sb + (PS_SUM + 2 * lane) * 4));
             }
+            ph = PH_EMPTY;
+          } else { // draw the child the path goes on with (e13search.c:304-311)
+            uint64_t rng = uni64(lds_ld64(c, sb + PS_RNG * 4));
+            const uint32_t nlive = (uint32_t)__popcll(live);
+            const uint32_t r = (uint32_t)(probe_mix(rng) % (uint64_t)nlive);
+            uint64_t m = live;
+            for (uint32_t j = 0; j < r; ++j) m &= m - 1;
+            const double w = __builtin_bit_cast(double, lds_ld64(c, sb + PS_W * 4)) * (double)nlive;
+            if (lane == 0) {
+              lds_st64(c, sb + PS_RNG * 4, rng);
+              lds_st64(c, sb + PS_W * 4, __builtin_bit_cast(uint64_t, w));
+              lds_st(c.sb, sb + PS_CHOSEN * 4, (uint32_t)__builtin_ctzll(m));
+            }
+            ph = PH_DRAWN;
+          }
+          if (lane == 0) lds_st(c.sb, sb + PS_PH * 4, (uint32_t)ph);
+          progress = true;
+          wave_lds_sync();
+        }
+      }
+      if (lane == 0) lds_st(c.sb, sb + PS_PH * 4, (uint32_t)ph);
+      busy += ph != PH_EMPTY;
+    }
+    wave_lds_sync();
+    // ---- work for idle lanes (wave-uniform) ----
+    bool fresh = false;
+    uint64_t need = ballot(!active);
+    for (int k = 0; k < PK && need; ++k) {
+      const int sb = slots + k * (PS_WORDS * 4);
+      const int ph = uni((int)lds_ld(c.sb, sb + PS_PH * 4));
+      if (ph == PH_DRAWN) { // the drawn child's re-run makes the next node
+        if (lane == __builtin_ctzll(need)) {
+          fresh = true;
+          tslot = k;
+          tchild = ADV_CHILD;
+        }
+        if (lane == 0) lds_st(c.sb, sb + PS_PH * 4, PH_ADVANCE);
+        progress = true;
+      } else if (ph == PH_CHILDREN) {
+        const int next = uni((int)lds_ld(c.sb, sb + PS_NEXT * 4));
+        const int count = uni((int)lds_ld(c.sb, sb + PS_COUNT * 4));
+        const int give = min(__popcll(need), count - next);
+        if (give > 0) {
+          const int r = (int)rank_in(need);
+          if (((need >> lane) & 1ull) && r < give) {
+            fresh = true;
+            tslot = k;
+            tchild = sbytes[sb + PS_LIST * 4 + next + r];
+          }
+          if (lane == 0) lds_st(c.sb, sb + PS_NEXT * 4, (uint32_t)(next + give));
+          progress = true;
+        }
+      }
+      need = ballot(!active && !fresh);
+    }
+    if (ballot(fresh)) fills++;
+    wave_lds_sync();
+    // ---- set up the lanes that got work ----
+    if (fresh) {
+      const int sb = slots + tslot * (PS_WORDS * 4);
+      const int pend = (int)lds_ld(c.sb, sb + PS_PEND * 4);
+      const int budget = (int)lds_ld(c.sb, sb + PS_BUDGET * 4);
+      const int nintro = (int)lds_ld(c.sb, sb + PS_NINTRO * 4);
+      const int ch = tchild == ADV_CHILD ? (int)lds_ld(c.sb, sb + PS_CHOSEN * 4) : tchild;
+      probe_child(ch, base[pend], budget, nintro, a.pb, tval, tnb, tni);
+      for (int wd = 0; wd < E13_TW; ++wd) lds_st(c.sb, c.tbl + wd * 256, lds_ld(c.sb, sb + (PS_TAPE + wd) * 4));
+      build_rows(c, sbytes + sb + PS_TAB * 4, nq, pend, tval, 1, nullptr);
+      const int r0 = sb + PS_REG * 4;
+      L.q = (int)lds_ld(c.sb, r0);
+      L.row = c.rbl + (int)lds_ld(c.sb, r0 + 4) * 256;
+      L.t = (int)lds_ld(c.sb, r0 + 8);
+      L.nt = (int)lds_ld(c.sb, r0 + 12);
+      L.stage = (int)lds_ld(c.sb, r0 + 16);
+      L.st0 = (int)lds_ld(c.sb, r0 + 20);
+      L.q0 = (int)lds_ld(c.sb, r0 + 24);
+      L.flags = (int)lds_ld(c.sb, r0 + 28);
+      L.ones = (int)lds_ld(c.sb, r0 + 32);
+      L.hb = lds_ld64(c, r0 + 36);
+      L.ch0 = lds_ld64(c, r0 + 44);
+      L.ch1 = lds_ld64(c, r0 + 52);
+      L.ch2 = lds_ld64(c, r0 + 60);
+      arm_stage(L, c);
+      t0 = L.t;
+      active = 1;
+    }
+    const uint64_t act = ballot(active);
+    if (act == 0) {
+      if (exhausted && busy == 0) break;
+      if (!progress) { // cannot happen: a slot that holds a probe always has a next step
+        if (lane == 0) atomicOr(&a.ctr->overflow, 8u);
+        break;
+      }
+      continue;
+    }
+    w_rounds++;
+    w_slots += (uint64_t)__popcll(act);
+    // ---- a round of steps ----
+    int ev = EV_NONE, evcell = -1;
+    const bool solo = GATE == GATE_COUNT && a.solo > 0 && __popcll(act) == 1;
+    if (solo) { // the whole wave runs the lone lane's loop (its block memo takes idle lanes' tapes)
+      solo_memo_reset(active, c, act, fills, solo_last);
+      ev = run_solo(L, c, a.solo, evcell, __ffsll((long long)act) - 1);
+      if (!active) ev = EV_NONE;
+    } else if (active) {
+      ev = run_slice<GATE>(L, c, a.slice, evcell);
+    }
+    // ---- finished runs ----
+    if (active && ev != EV_NONE) {
+      active = 0;
+      const int sb = slots + tslot * (PS_WORDS * 4);
+      const uint32_t dt = (uint32_t)(L.t - t0);
+      if (tchild == ADV_CHILD) { // the drawn child again: the next node (e13search.c:305-311)
+        adv_steps += dt;
+        if (ev == EV_STOP) {
+          if (probe_next_node(c, L, sbytes, sb, evcell, tval, tnb, tni, base)) {
+            tchild = 0;
+            t0 = L.t;
+            active = 1;
+          }
+        } else { // cannot happen: the drawn child stopped at this cell the first time
+          atomicOr(&a.ctr->overflow, 16u);
+          lds_st(c.sb, sb + PS_PH * 4, PH_EMPTY);
+        }
+      } else if (lds_ld64(c, sb + PS_EXIST * 4) == (1ull << tchild)) {
+        // the level's only child: its sums, the draw among one live child and
+        // the next node are this lane's (e13search.c:289-311)
+        my_steps += dt;
+        const bool ended = ev == EV_DEAD || ev == EV_SURV;
+        probe_single_add(c, sb, (int)dt, ended, ev == EV_SURV);
+        if (ended) { // e13search.c:303
+          const uint32_t p = lds_ld(c.sb, sb + PS_P * 4);
+          for (int j = 0; j < 4; ++j)
+            a.out[4 * (size_t)p + j] = __builtin_bit_cast(double, lds_ld64(c, sb + (PS_SUM + 2 * j) * 4));
+          lds_st(c.sb, sb + PS_PH * 4, PH_EMPTY);
+        } else {
+          uint64_t rng = lds_ld64(c, sb + PS_RNG * 4);
+          (void)probe_mix(rng); // splitmix(rng) % 1; w *= 1
+          lds_st64(c, sb + PS_RNG * 4, rng);
+          if (probe_next_node(c, L, sbytes, sb, evcell, tval, tnb, tni, base)) {
+            tchild = 0;
+            t0 = L.t;
+            active = 1;
           }
         }
+      } else {
+        my_steps += dt;
+        lds_st(c.sb, sb + (PS_DT + tchild) * 4, dt);
+        const uint32_t bit = 1u << (tchild & 31);
+        const int wo = tchild >> 5;
+        if (ev == EV_DEAD || ev == EV_SURV) atomicOr(&smem[(sb >> 2) + PS_ENDED + wo], bit);
+        if (ev == EV_SURV) atomicOr(&smem[(sb >> 2) + PS_SURV + wo], bit);
+        atomicSub(&smem[(sb >> 2) + PS_LEFT], 1u);
       }
-      const bool run = ok && !halt_keep;
-      if (run) {
-        for (int wd = 0; wd < E13_TW; ++wd) {
-          uint32_t v = 0u;
-          if (src >= 0) v = (wd < 32 && src_saved) ? save[wd * 64 + src] : lds_ld(c.sb, tbase + wd * 256 + src * 4);
-          lds_st(c.sb, c.tbl + wd * 256, v);
-        }
-        build_rows(c, ptab, nq, pend, val, 1, nullptr);
-        L.q = n_q;
-        L.row = c.rbl + n_st * 256;
-        L.t = n_t;
-        L.nt = n_nt;
-        L.stage = n_stage;
-        L.st0 = n_st0;
-        L.q0 = n_q0;
-        L.flags = n_flags;
-        L.ones = n_ones;
-        L.hb = n_hb;
-        L.ch0 = n_ch0;
-        L.ch1 = n_ch1;
-        L.ch2 = n_ch2;
-        arm_stage(L, c);
-      }
-      fills++;
-      int active = run ? 1 : 0, res = halt_keep ? EV_DEAD : EV_NONE, evc = -1;
-      bool saved = false;
-      uint64_t savedm = 0; // lanes whose words 0..31 went to save[]
-      // ---- run every child to its next stop ----
-      for (;;) {
-        const uint64_t act = ballot(active);
-        if (act == 0) break;
-        w_rounds++;
-        w_slots += (uint64_t)__popcll(act);
-        int ev = EV_NONE, evcell = -1;
-        const bool solo = GATE == GATE_COUNT && a.solo > 0 && __popcll(act) == 1;
-        if (solo) { // the whole wave runs the lone lane's loop
-          if (!saved) { // the memo is about to take idle lanes' words 0..31
-            saved = true;
-            savedm = ballot(res == EV_STOP);
-            if (res == EV_STOP)
-              for (int wd = 0; wd < 32; ++wd) save[wd * 64 + lane] = lds_ld(c.sb, c.tbl + wd * 256);
-            wave_mem_sync();
-          }
-          solo_memo_reset(active, c, act, fills, solo_last);
-          ev = run_solo(L, c, a.solo, evcell, __ffsll((long long)act) - 1);
-          if (!active) ev = EV_NONE;
-        } else if (active) {
-          ev = run_slice<GATE>(L, c, a.slice, evcell);
-        }
-        if (active && ev != EV_NONE) {
-          active = 0;
-          res = ev;
-          evc = evcell;
-        }
-      }
-      const int dt = run ? L.t - n_t : 0;
-      my_steps += (uint64_t)dt;
-      const uint64_t exist = ballot(ok), ended = ballot(res == EV_DEAD || res == EV_SURV);
-      const uint64_t surv = ballot(res == EV_SURV), live = ballot(ok && res == EV_STOP);
-      probe_add(S, w, exist, ended, surv, dt);
-      if (live == 0) break; // e13search.c:303
-      // ---- draw the child the path goes on with (e13search.c:304-311) ----
-      const uint32_t nlive = (uint32_t)__popcll(live);
-      const uint32_t r = (uint32_t)(probe_mix(rng) % (uint64_t)nlive);
-      uint64_t m = live;
-      for (uint32_t k = 0; k < r; ++k) m &= m - 1;
-      const int ch = __builtin_ctzll(m);
-      w = w * (double)nlive;
-      n_q = __builtin_amdgcn_readlane(L.q, ch);
-      n_st = __builtin_amdgcn_readlane((L.row - c.rbl) >> 8, ch);
-      n_t = __builtin_amdgcn_readlane(L.t, ch);
-      n_nt = __builtin_amdgcn_readlane(L.nt, ch);
-      n_stage = __builtin_amdgcn_readlane(L.stage, ch);
-      n_st0 = __builtin_amdgcn_readlane(L.st0, ch);
-      n_q0 = __builtin_amdgcn_readlane(L.q0, ch);
-      n_flags = __builtin_amdgcn_readlane(L.flags, ch);
-      n_ones = __builtin_amdgcn_readlane(L.ones, ch);
-      n_hb = rl64(L.hb, ch);
-      n_ch0 = rl64(L.ch0, ch);
-      n_ch1 = rl64(L.ch1, ch);
-      n_ch2 = rl64(L.ch2, ch);
-      budget = __builtin_amdgcn_readlane(nb, ch);
-      nintro = __builtin_amdgcn_readlane(ni, ch);
-      const int nval = __builtin_amdgcn_readlane(val, ch);
-      const int npend = __builtin_amdgcn_readlane(evc, ch);
-      // a child that finished before the scalar loop started has its words 0..31 in save[];
-      // the lone lane of that loop kept its own
-      src_saved = (savedm >> ch) & 1ull;
-      if (lane == 0) ptab[pend] = (uint8_t)nval;
-      pend = npend;
-      src = ch;
     }
-    if (lane == 0) {
-      double *o = a.out + 4 * (size_t)p;
-      o[0] = S.leaves;
-      o[1] = S.nodes;
-      o[2] = S.steps;
-      o[3] = S.surv;
-    }
+    wave_lds_sync();
   }
-  const uint64_t s_steps = wave_sum64(my_steps);
+  const uint64_t s_steps = wave_sum64(my_steps), s_adv = wave_sum64(adv_steps);
   if (lane == 0) {
     atomicAdd((unsigned long long *)&a.ctr->steps, (unsigned long long)s_steps);
+    atomicAdd((unsigned long long *)&a.ctr->cyc[0], (unsigned long long)s_adv); // re-run steps
     atomicAdd((unsigned long long *)&a.ctr->rounds, (unsigned long long)w_rounds);
     atomicAdd((unsigned long long *)&a.ctr->lane_slots, (unsigned long long)w_slots);
   }
@@ -1984,10 +2205,14 @@ int e13k_launch_batch(const E13BatchArgs *a, int grid, int wg_threads, int gate,
   return (int)hipGetLastError();
 }
 
-// The probe kernel uses the expansion's LDS layout (rows, tapes, block keys,
-// 64 bytes per wave for the path's table).
+// The probe kernel: the expansion's LDS layout (rows, tapes, block keys, 64
+// bytes per wave) and each wave's PK probe slots.
+size_t e13k_probe_lds_bytes(int wg_threads, int gate) {
+  return e13k_lds_bytes(wg_threads, gate) + (size_t)(wg_threads / WAVE) * PK * PS_WORDS * 4;
+}
+
 int e13k_probe_max_wgs(int wg_threads, int gate) {
-  const size_t lds = e13k_lds_bytes(wg_threads, gate);
+  const size_t lds = e13k_probe_lds_bytes(wg_threads, gate);
   int n = 0;
   const hipError_t e = gate == GATE_HASH
                            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_probe<GATE_HASH>, wg_threads, lds)
@@ -1996,7 +2221,7 @@ int e13k_probe_max_wgs(int wg_threads, int gate) {
 }
 
 int e13k_launch_probe(const E13ProbeArgs *a, int grid, int wg_threads, int gate, hipStream_t s) {
-  const size_t lds = e13k_lds_bytes(wg_threads, gate);
+  const size_t lds = e13k_probe_lds_bytes(wg_threads, gate);
   if (gate == GATE_HASH) hipLaunchKernelGGL((k_probe<GATE_HASH>), dim3(grid), dim3(wg_threads), lds, s, *a);
   else hipLaunchKernelGGL((k_probe<GATE_COUNT>), dim3(grid), dim3(wg_threads), lds, s, *a);
   return (int)hipGetLastError();

@@ -213,7 +213,8 @@ typedef struct {
   int64_t probes;
   double sum[4], sq[4];   /* leaves, nodes, steps, survivors: sums of the per-probe values and squares */
   double mean[4], se[4];  /* the estimates and their standard errors */
-  uint64_t sim_steps;     /* machine steps simulated on the device (unweighted) */
+  uint64_t sim_steps;     /* machine steps of the probes' children, simulated on the device (unweighted) */
+  uint64_t rerun_steps;   /* steps of the re-runs of drawn children that make each path's next node */
   double kernel_ms;       /* HIP-event time of the probe kernels */
   int64_t launches, rounds, lane_slots;
 } e13_estimate_sum;
