@@ -99,7 +99,8 @@ class Tuning(ctypes.Structure):
                 ("seed_per_wave", ctypes.c_int32), ("sched_every", ctypes.c_int32), ("donate_max", ctypes.c_int32),
                 ("idle_naps", ctypes.c_int32), ("mail_every", ctypes.c_int32), ("trace_events", ctypes.c_int32),
                 ("graph_chain", ctypes.c_int32), ("cost_log", ctypes.c_int32),
-                ("learned", ctypes.c_int32), ("solo_steps", ctypes.c_int32), ("evict_after", ctypes.c_int32)]
+                ("learned", ctypes.c_int32), ("solo_steps", ctypes.c_int32), ("evict_after", ctypes.c_int32),
+                ("probe_slice", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -53,7 +53,7 @@ int e13_estimate(e13_ctx *c, const e13_problem *p, uint64_t seed, int64_t probes
   a.ctr = c->d_ctr;
   a.seed = seed;
   a.nq = p->nq;
-  a.slice = c->tune.slice;
+  a.slice = c->tune.probe_slice;
   a.budget = p->budget;
   a.nintro = p->sym_n0;
   a.solo = c->tune.solo_steps;

@@ -172,6 +172,7 @@ static void tuning_defaults(e13_tuning *t) {
   t->graph_chain = 0;
   t->solo_steps = 4096;
   t->evict_after = 16384;
+  t->probe_slice = 32;
 }
 
 static void tuning_merge(e13_tuning *dst, const e13_tuning *src) {
@@ -203,11 +204,13 @@ static void tuning_merge(e13_tuning *dst, const e13_tuning *src) {
   if (src->evict_after > 0) dst->evict_after = src->evict_after;
   if (src->evict_after < 0) dst->evict_after = 0;
   if (src->learned < 0) dst->learned = 0;
+  if (src->probe_slice > 0) dst->probe_slice = src->probe_slice;
 }
 
 static int tuning_valid(const e13_tuning *t) {
   if (t->form < 1 || t->form > 3) return 0;
   if (t->slice < 1 || t->slice > 1 << 16) return 0;
+  if (t->probe_slice < 1 || t->probe_slice > 1 << 16) return 0;
   if (t->wg_threads < 64 || t->wg_threads > 256 || t->wg_threads % 64) return 0;
   if (t->wgs_per_cu < 1 || t->wgs_per_cu > 16) return 0;
   if (t->cap < 1) return 0;

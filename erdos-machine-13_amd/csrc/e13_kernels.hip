@@ -1696,7 +1696,7 @@ __device__ __forceinline__ uint64_t probe_stream(uint64_t seed, uint64_t p) {
   return probe_mix(s);
 }
 
-#define PK 4          // probes in flight per wave
+#define PK 6          // probes in flight per wave
 // a slot, in words from its base
 #define PS_TAPE 0     // the node's tape (43 words)
 #define PS_REG 44     // the node's state: q st t nt stage st0 q0 flags ones hb(2) ch0(2) ch1(2) ch2(2)

@@ -108,6 +108,7 @@ typedef struct {
   int32_t evict_after;    /* > 0: while waves are hungry, a wave with nothing to give hands over machines
                              this many steps from the blank tape, and a wave whose only running machine is
                              one keeps it alone (register loop); negative: off */
+  int32_t probe_slice;    /* probe estimator (e13_estimate): steps per round between hand-outs of child runs */
 } e13_tuning;
 
 typedef struct {

@@ -101,7 +101,7 @@ def test_probes_match_oracle(e13, ectx, name, seed, n):
                                                                            oes.se[j]])
 
 
-@pytest.mark.parametrize("settings", [dict(solo_steps=-1), dict(slice=16), dict(slice=256, solo_steps=64),
+@pytest.mark.parametrize("settings", [dict(solo_steps=-1), dict(probe_slice=8), dict(probe_slice=256, solo_steps=64),
                                       dict(wg_threads=64)])
 def test_probes_do_not_depend_on_settings(e13, ectx, settings):
     """Scalar loop on or off, round length, workgroup size: the same probes."""
