@@ -233,7 +233,7 @@ def estimator_workload(e13, ctx, base, probes=1 << 20):
                          "exact_leaves": 3078656272, "exact_steps": 319112357176}}
 
 
-def cpu_estimate_baseline(probes=400000, max_s=120):
+def cpu_estimate_baseline(probes=8000000, max_s=120):
     """The reference program's own estimate mode on the same problem, all host cores."""
     ref = os.path.join(ROOT, "oracle", "_ref", "e13search")
     port = os.path.join(ROOT, "oracle", "_build", "e13oracle")

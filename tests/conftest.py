@@ -227,6 +227,29 @@ def oracle_probes(L, p, seed, n):
     return out
 
 
+def v1c0_rows():
+    """Every value row of 4_division_6/v1/v1c0_est.txt: (value, machine path,
+    leaves, leaves standard error, steps, steps standard error), the machine
+    being c0_0LD.txt with C0 set to the value (the file's own definition)."""
+    import tempfile
+    base = read_text(MACHINES, "c0_0LD.txt").split("\n")
+    d = os.path.join(tempfile.gettempdir(), "e13_c0_machines")
+    os.makedirs(d, exist_ok=True)
+    rows = []
+    for line in read_text(REF_RESULTS, "4_division_6_v1_v1c0_est.txt").splitlines():
+        f = line.split()
+        if not line.startswith("c0_") or len(f) < 7:
+            continue
+        v = f[0][3:]
+        rows_m = list(base)
+        rows_m[2] = f"{v[0]}{v[1]}{ord(v[2]) - 65} " + rows_m[2].split()[1]
+        path = os.path.join(d, f"c0_{v}.txt")
+        with open(path, "w") as fh:
+            fh.write("\n".join(rows_m))
+        rows.append((v, path, float(f[2]), float(f[3]), float(f[5]), float(f[6])))
+    return rows
+
+
 def v1c0_exact_rows():
     """Rows of 4_division_6/v1/v1c0_est.txt whose standard errors are 0 (the
     value of C0 leaves a one-leaf tree, so the estimate is exact): (value,

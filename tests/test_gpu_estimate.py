@@ -15,7 +15,7 @@ import struct
 import pytest
 
 from conftest import (MACHINES, PROBE_CASES, oracle_lib, oracle_probe_problem, probe_fixture, read_text,
-                      v1c0_exact_rows)
+                      v1c0_exact_rows, v1c0_rows)
 
 pytestmark = pytest.mark.gpu
 
@@ -118,6 +118,31 @@ def test_probes_do_not_depend_on_settings(e13, ectx, settings):
     assert [bits(t) for t in per] == [bits(t) for t in ref]
 
 
+def test_probes_across_launches(e13, ectx):
+    """More probes than one launch holds (2^22 per launch): probe numbering and
+    streams carry across launches, and the totals are the sums of every probe."""
+    args, _ = probe_fixture("free_m1n0")
+    n = (1 << 22) + 3000
+    say(f"estimate free_m1n0: {n} probes (two launches)")
+    es, per = ectx.estimate(gpu_problem(e13, args), n, seed=5, per_probe=True)
+    assert es.launches == 2 and es.probes == n
+    L = oracle_lib()
+    rc, p = oracle_probe_problem(args)
+    assert rc == 0
+    L.eo_probe_stream.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+    L.eo_probe_stream.restype = ctypes.c_uint64
+    for i in list(range(200)) + list(range((1 << 22) - 300, (1 << 22) + 300)) + list(range(n - 200, n)):
+        rng = ctypes.c_uint64(L.eo_probe_stream(5, i))
+        o = (ctypes.c_double * 4)()
+        L.eo_probe(ctypes.byref(p), ctypes.byref(rng), o)
+        assert bits(per[i]) == bits(o), i
+    tot = [0.0] * 4
+    for t in per:
+        for j in range(4):
+            tot[j] += t[j]
+    assert bits(tot) == bits(es.sum)
+
+
 def test_v1c0_one_leaf_rows_are_exact(e13, ectx):
     """results/4_division_6/v1/v1c0_est.txt: values of C0 whose tree is one
     leaf give the same estimate from every probe."""
@@ -130,3 +155,21 @@ def test_v1c0_one_leaf_rows_are_exact(e13, ectx):
         p = e13.setup(base, 9, 0, cells, (3, 9))
         es = ectx.estimate(p, 256, seed=1)
         assert (es.mean[0], es.mean[2], es.se[0], es.se[2]) == (leaves, steps, 0.0, 0.0), v
+
+
+def test_v1c0_sizes_agree_with_the_reference(e13, ectx):
+    """The run sizes the reference estimated for V1 (v1c0_est.txt, 4,000,000
+    probes per value of C0 on its own streams): the GPU's estimates on its
+    streams (fixed seed, so the check is deterministic) agree within 4 combined
+    standard errors for every value, the four large subtrees (about 1e13
+    leaves) included."""
+    cells = "D0 D1 E0 E1 F0 F1 G0 G1 H0 H1 I0 I1".split()
+    rows = [r for r in v1c0_rows() if r[3] > 0]
+    assert len(rows) == 4
+    say(f"estimate: the {len(rows)} large values of C0, 4M probes each")
+    for v, path, leaves, lse, steps, sse in rows:
+        with open(path) as fh:
+            base = e13.parse_machine(fh.read())
+        es = ectx.estimate(e13.setup(base, 9, 0, cells, (3, 9)), 4000000, seed=3)
+        assert abs(es.mean[0] - leaves) < 4 * (lse ** 2 + es.se[0] ** 2) ** 0.5, (v, es.mean[0], leaves)
+        assert abs(es.mean[2] - steps) < 4 * (sse ** 2 + es.se[2] ** 2) ** 0.5, (v, es.mean[2], steps)
