@@ -92,6 +92,9 @@ struct e13_ctx {
   double ckpt_every_s;
   int64_t ckpt_max_launches;
   int ckpt_slot;         /* 0: the next save goes to <path>.a, 1: to <path>.b */
+  uint32_t *h_stop, *d_stop; /* the persistent form's clean-stop flag: pinned host memory, its device address */
+  int64_t *d_wave_state;     /* per wave: stack range and last delivery tag, written when a wave leaves */
+  size_t wave_state_cap;
 };
 
 /* The MLP kernels (e13_mlp.hip). */
@@ -115,6 +118,12 @@ E13_INTERNAL void e13i_free_cost(e13_ctx *c);
 E13_INTERNAL void e13i_free_estimate(e13_ctx *c);
 E13_INTERNAL int e13i_ckpt_save(e13_ctx *c, const e13_problem *p, size_t top, size_t peak, int64_t launches,
                                 double wall_ms);
+E13_INTERNAL int e13i_ckpt_stop_requested(e13_ctx *c);
+E13_INTERNAL void e13i_ckpt_stop_clear(e13_ctx *c);
+E13_INTERNAL int e13i_ckpt3_prepare(e13_ctx *c, int total_waves);
+E13_INTERNAL int e13i_ckpt3_gather(e13_ctx *c, int total_waves, int stk_cap, size_t *nseeds);
+E13_INTERNAL int e13i_ckpt3_save(e13_ctx *c, const e13_problem *ps, int np, const e13_counts *out, size_t nseeds);
+E13_INTERNAL int e13i_ckpt3_load(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, size_t *nseeds);
 E13_INTERNAL int e13i_ckpt_load(e13_ctx *c, const e13_problem *p, size_t *top, size_t *peak, int64_t *launches,
                                 double *wall_ms);
 /* stored per-device tuning (e13_tune.c) */

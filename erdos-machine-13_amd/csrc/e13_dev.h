@@ -170,6 +170,10 @@ typedef struct E13SearchArgs {
   uint64_t *seed_cnt;     /* cost logging: per seed, leaves and steps of its subtree */
   uint64_t *trace;        /* optional: trace_cap words per wave, word 0 = events logged, then
                              (realtime << 16 | aux << 8 | event) */
+  const uint32_t *stop;   /* optional (clean stop): a flag in host memory; once it is nonzero every wave
+                             pauses its running machines as resumable nodes on its stack and leaves */
+  int64_t *wave_state;    /* with stop: per wave, its stack range [bot, top) and the tag of the last
+                             delivery it took from its mailbox, written when it leaves */
 } E13SearchArgs;
 
 typedef struct E13BatchArgs {

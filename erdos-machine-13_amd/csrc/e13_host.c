@@ -325,6 +325,8 @@ void e13_close(e13_ctx *c) {
   hipFree(c->d_trace);
   e13i_free_cost(c);
   e13i_free_estimate(c);
+  hipFree(c->d_wave_state);
+  if (c->h_stop) hipHostFree(c->h_stop);
   hipFree(c->d_chain);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
@@ -556,8 +558,8 @@ static int run_chain(e13_ctx *c, E13ExpandArgs *a, size_t batch, int grid, int w
 int e13_search(e13_ctx *c, const e13_problem *p, e13_counts *out, e13_survivor *surv, uint64_t surv_cap,
                uint64_t *n_surv) {
   if (!c || !out || !check_problem(p)) return E13_EINVAL;
-  const int ckpt = c->ckpt_path[0] != 0; /* checkpoints need the host loop: form 3 runs as form 2 */
-  if (c->tune.form == 3 && !ckpt) return e13_search_many(c, p, 1, out, surv, surv_cap, n_surv);
+  const int ckpt = c->ckpt_path[0] != 0;
+  if (c->tune.form == 3) return e13_search_many(c, p, 1, out, surv, surv_cap, n_surv);
   HIPCHK(hipSetDevice(c->device));
   const e13_tuning *tu = &c->tune;
   const int form = tu->form == 1 ? 1 : 2;
@@ -962,7 +964,15 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
   HIPCHK(hipMemsetAsync(c->d_ctr, 0, sizeof(E13Counters), c->stream));
   size_t nseeds = 0;
   uint64_t prev[3 + E13_NS + 1] = {0};
-  for (int k = 0; k < np; k++) {
+  /* checkpoints: a search with a checkpoint present resumes from its nodes and counts */
+  const int ckpt = c->ckpt_path[0] != 0;
+  int resumed = 0;
+  if (ckpt) {
+    if ((rc = e13i_ckpt3_load(c, ps, np, out, &nseeds)) < 0) return rc;
+    resumed = rc == 1;
+    e13i_ckpt_stop_clear(c); /* left over from the last clean stop */
+  }
+  for (int k = 0; k < np && !resumed; k++) {
     const int nslot = 4 * ps[k].nq + 1;
     const size_t target_k = (seed_target + (size_t)np - 1) / (size_t)np;
     E13Node root;
@@ -1035,9 +1045,9 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
     prev[2] = h1.survivors;
     for (int j = 0; j <= E13_NS; j++) prev[3 + j] = h1.hist[j];
   }
-  if (learned && nseeds > 1 && (rc = learned_order(c, ps, c->d_seeds, nseeds))) return rc;
-  if (tu->learned == 2 && nseeds > 1 && (rc = random_order(c, c->d_seeds, nseeds))) return rc;
-  const int logging = tu->cost_log == 1;
+  if (!resumed && learned && nseeds > 1 && (rc = learned_order(c, ps, c->d_seeds, nseeds))) return rc;
+  if (!resumed && tu->learned == 2 && nseeds > 1 && (rc = random_order(c, c->d_seeds, nseeds))) return rc;
+  const int logging = tu->cost_log == 1 && !ckpt; /* subtree costs need the frontier of one launch */
   if (logging) { /* number the frontier and count each subtree */
     if ((rc = e13i_ensure_buf((void **)&c->d_seed_cnt, &c->seed_cnt_cap, nseeds > 0 ? 2 * nseeds : 2,
                               sizeof(uint64_t))))
@@ -1046,6 +1056,18 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
     if (e13k_launch_seed_ids(c->d_seeds, (uint32_t)nseeds, c->stream)) return E13_EHIP;
   }
   E13Sched sch;
+  if (ckpt && (rc = e13i_ckpt3_prepare(c, total_waves))) return rc;
+  int64_t call_launches = 0;
+  double t_ck = host_ms(), last_stop_check = 0;
+  E13Counters h;
+  float ms = 0;
+  for (;;) { /* one launch; with checkpoints, again from the nodes a clean stop left */
+  if (nseeds > (size_t)total_waves * (size_t)stk_cap) return E13_ESTACK;
+  if (call_launches > 0) { /* per-launch figures restart (the survivor count goes on) */
+    HIPCHK(hipMemsetAsync(&c->d_ctr->steps, 0, sizeof(uint64_t), c->stream));
+    HIPCHK(hipMemsetAsync(&c->d_ctr->lane_slots, 0, 2 * sizeof(uint64_t), c->stream));
+    HIPCHK(hipMemsetAsync(c->d_ctr->cyc, 0, sizeof c->d_ctr->cyc, c->stream));
+  }
   memset(&sch, 0, sizeof sch);
   sch.busy = nseeds < (size_t)total_waves ? (int32_t)nseeds : total_waves; /* waves dealt a seed */
   HIPCHK(hipMemcpyAsync(c->d_sch, &sch, sizeof sch, hipMemcpyHostToDevice, c->stream));
@@ -1087,6 +1109,12 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
     a.seed_cnt = c->d_seed_cnt;
     a.nseed_cnt = (uint32_t)nseeds;
   }
+  if (ckpt) {
+    *c->h_stop = 0u;
+    __atomic_thread_fence(__ATOMIC_SEQ_CST);
+    a.stop = c->d_stop;
+    a.wave_state = c->d_wave_state;
+  }
   c->trace_waves = c->trace_per_wave = 0;
   if (tu->trace_events > 0) {
     const size_t per = (size_t)tu->trace_events + 1, words = per * (size_t)total_waves;
@@ -1108,15 +1136,32 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
     return E13_EHIP;
   }
   HIPCHK(hipEventRecord(c->ev1, c->stream));
+  int stop_file = 0;
+  if (ckpt) { /* wait, raising the stop flag for the stop file or the checkpoint interval */
+    int raised = 0;
+    for (;;) {
+      const hipError_t q = hipStreamQuery(c->stream);
+      if (q == hipSuccess) break;
+      if (q != hipErrorNotReady) return E13_EHIP;
+      const double now = host_ms();
+      if (!raised && now - last_stop_check > 1000.0) {
+        last_stop_check = now;
+        stop_file = e13i_ckpt_stop_requested(c);
+      }
+      if (!raised && (stop_file || now - t_ck > 1000.0 * c->ckpt_every_s)) {
+        __atomic_store_n(c->h_stop, 1u, __ATOMIC_SEQ_CST);
+        raised = 1;
+      }
+      usleep(1000);
+    }
+  }
   uint64_t pcnt[E13_MAXPROB * E13_PCNT];
-  E13Counters h;
   HIPCHK(hipMemcpyAsync(pcnt, c->d_pcnt, (size_t)np * E13_PCNT * sizeof(uint64_t), hipMemcpyDeviceToHost,
                         c->stream));
   HIPCHK(hipMemcpyAsync(&h, c->d_ctr, sizeof h, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipMemcpyAsync(&sch, c->d_sch, sizeof sch, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   if (timing) tm[2] = host_ms();
-  float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->stats.kernel_ms += ms;
   c->stats.launches++;
@@ -1149,6 +1194,19 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
     out[k].survivors += q[2];
     for (int j = 0; j <= E13_NS; j++) out[k].dead_at_stage[j] += q[3 + j];
   }
+  if (!ckpt) break;
+  /* what a clean stop left: every wave's stack and the deliveries still in mailboxes */
+  call_launches++;
+  if ((rc = e13i_ckpt3_gather(c, total_waves, stk_cap, &nseeds))) return rc;
+  if (nseeds == 0) break; /* the search ended before the flag was seen */
+  if ((rc = e13i_ckpt3_save(c, ps, np, out, nseeds))) return rc;
+  t_ck = host_ms();
+  if (stop_file) {
+    e13i_ckpt_stop_clear(c);
+    return E13_ESTOPPED;
+  }
+  if (c->ckpt_max_launches > 0 && call_launches >= c->ckpt_max_launches) return E13_ESTOPPED;
+  } /* next launch, from the gathered nodes */
   uint64_t ns = h.nsurv;
   if (n_surv) *n_surv = ns;
   uint64_t keep = ns < c->surv_cap ? ns : c->surv_cap;

@@ -1448,9 +1448,12 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
       const uint64_t now = __builtin_readsteadycounter();
       if (idle_since == 0) idle_since = now;
       if ((++idle_polls & 63) == 0) { // an abort may have overwritten the end mark of this mailbox
-        int err = 0;
+        int err = 0, stp = 0;
         if (lane == 0) err = ld_agent_i(&a.sch->error);
-        if (uni(err)) break;
+        if (a.stop != nullptr && lane == 0)
+          stp = (int)__hip_atomic_load(a.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        if (uni(err) || uni(stp)) break; // a clean stop: nothing here to keep (a delivery still on its way
+                                         // stays in the mailbox for the host)
       }
       if (now - idle_since > (uint64_t)a.idle_limit) { // watchdog: no delivery and no end for this long
         if (lane == 0) atomicMax(&a.sch->error, 3);
@@ -1470,10 +1473,12 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
     // shared state, read every few rounds (staggered over the waves): the abort
     // flag and the hungry map; the loads complete while the lanes step
     const bool refresh = --refresh_in <= 0;
-    int s_err = 0;
+    int s_err = 0, s_stop = 0;
     uint64_t hw = 0; // hungry-map word `lane` (maps of up to 4096 waves)
     if (refresh) {
       if (lane == 0) s_err = ld_agent_i(&a.sch->error);
+      if (a.stop != nullptr && lane == 0)
+        s_stop = (int)__hip_atomic_load(a.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
       const bool can_give = top - bot >= 2 || (top - bot == 1 && staged == top - 1 && cur_end - cur_slot >= 2) ||
                             (a.evict_after > 0 && top == bot && __popcll(act) >= 2);
       if (lane < a.hungry_words && can_give) hw = ld_agent64(&a.hungry[lane]);
@@ -1495,6 +1500,10 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
     tn = stamp();
     cyc1 += tn - ts;
     ts = tn;
+    // ---- a clean stop: every running machine is paused as a resumable node, as at
+    // a launch cap, and goes onto this wave's stack below ----
+    const bool stopping = refresh && uni(s_stop) != 0;
+    if (stopping && active && ev == EV_NONE) ev = EV_CAP;
     // ---- finished machines ----
     // ---- long runners to hungry waves: with nothing on its stack to give, a wave
     // pauses machines that are far along (evict_after steps from the blank tape:
@@ -1577,6 +1586,11 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
     tn = stamp();
     cyc2 += tn - ts;
     ts = tn;
+    if (stopping) { // the staged node's cursor goes with it; the stack is this wave's whole state
+      if (staged >= bot && staged < top && lane == 0)
+        stk_node(a, wid, staged)->next_slot = slot_range(cur_slot, cur_end);
+      break;
+    }
     if (refresh) {
       if (uni(s_err)) break;
       // ---- deliver work to hungry waves: the oldest node, or with a single
@@ -1641,6 +1655,11 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
     ts = tn;
   }
   // ---- flush ----
+  if (a.wave_state != nullptr && lane == 0) {
+    a.wave_state[3 * wid + 0] = bot;
+    a.wave_state[3 * wid + 1] = top;
+    a.wave_state[3 * wid + 2] = (int64_t)last_tag;
+  }
   if (lane == 0) {
     atomicAdd(&a.sch->exited, 1);
     atomicAdd(&a.sch->donations, (int)w_gives);

@@ -123,3 +123,105 @@ def test_cli_exit_code_3_and_resume(tmp_path):
     ref = parse_summaries(read_text(REF_DUMP, "delete_f3.txt"))[12]
     assert (got["leaves"], got["steps"], got["survivors"], got["hist"]) == (ref["leaves"], ref["steps"],
                                                                              ref["survivors"], ref["hist"])
+
+
+COUNTER = dict(leaves=3078656272, steps=319112357176, survivors=61,
+               hist=[0, 2957671647, 119843176, 1115937, 22970, 2236, 1, 0, 244, 0])
+
+
+@pytest.fixture(scope="module")
+def pctx(e13):
+    """A context of its own on the persistent form (the default settings)."""
+    ctx = e13.Context(0)
+    yield ctx
+    ctx.close()
+
+
+def counter_problem(e13):
+    base = e13.parse_machine(read_machine())
+    return e13.setup(base, 13, free_cells="D0 G0 K0 K1 L1 M1".split(), sym=(10, 13))
+
+
+def test_persistent_stops_and_relaunches(e13, pctx, tmp_path):
+    """The persistent search with a checkpoint every 50 ms: each launch is
+    stopped cleanly (running machines paused as nodes on the waves' stacks),
+    saved, and relaunched from the gathered nodes; the counter search still
+    gives the reference's counts and 61 survivors."""
+    p = counter_problem(e13)
+    path = str(tmp_path / "p3")
+    pctx.set_checkpoint(path, every_sec=0.05)
+    say("counter search, persistent form, checkpoint every 50 ms")
+    try:
+        c, sv, n = pctx.search(p)
+    finally:
+        pctx.set_checkpoint(None)
+    d = c.as_dict()
+    assert (d["leaves"], d["steps"], d["survivors"], d["hist"]) == (COUNTER["leaves"], COUNTER["steps"],
+                                                                     COUNTER["survivors"], COUNTER["hist"])
+    assert n == 61 and all(s.recheck_ok == 1 for s in sv)
+    assert os.path.exists(path)  # it did stop and save at least once
+    assert pctx.stats()["search_launches"] >= 3
+
+
+def test_persistent_clean_stop_and_resume(e13, pctx, tmp_path):
+    """E13_ESTOPPED after two stopped launches, resumed call after call until
+    the search ends: FMAX=4 over X=0..13 in one persistent search per call gives
+    the reference's per-X counts; a checkpoint of the host loop or of another
+    search is refused."""
+    base = e13.parse_machine(read_machine())
+    ps = [e13.setup(base, X, 4) for X in range(14)]
+    path = str(tmp_path / "p4")
+    pctx.set_checkpoint(path, every_sec=0.25, max_launches=2)
+    stops = 0
+    say("delete FMAX=4 X=0..13, persistent form, stop after every 2 launches of 250 ms")
+    try:
+        while True:
+            try:
+                cs, sv, n = pctx.search_many(ps)
+                break
+            except e13.E13Error as ex:
+                assert ex.code == -9
+                stops += 1
+                assert stops < 200
+        with pytest.raises(e13.E13Error) as ex:  # the same file for other problems
+            pctx.search_many(ps[:13])
+        assert ex.value.code == -10
+    finally:
+        pctx.set_checkpoint(None)
+    assert stops >= 1
+    ref = parse_summaries(read_text(REF_DUMP, "delete_f4.txt"))
+    for X in range(14):
+        d = cs[X].as_dict()
+        assert (d["leaves"], d["steps"], d["hist"]) == (ref[X]["leaves"], ref[X]["steps"], ref[X]["hist"]), X
+    assert n == sum(ref[X]["survivors"] for X in range(14)) == 57
+    assert all(s.recheck_ok == 1 for s in sv)
+
+
+def test_persistent_stop_file(e13, pctx, tmp_path):
+    """The stop file (<path>.stop) stops a running persistent search within
+    about a second; the same call again resumes and finishes it."""
+    import threading
+    import time
+    p = counter_problem(e13)
+    base = e13.parse_machine(read_machine())
+    ps = [e13.setup(base, X, 4) for X in range(14)]
+    path = str(tmp_path / "p5")
+    pctx.set_checkpoint(path, every_sec=3600)
+    timer = threading.Timer(0.4, lambda: open(path + ".stop", "w").close())
+    say("delete FMAX=4 X=0..13, persistent form, stop file after 0.4 s")
+    try:
+        timer.start()
+        t0 = time.time()
+        with pytest.raises(e13.E13Error) as ex:
+            pctx.search_many(ps)
+        assert ex.value.code == -9 and time.time() - t0 < 2.5
+        assert not os.path.exists(path + ".stop")
+        say("resume")
+        cs, sv, n = pctx.search_many(ps)
+    finally:
+        timer.cancel()
+        pctx.set_checkpoint(None)
+    ref = parse_summaries(read_text(REF_DUMP, "delete_f4.txt"))
+    assert sum(cs[X].leaves for X in range(14)) == sum(ref[X]["leaves"] for X in range(14))
+    assert sum(cs[X].steps for X in range(14)) == sum(ref[X]["steps"] for X in range(14))
+    assert n == 57
