@@ -27,9 +27,10 @@
  * E13_TUNING_FILE (stored settings; <name>_small.<ext> beside it for small searches); E13_TRACE=N with
  * E13_TRACE_FILE=path writes the persistent search's wave event log (N events per wave) to path; the
  * library prints its host phase times with E13_TIMING=1 and a line per autotuner trial with E13_TUNE_LOG=1.
- * Checkpoints (e13gpu.py:19-23): E13_CKPT=file saves the host-loop search every E13_CKPT_SEC seconds
- * (default 1800) and after E13_CKPT_LAUNCHES launches if set; creating <file>.stop saves and exits with
- * code 3; the same command resumes.  With a checkpoint file the search runs the host loop (batched form).
+ * Checkpoints (e13gpu.py:19-23): E13_CKPT=file saves the search every E13_CKPT_SEC seconds (default
+ * 1800) and after E13_CKPT_LAUNCHES launches if set; creating <file>.stop saves and exits with code 3;
+ * the same command resumes.  The persistent form keeps one file for all X of a delete search; the host
+ * loop (E13_FORM=1, 2) one per X (<file>.X<n>).
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -114,6 +115,18 @@ static char *read_file(const char *path) {
 static const char *g_ckpt;
 static int g_ckpt_per_x;
 
+static void set_checkpoint(e13_ctx *ctx, int X) {
+  char path[1024];
+  if (g_ckpt_per_x) snprintf(path, sizeof path, "%s.X%d", g_ckpt, X);
+  else snprintf(path, sizeof path, "%s", g_ckpt);
+  const char *sec = getenv("E13_CKPT_SEC");
+  const char *nl = getenv("E13_CKPT_LAUNCHES");
+  if (e13_set_checkpoint(ctx, path, sec ? atof(sec) : 0.0, nl ? atoll(nl) : 0)) {
+    fprintf(stderr, "bad checkpoint path %s\n", path);
+    exit(1);
+  }
+}
+
 static int run_one(e13_ctx *ctx, const int32_t *base, int nq_base, int X, int fmax, const uint8_t *freebase,
                    int lo, int hi) {
   e13_problem p;
@@ -140,17 +153,7 @@ static int run_one(e13_ctx *ctx, const int32_t *base, int nq_base, int X, int fm
   uint64_t ns = 0;
   e13_stats st0, st1;
   e13_get_stats(ctx, &st0);
-  if (g_ckpt) {
-    char path[1024];
-    if (g_ckpt_per_x) snprintf(path, sizeof path, "%s.X%d", g_ckpt, X);
-    else snprintf(path, sizeof path, "%s", g_ckpt);
-    const char *sec = getenv("E13_CKPT_SEC");
-    const char *nl = getenv("E13_CKPT_LAUNCHES");
-    if (e13_set_checkpoint(ctx, path, sec ? atof(sec) : 0.0, nl ? atoll(nl) : 0)) {
-      fprintf(stderr, "bad checkpoint path %s\n", path);
-      exit(1);
-    }
-  }
+  if (g_ckpt) set_checkpoint(ctx, X);
   double w0 = now_s();
   rc = e13_search(ctx, &p, &cnt, sv, cap, &ns);
   double wall = now_s() - w0;
@@ -227,9 +230,14 @@ static int run_many(e13_ctx *ctx, const int32_t *base, int nq_base, int nx, char
   uint64_t ns = 0;
   e13_stats st0, st1;
   e13_get_stats(ctx, &st0);
+  if (g_ckpt) set_checkpoint(ctx, 0); /* one file for the whole persistent search */
   double w0 = now_s();
   int rc = e13_search_many(ctx, ps, np, cnt, sv, cap, &ns);
   double wall = now_s() - w0;
+  if (rc == E13_ESTOPPED) { /* e13gpu.py:398-409 */
+    fprintf(stderr, "STOPPED cleanly at wall=%.0fs; run the same command to resume\n", wall);
+    exit(3);
+  }
   if (rc) {
     fprintf(stderr, "search failed: %s\n", e13_strerror(rc));
     exit(1);
@@ -390,8 +398,8 @@ int main(int argc, char **argv) {
     int fmax = atoi(argv[3]);
     e13_tuning cur;
     e13_get_tuning(ctx, &cur);
-    g_ckpt_per_x = argc - 4 > 1;
-    if (cur.form == 3 && argc - 4 > 1 && !g_ckpt)
+    g_ckpt_per_x = cur.form != 3 && argc - 4 > 1; /* the host loop keeps one file per X */
+    if (cur.form == 3 && argc - 4 > 1)
       run_many(ctx, base, nq, argc - 4, argv + 4, fmax);
     else
       for (int i = 4; i < argc; i++) run_one(ctx, base, nq, atoi(argv[i]), fmax, NULL, -1, -1);

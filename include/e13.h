@@ -183,20 +183,23 @@ int e13_simulate_batch(e13_ctx *ctx, const int8_t *tabs, int ncells, int n, cons
 int e13_simulate_batch_device(e13_ctx *ctx, const void *tabs_dev, int nq, int n, const void *caps_dev, void *out_dev);
 int e13_sync(e13_ctx *ctx);
 
-/* Checkpoints of the host-loop search: replace the GPU driver's E13_CKPT /
+/* Checkpoints, clean stop and resume: replace the GPU driver's E13_CKPT /
    E13_CKPT_SEC / <file>.stop handling (e13gpu.py:19-23, 349-409).  With a
-   path set, e13_search runs the host loop (form 1 as set, otherwise the
-   batched form 2: the persistent form has no point between launches where the
-   state is all in the stack) and, between launches, saves the stack, counters
-   and survivor records to <path>.a / <path>.b in turn, the pointer file <path>
-   naming the last complete one: every every_sec seconds (<= 0: 1800), when
-   the file <path>.stop appears (checked at most once a second; the file is
-   removed), or after max_launches launches of this call (> 0; 0 = no limit).
-   A stop returns E13_ESTOPPED after the save.  A search started with a
-   checkpoint present resumes from it (E13_ECKPT if it belongs to another
-   problem or is damaged); results are those of an uninterrupted run.  A stop
-   file left from an earlier clean stop is removed at the start.  path NULL or
-   "": off. */
+   path set, e13_search / e13_search_many save the remaining search to
+   <path>.a / <path>.b in turn, the pointer file <path> naming the last complete
+   one: every every_sec seconds (<= 0: 1800), when the file <path>.stop appears
+   (checked at most once a second; the file is removed), or after max_launches
+   launches of this call (> 0; 0 = no limit).  Host-loop forms (1, 2): between
+   launches, the node stack, counters and survivor records.  Persistent form
+   (3): the launch is stopped cleanly (every wave pauses its running machines as
+   resumable nodes on its stack and leaves), and the waves' stacks plus any
+   delivery still in a mailbox are saved with the counters and survivor
+   records; on a timed checkpoint the search relaunches from them at once.  A
+   stop returns E13_ESTOPPED after the save.  A search started with a
+   checkpoint present resumes from it (E13_ECKPT if it belongs to other
+   problems, to the other form, or is damaged); results are those of an
+   uninterrupted run.  A stop file left from an earlier clean stop is removed
+   at the start; the files stay after the search ends.  path NULL or "": off. */
 int e13_set_checkpoint(e13_ctx *ctx, const char *path, double every_sec, int64_t max_launches);
 
 /* Probe estimator: replaces the estimate mode of the reference

@@ -225,3 +225,26 @@ def test_persistent_stop_file(e13, pctx, tmp_path):
     assert sum(cs[X].leaves for X in range(14)) == sum(ref[X]["leaves"] for X in range(14))
     assert sum(cs[X].steps for X in range(14)) == sum(ref[X]["steps"] for X in range(14))
     assert n == 57
+
+
+def test_cli_persistent_multi_x_checkpoint(tmp_path):
+    """The command line on the persistent form: delete FMAX=3 over X=0..13 in
+    one search with one checkpoint file, stopped after every launch of 20 ms
+    (exit code 3) and re-run until it ends with the reference's summaries."""
+    ck = str(tmp_path / "cli_p3")
+    env = dict(os.environ, E13_CKPT=ck, E13_CKPT_SEC="0.02", E13_CKPT_LAUNCHES="1")
+    args = [CLI, "delete", machine_path(), "3"] + [str(x) for x in range(14)]
+    codes = []
+    for _ in range(200):
+        say(f"e13search_gpu delete FMAX=3 X=0..13 with E13_CKPT, run {len(codes) + 1}")
+        r = subprocess.run(args, env=env, capture_output=True, text=True, timeout=120)
+        codes.append(r.returncode)
+        if r.returncode != 3:
+            break
+        assert "STOPPED cleanly" in r.stderr
+    assert codes[-1] == 0 and codes.count(3) >= 1, codes
+    got = parse_summaries(r.stderr)
+    ref = parse_summaries(read_text(REF_DUMP, "delete_f3.txt"))
+    for X in range(14):
+        assert (got[X]["leaves"], got[X]["steps"], got[X]["survivors"], got[X]["hist"]) == (
+            ref[X]["leaves"], ref[X]["steps"], ref[X]["survivors"], ref[X]["hist"]), X
