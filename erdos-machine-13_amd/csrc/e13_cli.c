@@ -5,6 +5,8 @@
  *   e13search_gpu delete  machine.txt FMAX X...
  *   e13search_gpu free    machine.txt X CELL...
  *   e13search_gpu freesym machine.txt X LO HI CELL...
+ *   e13search_gpu gpu     machine.txt X LO HI CELL...   (the GPU driver's argv, src/e13gpu.py:293-301;
+ *                                       LO = -1: no symmetry)
  *   e13search_gpu estimate machine.txt PROBES X FMAX LO HI [CELL...]   (probe estimator; LO < 0: no
  *                                       symmetry; cells: free mode; probe i draws from its own stream of
  *                                       seed $E13_SEED, default 0)
@@ -490,6 +492,29 @@ int main(int argc, char **argv) {
             es.kernel_ms, es.kernel_ms > 0 ? 1e3 * (double)probes / es.kernel_ms : 0.0,
             es.kernel_ms > 0 ? 1e3 * (double)es.sim_steps / es.kernel_ms : 0.0, (unsigned long long)es.sim_steps,
             (long long)es.launches, es.rounds ? (double)es.lane_slots / (double)es.rounds : 0.0);
+  } else if (!strcmp(mode, "gpu")) { /* the GPU driver's argv: machine.txt X LO HI CELL... (e13gpu.py:293-301) */
+    if (argc < 6) {
+      fprintf(stderr, "usage: e13search_gpu gpu machine.txt X LO HI CELL...   (LO = -1: no symmetry)\n");
+      return 1;
+    }
+    uint8_t fb[32] = {0};
+    const int X = atoi(argv[3]), lo = atoi(argv[4]), hi = atoi(argv[5]);
+    for (int i = 6; i < argc; i++) {
+      int q = argv[i][0] - 'A', s = argv[i][1] - '0';
+      if (q < 0 || q >= nq || (s != 0 && s != 1)) {
+        fprintf(stderr, "bad cell %s\n", argv[i]);
+        return 1;
+      }
+      fb[2 * q + s] = 1;
+    }
+    e13_problem p;
+    if (e13_setup(base, nq, X, 0, fb, lo, lo >= 0 ? hi : -1, &p) == E13_OK) { /* the driver's first line (:318) */
+      e13_tuning cur;
+      e13_get_tuning(ctx, &cur);
+      fprintf(stderr, "X=%d nq=%d free=%d sym=%d..%d n0=%d node=%dB form=%d\n", X, p.nq, argc - 6, p.sym_lo, p.sym_hi,
+              p.sym_n0, 304, cur.form);
+    }
+    run_one(ctx, base, nq, X, 0, fb, lo, lo >= 0 ? hi : -1);
   } else if (!strcmp(mode, "free") || !strcmp(mode, "freesym")) {
     if (argc < 4) return 1;
     uint8_t fb[32] = {0};

@@ -248,3 +248,4 @@ def test_cli_persistent_multi_x_checkpoint(tmp_path):
     for X in range(14):
         assert (got[X]["leaves"], got[X]["steps"], got[X]["survivors"], got[X]["hist"]) == (
             ref[X]["leaves"], ref[X]["steps"], ref[X]["survivors"], ref[X]["hist"]), X
+
