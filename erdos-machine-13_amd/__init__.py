@@ -146,7 +146,8 @@ EXPORTS = ("e13_open", "e13_close", "e13_strerror", "e13_set_tuning", "e13_get_t
            "e13_synth_table", "e13_get_trace", "e13_batch_digest", "e13_synth_run", "e13_get_cost_log",
            "e13_cost_targets", "e13_mlp_init", "e13_mlp_forward", "e13_mlp_grad", "e13_mlp_train",
            "e13_set_cost_model", "e13_autotune", "e13_save_tuning", "e13_load_tuning", "e13_tuning_file",
-           "e13_tuning_file_small", "e13_search_profile", "e13_estimate", "e13_set_checkpoint")
+           "e13_tuning_file_small", "e13_search_profile", "e13_estimate", "e13_set_checkpoint",
+           "e13_deep_check_line", "e13_deep_check")
 
 _lib = None
 
@@ -214,6 +215,8 @@ def lib():
     L.e13_tuning_file.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
     L.e13_tuning_file_small.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
     L.e13_search_profile.argtypes = [ctypes.c_void_p, P(Problem), ctypes.c_int, P(Tuning)]
+    L.e13_deep_check_line.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    L.e13_deep_check.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int64]
     L.e13_set_checkpoint.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_double, ctypes.c_int64]
     L.e13_estimate.argtypes = [ctypes.c_void_p, P(Problem), ctypes.c_uint64, ctypes.c_int64, P(ctypes.c_double),
                                P(EstimateSum)]
@@ -336,6 +339,21 @@ def load_tuning(path):
     t = Tuning()
     _check(lib().e13_load_tuning(path.encode(), ctypes.byref(t)), "e13_load_tuning")
     return t.as_dict()
+
+
+def deep_check_line(line, survivor_id, nmax):
+    """The deep stage check (src/deep.c) of one SURVIVOR line: its result line."""
+    buf = ctypes.create_string_buffer(4096)
+    _check(min(lib().e13_deep_check_line(line.encode(), survivor_id, nmax, buf, 4096), 0), "e13_deep_check_line")
+    return buf.value.decode()
+
+
+def deep_check(text, nmax, threads=8):
+    """The deep stage check of every SURVIVOR line of text, numbered from 1."""
+    cap = 2048 * (len(text) // 64 + 16)
+    buf = ctypes.create_string_buffer(cap)
+    _check(min(lib().e13_deep_check(text.encode(), nmax, threads, buf, cap), 0), "e13_deep_check")
+    return buf.value.decode()
 
 
 def recheck(tab, nq, st0, head0, abs_ok, rel_ok, steps):

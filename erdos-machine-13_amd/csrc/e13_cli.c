@@ -11,6 +11,7 @@
  *                                       symmetry; cells: free mode; probe i draws from its own stream of
  *                                       seed $E13_SEED, default 0)
  *   e13search_gpu synth   SEED0 K0 N      (the synthetic stress batch; one summary line)
+ *   e13search_gpu deep    NMAX < survivors.txt   (src/deep.c's stage check to NMAX, host threads)
  *   e13search_gpu tune    machine.txt FMAX X...   (autotune on that search; saves the settings
  *   e13search_gpu tune    machine.txt freesym X LO HI CELL...   to $E13_TUNING_OUT or to the
  *                                       device's tuning file of the profile the search runs with)
@@ -38,6 +39,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
+#include <unistd.h>
 
 #include "e13.h"
 
@@ -283,6 +285,34 @@ int main(int argc, char **argv) {
   if (argc < 3) {
     fprintf(stderr, "usage: e13search_gpu base|mutant|delete|free|freesym machine.txt [...]\n");
     return 1;
+  }
+  if (!strcmp(argv[1], "deep")) { /* deep NMAX < survivors.txt (src/deep.c): host threads, no GPU */
+    const int nmax = atoi(argv[2]);
+    size_t cap = 1 << 20, len = 0;
+    char *text = (char *)malloc(cap);
+    for (size_t k; text && (k = fread(text + len, 1, cap - len - 1, stdin)) > 0;) {
+      len += k;
+      if (len + 1 == cap) {
+        char *t2 = (char *)realloc(text, cap *= 2);
+        if (!t2) free(text);
+        text = t2;
+      }
+    }
+    if (!text) return 1;
+    text[len] = 0;
+    const char *th = getenv("OMP_NUM_THREADS");
+    const int threads = th && atoi(th) > 0 ? atoi(th) : (int)sysconf(_SC_NPROCESSORS_ONLN);
+    const int64_t outlen = 2048 * (int64_t)(len / 64 + 16);
+    char *out = (char *)malloc((size_t)outlen);
+    const int rc = out ? e13_deep_check(text, nmax, threads, out, outlen) : E13_ENOMEM;
+    if (rc < 0) {
+      fprintf(stderr, "deep: %s\n", e13_strerror(rc));
+      return 1;
+    }
+    fputs(out, stdout);
+    free(out);
+    free(text);
+    return 0;
   }
   if (!strcmp(argv[1], "synth")) { /* synth SEED0 K0 N: the synthetic stress batch */
     e13_tuning st;

@@ -231,6 +231,20 @@ int e13_estimate(e13_ctx *ctx, const e13_problem *p, uint64_t seed, int64_t prob
 int e13_recheck(const int8_t *tab, int nq, int st0, int head0, int abs_ok, int rel_ok, int64_t steps,
                 uint32_t *read_mask);
 
+/* Deep stage check (replaces src/deep.c): each SURVIVOR line's table run from
+   the blank tape to stage nmax (<= 24) on a tape of 4096 + 2^nmax + 4096
+   cells, finding the stage starts (state st0, head at head0 or, for rel=1
+   abs=0, at head0 - 2 + 2^n, the block -1 .. 2^n - 2 exactly the ones at
+   cells >= -2) and printing the counter's ones left of cell -2 at each, as
+   deep.c does; it stops at a stage not reached within 64*4^n + 100000 more
+   steps, a halt, a never-read free cell ('?---') or off the tape.
+   e13_deep_check_line: one line as survivor `id`, the result line (with its
+   newline) into out; returns its length.  e13_deep_check: every line of text
+   with table= and st0=, numbered from 1, on `threads` host threads; the result
+   lines into out in input order; returns the number of survivors. */
+int e13_deep_check_line(const char *line, int id, int nmax, char *out, int outlen);
+int e13_deep_check(const char *text, int nmax, int threads, char *out, int64_t outlen);
+
 /* The SURVIVOR line of report() (e13search.c:184-200) and the extra HASH line;
    returns the length written (without the NUL). */
 int e13_format_survivor(const e13_problem *p, const e13_survivor *s, char *buf, int buflen);

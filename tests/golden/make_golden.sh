@@ -30,10 +30,16 @@ for f in 1_local_edits/log_f2.txt 1_local_edits/log_f3.txt 1_local_edits/log_f4.
          4_division_6/v3/log_v3.txt 4_division_6/v3/g_v3.log \
          validation/base_erdos14.txt validation/mutants.txt \
          validation/split_test.txt validation/split_test_free.txt \
-         4_division_6/v1/v1c0_est.txt; do
+         4_division_6/v1/v1c0_est.txt validation/deep_base14.txt validation/check_erdos14.txt \
+         4_division_6/v2/surv_v2.txt 4_division_6/v2/deep_v2.txt; do
   # fixtures keep a .txt suffix whatever the source suffix
   cp "$REF/results/$f" "$HERE/ref_results/$(echo "$f" | tr '/' '_' | sed 's/\.log$/.txt/')"
 done
+
+# deep stage check (src/deep.c): the first survivors of V1's 1RD search and
+# three that leave the tape at stage 12, with the reference's results
+sed -n '1,4p;104,106p' "$REF/results/4_division_6/v1/surv_1RD.txt" > "$HERE/ref_results/4_division_6_v1_surv_1RD_sample.txt"
+sed -n '1,4p;104,106p' "$REF/results/4_division_6/v1/deep15_1RD.txt" > "$HERE/ref_results/4_division_6_v1_deep15_1RD_sample.txt"
 
 M=$REF/machines/erdos_14.txt
 XS="0 1 2 3 4 5 6 7 8 9 10 11 12 13"

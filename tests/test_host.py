@@ -247,3 +247,48 @@ def test_lone_lane_loop_accelerations_match_plain_stepping(tmp_path):
                 assert bad == 0 and r.returncode == 0, r.stdout
             else:
                 assert bad > 0 and r.returncode == 1, (plant, r.stdout)
+
+
+def test_deep_check_of_erdos14_matches_the_reference(e13):
+    """src/deep.c's stage check of erdos_14 to stage 14
+    (results/validation/deep_base14.txt), on the survivor line of
+    results/validation/check_erdos14.txt."""
+    line = [l for l in read_text(REF_RESULTS, "validation_check_erdos14.txt").splitlines()
+            if l.startswith("SURVIVOR")][0]
+    want = read_text(REF_RESULTS, "validation_deep_base14.txt").splitlines()[0]
+    assert e13.deep_check_line(line, 1, 14).rstrip("\n") == want
+
+
+def test_deep_check_of_v1_survivors_matches_the_reference(e13):
+    """V1's 1RD survivors to stage 15 (results/4_division_6/v1/deep15_1RD.txt):
+    the first four, which reach stage 15, and numbers 104-106, which leave the
+    tape at stage 12; each numbered as in the reference's file."""
+    from concurrent.futures import ThreadPoolExecutor
+    lines = read_text(REF_RESULTS, "4_division_6_v1_surv_1RD_sample.txt").splitlines()
+    want = read_text(REF_RESULTS, "4_division_6_v1_deep15_1RD_sample.txt").splitlines()
+    ids = [1, 2, 3, 4, 104, 105, 106]
+    with ThreadPoolExecutor(len(ids)) as ex:
+        got = list(ex.map(lambda a: e13.deep_check_line(a[0], a[1], 15).rstrip("\n"), zip(lines, ids)))
+    assert got == want
+
+
+def test_deep_check_of_v2_survivors_matches_the_reference(e13):
+    """All 12 survivors of V2 to stage 15 on host threads, in input order
+    (results/4_division_6/v2/deep_v2.txt)."""
+    text = read_text(REF_RESULTS, "4_division_6_v2_surv_v2.txt")
+    want = read_text(REF_RESULTS, "4_division_6_v2_deep_v2.txt")
+    assert e13.deep_check(text, 15, threads=8) == want
+
+
+def test_deep_check_command_line():
+    """`e13search_gpu deep NMAX < survivors` (the reference's `deep NMAX <
+    survivors`): no GPU is needed; the lines are the library's."""
+    import subprocess
+    cli = os.path.join(ROOT, "erdos-machine-13_amd", "lib", "e13search_gpu")
+    text = read_text(REF_RESULTS, "validation_check_erdos14.txt")
+    r = subprocess.run([cli, "deep", "12"], input=text, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    want = read_text(REF_RESULTS, "validation_deep_base14.txt").splitlines()[0]
+    first = r.stdout.splitlines()[0]
+    assert want.startswith(first) and first.endswith("n12:3@-5")
+    assert len(r.stdout.splitlines()) == len([l for l in text.splitlines() if "table=" in l and "st0=" in l])
