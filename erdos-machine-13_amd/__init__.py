@@ -147,7 +147,7 @@ EXPORTS = ("e13_open", "e13_close", "e13_strerror", "e13_set_tuning", "e13_get_t
            "e13_cost_targets", "e13_mlp_init", "e13_mlp_forward", "e13_mlp_grad", "e13_mlp_train",
            "e13_set_cost_model", "e13_autotune", "e13_save_tuning", "e13_load_tuning", "e13_tuning_file",
            "e13_tuning_file_small", "e13_search_profile", "e13_estimate", "e13_set_checkpoint",
-           "e13_deep_check_line", "e13_deep_check")
+           "e13_deep_check_line", "e13_deep_check", "e13_recheck_counter_line", "e13_recheck_counter")
 
 _lib = None
 
@@ -216,6 +216,8 @@ def lib():
     L.e13_tuning_file_small.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
     L.e13_search_profile.argtypes = [ctypes.c_void_p, P(Problem), ctypes.c_int, P(Tuning)]
     L.e13_deep_check_line.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    L.e13_recheck_counter_line.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    L.e13_recheck_counter.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int64]
     L.e13_deep_check.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int64]
     L.e13_set_checkpoint.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_double, ctypes.c_int64]
     L.e13_estimate.argtypes = [ctypes.c_void_p, P(Problem), ctypes.c_uint64, ctypes.c_int64, P(ctypes.c_double),
@@ -345,6 +347,22 @@ def deep_check_line(line, survivor_id, nmax):
     """The deep stage check (src/deep.c) of one SURVIVOR line: its result line."""
     buf = ctypes.create_string_buffer(4096)
     _check(min(lib().e13_deep_check_line(line.encode(), survivor_id, nmax, buf, 4096), 0), "e13_deep_check_line")
+    return buf.value.decode()
+
+
+def recheck_counter_line(line, survivor_id, nmax):
+    """The counter re-check (src/recheck.c) of one SURVIVOR line: its verdict line."""
+    buf = ctypes.create_string_buffer(65536)
+    _check(min(lib().e13_recheck_counter_line(line.encode(), survivor_id, nmax, buf, 65536), 0),
+           "e13_recheck_counter_line")
+    return buf.value.decode()
+
+
+def recheck_counter(text, nmax, threads=8):
+    """The counter re-check of every line of text with table=, numbered from 1."""
+    cap = 20480 * (len(text) // 64 + 16)
+    buf = ctypes.create_string_buffer(cap)
+    _check(min(lib().e13_recheck_counter(text.encode(), nmax, threads, buf, cap), 0), "e13_recheck_counter")
     return buf.value.decode()
 
 

@@ -12,6 +12,7 @@
  *                                       seed $E13_SEED, default 0)
  *   e13search_gpu synth   SEED0 K0 N      (the synthetic stress batch; one summary line)
  *   e13search_gpu deep    NMAX < survivors.txt   (src/deep.c's stage check to NMAX, host threads)
+ *   e13search_gpu recheck NMAX < survivors.txt   (src/recheck.c's counter re-check, host threads)
  *   e13search_gpu tune    machine.txt FMAX X...   (autotune on that search; saves the settings
  *   e13search_gpu tune    machine.txt freesym X LO HI CELL...   to $E13_TUNING_OUT or to the
  *                                       device's tuning file of the profile the search runs with)
@@ -286,7 +287,9 @@ int main(int argc, char **argv) {
     fprintf(stderr, "usage: e13search_gpu base|mutant|delete|free|freesym machine.txt [...]\n");
     return 1;
   }
-  if (!strcmp(argv[1], "deep")) { /* deep NMAX < survivors.txt (src/deep.c): host threads, no GPU */
+  if (!strcmp(argv[1], "deep") || !strcmp(argv[1], "recheck")) {
+    /* deep NMAX < survivors.txt (src/deep.c), recheck NMAX < survivors.txt (src/recheck.c): host threads */
+    const int rech = !strcmp(argv[1], "recheck");
     const int nmax = atoi(argv[2]);
     size_t cap = 1 << 20, len = 0;
     char *text = (char *)malloc(cap);
@@ -302,14 +305,21 @@ int main(int argc, char **argv) {
     text[len] = 0;
     const char *th = getenv("OMP_NUM_THREADS");
     const int threads = th && atoi(th) > 0 ? atoi(th) : (int)sysconf(_SC_NPROCESSORS_ONLN);
-    const int64_t outlen = 2048 * (int64_t)(len / 64 + 16);
+    const int64_t outlen = (rech ? 20480 : 2048) * (int64_t)(len / 64 + 16);
     char *out = (char *)malloc((size_t)outlen);
-    const int rc = out ? e13_deep_check(text, nmax, threads, out, outlen) : E13_ENOMEM;
+    const int rc = !out ? E13_ENOMEM
+                        : rech ? e13_recheck_counter(text, nmax, threads, out, outlen)
+                               : e13_deep_check(text, nmax, threads, out, outlen);
     if (rc < 0) {
-      fprintf(stderr, "deep: %s\n", e13_strerror(rc));
+      fprintf(stderr, "%s: %s\n", argv[1], e13_strerror(rc));
       return 1;
     }
     fputs(out, stdout);
+    if (rech) { /* recheck.c:100 */
+      int good = 0;
+      for (const char *l = out; (l = strstr(l, ": CONSISTENT")) != NULL; l++) good++;
+      fprintf(stderr, "machines %d consistent %d failed %d\n", rc, good, rc - good);
+    }
     free(out);
     free(text);
     return 0;

@@ -245,6 +245,19 @@ int e13_recheck(const int8_t *tab, int nq, int st0, int head0, int abs_ok, int r
 int e13_deep_check_line(const char *line, int id, int nmax, char *out, int outlen);
 int e13_deep_check(const char *text, int nmax, int threads, char *out, int64_t outlen);
 
+/* Counter re-check (replaces src/recheck.c): the same stage starts to nmax
+   (<= 19) on a tape of 4e6 cells each side, with the exact set of counter ones
+   (cells <= -3) at each start; the verdict line "id: ..." is recheck.c's:
+   stage not reached (limit 8*4^n + 100000 more steps), off tape, free cell
+   read, halt, "counter changes at n=..." (the set differs from the first
+   stage of its class c(n): 0 for n <= 2, 1 for 3..8, 2 from 9), "counter
+   equal for different c", or "CONSISTENT to n=... (c2 counter ...)" (for nmax
+   < 9 the c2 counter is printed empty).  Returns the line's length. */
+int e13_recheck_counter_line(const char *line, int id, int nmax, char *out, int outlen);
+/* Every line of text with table=, numbered from 1, on `threads` host threads;
+   the verdict lines into out in input order; returns the number of lines. */
+int e13_recheck_counter(const char *text, int nmax, int threads, char *out, int64_t outlen);
+
 /* The SURVIVOR line of report() (e13search.c:184-200) and the extra HASH line;
    returns the length written (without the NUL). */
 int e13_format_survivor(const e13_problem *p, const e13_survivor *s, char *buf, int buflen);

@@ -40,6 +40,12 @@ done
 # three that leave the tape at stage 12, with the reference's results
 sed -n '1,4p;104,106p' "$REF/results/4_division_6/v1/surv_1RD.txt" > "$HERE/ref_results/4_division_6_v1_surv_1RD_sample.txt"
 sed -n '1,4p;104,106p' "$REF/results/4_division_6/v1/deep15_1RD.txt" > "$HERE/ref_results/4_division_6_v1_deep15_1RD_sample.txt"
+# the counter re-check (src/recheck.c 15, validation/recheck_1RD.txt, run in chunks of ~359 lines):
+# survivor lines 1, 2, 53, 58 (chunk 0) and 2840-2842 (chunk 7, ids 330-332)
+sed -n '1p;2p;53p;58p;2840p;2841p;2842p' "$REF/results/4_division_6/v1/surv_1RD.txt" \
+    > "$HERE/ref_results/4_division_6_v1_surv_1RD_recheck_sample.txt"
+sed -n '4p;5p;56p;61p;2850p;2851p;2852p' "$REF/results/validation/recheck_1RD.txt" \
+    > "$HERE/ref_results/validation_recheck_1RD_sample.txt"
 
 M=$REF/machines/erdos_14.txt
 XS="0 1 2 3 4 5 6 7 8 9 10 11 12 13"

@@ -292,3 +292,37 @@ def test_deep_check_command_line():
     first = r.stdout.splitlines()[0]
     assert want.startswith(first) and first.endswith("n12:3@-5")
     assert len(r.stdout.splitlines()) == len([l for l in text.splitlines() if "table=" in l and "st0=" in l])
+
+
+def test_counter_recheck_matches_the_reference(e13):
+    """src/recheck.c 15 on V1's 1RD survivors (results/validation/recheck_1RD.txt,
+    run there in chunks whose ids restart at 1): survivor lines 1 and 2 (the
+    counter changes at stage 15), 53 and 58 (at stage 13) and 2840-2842
+    (stage 12 not reached), with the verdict lines' own ids."""
+    from concurrent.futures import ThreadPoolExecutor
+    lines = read_text(REF_RESULTS, "4_division_6_v1_surv_1RD_recheck_sample.txt").splitlines()
+    want = read_text(REF_RESULTS, "validation_recheck_1RD_sample.txt").splitlines()
+    ids = [int(w.split(":")[0]) for w in want]
+    with ThreadPoolExecutor(len(ids)) as ex:
+        got = list(ex.map(lambda a: e13.recheck_counter_line(a[0], a[1], 15).rstrip("\n"), zip(lines, ids)))
+    assert got == want
+
+
+def test_counter_recheck_many_and_command_line(e13):
+    """The threaded form over a text of survivor lines (numbered from 1, in
+    input order) and `e13search_gpu recheck NMAX < survivors` with recheck.c's
+    summary line; stage 13 keeps it short (the verdicts of lines 53 and 58
+    and of 2840-2842 are reached by then)."""
+    import subprocess
+    text = read_text(REF_RESULTS, "4_division_6_v1_surv_1RD_recheck_sample.txt")
+    want = [w.split(": ", 1)[1] for w in read_text(REF_RESULTS, "validation_recheck_1RD_sample.txt").splitlines()]
+    got = e13.recheck_counter(text, 13, threads=8).splitlines()
+    assert [g.split(": ", 1)[0] for g in got] == [str(i) for i in range(1, 8)]
+    verdicts = [g.split(": ", 1)[1] for g in got]
+    assert verdicts[2:] == want[2:]
+    assert verdicts[0].startswith("CONSISTENT to n=13")
+    cli = os.path.join(ROOT, "erdos-machine-13_amd", "lib", "e13search_gpu")
+    r = subprocess.run([cli, "recheck", "13"], input=text, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.splitlines() == got
+    assert "machines 7 consistent 2 failed 5" in r.stderr
