@@ -147,7 +147,8 @@ EXPORTS = ("e13_open", "e13_close", "e13_strerror", "e13_set_tuning", "e13_get_t
            "e13_cost_targets", "e13_mlp_init", "e13_mlp_forward", "e13_mlp_grad", "e13_mlp_train",
            "e13_set_cost_model", "e13_autotune", "e13_save_tuning", "e13_load_tuning", "e13_tuning_file",
            "e13_tuning_file_small", "e13_search_profile", "e13_estimate", "e13_set_checkpoint",
-           "e13_deep_check_line", "e13_deep_check", "e13_recheck_counter_line", "e13_recheck_counter")
+           "e13_deep_check_line", "e13_deep_check", "e13_recheck_counter_line", "e13_recheck_counter",
+           "e13_check_survivor")
 
 _lib = None
 
@@ -218,6 +219,7 @@ def lib():
     L.e13_deep_check_line.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
     L.e13_recheck_counter_line.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
     L.e13_recheck_counter.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int64]
+    L.e13_check_survivor.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
     L.e13_deep_check.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int64]
     L.e13_set_checkpoint.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_double, ctypes.c_int64]
     L.e13_estimate.argtypes = [ctypes.c_void_p, P(Problem), ctypes.c_uint64, ctypes.c_int64, P(ctypes.c_double),
@@ -363,6 +365,13 @@ def recheck_counter(text, nmax, threads=8):
     cap = 20480 * (len(text) // 64 + 16)
     buf = ctypes.create_string_buffer(cap)
     _check(min(lib().e13_recheck_counter(text.encode(), nmax, threads, buf, cap), 0), "e13_recheck_counter")
+    return buf.value.decode()
+
+
+def check_survivor(line):
+    """The crafted-start check (src/check.py) of one SURVIVOR line: its lines."""
+    buf = ctypes.create_string_buffer(1 << 16)
+    _check(min(lib().e13_check_survivor(line.encode(), buf, 1 << 16), 0), "e13_check_survivor")
     return buf.value.decode()
 
 

@@ -13,6 +13,7 @@
  *   e13search_gpu synth   SEED0 K0 N      (the synthetic stress batch; one summary line)
  *   e13search_gpu deep    NMAX < survivors.txt   (src/deep.c's stage check to NMAX, host threads)
  *   e13search_gpu recheck NMAX < survivors.txt   (src/recheck.c's counter re-check, host threads)
+ *   e13search_gpu check   survivors.txt          (src/check.py's crafted-start check, host only)
  *   e13search_gpu tune    machine.txt FMAX X...   (autotune on that search; saves the settings
  *   e13search_gpu tune    machine.txt freesym X LO HI CELL...   to $E13_TUNING_OUT or to the
  *                                       device's tuning file of the profile the search runs with)
@@ -322,6 +323,28 @@ int main(int argc, char **argv) {
     }
     free(out);
     free(text);
+    return 0;
+  }
+  if (!strcmp(argv[1], "check")) { /* check survivors.txt (src/check.py): host only, no GPU */
+    FILE *f = fopen(argv[2], "r");
+    if (!f) {
+      fprintf(stderr, "cannot open %s\n", argv[2]);
+      return 1;
+    }
+    char line[8192];
+    char *out = (char *)malloc(1 << 16);
+    while (out && fgets(line, sizeof line, f)) {
+      if (!strstr(line, "table=")) continue;
+      const int rc = e13_check_survivor(line, out, 1 << 16);
+      if (rc < 0) {
+        fprintf(stderr, "check: %s\n", e13_strerror(rc));
+        return 1;
+      }
+      fputs(out, stdout);
+      fflush(stdout);
+    }
+    fclose(f);
+    free(out);
     return 0;
   }
   if (!strcmp(argv[1], "synth")) { /* synth SEED0 K0 N: the synthetic stress batch */

@@ -258,6 +258,16 @@ int e13_recheck_counter_line(const char *line, int id, int nmax, char *out, int 
    the verdict lines into out in input order; returns the number of lines. */
 int e13_recheck_counter(const char *text, int nmax, int threads, char *out, int64_t outlen);
 
+/* Independent check with crafted stage starts (replaces src/check.py): the
+   stage starts to 11 from the blank tape with the counter depending only on
+   c(n) = #{1 <= k < n : 2^k has no digit 2 in base 3}, then from crafted
+   starts (the counter of c, a block of m = 2..40 ones) a halt iff c = 2 and m
+   has no digit 2, else the start with block 2m and the counter of c or c + 1;
+   with no halt cell the halt goes into each never-read cell in turn.  Writes
+   check.py's lines for one survivor (the line itself, then one "   halt@...:
+   PASS|FAIL ..." line per placement); returns their length. */
+int e13_check_survivor(const char *line, char *out, int outlen);
+
 /* The SURVIVOR line of report() (e13search.c:184-200) and the extra HASH line;
    returns the length written (without the NUL). */
 int e13_format_survivor(const e13_problem *p, const e13_survivor *s, char *buf, int buflen);

@@ -31,7 +31,7 @@ for f in 1_local_edits/log_f2.txt 1_local_edits/log_f3.txt 1_local_edits/log_f4.
          validation/base_erdos14.txt validation/mutants.txt \
          validation/split_test.txt validation/split_test_free.txt \
          4_division_6/v1/v1c0_est.txt validation/deep_base14.txt validation/check_erdos14.txt \
-         4_division_6/v2/surv_v2.txt 4_division_6/v2/deep_v2.txt; do
+         4_division_6/v2/surv_v2.txt 4_division_6/v2/deep_v2.txt 4_division_6/v2/check_v2.txt; do
   # fixtures keep a .txt suffix whatever the source suffix
   cp "$REF/results/$f" "$HERE/ref_results/$(echo "$f" | tr '/' '_' | sed 's/\.log$/.txt/')"
 done

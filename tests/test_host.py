@@ -326,3 +326,22 @@ def test_counter_recheck_many_and_command_line(e13):
     assert r.returncode == 0, r.stderr
     assert r.stdout.splitlines() == got
     assert "machines 7 consistent 2 failed 5" in r.stderr
+
+
+def test_crafted_start_check_matches_the_reference(e13):
+    """src/check.py's lines on erdos_14 (results/validation/check_erdos14.txt:
+    PASS) and on all 12 survivors of V2 (results/4_division_6/v2/check_v2.txt:
+    FAIL with 107 crafted starts that miss, the first four listed), through the
+    library and through `e13search_gpu check`."""
+    import subprocess
+    ref14 = read_text(REF_RESULTS, "validation_check_erdos14.txt").splitlines()
+    assert e13.check_survivor(ref14[1]).splitlines() == ref14[2:4]
+    want = read_text(REF_RESULTS, "4_division_6_v2_check_v2.txt").splitlines()
+    got = []
+    for line in read_text(REF_RESULTS, "4_division_6_v2_surv_v2.txt").splitlines():
+        got += e13.check_survivor(line).splitlines()
+    assert got == want
+    cli = os.path.join(ROOT, "erdos-machine-13_amd", "lib", "e13search_gpu")
+    path = os.path.join(REF_RESULTS, "4_division_6_v2_surv_v2.txt")
+    r = subprocess.run([cli, "check", path], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and r.stdout.splitlines() == want
