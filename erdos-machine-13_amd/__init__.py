@@ -148,7 +148,7 @@ EXPORTS = ("e13_open", "e13_close", "e13_strerror", "e13_set_tuning", "e13_get_t
            "e13_set_cost_model", "e13_autotune", "e13_save_tuning", "e13_load_tuning", "e13_tuning_file",
            "e13_tuning_file_small", "e13_search_profile", "e13_estimate", "e13_set_checkpoint",
            "e13_deep_check_line", "e13_deep_check", "e13_recheck_counter_line", "e13_recheck_counter",
-           "e13_check_survivor")
+           "e13_check_survivor", "e13_classify_deep")
 
 _lib = None
 
@@ -216,11 +216,14 @@ def lib():
     L.e13_tuning_file.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
     L.e13_tuning_file_small.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
     L.e13_search_profile.argtypes = [ctypes.c_void_p, P(Problem), ctypes.c_int, P(Tuning)]
-    L.e13_deep_check_line.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    L.e13_deep_check_line.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
+                                      ctypes.c_int]
     L.e13_recheck_counter_line.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
     L.e13_recheck_counter.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int64]
+    L.e13_classify_deep.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int64]
     L.e13_check_survivor.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
-    L.e13_deep_check.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int64]
+    L.e13_deep_check.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
+                                 ctypes.c_int64]
     L.e13_set_checkpoint.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_double, ctypes.c_int64]
     L.e13_estimate.argtypes = [ctypes.c_void_p, P(Problem), ctypes.c_uint64, ctypes.c_int64, P(ctypes.c_double),
                                P(EstimateSum)]
@@ -345,10 +348,11 @@ def load_tuning(path):
     return t.as_dict()
 
 
-def deep_check_line(line, survivor_id, nmax):
-    """The deep stage check (src/deep.c) of one SURVIVOR line: its result line."""
+def deep_check_line(line, survivor_id, nmax, big=False):
+    """The deep stage check (src/deep.c; big: src/deep_big.c) of one SURVIVOR line: its result line."""
     buf = ctypes.create_string_buffer(4096)
-    _check(min(lib().e13_deep_check_line(line.encode(), survivor_id, nmax, buf, 4096), 0), "e13_deep_check_line")
+    _check(min(lib().e13_deep_check_line(line.encode(), survivor_id, nmax, int(big), buf, 4096), 0),
+           "e13_deep_check_line")
     return buf.value.decode()
 
 
@@ -368,6 +372,14 @@ def recheck_counter(text, nmax, threads=8):
     return buf.value.decode()
 
 
+def classify_deep(survivors, deep):
+    """classify_deep.py's lines for survivor lines and their deep-check lines."""
+    cap = 1 << 20
+    buf = ctypes.create_string_buffer(cap)
+    _check(min(lib().e13_classify_deep(survivors.encode(), deep.encode(), buf, cap), 0), "e13_classify_deep")
+    return buf.value.decode()
+
+
 def check_survivor(line):
     """The crafted-start check (src/check.py) of one SURVIVOR line: its lines."""
     buf = ctypes.create_string_buffer(1 << 16)
@@ -375,11 +387,11 @@ def check_survivor(line):
     return buf.value.decode()
 
 
-def deep_check(text, nmax, threads=8):
+def deep_check(text, nmax, threads=8, big=False):
     """The deep stage check of every SURVIVOR line of text, numbered from 1."""
     cap = 2048 * (len(text) // 64 + 16)
     buf = ctypes.create_string_buffer(cap)
-    _check(min(lib().e13_deep_check(text.encode(), nmax, threads, buf, cap), 0), "e13_deep_check")
+    _check(min(lib().e13_deep_check(text.encode(), nmax, int(big), threads, buf, cap), 0), "e13_deep_check")
     return buf.value.decode()
 
 

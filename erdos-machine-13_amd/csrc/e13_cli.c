@@ -12,8 +12,10 @@
  *                                       seed $E13_SEED, default 0)
  *   e13search_gpu synth   SEED0 K0 N      (the synthetic stress batch; one summary line)
  *   e13search_gpu deep    NMAX < survivors.txt   (src/deep.c's stage check to NMAX, host threads)
+ *   e13search_gpu deep_big NMAX < survivors.txt  (src/deep_big.c: 4M cells each side)
  *   e13search_gpu recheck NMAX < survivors.txt   (src/recheck.c's counter re-check, host threads)
  *   e13search_gpu check   survivors.txt          (src/check.py's crafted-start check, host only)
+ *   e13search_gpu classify survivors.txt deep.txt...   (src/classify_deep.py, host only)
  *   e13search_gpu tune    machine.txt FMAX X...   (autotune on that search; saves the settings
  *   e13search_gpu tune    machine.txt freesym X LO HI CELL...   to $E13_TUNING_OUT or to the
  *                                       device's tuning file of the profile the search runs with)
@@ -288,9 +290,10 @@ int main(int argc, char **argv) {
     fprintf(stderr, "usage: e13search_gpu base|mutant|delete|free|freesym machine.txt [...]\n");
     return 1;
   }
-  if (!strcmp(argv[1], "deep") || !strcmp(argv[1], "recheck")) {
-    /* deep NMAX < survivors.txt (src/deep.c), recheck NMAX < survivors.txt (src/recheck.c): host threads */
-    const int rech = !strcmp(argv[1], "recheck");
+  if (!strcmp(argv[1], "deep") || !strcmp(argv[1], "deep_big") || !strcmp(argv[1], "recheck")) {
+    /* deep NMAX < survivors.txt (src/deep.c; deep_big: src/deep_big.c), recheck NMAX < survivors.txt
+       (src/recheck.c): host threads */
+    const int rech = !strcmp(argv[1], "recheck"), big = !strcmp(argv[1], "deep_big");
     const int nmax = atoi(argv[2]);
     size_t cap = 1 << 20, len = 0;
     char *text = (char *)malloc(cap);
@@ -310,7 +313,7 @@ int main(int argc, char **argv) {
     char *out = (char *)malloc((size_t)outlen);
     const int rc = !out ? E13_ENOMEM
                         : rech ? e13_recheck_counter(text, nmax, threads, out, outlen)
-                               : e13_deep_check(text, nmax, threads, out, outlen);
+                               : e13_deep_check(text, nmax, big, threads, out, outlen);
     if (rc < 0) {
       fprintf(stderr, "%s: %s\n", argv[1], e13_strerror(rc));
       return 1;
@@ -323,6 +326,39 @@ int main(int argc, char **argv) {
     }
     free(out);
     free(text);
+    return 0;
+  }
+  if (!strcmp(argv[1], "classify") && argc >= 4) { /* classify SURVIVORS DEEPOUT... (src/classify_deep.py) */
+    char *txt[64] = {0};
+    size_t len[64] = {0};
+    for (int i = 2; i < argc && i < 64; i++) { /* the files, read whole */
+      FILE *f = fopen(argv[i], "rb");
+      if (!f) {
+        fprintf(stderr, "cannot open %s\n", argv[i]);
+        return 1;
+      }
+      fseek(f, 0, SEEK_END);
+      len[i] = (size_t)ftell(f);
+      fseek(f, 0, SEEK_SET);
+      txt[i] = (char *)calloc(len[i] + 2, 1);
+      if (!txt[i] || fread(txt[i], 1, len[i], f) != len[i]) return 1;
+      if (len[i] && txt[i][len[i] - 1] != '\n') txt[i][len[i]++] = '\n';
+      fclose(f);
+    }
+    size_t dl = 0;
+    for (int i = 3; i < argc && i < 64; i++) dl += len[i];
+    char *deep = (char *)calloc(dl + 1, 1), *out = (char *)malloc(1 << 20);
+    if (!deep || !out) return 1;
+    for (int i = 3, o = 0; i < argc && i < 64; i++) {
+      memcpy(deep + o, txt[i], len[i]);
+      o += (int)len[i];
+    }
+    const int rc = e13_classify_deep(txt[2], deep, out, 1 << 20);
+    if (rc < 0) {
+      fprintf(stderr, "classify: %s\n", e13_strerror(rc));
+      return 1;
+    }
+    fputs(out, stdout);
     return 0;
   }
   if (!strcmp(argv[1], "check")) { /* check survivors.txt (src/check.py): host only, no GPU */

@@ -18,7 +18,14 @@
 
 #include "e13.h"
 
-#define DEEP_LEFT 4096 /* cells left of cell 0 (deep.c:13) */
+#define DEEP_LEFT 4096         /* cells left of cell 0 (deep.c:13) */
+#define DEEP_BIG_LEFT (1 << 22) /* deep_big.c:14: cells each side beyond the block */
+
+/* The tape of one run: deep.c's, or deep_big.c's 4M cells each side. */
+static long long deep_left(int big) { return big ? DEEP_BIG_LEFT : DEEP_LEFT; }
+static long long deep_size(int nmax, int big) {
+  return big ? DEEP_BIG_LEFT + (1LL << nmax) + DEEP_BIG_LEFT + 4096 : DEEP_LEFT + (1LL << nmax) + 4096;
+}
 
 typedef struct {
   int st0, rel, nq;
@@ -97,11 +104,12 @@ static int appendf(char *out, int len, int *pos, const char *fmt, ...) {
   return 0;
 }
 
-/* One survivor (number id) to stage nmax on tape[0 .. DEEP_LEFT + 2^nmax + 4096). */
-static int deep_run(const deep_machine *m, int id, int nmax, uint8_t *tape, char *out, int len) {
-  const long long size = DEEP_LEFT + (1LL << nmax) + 4096;
+/* One survivor (number id) to stage nmax on tape[0 .. deep_size).  big:
+   deep_big.c's tape and its report of head, state and ones at the limit. */
+static int deep_run(const deep_machine *m, int id, int nmax, int big, uint8_t *tape, char *out, int len) {
+  const long long size = deep_size(nmax, big), left = deep_left(big);
   memset(tape, 0, (size_t)size);
-  uint8_t *t0 = tape + DEEP_LEFT; /* cell k at t0[k] */
+  uint8_t *t0 = tape + left; /* cell k at t0[k] */
   long long head = 0, t = 0, ones = 0; /* ones at cells >= -2 */
   int st = 0, pos = 0, failed = 0;
   if (snprintf(out, (size_t)len, "survivor %d (nq=%d st0=%d%s):", id, m->nq, m->st0, m->rel ? " rel" : "") >= len)
@@ -118,10 +126,22 @@ static int deep_run(const deep_machine *m, int id, int nmax, uint8_t *tape, char
       }
       if (t >= limit) {
         if (appendf(out, len, &pos, " | stage %d NOT reached (limit, t=%lld)", n, t)) return -1;
+        if (big) { /* deep_big.c:34: where the machine is and the extent of its ones */
+          long long lo1 = 0, hi1 = 0, cnt = 0;
+          for (long long k = -left + 1; k < size - left; k++)
+            if (t0[k]) {
+              if (!cnt) lo1 = k;
+              hi1 = k;
+              cnt++;
+            }
+          if (appendf(out, len, &pos, " head=%lld state=%c ones=%lld in [%lld,%lld]", head, (char)(65 + st), cnt, lo1,
+                      hi1))
+            return -1;
+        }
         failed = 1;
         break;
       }
-      if (head < -DEEP_LEFT + 2 || head > blk + 2000) {
+      if (head < -left + 2 || head > blk + (big ? DEEP_BIG_LEFT : 2000)) {
         if (appendf(out, len, &pos, " | stage %d: off tape at %lld", n, head)) return -1;
         failed = 1;
         break;
@@ -148,7 +168,7 @@ static int deep_run(const deep_machine *m, int id, int nmax, uint8_t *tape, char
     }
     if (failed) break;
     long long cnt = 0, lo = 0; /* the counter: ones left of cell -2 */
-    for (long long k = -3; k >= -DEEP_LEFT + 1; k--)
+    for (long long k = -3; k >= -left + 1; k--)
       if (t0[k]) {
         cnt++;
         lo = k;
@@ -160,13 +180,13 @@ static int deep_run(const deep_machine *m, int id, int nmax, uint8_t *tape, char
   return pos;
 }
 
-int e13_deep_check_line(const char *line, int id, int nmax, char *out, int outlen) {
+int e13_deep_check_line(const char *line, int id, int nmax, int big, char *out, int outlen) {
   if (!line || !out || outlen < 2 || nmax < 1 || nmax > 24) return E13_EINVAL;
   deep_machine m;
   if (!deep_parse(line, &m)) return E13_EINVAL;
-  uint8_t *tape = (uint8_t *)malloc((size_t)(DEEP_LEFT + (1LL << nmax) + 4096));
+  uint8_t *tape = (uint8_t *)malloc((size_t)deep_size(nmax, big));
   if (!tape) return E13_ENOMEM;
-  const int r = deep_run(&m, id, nmax, tape, out, outlen);
+  const int r = deep_run(&m, id, nmax, big, tape, out, outlen);
   free(tape);
   return r < 0 ? E13_EINVAL : r;
 }
@@ -176,7 +196,7 @@ int e13_deep_check_line(const char *line, int id, int nmax, char *out, int outle
    in the same order. */
 typedef struct {
   const char **lines;
-  int n, nmax, recheck; /* recheck: the counter re-check instead of the deep check */
+  int n, nmax, recheck, big; /* recheck: the counter re-check instead of the deep check; big: deep_big.c */
   char *res;            /* n x DEEP_OUT */
   int next, err;
   pthread_mutex_t mu;
@@ -189,8 +209,7 @@ static int deep_parse_recheck(const char *line, deep_machine *m);
 
 static void *deep_worker(void *arg) {
   deep_job *j = (deep_job *)arg;
-  uint8_t *tape = (uint8_t *)malloc(j->recheck ? (size_t)(2 * 4000000LL + 1) :
-                                                 (size_t)(DEEP_LEFT + (1LL << j->nmax) + 4096));
+  uint8_t *tape = (uint8_t *)malloc(j->recheck ? (size_t)(2 * 4000000LL + 1) : (size_t)deep_size(j->nmax, j->big));
   char (*ctr)[8192] = j->recheck ? (char (*)[8192])calloc(20, 8192) : NULL;
   for (;;) {
     pthread_mutex_lock(&j->mu);
@@ -202,7 +221,7 @@ static void *deep_worker(void *arg) {
     const int ok = tape && (j->recheck ? ctr && deep_parse_recheck(j->lines[i], &m) &&
                                              rc_run(&m, i + 1, j->nmax, tape, ctr, o, DEEP_OUT) >= 0
                                        : deep_parse(j->lines[i], &m) &&
-                                             deep_run(&m, i + 1, j->nmax, tape, o, DEEP_OUT) >= 0);
+                                             deep_run(&m, i + 1, j->nmax, j->big, tape, o, DEEP_OUT) >= 0);
     if (!ok) {
       pthread_mutex_lock(&j->mu);
       j->err = 1;
@@ -214,7 +233,7 @@ static void *deep_worker(void *arg) {
   return NULL;
 }
 
-static int deep_many(const char *text, int nmax, int threads, char *out, int64_t outlen, int recheck) {
+static int deep_many(const char *text, int nmax, int threads, char *out, int64_t outlen, int recheck, int big) {
   /* the survivor lines, in order */
   char *copy = strdup(text);
   if (!copy) return E13_ENOMEM;
@@ -240,6 +259,7 @@ static int deep_many(const char *text, int nmax, int threads, char *out, int64_t
   j.n = n;
   j.nmax = nmax;
   j.recheck = recheck;
+  j.big = big;
   j.res = n ? (char *)calloc((size_t)n, DEEP_OUT) : NULL;
   int rc = lines && (!n || j.res) ? E13_OK : E13_ENOMEM;
   if (!rc && n) {
@@ -374,12 +394,186 @@ int e13_recheck_counter_line(const char *line, int id, int nmax, char *out, int 
   return r;
 }
 
-int e13_deep_check(const char *text, int nmax, int threads, char *out, int64_t outlen) {
+int e13_deep_check(const char *text, int nmax, int big, int threads, char *out, int64_t outlen) {
   if (!text || !out || outlen < 1 || nmax < 1 || nmax > 24) return E13_EINVAL;
-  return deep_many(text, nmax, threads, out, outlen, 0);
+  return deep_many(text, nmax, threads, out, outlen, 0, big);
 }
 
 int e13_recheck_counter(const char *text, int nmax, int threads, char *out, int64_t outlen) {
   if (!text || !out || outlen < 1 || nmax < 1 || nmax > 19) return E13_EINVAL;
-  return deep_many(text, nmax, threads, out, outlen, 1);
+  return deep_many(text, nmax, threads, out, outlen, 1, 0);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Classification of deep-check lines (the reference's src/classify_deep.py):*/
+/* per survivor, the first counter change within a class c(n), counters     */
+/* equal across classes, the failure the line ends with (times and places   */
+/* removed), or a candidate; the kinds counted apart for absolute and       */
+/* relative starts, most frequent first, then an example survivor line for  */
+/* every candidate or free-cell kind.                                        */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  char kind[512];
+  int rel, count;
+  const char *example; /* the first survivor line of this kind */
+} cl_kind;
+
+static int cl_c(int n) { return n <= 2 ? 0 : (n <= 8 ? 1 : 2); }
+
+/* The failure text with "(t=N)", "(limit, t=N)" and "at N" removed, as the
+   reference's regular expression does, scanning left to right. */
+static void cl_strip(const char *a, size_t n, char *o, size_t cap) {
+  size_t j = 0;
+  for (size_t i = 0; i < n && j + 1 < cap;) {
+    size_t k = 0;
+    if (!strncmp(a + i, "(t=", 3) || !strncmp(a + i, "(limit, t=", 10)) {
+      size_t p = i + (a[i + 1] == 't' ? 3 : 10), q = p;
+      while (q < n && a[q] >= '0' && a[q] <= '9') q++;
+      if (q > p && q < n && a[q] == ')') k = q + 1 - i;
+    } else if (!strncmp(a + i, "at ", 3)) {
+      size_t p = i + 3;
+      if (p < n && a[p] == '-') p++;
+      size_t q = p;
+      while (q < n && a[q] >= '0' && a[q] <= '9') q++;
+      if (q > p) k = q - i;
+    }
+    if (k) i += k;
+    else o[j++] = a[i++];
+  }
+  o[j] = 0;
+}
+
+static char *cl_line(const char **p) { /* the next line of a text, or NULL */
+  if (!**p) return NULL;
+  const char *e = strchr(*p, '\n');
+  const size_t n = e ? (size_t)(e - *p) : strlen(*p);
+  char *l = (char *)malloc(n + 1);
+  if (l) {
+    memcpy(l, *p, n);
+    l[n] = 0;
+  }
+  *p += n + (e ? 1 : 0);
+  return l;
+}
+
+int e13_classify_deep(const char *survivors, const char *deep, char *out, int64_t outlen) {
+  if (!survivors || !deep || !out || outlen < 1) return E13_EINVAL;
+  int cap = 4096, ns = 0, nd = 0, nk = 0, rc = E13_OK;
+  char **sv = (char **)malloc(sizeof(char *) * (size_t)cap), **dv = (char **)malloc(sizeof(char *) * (size_t)cap);
+  cl_kind *kinds = (cl_kind *)calloc(256, sizeof(cl_kind));
+  if (!sv || !dv || !kinds) rc = E13_ENOMEM;
+  for (const char *p = survivors; !rc;) { /* lines with table= */
+    char *l = cl_line(&p);
+    if (!l) break;
+    if (!strstr(l, "table=")) {
+      free(l);
+      continue;
+    }
+    if (ns == cap) rc = E13_EINVAL;
+    else sv[ns++] = l;
+  }
+  for (const char *p = deep; !rc;) { /* lines that start with "survivor" */
+    char *l = cl_line(&p);
+    if (!l) break;
+    if (strncmp(l, "survivor", 8)) {
+      free(l);
+      continue;
+    }
+    if (nd == cap) rc = E13_EINVAL;
+    else dv[nd++] = l;
+  }
+  if (!rc && ns != nd) rc = E13_EINVAL;
+  for (int i = 0; !rc && i < nd; i++) {
+    const char *d = dv[i], *body = strstr(d, "):");
+    if (!body) {
+      rc = E13_EINVAL;
+      break;
+    }
+    body += 2;
+    const char *bar = strchr(body, '|');
+    const size_t slen = bar ? (size_t)(bar - body) : strlen(body);
+    /* n -> its "cnt@lo" text, from " nK:VALUE" items before the '|' */
+    char val[64][64];
+    int have[64] = {0}, nmax = 0;
+    for (const char *q = body; q < body + slen; q++) {
+      if (*q != 'n' || q[1] < '0' || q[1] > '9') continue;
+      char *e;
+      const long n = strtol(q + 1, &e, 10);
+      if (*e != ':' || n < 1 || n > 63) continue;
+      const char *v = e + 1;
+      size_t k = 0;
+      while (v + k < body + slen && v[k] != ' ' && v[k] != '\t' && k < 63) k++;
+      if (!k) continue;
+      memcpy(val[n], v, k);
+      val[n][k] = 0;
+      have[n] = 1;
+      if (n > nmax) nmax = (int)n;
+      q = v + k - 1;
+    }
+    char kind[512];
+    kind[0] = 0;
+    int first[3] = {0, 0, 0};
+    for (int n = 1; n <= nmax && !kind[0]; n++) {
+      if (!have[n]) continue;
+      const int c = cl_c(n);
+      if (first[c] && strcmp(val[first[c]], val[n]))
+        snprintf(kind, sizeof kind, "counter changes at n=%d (%s -> %s)", n, val[first[c]], val[n]);
+      else if (!first[c])
+        first[c] = n;
+    }
+    if (!kind[0]) {
+      for (int a = 0; a < 3; a++)
+        for (int b = a + 1; b < 3; b++)
+          if (first[a] && first[b] && !strcmp(val[first[a]], val[first[b]]))
+            snprintf(kind, sizeof kind, "counter equal for different c");
+    }
+    if (!kind[0] && bar) { /* the failure the line ends with */
+      const char *a = bar + 1, *b = d + strlen(d);
+      while (a < b && (*a == ' ' || *a == '\t')) a++;
+      while (b > a && (b[-1] == ' ' || b[-1] == '\t' || b[-1] == '\r')) b--;
+      if (b > a) cl_strip(a, (size_t)(b - a), kind, sizeof kind);
+    }
+    if (!kind[0]) snprintf(kind, sizeof kind, "CANDIDATE (consistent to n=%d)", nmax);
+    const int rel = strstr(d, " rel)") != NULL;
+    int j = 0;
+    while (j < nk && (kinds[j].rel != rel || strcmp(kinds[j].kind, kind))) j++;
+    if (j == nk) {
+      if (nk == 256) {
+        rc = E13_EINVAL;
+        break;
+      }
+      snprintf(kinds[nk].kind, sizeof kinds[nk].kind, "%s", kind);
+      kinds[nk].rel = rel;
+      kinds[nk].example = sv[i];
+      nk++;
+    }
+    kinds[j].count++;
+  }
+  int64_t pos = 0;
+  int done[256] = {0};
+  for (int r = 0; !rc && r < nk; r++) { /* most frequent first; ties in order of first appearance */
+    int best = -1;
+    for (int j = 0; j < nk; j++)
+      if (!done[j] && (best < 0 || kinds[j].count > kinds[best].count)) best = j;
+    done[best] = 1;
+    const int k = snprintf(out + pos, (size_t)(outlen - pos), "%6d  %s  %s\n", kinds[best].count,
+                           kinds[best].rel ? "rel" : "abs", kinds[best].kind);
+    if (k < 0 || pos + k >= outlen) rc = E13_EINVAL;
+    else pos += k;
+  }
+  for (int j = 0; !rc && j < nk; j++) {
+    if (strncmp(kinds[j].kind, "CANDIDATE", 9) && !strstr(kinds[j].kind, "FREE")) continue;
+    const char q = strchr(kinds[j].kind, '\'') ? '"' : '\''; /* Python's repr of the (kind, rel) pair */
+    const int k = snprintf(out + pos, (size_t)(outlen - pos), "example (%c%s%c, '%s') %s\n", q, kinds[j].kind, q,
+                           kinds[j].rel ? "rel" : "abs", kinds[j].example);
+    if (k < 0 || pos + k >= outlen) rc = E13_EINVAL;
+    else pos += k;
+  }
+  if (!rc) out[pos] = 0;
+  for (int i = 0; i < ns; i++) free(sv[i]);
+  for (int i = 0; i < nd; i++) free(dv[i]);
+  free(sv);
+  free(dv);
+  free(kinds);
+  return rc ? rc : nd;
 }

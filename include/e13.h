@@ -238,12 +238,23 @@ int e13_recheck(const int8_t *tab, int nq, int st0, int head0, int abs_ok, int r
    cells >= -2) and printing the counter's ones left of cell -2 at each, as
    deep.c does; it stops at a stage not reached within 64*4^n + 100000 more
    steps, a halt, a never-read free cell ('?---') or off the tape.
+   big = 1: src/deep_big.c instead, 4M cells each side, and at the step limit
+   the head, state and extent of the ones are reported as it does.
    e13_deep_check_line: one line as survivor `id`, the result line (with its
    newline) into out; returns its length.  e13_deep_check: every line of text
    with table= and st0=, numbered from 1, on `threads` host threads; the result
    lines into out in input order; returns the number of survivors. */
-int e13_deep_check_line(const char *line, int id, int nmax, char *out, int outlen);
-int e13_deep_check(const char *text, int nmax, int threads, char *out, int64_t outlen);
+int e13_deep_check_line(const char *line, int id, int nmax, int big, char *out, int outlen);
+int e13_deep_check(const char *text, int nmax, int big, int threads, char *out, int64_t outlen);
+
+/* Classification of deep-check output (replaces src/classify_deep.py): the
+   survivors text (lines with table=) and the deep output (lines starting with
+   "survivor", in the same order) to classify_deep.py's lines -- per kind (the
+   first counter change within a class c(n), counters equal across classes,
+   the failure with its times and places removed, or CANDIDATE) and start (abs
+   or rel), counts most frequent first, then an example line for candidate
+   and free-cell kinds.  Returns the number of survivors. */
+int e13_classify_deep(const char *survivors, const char *deep, char *out, int64_t outlen);
 
 /* Counter re-check (replaces src/recheck.c): the same stage starts to nmax
    (<= 19) on a tape of 4e6 cells each side, with the exact set of counter ones

@@ -31,7 +31,8 @@ for f in 1_local_edits/log_f2.txt 1_local_edits/log_f3.txt 1_local_edits/log_f4.
          validation/base_erdos14.txt validation/mutants.txt \
          validation/split_test.txt validation/split_test_free.txt \
          4_division_6/v1/v1c0_est.txt validation/deep_base14.txt validation/check_erdos14.txt \
-         4_division_6/v2/surv_v2.txt 4_division_6/v2/deep_v2.txt 4_division_6/v2/check_v2.txt; do
+         4_division_6/v2/surv_v2.txt 4_division_6/v2/deep_v2.txt 4_division_6/v2/check_v2.txt \
+         4_division_6/v1/classify_1RD.txt 4_division_6/v1/offtape3.txt 4_division_6/v1/runaways_deep_big.txt; do
   # fixtures keep a .txt suffix whatever the source suffix
   cp "$REF/results/$f" "$HERE/ref_results/$(echo "$f" | tr '/' '_' | sed 's/\.log$/.txt/')"
 done
@@ -40,6 +41,9 @@ done
 # three that leave the tape at stage 12, with the reference's results
 sed -n '1,4p;104,106p' "$REF/results/4_division_6/v1/surv_1RD.txt" > "$HERE/ref_results/4_division_6_v1_surv_1RD_sample.txt"
 sed -n '1,4p;104,106p' "$REF/results/4_division_6/v1/deep15_1RD.txt" > "$HERE/ref_results/4_division_6_v1_deep15_1RD_sample.txt"
+# all of V1's 1RD survivors and their deep check, compressed (the classification's input)
+gzip -c -n "$REF/results/4_division_6/v1/surv_1RD.txt" > "$HERE/ref_results/4_division_6_v1_surv_1RD.txt.gz"
+gzip -c -n "$REF/results/4_division_6/v1/deep15_1RD.txt" > "$HERE/ref_results/4_division_6_v1_deep15_1RD.txt.gz"
 # the counter re-check (src/recheck.c 15, validation/recheck_1RD.txt, run in chunks of ~359 lines):
 # survivor lines 1, 2, 53, 58 (chunk 0) and 2840-2842 (chunk 7, ids 330-332)
 sed -n '1p;2p;53p;58p;2840p;2841p;2842p' "$REF/results/4_division_6/v1/surv_1RD.txt" \

@@ -345,3 +345,35 @@ def test_crafted_start_check_matches_the_reference(e13):
     path = os.path.join(REF_RESULTS, "4_division_6_v2_surv_v2.txt")
     r = subprocess.run([cli, "check", path], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0 and r.stdout.splitlines() == want
+
+
+def test_classify_deep_matches_the_reference(e13):
+    """src/classify_deep.py over all 2868 survivors of V1's 1RD search and
+    their deep check (results/4_division_6/v1/classify_1RD.txt), through the
+    library and through `e13search_gpu classify`."""
+    import gzip
+    import subprocess
+    import tempfile
+    surv = gzip.open(os.path.join(REF_RESULTS, "4_division_6_v1_surv_1RD.txt.gz"), "rt").read()
+    deep = gzip.open(os.path.join(REF_RESULTS, "4_division_6_v1_deep15_1RD.txt.gz"), "rt").read()
+    want = read_text(REF_RESULTS, "4_division_6_v1_classify_1RD.txt")
+    assert e13.classify_deep(surv, deep) == want
+    with tempfile.TemporaryDirectory() as d:
+        ps, pd = os.path.join(d, "s.txt"), os.path.join(d, "d.txt")
+        with open(ps, "w") as f:
+            f.write(surv)
+        with open(pd, "w") as f:
+            f.write(deep)
+        cli = os.path.join(ROOT, "erdos-machine-13_amd", "lib", "e13search_gpu")
+        r = subprocess.run([cli, "classify", ps, pd], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and r.stdout == want
+
+
+def test_deep_big_runaways_match_the_reference(e13):
+    """src/deep_big.c 16 on the three survivors that leave deep.c's tape
+    (results/4_division_6/v1/runaways_deep_big.txt): 4M cells each side, each
+    stopped at stage 12's step limit with its head, state and ones reported."""
+    text = read_text(REF_RESULTS, "4_division_6_v1_offtape3.txt")
+    want = [l for l in read_text(REF_RESULTS, "4_division_6_v1_runaways_deep_big.txt").splitlines()
+            if l.startswith("survivor")]
+    assert e13.deep_check(text, 16, threads=3, big=True).splitlines() == want
