@@ -287,7 +287,9 @@ static int run_many(e13_ctx *ctx, const int32_t *base, int nq_base, int nx, char
 
 int main(int argc, char **argv) {
   if (argc < 3) {
-    fprintf(stderr, "usage: e13search_gpu base|mutant|delete|free|freesym machine.txt [...]\n");
+    fprintf(stderr, "usage: e13search_gpu base|mutant|delete|free|freesym|gpu|estimate|tune machine.txt [...]\n"
+                    "       e13search_gpu synth SEED0 K0 N | deep|deep_big|recheck NMAX < survivors.txt |\n"
+                    "       e13search_gpu check survivors.txt | classify survivors.txt deep.txt...\n");
     return 1;
   }
   if (!strcmp(argv[1], "deep") || !strcmp(argv[1], "deep_big") || !strcmp(argv[1], "recheck")) {
