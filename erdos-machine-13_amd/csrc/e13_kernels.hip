@@ -364,15 +364,27 @@ __device__ __forceinline__ int run_solo(Lane &L, const Ctx &c, int nsteps_in, in
       // run up to the end of this tape word, the window edge and the deadline.
       const uint32_t diff = word ^ (0u - sym); // bits that differ from sym
       int k;
+      bool to_end; // the run reaches the end of this tape word
       if (e & 2u) { // right: cells qb, qb+1, ...
         const uint32_t up = diff >> qb;
+        to_end = up == 0u;
         k = up ? __builtin_ctz(up) : 32 - (int)qb;
-        k = min(k, E13_QMAX - q);
       } else { // left: cells qb, qb-1, ...
         const uint32_t dn = diff << (31u - qb);
+        to_end = dn == 0u;
         k = dn ? __builtin_clz(dn) : (int)qb + 1;
-        k = min(k, q);
       }
+      if (to_end) { // it may go on: the following words of the tape, one per lane, in one read
+        const int wj = wi + ((e & 2u) ? 1 + lane : -1 - lane);
+        const uint32_t fill = 0u - sym;
+        uint32_t wv = ~fill; // past the tape: the run ends there
+        if (wj >= 0 && wj < E13_TW) wv = lds_ld(c.sb, (wj << 8) + tbl);
+        const uint64_t mism = __ballot(wv != fill);
+        const int j = __builtin_ctzll(mism); // the first word where the run ends (always one)
+        const uint32_t wd = (uint32_t)__builtin_amdgcn_readlane((int)wv, j) ^ fill;
+        k += 32 * j + ((e & 2u) ? __builtin_ctz(wd) : __builtin_clz(wd));
+      }
+      k = (e & 2u) ? min(k, E13_QMAX - q) : min(k, q);
       k = min(k, n_eff - r);
       if (ones == target) { // the gate can fire only in the stage-1 state at qA or qR (or anywhere at nt 1)
         if (nt1 | ((st == st0r) & ((q == qA) | (q == qR)))) k = 0;

@@ -17,7 +17,8 @@
  *
  * Usage: solo_loop_check SEED RUNS; prints "runs=N mismatches=M stays=S" and
  * exits 1 on a mismatch.  Built with -DPLANT=1 (ones mask of the boundary
- * block), -DPLANT=2 (gate margin 8 instead of 16) or -DPLANT=3 (stay position
+ * block), -DPLANT=2 (gate margin 8 instead of 16), -DPLANT=4 (a word of a run
+ * that crosses words counted as 33 cells) or -DPLANT=3 (stay position
  * off by one) it carries a deliberate slip, which the check must catch.
  */
 #include <stdint.h>
@@ -100,14 +101,29 @@ static int fast(Machine *m, int n, int *why, int me) {
     const uint32_t e = entry(st, sym), nx = e & 0x0F00u;
     if ((((sym ? sw1 : sw0) >> st) & 1u) != 0u) { /* a sweep */
       const uint32_t diff = word ^ (0u - sym);
-      int k;
+      int k, to_end;
       if (e & 2u) {
         const uint32_t up = diff >> qb;
+        to_end = up == 0u;
         k = up ? __builtin_ctz(up) : 32 - (int)qb;
-        if (k > QMAX - q) k = QMAX - q;
       } else {
         const uint32_t dn = diff << (31u - qb);
+        to_end = dn == 0u;
         k = dn ? __builtin_clz(dn) : (int)qb + 1;
+      }
+      if (to_end) { /* the run goes on through the following words while they are all sym */
+        const uint32_t fill = 0u - sym;
+        int wj = wi + ((e & 2u) ? 1 : -1);
+        while (wj >= 0 && wj < 43 && m->tape[wj] == fill) {
+          k += PLANT == 4 ? 33 : 32;
+          wj += (e & 2u) ? 1 : -1;
+        }
+        const uint32_t wd = (wj >= 0 && wj < 43 ? m->tape[wj] : ~fill) ^ fill;
+        k += (e & 2u) ? __builtin_ctz(wd) : __builtin_clz(wd);
+      }
+      if (e & 2u) {
+        if (k > QMAX - q) k = QMAX - q;
+      } else {
         if (k > q) k = q;
       }
       if (k > n - r) k = n - r;
@@ -238,6 +254,11 @@ int main(int argc, char **argv) {
     Machine a;
     memset(&a, 0, sizeof a);
     for (int w = 0; w < 43; w++) a.tape[w] = (uint32_t)splitmix(&s) & (uint32_t)splitmix(&s);
+    if (it % 4 == 3) /* long runs: whole words of zeros or of ones, so sweeps cross words */
+      for (int w = 0; w < 43; w++) {
+        const uint64_t x = splitmix(&s) % 4;
+        if (x) a.tape[w] = x == 1 ? ~0u : 0u;
+      }
     a.tape[42] &= (1u << (QMAX + 1 - 1344)) - 1u;
     const int where = (int)(it % 8);
     a.q = where == 1 ? 52 + (int)(splitmix(&s) % 20)
