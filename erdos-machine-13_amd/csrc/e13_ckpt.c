@@ -44,6 +44,10 @@ int e13_set_checkpoint(e13_ctx *c, const char *path, double every_sec, int64_t m
   return E13_OK;
 }
 
+/* Record counts above this mark a damaged header (and keep the size
+   arithmetic below far from overflow). */
+#define CKPT_MAX_RECORDS (1ull << 32)
+
 static int write_all(FILE *f, const void *p, size_t n) { return fwrite(p, 1, n, f) == n ? 0 : -1; }
 
 /* Device bytes [src, src+n) appended to f through a host staging buffer. */
@@ -128,7 +132,8 @@ int e13i_ckpt_load(e13_ctx *c, const e13_problem *p, size_t *top, size_t *peak, 
   ckpt_head h;
   int rc = E13_OK;
   if (fread(&h, 1, sizeof h, f) != sizeof h || h.magic != CKPT_MAGIC || h.node_bytes != sizeof(E13Node) ||
-      h.surv_bytes != sizeof(E13SurvRec) || memcmp(&h.prob, p, sizeof *p) != 0 || h.kept > h.nsurv)
+      h.surv_bytes != sizeof(E13SurvRec) || memcmp(&h.prob, p, sizeof *p) != 0 || h.kept > h.nsurv ||
+      h.kept > CKPT_MAX_RECORDS || h.top > CKPT_MAX_RECORDS)
     rc = E13_ECKPT;
   if (!rc) { /* the file holds exactly what the header says */
     const long long want = (long long)(sizeof h + h.kept * sizeof(E13SurvRec) + h.top * sizeof(E13Node));
@@ -306,7 +311,7 @@ int e13i_ckpt3_load(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
   int rc = E13_OK;
   if (fread(&h, 1, sizeof h, f) != sizeof h || h.magic != CKPT3_MAGIC || h.node_bytes != sizeof(E13Node) ||
       h.surv_bytes != sizeof(E13SurvRec) || h.np != np || memcmp(h.prob, ps, (size_t)np * sizeof *ps) != 0 ||
-      h.kept > h.nsurv)
+      h.kept > h.nsurv || h.kept > CKPT_MAX_RECORDS || h.nseeds > CKPT_MAX_RECORDS)
     rc = E13_ECKPT;
   if (!rc) {
     const long long want = (long long)(sizeof h + h.kept * sizeof(E13SurvRec) + h.nseeds * sizeof(E13Node));
