@@ -43,9 +43,9 @@ static int32_t *knob(e13_tuning *t, int k) {
 }
 
 /* Candidate values (-1: off / none, as tuning_merge reads it). */
-static const int32_t cand[NKNOB][5] = {
-    {1, 2, 3, 4, 0},      {16, 32, 64, 128, 0}, {1, 2, 4, 0, 0},  {-1, 1024, 4096, 16384, 0}, {-1, 2, 8, 0, 0},
-    {2, 4, 8, 16, 0},     {8, 32, 128, 0, 0},   {128, 256, 0, 0, 0}, {1, 4, 16, 0, 0},        {-1, 4096, 16384, 65536, 0}};
+static const int32_t cand[NKNOB][6] = {
+    {1, 2, 3, 4, 0},  {16, 32, 64, 128, 0}, {1, 2, 4, 0, 0},     {-1, 1024, 4096, 16384, 65536, 0}, {-1, 2, 8, 0, 0},
+    {2, 4, 8, 16, 0}, {8, 32, 128, 0, 0},   {128, 256, 0, 0, 0}, {1, 4, 16, 0, 0},                  {-1, 4096, 16384, 65536, 0}};
 
 static int same_counts(const e13_counts *a, const e13_counts *b, int np) {
   for (int k = 0; k < np; k++) {
@@ -120,7 +120,7 @@ int e13_autotune(e13_ctx *c, const e13_problem *ps, int np, int reps, e13_tuning
     for (int k = 0; k < NKNOB; k++) {
       const int32_t keep = *knob(&cur, k);
       int32_t pick = keep;
-      for (int i = 0; i < 5 && cand[k][i] != 0; i++) {
+      for (int i = 0; i < 6 && cand[k][i] != 0; i++) {
         const int32_t v = cand[k][i];
         /* the current value (as stored: off is 0 after merging) is already measured */
         if (v == keep || (v < 0 && keep == 0)) continue;
