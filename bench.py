@@ -1,34 +1,40 @@
 """Throughput of the TNF enumeration + simulate-and-filter path on one MI355X.
 
 Workload (one bench step): the local-edit search around erdos_14 with at most
-3 changed cells, every deleted state X = 0..13 -- 36,383,467 leaves and
-4,006,276,021 simulated steps.  Each step's leaf counts, step totals and
-dead-at-stage histograms are compared with the reference's (the golden
-fixture tests/golden/ref_dump/delete_f3.txt); a mismatch aborts the run.
+4 changed cells, every deleted state X = 0..13 -- 5,986,617,022 leaves and
+586,158,995,069 simulated steps, the largest search of the reference's
+configurations that one GPU runs.  Each step's leaf counts, step totals and
+dead-at-stage histograms are compared with the reference's (the golden fixture
+tests/golden/ref_dump/delete_f4.txt, the compiled reference's own run); a
+mismatch aborts the run.
 
 value = simulated TM steps per second over the K timed steps (the whole
-search: every X in one persistent launch, plus the survivor re-check), with
-the device's stored autotuned settings (erdos-machine-13_amd/tuning/).
-Leaves/s and the CPU baseline are reported beside it.  Once each, with counts
-checked: the symmetric 3-state counter search (3,078,656,272 leaves) and the
-FMAX=4 search (5,986,617,022 leaves); the 2^22-table synthetic stress batch
-(refilling vs plain form, digest checked); the counter search in the plain
-per-leaf-launch form vs the persistent form; and FMAX=4 in the plain form, with
-the static split and with the split learned by the cost model (trained here,
-on the GPU, from the FMAX=2 and FMAX=3 searches' cost logs); and the probe
-estimator (the reference's estimate mode) on the counter search, its first
-probes matched bit for bit against the dump of the reference's probe().
+search: every X in one persistent launch after a breadth-first frontier, plus
+the survivor re-check), with the device's stored autotuned settings
+(erdos-machine-13_amd/tuning/).  Leaves/s and the CPU baseline are reported
+beside it.  Once each, with counts checked: the symmetric 3-state counter
+search (3,078,656,272 leaves), the FMAX=3 search (36,383,467 leaves), the
+2^22-table synthetic stress batch (refilling vs plain form, digest checked),
+the counter search in the plain per-leaf-launch form vs the persistent form,
+FMAX=4 in the plain form and with the static, probe-ordered and learned
+frontier orders, and the probe estimator (the reference's estimate mode) on
+the counter search, its first probes matched bit for bit against the dump of
+the reference's probe().
 
-roofline: the dominant kernel is the persistent search kernel.  Its
-algorithmic work is 13 integer lane-ops and 8 LDS bytes per simulated step;
-achieved = steps x 13 / kernel time, with the kernel time taken from HIP events
-on the library's own stream around every launch of the timed region.
+roofline: the dominant kernel is the persistent search kernel (k_search).
+Its algorithmic work is 13 integer lane-ops and 8 LDS bytes per simulated
+step; achieved = steps simulated inside it x 13 / its time, the time from HIP
+events on the library's own stream around every launch of the timed region.
 peak = the gfx950 VALU int32 issue rate, 256 CU x 64 lanes/clk x 2.4 GHz =
-3.93e13 op/s (LDS: 256 CU x 128 B/clk x 2.4 GHz).
+3.93e13 op/s (LDS: 256 CU x 128 B/clk x 2.4 GHz).  traffic = HBM bytes per
+launch from the committed rocprofv3 counter summary of the same workload.
 
 cpu_baseline: the reference program compiled from its own source
 (oracle/_ref/e13search, kind "reference"), or the oracle's restatement when
-that binary is absent (kind "port"), on the same search, timed on this host.
+that binary is absent (kind "port"), on a bounded sample of the same workload:
+the FMAX=4 search of deleted state X=4 alone (52,214,301 leaves,
+5,601,350,435 steps), on all host cores, median of 3 wall times; the GPU is
+timed on the same sample beside it.
 """
 import argparse
 import json
@@ -41,19 +47,19 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 MACHINE = os.path.join(ROOT, "tests", "golden", "machines", "erdos_14.txt")
-GOLDEN = os.path.join(ROOT, "tests", "golden", "ref_dump", "delete_f3.txt")
+REF_DUMP = os.path.join(ROOT, "tests", "golden", "ref_dump")
 PROFILES = os.path.join(ROOT, "profiles")
-FMAX = 3
 XS = list(range(14))
+CPU_SAMPLE_X = 4  # the bounded CPU sample: FMAX=4 for this deleted state alone
 OPS_PER_STEP = 13
 VALU_PEAK = 256 * 64 * 2.4e9  # int32 lane-ops/s
 LDS_BYTES_PER_STEP = 8        # one tape word read and written back
 LDS_PEAK = 256 * 128 * 2.4e9  # B/s
 
 
-def golden():
+def golden(fmax):
     want = {}
-    with open(GOLDEN) as f:
+    with open(os.path.join(REF_DUMP, f"delete_f{fmax}.txt")) as f:
         for line in f:
             m = re.search(r"X=(\d+) .*leaves=(\d+) survivors=(\d+) steps=(\d+) dead-at-stage\(0\.\.9\):((?: \d+)+)",
                           line)
@@ -116,7 +122,7 @@ def synth_workload(e13, ctx, n=1 << 22):
         sm = ctx.synth_run(0x5EED, 0, n)
         out[name] = {"kernel_ms": sm.kernel_ms, "steps_per_s": sm.steps / (sm.kernel_ms * 1e-3),
                      "tables_per_s": n / (sm.kernel_ms * 1e-3), "bit_identical_digest": sm.line(0x5EED, 0) == want}
-    ctx.set_tuning(form=3)
+    ctx.reset_tuning()
     out["steps"] = sm.steps
     out["refill_over_plain"] = out["plain"]["kernel_ms"] / out["refill"]["kernel_ms"]
     return out
@@ -140,7 +146,7 @@ def plain_vs_batched(e13, ctx, problem):
         res[f"form{form}"] = {"seconds": dt, "kernel_ms": s1["kernel_ms"] - s0["kernel_ms"],
                               "launches": s1["launches"] - s0["launches"], "steps_per_s": cs[0].steps / dt,
                               "bit_identical_counts": ok}
-    ctx.set_tuning(form=3)
+    ctx.reset_tuning()
     res["persistent_over_plain"] = res["form1"]["seconds"] / res["form3"]["seconds"]
     return res
 
@@ -182,7 +188,6 @@ def learned_vs_static(e13, ctx, base):
                 np.array([e13.cost_targets(r.leaves, r.steps) for r in recs], dtype=np.float32))
 
     f4 = [e13.setup(base, X, 4) for X in XS]
-    before = ctx.tuning()
     out = {"workload": "delete FMAX=4 X=0..13: plain per-leaf-launch form vs persistent static split vs learned split"}
     try:
         x2, t2 = log(2)
@@ -197,7 +202,7 @@ def learned_vs_static(e13, ctx, base):
         out["learned_split"] = run(f4, seed_per_wave=4, learned=1)
         out["plain_form1"] = run(f4, form=1, seed_per_wave=-1, learned=-1)
     finally:
-        ctx.restore_tuning(before)
+        ctx.reset_tuning()
         ctx.set_cost_model(None)
     out["learned_over_plain"] = out["plain_form1"]["seconds"] / out["learned_split"]["seconds"]
     out["learned_over_static"] = out["static_split"]["seconds"] / out["learned_split"]["seconds"]
@@ -253,8 +258,22 @@ def cpu_estimate_baseline(probes=8000000, max_s=120):
             "sample": f"{probes} probes of the same estimate problem in {wall:.1f} s wall"}
 
 
-def cpu_baseline(max_s=120):
-    """The reference CPU search (or the oracle port) on the same workload."""
+def host_cpu():
+    """The host's CPU model and core count (lscpu / nproc)."""
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        m = re.search(r"^Model name:\s*(.+)$", out, re.M)
+        model = m.group(1).strip() if m else None
+    except (OSError, subprocess.TimeoutExpired):
+        pass
+    return model, os.cpu_count()
+
+
+def cpu_baseline(runs=3, max_s=150):
+    """The reference CPU search (or the oracle port) on the bounded sample:
+    delete FMAX=4 for X=CPU_SAMPLE_X alone, all host cores, median of `runs`
+    wall times (process start to exit); counts checked against the fixture."""
     ref = os.path.join(ROOT, "oracle", "_ref", "e13search")
     port = os.path.join(ROOT, "oracle", "_build", "e13oracle")
     exe, kind = (ref, "reference") if os.path.exists(ref) else (port, "port")
@@ -262,26 +281,53 @@ def cpu_baseline(max_s=120):
         return None
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     env = dict(os.environ, OMP_NUM_THREADS=str(threads))
-    t0 = time.time()
-    try:
-        out = subprocess.run([exe, "delete", MACHINE, str(FMAX)] + [str(x) for x in XS], capture_output=True,
-                             text=True, env=env, timeout=max_s)
-    except subprocess.TimeoutExpired:
-        return None
-    wall = time.time() - t0
-    summaries = [l for l in out.stderr.splitlines() if "leaves=" in l]  # not the progress lines
-    steps = sum(int(re.search(r"steps=(\d+)", l).group(1)) for l in summaries)
-    leaves = sum(int(re.search(r"leaves=(\d+)", l).group(1)) for l in summaries)
+    want = golden(4)[CPU_SAMPLE_X]
+    walls = []
+    for _ in range(runs):
+        t0 = time.time()
+        try:
+            out = subprocess.run([exe, "delete", MACHINE, "4", str(CPU_SAMPLE_X)], capture_output=True, text=True,
+                                 env=env, timeout=max_s)
+        except subprocess.TimeoutExpired:
+            return None
+        walls.append(time.time() - t0)
+        summ = [l for l in out.stderr.splitlines() if "leaves=" in l]  # not the progress lines
+        leaves = int(re.search(r"leaves=(\d+)", summ[0]).group(1))
+        steps = int(re.search(r"steps=(\d+)", summ[0]).group(1))
+        if (leaves, steps) != (want[0], want[2]):
+            raise SystemExit(f"cpu baseline: counts differ from the reference fixture: {leaves} {steps}")
+        print(f"bench: cpu sample run {len(walls)}: {walls[-1]:.1f} s", file=sys.stderr, flush=True)
+    wall = sorted(walls)[len(walls) // 2]
+    model, ncpu = host_cpu()
     return {"value": steps / wall, "unit": "steps/s", "cores": threads, "kind": kind,
-            "sample": f"the full workload (delete FMAX={FMAX}, X=0..13): {leaves} leaves, {steps} steps in "
-                      f"{wall:.1f} s wall, OpenMP schedule(dynamic,1), E13_SPLIT=2",
-            "leaves_per_s": leaves / wall}
+            "sample": f"delete FMAX=4 X={CPU_SAMPLE_X} alone ({leaves} leaves, {steps} steps), median of "
+                      f"{runs} wall times {[round(w, 2) for w in walls]} s, OpenMP schedule(dynamic,1), E13_SPLIT=2",
+            "leaves_per_s": leaves / wall, "cpu_model": model, "nproc": ncpu}
 
 
-def traffic_from_profiles():
-    """HBM bytes per expansion launch from the committed rocprofv3 counter
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
-    path = os.path.join(PROFILES, "k_search_fmax3_counters.txt")
+def gpu_on_cpu_sample(e13, ctx, base, runs=3):
+    """The GPU on the CPU baseline's sample (FMAX=4, X=CPU_SAMPLE_X alone):
+    median of `runs` search calls, counts checked."""
+    want = golden(4)[CPU_SAMPLE_X]
+    p = e13.setup(base, CPU_SAMPLE_X, 4)
+    walls = []
+    for _ in range(runs):
+        ctx.sync()
+        t0 = time.perf_counter()
+        cs, sv, ns = ctx.search_many([p], surv_cap=1024)
+        walls.append(time.perf_counter() - t0)
+        if (cs[0].leaves, cs[0].survivors, cs[0].steps, list(cs[0].dead_at_stage)) != want:
+            raise SystemExit("bench: parity failure on the CPU sample")
+    wall = sorted(walls)[len(walls) // 2]
+    return {"steps_per_s": want[2] / wall, "leaves_per_s": want[0] / wall, "seconds": wall,
+            "runs_s": [round(w, 4) for w in walls]}
+
+
+def traffic_from_profiles(workload):
+    """HBM bytes per k_search launch from the committed rocprofv3 counter
+    summary of the same workload (FETCH_SIZE + WRITE_SIZE, scaled by the
+    copy-kernel calibration)."""
+    path = os.path.join(PROFILES, f"k_search_{workload}_counters.txt")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -289,15 +335,22 @@ def traffic_from_profiles():
     return float(m.group(1)) if m else None
 
 
+WORKLOADS = {
+    "fmax4": "delete FMAX=4 X=0..13 (5,986,617,022 leaves, 586,158,995,069 steps per pass)",
+    "fmax3": "delete FMAX=3 X=0..13 (36,383,467 leaves, 4,006,276,021 steps per pass)",
+    "counter3": "freesym X=13 10..13 D0 G0 K0 K1 L1 M1 (3,078,656,272 leaves, 319,112,357,176 steps per pass)",
+}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-extra", action="store_true", help="skip the 3-state counter and FMAX=4 searches")
-    ap.add_argument("--workload", choices=["fmax3", "counter3"], default="fmax3",
-                    help="timed workload (fmax3 is the bench line; counter3 for profiling the kernel)")
+    ap.add_argument("--no-extra", action="store_true", help="skip the further north-star workloads")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="fmax4",
+                    help="timed workload (fmax4 is the bench line; the others for profiling the kernel)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (rocprof FETCH/WRITE_SIZE calibration)")
     args = ap.parse_args()
@@ -325,11 +378,14 @@ def main():
 
     with open(MACHINE) as f:
         base = e13.parse_machine(f.read())
-    want = golden()
-    problems = [(X, e13.setup(base, X, FMAX)) for X in XS]
+    counter = e13.setup(base, 13, free_cells="D0 G0 K0 K1 L1 M1".split(), sym=(10, 13))
     if args.workload == "counter3":
-        problems = [(13, e13.setup(base, 13, free_cells="D0 G0 K0 K1 L1 M1".split(), sym=(10, 13)))]
+        problems = [(13, counter)]
         want = {13: (3078656272, 61, 319112357176, [0, 2957671647, 119843176, 1115937, 22970, 2236, 1, 0, 244, 0])}
+    else:
+        fmax = 4 if args.workload == "fmax4" else 3
+        problems = [(X, e13.setup(base, X, fmax)) for X in XS]
+        want = golden(fmax)
     ctx = e13.Context(0)
     # the stored settings profile this search runs with (small searches have their own)
     profile, tuning = ctx.search_profile([p for _, p in problems])
@@ -337,14 +393,17 @@ def main():
     tuning_file = tuning_file if os.path.exists(os.path.join(ROOT, tuning_file)) else None
     for _ in range(args.warmup):
         one_pass(e13, ctx, problems, want)
+        print("bench: warmup pass done", file=sys.stderr, flush=True)
     ctx.sync()
     s0 = ctx.stats()
     t0 = time.perf_counter()
     leaves = steps = 0
-    for _ in range(args.steps):
-        l, s = one_pass(e13, ctx, problems, want)
+    for k in range(args.steps):
+        l, st = one_pass(e13, ctx, problems, want)
         leaves += l
-        steps += s
+        steps += st
+        if k % 5 == 4:
+            print(f"bench: {k + 1} timed passes", file=sys.stderr, flush=True)
     ctx.sync()
     dt = time.perf_counter() - t0
     s1 = ctx.stats()
@@ -358,21 +417,29 @@ def main():
     cyc = [b - a for a, b in zip(s0["cycles"], s1["cycles"])]
     extra = []
     if not args.no_extra:
-        counter = e13.setup(base, 13, free_cells="D0 G0 K0 K1 L1 M1".split(), sym=(10, 13))
         extra.append(extra_workload(e13, ctx, "3-state counter, freesym X=13 10..13 D0 G0 K0 K1 L1 M1", [counter],
                                     3078656272, 319112357176, 61))
-        f4 = [e13.setup(base, X, 4) for X in XS]
-        extra.append(extra_workload(e13, ctx, "delete FMAX=4 X=0..13", f4, 5986617022, 586158995069, 57))
+        print("bench: counter search done", file=sys.stderr, flush=True)
+        f3 = [e13.setup(base, X, 3) for X in XS]
+        extra.append(extra_workload(e13, ctx, "delete FMAX=3 X=0..13", f3, 36383467, 4006276021, 11))
         extra.append(synth_workload(e13, ctx))
+        print("bench: synthetic batch done", file=sys.stderr, flush=True)
         extra.append(plain_vs_batched(e13, ctx, counter))
+        print("bench: plain vs batched done", file=sys.stderr, flush=True)
         extra.append(learned_vs_static(e13, ctx, base))
+        print("bench: frontier orders done", file=sys.stderr, flush=True)
         est = estimator_workload(e13, ctx, base)
         if not args.no_cpu:
             est["cpu"] = cpu_estimate_baseline()
         extra.append(est)
+    cpu = None
+    if not args.no_cpu and args.workload == "fmax4":
+        cpu = cpu_baseline()
+        if cpu is not None:
+            cpu["gpu_same_sample"] = gpu_on_cpu_sample(e13, ctx, base)
+            cpu["gpu_over_cpu_same_sample"] = cpu["gpu_same_sample"]["steps_per_s"] / cpu["value"]
     line = {
-        "metric": "simulated TM steps/s, local-edit search FMAX=3 (X=0..13) on erdos_14, bit-identical to the "
-                  "reference",
+        "metric": "simulated TM steps/s (leaves/s beside it) on 1 MI355X, bit-identical survivors, vs CPU",
         "value": steps / dt,
         "unit": "steps/s",
         "n_gpus": 1,
@@ -383,11 +450,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int32",
-        "data": "the reference's own search input (erdos_14, deleted states 0..13, <=3 changed cells); no "
-                "synthetic data",
-        "config": {"workload": "delete FMAX=3 X=0..13 (36,383,467 leaves, 4,006,276,021 steps per pass)"
-                   if args.workload == "fmax3" else
-                   "freesym X=13 10..13 D0 G0 K0 K1 L1 M1 (3,078,656,272 leaves, 319,112,357,176 steps)",
+        "data": "the reference's own search input (erdos_14 and its edits); no synthetic data",
+        "config": {"workload": WORKLOADS[args.workload],
                    "form": s1["form"], "slice": s1["slice"], "wg_threads": s1["wg_threads"], "grid": s1["grid"],
                    "gate": "ones-count" if s1["gate"] == 1 else "zobrist-hash",
                    "settings_profile": "small-search" if s1["profile"] == 1 else "main",
@@ -398,7 +462,7 @@ def main():
         "leaves_per_s": leaves / dt,
         "bit_identical": True,
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_PEAK / 1e12, "unit": "TOP/s",
-                     "frac": achieved / VALU_PEAK, "traffic": traffic_from_profiles(),
+                     "frac": achieved / VALU_PEAK, "traffic": traffic_from_profiles(args.workload),
                      "lds": {"achieved_GBps": ksteps * LDS_BYTES_PER_STEP / kernel_s / 1e9,
                              "peak_GBps": LDS_PEAK / 1e9, "frac": ksteps * LDS_BYTES_PER_STEP / kernel_s / LDS_PEAK},
                      "kernel": "k_search", "launches": launches,
@@ -414,7 +478,7 @@ def main():
                      "wave_cycle_split": {"refill": cyc[0] / max(sum(cyc), 1), "steps": cyc[1] / max(sum(cyc), 1),
                                           "finish": cyc[2] / max(sum(cyc), 1), "pool_idle": cyc[3] / max(sum(cyc), 1)}},
         "north_star_workloads": extra,
-        "cpu_baseline": None if args.no_cpu else cpu_baseline(),
+        "cpu_baseline": cpu,
     }
     ctx.close()
     print(json.dumps(line))

@@ -112,7 +112,8 @@ class Stats(ctypes.Structure):
                 ("wg_threads", ctypes.c_int32), ("grid", ctypes.c_int32), ("gate", ctypes.c_int32),
                 ("form", ctypes.c_int32), ("profile", ctypes.c_int32), ("donations", ctypes.c_int64),
                 ("pool_takes", ctypes.c_int64), ("cycles", ctypes.c_int64 * 4), ("graphs", ctypes.c_int64),
-                ("search_ms", ctypes.c_double), ("search_launches", ctypes.c_int64), ("search_steps", ctypes.c_int64)]
+                ("search_ms", ctypes.c_double), ("search_launches", ctypes.c_int64), ("search_steps", ctypes.c_int64),
+                ("frontier", ctypes.c_int64)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k != "cycles"}
@@ -433,6 +434,10 @@ class Context:
 
     def set_tuning(self, **kw):
         _check(lib().e13_set_tuning(self._h, ctypes.byref(Tuning(**kw))), "e13_set_tuning")
+
+    def reset_tuning(self):
+        """Back to the settings the context opened with (stored profiles, automatic choice)."""
+        _check(lib().e13_set_tuning(self._h, None), "e13_set_tuning")
 
     # fields where 0 reads back as "off" but a set of 0 means "keep"
     _OFF_FIELDS = ("seed_per_wave", "trace_events", "graph_chain", "cost_log", "learned", "solo_steps", "evict_after")

@@ -135,8 +135,24 @@ static void set_checkpoint(e13_ctx *ctx, int X) {
   }
 }
 
+/* The summary line of the C program (e13search.c:413-416): split is
+   $E13_SPLIT (default 2) as there, front the nodes the parallel phase started
+   from (this library's breadth-first frontier); split, front and wall depend
+   on the schedule, the other fields do not. */
+static void c_summary(int X, int budget, const e13_stats *st, const e13_counts *cnt, double wall) {
+  const char *sp = getenv("E13_SPLIT");
+  fprintf(stderr, "X=%d budget=%d split=%d front=%lld leaves=%llu survivors=%llu steps=%llu wall=%.1fs "
+                  "dead-at-stage(0..%d):",
+          X, budget, sp ? atoi(sp) : 2, (long long)st->frontier, (unsigned long long)cnt->leaves,
+          (unsigned long long)cnt->survivors, (unsigned long long)cnt->steps, wall, E13_NS);
+  for (int j = 0; j <= E13_NS; j++) fprintf(stderr, " %llu", (unsigned long long)cnt->dead_at_stage[j]);
+  fprintf(stderr, "\n");
+}
+
+/* gpu_line = 1: the GPU driver's summary line (e13gpu.py:456-459) instead of
+   the C program's. */
 static int run_one(e13_ctx *ctx, const int32_t *base, int nq_base, int X, int fmax, const uint8_t *freebase,
-                   int lo, int hi) {
+                   int lo, int hi, int gpu_line) {
   e13_problem p;
   int rc = e13_setup(base, nq_base, X, fmax, freebase, lo, hi, &p);
   if (rc == E13_EFORCED) {
@@ -190,12 +206,17 @@ static int run_one(e13_ctx *ctx, const int32_t *base, int nq_base, int X, int fm
     digest += text_hash(l2) ^ mix64(text_hash(h2));
   }
   fflush(stdout);
-  fprintf(stderr, "X=%d budget=%d gpu leaves=%llu survivors=%llu steps=%llu wall=%.1fs dead-at-stage(0..%d):", X,
-          p.budget, (unsigned long long)cnt.leaves, (unsigned long long)cnt.survivors,
-          (unsigned long long)cnt.steps, wall, E13_NS);
-  for (int j = 0; j <= E13_NS; j++) fprintf(stderr, " %llu", (unsigned long long)cnt.dead_at_stage[j]);
+  if (gpu_line) {
+    fprintf(stderr, "X=%d budget=%d gpu leaves=%llu survivors=%llu steps=%llu wall=%.1fs dead-at-stage(0..%d):", X,
+            p.budget, (unsigned long long)cnt.leaves, (unsigned long long)cnt.survivors,
+            (unsigned long long)cnt.steps, wall, E13_NS);
+    for (int j = 0; j <= E13_NS; j++) fprintf(stderr, " %llu", (unsigned long long)cnt.dead_at_stage[j]);
+    fprintf(stderr, "\n");
+  } else {
+    c_summary(X, p.budget, &st1, &cnt, wall);
+  }
   fprintf(stderr,
-          "\n  launches=%lld peak stack=%lld nodes kernel=%.1fms %.2e steps/s rounds=%lld lane-slots=%lld "
+          "  launches=%lld peak stack=%lld nodes kernel=%.1fms %.2e steps/s rounds=%lld lane-slots=%lld "
           "donations=%lld pool-takes=%lld settings=%s\nX=%d digest=%016llx\n",
           (long long)(st1.launches - st0.launches), (long long)st1.peak_stack, st1.kernel_ms - st0.kernel_ms,
           wall > 0 ? (double)cnt.steps / wall : 0.0, (long long)(st1.rounds - st0.rounds),
@@ -267,11 +288,8 @@ static int run_many(e13_ctx *ctx, const int32_t *base, int nq_base, int nx, char
     }
   fflush(stdout);
   for (int k = 0; k < np; k++) {
-    fprintf(stderr, "X=%d budget=%d gpu leaves=%llu survivors=%llu steps=%llu wall=%.1fs dead-at-stage(0..%d):", xv[k],
-            ps[k].budget, (unsigned long long)cnt[k].leaves, (unsigned long long)cnt[k].survivors,
-            (unsigned long long)cnt[k].steps, wall, E13_NS);
-    for (int j = 0; j <= E13_NS; j++) fprintf(stderr, " %llu", (unsigned long long)cnt[k].dead_at_stage[j]);
-    fprintf(stderr, "\nX=%d digest=%016llx\n", xv[k], (unsigned long long)digest[k]);
+    c_summary(xv[k], ps[k].budget, &st1, &cnt[k], wall);
+    fprintf(stderr, "X=%d digest=%016llx\n", xv[k], (unsigned long long)digest[k]);
     tot += cnt[k].steps;
   }
   fprintf(stderr, "  one launch for %d problems (%s settings): kernel=%.1fms %.2e steps/s, rounds=%lld lane-slots=%lld "
@@ -505,7 +523,7 @@ int main(int argc, char **argv) {
     if (cur.form == 3 && argc - 4 > 1)
       run_many(ctx, base, nq, argc - 4, argv + 4, fmax);
     else
-      for (int i = 4; i < argc; i++) run_one(ctx, base, nq, atoi(argv[i]), fmax, NULL, -1, -1);
+      for (int i = 4; i < argc; i++) run_one(ctx, base, nq, atoi(argv[i]), fmax, NULL, -1, -1, 0);
   } else if (!strcmp(mode, "tune")) { /* tune machine.txt FMAX X... | tune machine.txt freesym X LO HI CELL... */
     if (argc < 5) return 1;
     int np = 0;
@@ -615,7 +633,7 @@ int main(int argc, char **argv) {
       fprintf(stderr, "X=%d nq=%d free=%d sym=%d..%d n0=%d node=%dB form=%d\n", X, p.nq, argc - 6, p.sym_lo, p.sym_hi,
               p.sym_n0, 304, cur.form);
     }
-    run_one(ctx, base, nq, X, 0, fb, lo, lo >= 0 ? hi : -1);
+    run_one(ctx, base, nq, X, 0, fb, lo, lo >= 0 ? hi : -1, 1);
   } else if (!strcmp(mode, "free") || !strcmp(mode, "freesym")) {
     if (argc < 4) return 1;
     uint8_t fb[32] = {0};
@@ -634,7 +652,7 @@ int main(int argc, char **argv) {
       }
       fb[2 * q + s] = 1;
     }
-    run_one(ctx, base, nq, X, 0, fb, lo, hi);
+    run_one(ctx, base, nq, X, 0, fb, lo, hi, 0);
   } else {
     fprintf(stderr, "unknown mode %s\n", mode);
     return 1;

@@ -44,6 +44,7 @@ struct e13_ctx {
   hipEvent_t ev0, ev1;
   e13_tuning tune;
   e13_tuning tune_small; /* settings for small searches (valid when has_small) */
+  e13_tuning tune_open, tune_small_open; /* both profiles as e13_open installed them */
   int has_small, auto_profile; /* auto_profile: no e13_set_tuning since e13_open */
   e13_stats stats;
   E13Counters *d_ctr;

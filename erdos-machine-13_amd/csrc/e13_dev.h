@@ -129,16 +129,21 @@ typedef struct E13Sched {
   uint64_t pad[3];
 } E13Sched;
 
-/* A wave's mailbox.  A donor that won the wave's bit in the hungry map writes
-   the node as 64-bit elements (delivery tag << 32 | node word), then sets
-   flag = tag.  The wave polls only its own flag; a flag that differs from the
-   last tag it took is a delivery, and the node is complete when every element
-   carries that tag (stale words from an earlier delivery carry another tag).
-   flag = E13_MAIL_DONE: the search is over.  Tags are never 0. */
+/* A wave's mailbox.  A donor that won the wave's bit in the hungry map takes
+   the mailbox's next delivery number from `seq` (an atomic add), writes the
+   node as 64-bit elements (delivery tag << 32 | node word), then sets
+   flag = tag, with tag = seq mod (2^32 - 1) + 1.  The wave polls only its own
+   flag; a flag that differs from the last tag it took is a delivery, and the
+   node is complete when every element carries that tag (stale words from an
+   earlier delivery to this mailbox carry an earlier number).  Numbers are per
+   mailbox, so two deliveries to one wave carry the same tag only 2^32 - 1
+   deliveries apart.  flag = E13_MAIL_DONE: the search is over.  Tags are
+   never 0. */
 #define E13_MAIL_DONE 0xFFFFFFFFFFFFFFFFull
 typedef struct E13Mail {
   uint64_t flag;
-  uint64_t pad[7];
+  uint64_t seq;   /* deliveries numbered so far */
+  uint64_t pad[6];
   uint64_t w[E13_NODE_WORDS];
 } E13Mail;
 

@@ -37,6 +37,9 @@ const char *e13_strerror(int code) {
   case E13_ENODEV: return "no HIP device";
   case E13_ESTOPPED: return "stopped cleanly; the checkpoint is saved";
   case E13_ECKPT: return "checkpoint file unreadable, damaged or for another search";
+  case E13_ELOST: return "persistent search: a mailbox delivery never arrived complete";
+  case E13_EWATCHDOG: return "persistent search: idle watchdog (no work and no end mark)";
+  case E13_ELDS: return "dynamic LDS does not start at LDS address 0";
   default: return "unknown error";
   }
 }
@@ -124,11 +127,16 @@ int e13_setup(const int32_t *base, int nq_base, int X, int fmax, const uint8_t *
   out->nforced = nforced;
   out->sym_lo = out->sym_hi = -1;
   out->sym_n0 = 0;
-  if (sym_lo >= 0) { /* set_sym (e13search.c:423-426) + the checks at :336-351 */
-    int lo = sym_lo - (sym_lo > X), hi = sym_hi - (sym_hi >= X);
-    if (lo < 0 || hi < lo || hi >= row) return E13_EINVAL;
+  /* set_sym (e13search.c:423-426): LO and HI are base states, shifted down
+     when above X.  With X = -1 both move down by one (lo > -1 holds for every
+     lo >= 0), and a LO that lands below 0 switches symmetry off. */
+  const int lo = sym_lo >= 0 ? sym_lo - (sym_lo > X) : -1, hi = sym_hi - (sym_hi >= X);
+  if (lo >= 0) { /* the checks at e13search.c:336-351 */
+    if (hi >= row) return E13_EINVAL; /* the reference would read past its table */
     uint8_t named[16] = {0};
-    for (int i = 0; i < 2 * row; i++) {
+    /* the scan covers 2*(NQ-1) cells, NQ the base states (e13search.c:341): the
+       whole table when a state is deleted, all but the last row when none is */
+    for (int i = 0; i < 2 * (nq_base - 1); i++) {
       int c = out->tab[i];
       if (c >= 0 && (c >> 2) >= lo && (c >> 2) <= hi) named[(c >> 2) - lo] = 1;
     }
@@ -272,6 +280,8 @@ int e13_open(int device, const e13_tuning *t, e13_ctx **out) {
     free(c);
     return E13_EINVAL;
   }
+  c->tune_open = c->tune;
+  c->tune_small_open = c->tune_small;
   if (hipStreamCreate(&c->stream) != hipSuccess || hipEventCreate(&c->ev0) != hipSuccess ||
       hipEventCreate(&c->ev1) != hipSuccess) {
     free(c);
@@ -336,6 +346,12 @@ void e13_close(e13_ctx *c) {
 
 int e13_set_tuning(e13_ctx *c, const e13_tuning *t) {
   if (!c) return E13_EINVAL;
+  if (!t) { /* back to what e13_open installed, with the automatic profile choice */
+    c->tune = c->tune_open;
+    c->tune_small = c->tune_small_open;
+    c->auto_profile = 1;
+    return E13_OK;
+  }
   e13_tuning n = c->tune;
   tuning_merge(&n, t);
   if (!tuning_valid(&n)) return E13_EINVAL;
@@ -542,7 +558,7 @@ static int run_chain(e13_ctx *c, E13ExpandArgs *a, size_t batch, int grid, int w
     c->stats.graphs += 1;
     if (ch.error) {
       if (ch.error == 2) fprintf(stderr, "libe13: dynamic LDS does not start at LDS address 0\n");
-      rc = ch.error == 2 ? E13_EHIP : E13_ESTACK;
+      rc = ch.error == 2 ? E13_ELDS : E13_ESTACK;
       break;
     }
     if (ch.top == 0) break;
@@ -565,6 +581,7 @@ int e13_search(e13_ctx *c, const e13_problem *p, e13_counts *out, e13_survivor *
   const int form = tu->form == 1 ? 1 : 2;
   const int nslot = 4 * p->nq + 1;
   const size_t batch = (size_t)tu->batch;
+  c->stats.frontier = 1; /* the host loop starts from the root */
   int rc;
   if ((rc = e13i_ensure_buf((void **)&c->d_stack, &c->stack_cap, (size_t)tu->stack_nodes, sizeof(E13Node))))
     return rc;
@@ -669,7 +686,7 @@ int e13_search(e13_ctx *c, const e13_problem *p, e13_counts *out, e13_survivor *
     uint32_t nout = io[1];
     if (io[3] & 4u) {
       fprintf(stderr, "libe13: dynamic LDS does not start at LDS address 0\n");
-      return E13_EHIP;
+      return E13_ELDS;
     }
     if (io[3]) return E13_ESTACK; /* output buffer overflow (cannot happen: one child per item) */
     if (base_i + nout > c->stack_cap) return E13_ESTACK;
@@ -1022,7 +1039,7 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
       float ems = 0;
       if (hipEventElapsedTime(&ems, c->ev0, c->ev1) == hipSuccess) c->stats.kernel_ms += ems;
       c->stats.launches++;
-      if (io[3]) return io[3] & 4u ? E13_EHIP : E13_ESTACK;
+      if (io[3]) return io[3] & 4u ? E13_ELDS : E13_ESTACK;
       if (io[1])
         HIPCHK(hipMemcpyAsync(c->d_stack, c->d_out, (size_t)io[1] * sizeof(E13Node), hipMemcpyDeviceToDevice,
                               c->stream));
@@ -1061,7 +1078,16 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
   double t_ck = host_ms(), last_stop_check = 0;
   E13Counters h;
   float ms = 0;
+  if (!resumed) c->stats.frontier = (int64_t)nseeds;
   for (;;) { /* one launch; with checkpoints, again from the nodes a clean stop left */
+  if (nseeds == 0) {
+    /* the breadth-first seeding finished every tree (or a resumed checkpoint
+       holds no nodes): the counts are complete, and a launch with no work
+       would have no wave to end it */
+    HIPCHK(hipMemcpyAsync(&h, c->d_ctr, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    break;
+  }
   if (nseeds > (size_t)total_waves * (size_t)stk_cap) return E13_ESTACK;
   if (call_launches > 0) { /* per-launch figures restart (the survivor count goes on) */
     HIPCHK(hipMemsetAsync(&c->d_ctr->steps, 0, sizeof(uint64_t), c->stream));
@@ -1181,7 +1207,7 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
             : sch.error == 2 ? "a mailbox delivery never arrived complete"
             : sch.error == 3 ? "idle watchdog"
                              : "dynamic LDS does not start at LDS address 0");
-    return E13_ESTACK;
+    return sch.error == 1 ? E13_ESTACK : sch.error == 2 ? E13_ELOST : sch.error == 3 ? E13_EWATCHDOG : E13_ELDS;
   }
   if (logging && (rc = collect_cost_log(c, ps, nseeds))) return rc;
   c->stats.search_ms += ms;
@@ -1340,7 +1366,7 @@ int e13_simulate_batch(e13_ctx *c, const int8_t *tabs, int ncells, int n, const 
     free(lr);
     free(h);
     fprintf(stderr, "libe13: dynamic LDS does not start at LDS address 0\n");
-    return E13_EHIP;
+    return E13_ELDS;
   }
   float ms = 0;
   if (hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) c->stats.kernel_ms += ms;
@@ -1530,7 +1556,7 @@ int e13_synth_run(e13_ctx *c, uint64_t seed0, uint64_t k0, int64_t n, e13_batch_
   }
   if (hc.overflow & 4u) {
     fprintf(stderr, "libe13: dynamic LDS does not start at LDS address 0\n");
-    rc = E13_EHIP;
+    rc = E13_ELDS;
     goto done;
   }
   c->stats.kernel_ms += ms;

@@ -1224,6 +1224,7 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
     if (threadIdx.x == 0) atomicMax(&a.sch->error, 4);
     return;
   }
+  if (a.nseeds == 0) return; // no work: no wave would ever count busy down to the end mark
   const int after_keys = c.keyb + (GATE == GATE_HASH ? KEYB_WORDS * 8 : 0);
   uint32_t *stg = (uint32_t *)(c.sb + after_keys) + wave * STG_WORDS;
   unsigned long long *pc = (unsigned long long *)(c.sb + after_keys + nwaves * STG_WORDS * 4); // np x 16 per WG
@@ -1626,15 +1627,19 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
         const int bsel = (__ffsll((long long)wr) - 1 + brot) & 63;
         const uint64_t bit = 1ull << bsel;
         int won = 0;
+        uint64_t dseq = 0; // the receiver's delivery number
+        const int dst_w = wsel * 64 + bsel;
         if (lane == 0) {
           won = (atomicAnd((unsigned long long *)&a.hungry[wsel], ~bit) & bit) != 0;
-          if (won) atomicAdd(&a.sch->busy, 1); // the delivery counts as busy until it is taken
+          if (won) {
+            atomicAdd(&a.sch->busy, 1); // the delivery counts as busy until it is taken
+            dseq = atomicAdd((unsigned long long *)&a.mail[dst_w].seq, 1ull);
+          }
         }
         won = uni(won);
         if (lane == wsel) hw &= ~bit;
         nz = ballot(hw != 0);
         if (!won) continue;
-        const int dst_w = wsel * 64 + bsel;
         uint32_t slots_word = 0xFFFFFFFFu; // the donated copy's slot range (split only)
         if (whole && staged == bot) { // a delivery may sit above it: its slot cursor travels with it
           if (lane == 0) stk_node(a, wid, staged)->next_slot = slot_range(cur_slot, cur_end);
@@ -1647,7 +1652,7 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
         wave_mem_sync();
         const uint32_t *src = (const uint32_t *)stk_node(a, wid, whole ? bot : staged);
         uint64_t *dst = a.mail[dst_w].w;
-        const uint64_t tag = ((uint64_t)(wid + 1) << 19) | ((w_gives + 1) & 0x7FFFFull); // unique, never 0
+        const uint64_t tag = (uint64_t)uni64((long long)dseq) % 0xFFFFFFFFull + 1ull; // 1 .. 2^32 - 1
         uint32_t v0 = src[lane];
         const uint32_t v1 = lane < E13_NODE_WORDS - 64 ? src[64 + lane] : 0u;
         if (!whole && lane == E13_NODE_W_SLOTS) v0 = slots_word;

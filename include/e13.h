@@ -33,6 +33,11 @@ extern "C" {
 #define E13_ENODEV (-8)     /* no HIP device */
 #define E13_ESTOPPED (-9)   /* a clean stop: the checkpoint is saved (e13gpu.py:398-409, exit code 3) */
 #define E13_ECKPT (-10)     /* checkpoint unreadable, damaged or written for another search (e13gpu.py:360-361) */
+/* Aborts of the persistent search's scheduler (no reference counterpart: the
+   reference's host loop has no device-side scheduling). */
+#define E13_ELOST (-11)     /* a mailbox delivery never arrived complete */
+#define E13_EWATCHDOG (-12) /* a wave waited longer than the idle watchdog for work or the end mark */
+#define E13_ELDS (-13)      /* dynamic LDS does not start at LDS address 0 */
 
 /* Result codes of a run (e13search.c:64-65). */
 #define E13_DEAD (-1)
@@ -126,6 +131,9 @@ typedef struct {
   double search_ms;              /* the persistent kernel alone (k_search): HIP-event time, */
   int64_t search_launches;       /* launches, */
   int64_t search_steps;          /* and the steps simulated inside it */
+  int64_t frontier;              /* nodes the last search's parallel phase started from: the breadth-first
+                                    frontier of the persistent form (the reference's front=, e13search.c:373),
+                                    1 (the root) for the host-loop forms */
 } e13_stats;
 
 typedef struct e13_ctx e13_ctx;
@@ -135,6 +143,9 @@ typedef struct e13_ctx e13_ctx;
 int e13_open(int device, const e13_tuning *t, e13_ctx **out);
 void e13_close(e13_ctx *ctx);
 const char *e13_strerror(int code);
+/* Install settings (fields <= 0 keep the current value); they then hold for
+   every search (no automatic profile choice).  t = NULL: back to the settings
+   e13_open installed, with the automatic choice of the small-search profile. */
 int e13_set_tuning(e13_ctx *ctx, const e13_tuning *t);
 int e13_get_tuning(e13_ctx *ctx, e13_tuning *t);
 int e13_get_stats(e13_ctx *ctx, e13_stats *s);

@@ -167,7 +167,9 @@ int eo_setup(eo_problem *p) {
   p->sym_n0 = 0;
   if (p->sym_lo >= 0) {
     uint8_t named[EO_MAXQ] = {0};
-    for (int i = 0; i < 2 * p->nq; i++) {
+    /* e13search.c:341 scans 2*(NQ-1) cells, NQ the base states: with no state
+       deleted (X = -1) the last row is left out */
+    for (int i = 0; i < 2 * (p->nq_base - 1); i++) {
       int c = p->tab[i];
       if (c >= 0 && (c >> 2) >= p->sym_lo && (c >> 2) <= p->sym_hi) named[(c >> 2) - p->sym_lo] = 1;
     }

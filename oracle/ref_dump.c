@@ -18,6 +18,13 @@
  *                               the estimate problem (argv as in its estimate
  *                               mode), probe i drawing from the splitmix state
  *                               splitmix(0x5EED0000 + (SEED << 40) + i)
+ *   e13ref_dump setup machine X FMAX LO HI [CELL...]
+ *                               what the reference's set_sym() + setup()
+ *                               leave for that problem (argv as in its
+ *                               estimate mode: LO < 0 no set_sym, cells give
+ *                               free mode): one line with the budget, SYM_LO,
+ *                               SYM_HI, SYM_N0 and the table codes; setup()'s
+ *                               own message and exit status 1 when it refuses
  */
 #define main e13ref_main
 #include REF_SRC
@@ -51,7 +58,7 @@ static int parse_tab(const char *tb, int *tab) {
 int main(int argc, char **argv) {
   ref_init();
   if (argc < 2) {
-    fprintf(stderr, "usage: e13ref_dump keys|hash|probe ...\n");
+    fprintf(stderr, "usage: e13ref_dump keys|hash|probe|setup ...\n");
     return 1;
   }
   if (!strcmp(argv[1], "keys")) {
@@ -101,6 +108,19 @@ int main(int argc, char **argv) {
              (unsigned long long)b[2], (unsigned long long)b[3]);
     }
     free(x);
+    return 0;
+  }
+  if (!strcmp(argv[1], "setup") && argc >= 7) {
+    load(argv[2]);
+    int X = atoi(argv[3]), fmax = atoi(argv[4]), tab[2 * MAXQ];
+    unsigned char freebase[2 * MAXQ] = {0};
+    if (atoi(argv[5]) >= 0) set_sym(X, atoi(argv[5]), atoi(argv[6]));
+    for (int i = 7; i < argc; i++) freebase[2 * (argv[i][0] - 'A') + (argv[i][1] - '0')] = 1;
+    int budget = setup(X, fmax, argc > 7 ? freebase : NULL, tab); /* exits 1 on a symmetry refusal */
+    int nq = NQ - (X >= 0);
+    printf("setup budget=%d sym_lo=%d sym_hi=%d sym_n0=%d tab=", budget, SYM_LO, SYM_HI, SYM_N0);
+    for (int i = 0; i < 2 * nq; i++) printf("%s%d", i ? "," : "", tab[i]);
+    printf("\n");
     return 0;
   }
   (void)e13ref_main;

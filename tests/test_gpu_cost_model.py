@@ -62,7 +62,7 @@ def cost_log(e13, ctx, base, fmax, want):
         cs, sv, ns = ctx.search_many([e13.setup(base, X, fmax) for X in XS], surv_cap=4096)
         log = ctx.cost_log()
     finally:
-        ctx.set_tuning(seed_per_wave=-1, cost_log=-1)
+        ctx.reset_tuning()
     for X, c in zip(XS, cs):
         assert (c.leaves, c.steps) == (want[X]["leaves"], want[X]["steps"]), X
     x = np.array([list(r.x) for r in log], dtype=np.float32)
@@ -124,7 +124,7 @@ def test_frontier_order_never_changes_results(e13, gpu_ctx, base, learned):
     try:
         cs, sv, ns = gpu_ctx.search_many([e13.setup(base, X, 3) for X in XS], surv_cap=4096)
     finally:
-        gpu_ctx.set_tuning(seed_per_wave=-1, learned=-1)
+        gpu_ctx.reset_tuning()
         gpu_ctx.set_cost_model(None)
     f3 = parse_summaries(read_text(REF_DUMP, "delete_f3.txt"))
     for X, c in zip(XS, cs):

@@ -85,7 +85,7 @@ def test_synthetic_batch_matches_oracle(e13, gpu_ctx, gate):
     try:
         lv = gpu_ctx.simulate_batch(tabs, caps)
     finally:
-        gpu_ctx.set_tuning(gate=1)
+        gpu_ctx.reset_tuning()
     L = oracle_lib()
     for k in range(n):
         want = oracle_full(L, tabs[k], [caps[k]] * 10)
@@ -103,7 +103,7 @@ def test_synthetic_stress_batch_matches_golden(e13, gpu_ctx, n, form, gate):
     try:
         sm = gpu_ctx.synth_run(0x5EED, 0, n)
     finally:
-        gpu_ctx.set_tuning(form=3, gate=1)
+        gpu_ctx.reset_tuning()
     assert sm.line(0x5EED, 0) == synth_golden()[n]
 
 
@@ -122,7 +122,7 @@ def test_local_edits_fmax2(e13, gpu_ctx, base, form, gate):
     try:
         sums, lines, hashes = search_all(e13, gpu_ctx, base, XS, fmax=2)
     finally:
-        gpu_ctx.set_tuning(form=3, gate=1)
+        gpu_ctx.reset_tuning()
     check_against(sums, parse_summaries(read_text(REF_DUMP, "delete_f2.txt")))
     check_against(sums, parse_summaries(read_text(REF_RESULTS, "1_local_edits_log_f2.txt")))
     assert lines == survivor_lines(read_text(REF_RESULTS, "1_local_edits_surv_f2.txt"))
@@ -240,12 +240,11 @@ def test_counter_3state_symmetric(e13, gpu_ctx, base):
                                   dict(form=3, wg_threads=64, wgs_per_cu=1, stack_nodes=1 << 16)])
 def test_results_invariant_under_tuning(e13, gpu_ctx, base, tune):
     say(f"search: delete FMAX=2 X=7,8,12 tuning={tune}")
-    before = gpu_ctx.tuning()
     gpu_ctx.set_tuning(**tune)
     try:
         sums, lines, hashes = search_all(e13, gpu_ctx, base, [7, 8, 12], fmax=2)
     finally:
-        gpu_ctx.restore_tuning(before)
+        gpu_ctx.reset_tuning()
     golden = parse_summaries(read_text(REF_DUMP, "delete_f2.txt"))
     check_against(sums, {X: golden[X] for X in (7, 8, 12)})
     assert lines == survivor_lines(read_text(REF_RESULTS, "1_local_edits_surv_f2.txt"))
@@ -255,14 +254,13 @@ def test_autotune_keeps_results(e13, gpu_ctx, base):
     """The autotuner's trials all give the same counts (it checks each against
     the first) and the settings it installs still give the reference's."""
     say("autotune: delete FMAX=2 X=0..13")
-    before = gpu_ctx.tuning()
     ps = [e13.setup(base, X, 2) for X in XS]
     try:
         best, rep = gpu_ctx.autotune(ps, reps=1)
         assert rep["trials"] >= 10 and rep["best_ms"] <= rep["first_ms"] * 1.0001
         cs, sv, ns = gpu_ctx.search_many(ps, surv_cap=4096)
     finally:
-        gpu_ctx.restore_tuning(before)
+        gpu_ctx.reset_tuning()
     golden = parse_summaries(read_text(REF_DUMP, "delete_f2.txt"))
     for X, c in zip(XS, cs):
         d = c.as_dict()

@@ -1,0 +1,157 @@
+"""GPU searches against reference fixtures that the parity suite's shared
+context does not cover: fresh contexts with the stored settings profiles
+(including searches whose breadth-first frontier finishes the whole tree),
+free-mode searches with no deleted state (X = -1), the raw 3-state counter
+search without symmetry, the division search V3, and FMAX=3 in the plain
+per-leaf-launch form.  Exact comparisons throughout."""
+import pytest
+
+from conftest import REF_DUMP, REF_RESULTS, XS, parse_summaries, read_machine, read_text, ref_hashes, survivor_lines
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def base(e13):
+    return e13.parse_machine(read_machine())
+
+
+def run_one(e13, ctx, p):
+    c, sv, ns = ctx.search(p)
+    assert ns == len(sv) == c.survivors
+    lines, hashes = [], []
+    for s in sv:
+        assert s.recheck_ok == 1
+        line = e13.format_survivor(p, s)
+        lines.append(line)
+        hashes.append((p.X, line.split("table=")[1].replace("?", ""), tuple(s.ch)))
+    d = c.as_dict()
+    return {"leaves": d["leaves"], "survivors": d["survivors"], "steps": d["steps"], "hist": d["hist"]}, \
+        sorted(lines), hashes
+
+
+def same(got, want):
+    for k in ("leaves", "survivors", "steps", "hist"):
+        if want.get(k) is not None:
+            assert got[k] == want[k], (k, got[k], want[k])
+
+
+@pytest.mark.parametrize("X", [2, 13, 8])
+def test_frontier_finishes_the_tree(e13, base, X):
+    """FMAX=2 searches of one deleted state each on a fresh context, in the
+    persistent form with a breadth-first frontier of 2 nodes per wave: for X=2
+    and X=13 the frontier expansion ends every machine of the tree, so no
+    persistent launch is needed (one with no work would never end); X=8 has
+    work left for the launch."""
+    golden = parse_summaries(read_text(REF_DUMP, "delete_f2.txt"))
+    with e13.Context(0) as ctx:
+        say(f"search: delete FMAX=2 X={X} alone, form 3, seed_per_wave=2, fresh context")
+        ctx.set_tuning(form=3, seed_per_wave=2)
+        got, lines, _ = run_one(e13, ctx, e13.setup(base, X, 2))
+        same(got, golden[X])
+        want = [l for l in survivor_lines(read_text(REF_RESULTS, "1_local_edits_surv_f2.txt")) if f"X={X} " in l]
+        assert lines == want
+
+
+def test_stored_profiles_fresh_context(e13, base):
+    """The settings a context opens with (the stored main and small-search
+    profiles, chosen per search): every X of FMAX=2 and FMAX=3 one at a time,
+    then the symmetric counter search."""
+    f2 = parse_summaries(read_text(REF_DUMP, "delete_f2.txt"))
+    f3 = parse_summaries(read_text(REF_DUMP, "delete_f3.txt"))
+    ref = ref_hashes()
+    with e13.Context(0) as ctx:
+        for fmax, golden, surv in ((2, f2, "1_local_edits_surv_f2.txt"), (3, f3, "1_local_edits_surv_f3.txt")):
+            lines = []
+            for X in XS:
+                say(f"search: delete FMAX={fmax} X={X}, stored profiles, fresh context")
+                got, ls, hs = run_one(e13, ctx, e13.setup(base, X, fmax))
+                same(got, golden[X])
+                lines += ls
+                for x, tab, ch in hs:
+                    assert ref[(x, tab)] == ch
+            assert sorted(lines) == survivor_lines(read_text(REF_RESULTS, surv))
+        say("search: freesym X=13 10..13 D0 G0 K0 K1 L1 M1, stored profiles, fresh context")
+        got, lines, _ = run_one(e13, ctx, e13.setup(base, 13, 0, "D0 G0 K0 K1 L1 M1".split(), (10, 13)))
+        same(got, parse_summaries(read_text(REF_RESULTS, "2_counter_3_log_modsym.txt"))[13])
+        assert lines == survivor_lines(read_text(REF_RESULTS, "2_counter_3_surv_modsym.txt"))
+
+
+def freesym_xm1_cases():
+    cases, cur = [], None
+    for line in read_text(REF_DUMP, "freesym_xm1.txt").splitlines():
+        if line.startswith("case "):
+            cur = {"args": line.split()[2:], "surv": [], "hash": []}
+            cases.append(cur)
+        elif line.startswith("SURVIVOR"):
+            cur["surv"].append(line.strip())
+        elif line.startswith("HASH"):
+            cur["hash"].append(tuple(int(f.split("=")[1], 16) for f in line.split()[2:5]))
+        elif "leaves=" in line:
+            cur["sum"] = parse_summaries(line)[-1]
+    return cases
+
+
+@pytest.mark.parametrize("case", freesym_xm1_cases(), ids=lambda c: "_".join(c["args"]))
+def test_freesym_no_deleted_state(e13, gpu_ctx, base, case):
+    """freesym with X = -1 (no deleted state): symmetry from the shifted LO..HI,
+    checked over 2*(NQ-1) cells, or off when LO lands below 0 -- the compiled
+    reference's counts, survivors and counter hashes."""
+    a = case["args"]
+    say(f"search: freesym {' '.join(a)}")
+    p = e13.setup(base, int(a[0]), 0, a[3:], (int(a[1]), int(a[2])))
+    got, lines, hashes = run_one(e13, gpu_ctx, p)
+    same(got, case["sum"])
+    assert lines == sorted(case["surv"])
+    # the fixture's HASH lines follow its SURVIVOR lines one for one
+    want = dict(zip(case["surv"], case["hash"]))
+    got_h = {tab: ch for _, tab, ch in hashes}
+    for line, ch in want.items():
+        assert got_h[line.split("table=")[1].replace("?", "")] == ch
+
+
+def test_counter_3state_raw(e13, gpu_ctx, base):
+    """The 3-state counter search without symmetry: 18,471,584,227 leaves and
+    366 = 61 * 3! survivors (results/2_counter_3/log_mod.txt, surv_mod.txt),
+    the symmetric search's leaves times the relabellings it removes."""
+    say("search: free X=13 D0 G0 K0 K1 L1 M1 (raw 3-state counter)")
+    p = e13.setup(base, 13, 0, "D0 G0 K0 K1 L1 M1".split())
+    got, lines, hashes = run_one(e13, gpu_ctx, p)
+    same(got, parse_summaries(read_text(REF_RESULTS, "2_counter_3_log_mod.txt"))[13])
+    assert lines == survivor_lines(read_text(REF_RESULTS, "2_counter_3_surv_mod.txt"))
+    ref = ref_hashes()
+    for X, tab, ch in hashes:
+        assert ref[(X, tab)] == ch
+    say(f"raw counter: {got['steps']} steps")
+
+
+def test_division_v3(e13, gpu_ctx, base):
+    """Division search V3: freesym X=9 3..9 E0..I1, 53,282,072,641 leaves and
+    5,803,777,946,442 steps (results/4_division_6/v3/log_v3.txt, g_v3.log)."""
+    say("search: freesym X=9 3..9 E0 E1 F0 F1 G0 G1 H0 H1 I0 I1 (V3)")
+    p = e13.setup(base, 9, 0, "E0 E1 F0 F1 G0 G1 H0 H1 I0 I1".split(), (3, 9))
+    got, lines, _ = run_one(e13, gpu_ctx, p)
+    same(got, parse_summaries(read_text(REF_RESULTS, "4_division_6_v3_log_v3.txt"))[9])
+    same(got, parse_summaries(read_text(REF_RESULTS, "4_division_6_v3_g_v3.txt"))[9])
+    assert lines == []
+
+
+def test_local_edits_fmax3_plain_form(e13, gpu_ctx, base):
+    """FMAX=3 in the plain form (one machine per lane, one launch per batch of
+    nodes, host loop)."""
+    f3 = parse_summaries(read_text(REF_DUMP, "delete_f3.txt"))
+    gpu_ctx.set_tuning(form=1)
+    lines = []
+    try:
+        for X in XS:
+            say(f"search: delete FMAX=3 X={X} form=1")
+            got, ls, _ = run_one(e13, gpu_ctx, e13.setup(base, X, 3))
+            same(got, f3[X])
+            lines += ls
+    finally:
+        gpu_ctx.reset_tuning()
+    assert sorted(lines) == survivor_lines(read_text(REF_RESULTS, "1_local_edits_surv_f3.txt"))

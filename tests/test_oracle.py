@@ -14,8 +14,13 @@ Pins, by oracle function:
                             split tests; the doubling logs; survivor lines.
   counter hashes ch[]       ref_dump/survivor_hashes.txt (the reference's own
                             run() on every fixture survivor table).
-  eo_recheck                every fixture survivor re-checks, with the '?' marks
-                            of its line equal to the unread cells.
+  eo_recheck                every fixture survivor line (502, including the raw
+                            counter search's 366) re-checks with the '?' marks of
+                            its line equal to the unread cells; claims one step,
+                            state, head or start kind off are refused; and on
+                            random edits of erdos_14 it accepts exactly the
+                            tables eo_run_full calls survivors, with their
+                            stage-1 state, head and step count.
   eo_synth_table            the brute-force simulator on the same tables.
   eo_probe / eo_estimate    per-probe sums of the reference's own probe()
                             (ref_dump/probe_*.txt, one stream per probe), the
@@ -312,3 +317,95 @@ def test_estimate_mean_is_unbiased_on_a_small_tree():
     for key, ex in (("leaves", exact["leaves"]), ("steps", exact["steps"]), ("survivors", exact["survivors"])):
         m, se = est[key]
         assert se > 0 and abs(m - ex) < 5 * se, (key, m, se, ex)
+
+
+# ---------------------------------------------------------------------------
+# eo_recheck: the plain unhashed re-check of a survivor claim
+# ---------------------------------------------------------------------------
+def recheck_fixture_lines():
+    out = []
+    for f in ("1_local_edits_surv_f2.txt", "1_local_edits_surv_f3.txt", "1_local_edits_surv_f4.txt",
+              "2_counter_3_surv_modsym.txt", "2_counter_3_surv_mod.txt", "validation_split_test.txt",
+              "validation_split_test_free.txt"):
+        out += [l.strip() for l in read_text(REF_RESULTS, f).splitlines() if l.startswith("SURVIVOR")]
+    return out
+
+
+def parse_claim(line):
+    f = dict(kv.split("=", 1) for kv in line.split(" table=")[0].split()[1:])
+    cells = []
+    for row in line.split("table=")[1].split("|"):
+        cells += row.split()
+    tab, unread = [], 0
+    for i, c in enumerate(cells):
+        if c.startswith("?"):
+            unread |= 1 << i
+            c = c[1:]
+        tab.append(-1 if c == "---" else (int(c[2:]) << 2) | ((c[1] == "R") << 1) | int(c[0]))
+    claim = (int(f["st0"]), int(f["head0"]), int(f["abs"]), int(f["rel"]), int(f["steps"]))
+    return tab, unread, claim
+
+
+def eo_recheck(L, tab, claim):
+    L.eo_recheck.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                             ctypes.c_int, ctypes.c_longlong, ctypes.POINTER(ctypes.c_uint32)]
+    t32 = (ctypes.c_int * 32)(*(list(tab) + [-1] * (32 - len(tab))))
+    m = ctypes.c_uint32()
+    ok = L.eo_recheck(t32, len(tab) // 2, *claim, ctypes.byref(m))
+    return ok, m.value
+
+
+def test_recheck_accepts_every_fixture_survivor():
+    L = oracle_lib()
+    lines = recheck_fixture_lines()
+    assert len(lines) >= 500
+    for line in lines:
+        tab, unread, claim = parse_claim(line)
+        ok, mask = eo_recheck(L, tab, claim)
+        assert ok == 1, line
+        assert mask == ((1 << len(tab)) - 1) & ~unread, line
+
+
+def test_recheck_refuses_wrong_claims():
+    """A claim one step long or short, with another stage-1 state or head, or
+    with the other start kind (abs <-> rel) where only one holds, is refused."""
+    L = oracle_lib()
+    refused = 0
+    for line in recheck_fixture_lines()[::7]:
+        tab, _, (st0, head0, a, r, steps) = parse_claim(line)
+        nq = len(tab) // 2
+        wrong = [(st0, head0, a, r, steps + 1), (st0, head0, a, r, steps - 1), ((st0 + 1) % nq, head0, a, r, steps),
+                 (st0, head0 + 1, a, r, steps), (st0, head0 - 1, a, r, steps)]
+        if a != r:
+            wrong.append((st0, head0, r, a, steps))
+        for w in wrong:
+            assert eo_recheck(L, tab, w)[0] == 0, (line, w)
+            refused += 1
+    assert refused > 350
+
+
+def test_recheck_agrees_with_run_on_random_edits():
+    """Random 1..3-cell edits of erdos_14 plus the cells that make fixture
+    survivors: eo_recheck accepts the claim that eo_run_full's final state
+    makes exactly when the run ends as a survivor."""
+    from conftest import oracle_full
+    L = oracle_lib()
+    conv = erdos14()
+    rng = random.Random(11)
+    tabs = []
+    for _ in range(300):
+        tab = list(conv)
+        for _ in range(rng.randint(1, 3)):
+            tab[rng.randrange(28)] = rng.randrange(-1, 56)
+        tabs.append(tab)
+    tabs += [parse_claim(l)[0] for l in recheck_fixture_lines()]
+    surv = dead = 0
+    for tab in tabs:
+        r, stage, t, st0, head0, a, rl, _ = oracle_full(L, tab, None)
+        if stage < 1:
+            continue
+        ok, _ = eo_recheck(L, tab, (st0, head0, a, rl, t))
+        assert ok == (1 if r == -2 else 0), (tab, r, stage)
+        surv += r == -2
+        dead += r != -2
+    assert surv >= 500 and dead >= 50
