@@ -100,7 +100,7 @@ class Tuning(ctypes.Structure):
                 ("idle_naps", ctypes.c_int32), ("mail_every", ctypes.c_int32), ("trace_events", ctypes.c_int32),
                 ("graph_chain", ctypes.c_int32), ("cost_log", ctypes.c_int32),
                 ("learned", ctypes.c_int32), ("solo_steps", ctypes.c_int32), ("evict_after", ctypes.c_int32),
-                ("probe_slice", ctypes.c_int32)]
+                ("probe_slice", ctypes.c_int32), ("cut_live", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -440,7 +440,8 @@ class Context:
         _check(lib().e13_set_tuning(self._h, None), "e13_set_tuning")
 
     # fields where 0 reads back as "off" but a set of 0 means "keep"
-    _OFF_FIELDS = ("seed_per_wave", "trace_events", "graph_chain", "cost_log", "learned", "solo_steps", "evict_after")
+    _OFF_FIELDS = ("seed_per_wave", "trace_events", "graph_chain", "cost_log", "learned", "solo_steps", "evict_after",
+                   "cut_live")
 
     def restore_tuning(self, t):
         """Set every field of a tuning dict read back by tuning() (off fields included)."""

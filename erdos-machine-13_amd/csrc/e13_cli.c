@@ -30,7 +30,8 @@
  * Environment: E13_FORM (1 plain, 2 batched), E13_SLICE, E13_CAP, E13_BATCH,
  * E13_GATE (1 count, 2 hash), E13_WG, E13_WGS_PER_CU, E13_DEVICE,
  * E13_REFILL_TRIES, E13_SEED_PER_WAVE, E13_SCHED_EVERY, E13_DONATE_MAX,
- * E13_IDLE_NAPS, E13_MAIL_EVERY, E13_GRAPH_CHAIN, E13_SOLO_STEPS, E13_EVICT_AFTER (negative: off), E13_PROBE_SLICE;
+ * E13_IDLE_NAPS, E13_MAIL_EVERY, E13_GRAPH_CHAIN, E13_SOLO_STEPS, E13_EVICT_AFTER, E13_CUT_LIVE (negative: off),
+ * E13_PROBE_SLICE;
  * E13_TUNING_FILE (stored settings; <name>_small.<ext> beside it for small searches); E13_TRACE=N with
  * E13_TRACE_FILE=path writes the persistent search's wave event log (N events per wave) to path; the
  * library prints its host phase times with E13_TIMING=1 and a line per autotuner trial with E13_TUNE_LOG=1.
@@ -469,6 +470,7 @@ int main(int argc, char **argv) {
   tu.graph_chain = env_int("E13_GRAPH_CHAIN");
   tu.solo_steps = env_int("E13_SOLO_STEPS");
   tu.evict_after = env_int("E13_EVICT_AFTER");
+  tu.cut_live = env_int("E13_CUT_LIVE");
   tu.probe_slice = env_int("E13_PROBE_SLICE");
   e13_ctx *ctx = NULL;
   int rc = e13_open(env_int("E13_DEVICE"), &tu, &ctx);
@@ -563,10 +565,10 @@ int main(int argc, char **argv) {
       path[0] = 0;
     fprintf(stderr, "autotune (%s profile): %d trials, %.1f ms -> %.1f ms; wgs_per_cu=%d slice=%d refill_tries=%d "
                     "solo_steps=%d seed_per_wave=%d donate_max=%d sched_every=%d wg_threads=%d idle_naps=%d "
-                    "evict_after=%d\n",
+                    "evict_after=%d cut_live=%d\n",
             profile == E13_PROFILE_SMALL ? "small-search" : "main", rep.trials, rep.first_ms, rep.best_ms,
             best.wgs_per_cu, best.slice, best.refill_tries, best.solo_steps, best.seed_per_wave, best.donate_max,
-            best.sched_every, best.wg_threads, best.idle_naps, best.evict_after);
+            best.sched_every, best.wg_threads, best.idle_naps, best.evict_after, best.cut_live);
     if (path[0] && e13_save_tuning(&best, path) == E13_OK) fprintf(stderr, "autotune: saved to %s\n", path);
   } else if (!strcmp(mode, "estimate")) { /* estimate machine.txt PROBES X FMAX LO HI [CELL...] (e13search.c:473-510) */
     if (argc < 8) {

@@ -171,6 +171,8 @@ typedef struct E13SearchArgs {
   int32_t idle_naps, mail_every, trace_cap;
   int32_t solo;           /* > 0: steps per round of the scalar loop for a wave's lone running lane */
   int32_t evict_after;    /* > 0: a machine this many steps from the blank tape may be handed to a hungry wave */
+  int32_t cut_live;       /* > 1: a round ends once fewer lanes than this still run while the wave has nodes */
+  int32_t pad_cut;
   uint32_t nseed_cnt;     /* cost logging: seeds with counters (0: off) */
   uint64_t *seed_cnt;     /* cost logging: per seed, leaves and steps of its subtree */
   uint64_t *trace;        /* optional: trace_cap words per wave, word 0 = events logged, then

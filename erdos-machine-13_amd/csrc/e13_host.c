@@ -213,6 +213,8 @@ static void tuning_merge(e13_tuning *dst, const e13_tuning *src) {
   if (src->evict_after < 0) dst->evict_after = 0;
   if (src->learned < 0) dst->learned = 0;
   if (src->probe_slice > 0) dst->probe_slice = src->probe_slice;
+  if (src->cut_live > 0) dst->cut_live = src->cut_live;
+  if (src->cut_live < 0) dst->cut_live = 0;
 }
 
 static int tuning_valid(const e13_tuning *t) {
@@ -230,6 +232,7 @@ static int tuning_valid(const e13_tuning *t) {
   if (t->sched_every < 1 || t->donate_max < 1 || t->idle_naps < 1 || t->mail_every < 1) return 0;
   if (t->trace_events < 0 || t->trace_events > 1 << 16) return 0;
   if (t->graph_chain < 0 || t->graph_chain > 4096) return 0;
+  if (t->cut_live < 0 || t->cut_live > 64) return 0;
   return 1;
 }
 
@@ -1124,6 +1127,7 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
   a.mail_every = tu->mail_every;
   a.solo = tu->solo_steps;
   a.evict_after = tu->evict_after;
+  a.cut_live = tu->cut_live;
   a.pool_min_idle = 16;
   a.total_waves = total_waves;
   a.surv_cap = (uint32_t)c->surv_cap;

@@ -6,7 +6,7 @@
 // LDS, per wave:
 //   rows  16 x 64 words at a 4096-byte aligned base: the word of state s for
 //         lane l holds the two rule entries (symbol 0 in bits 0..15, symbol 1 in
-//         bits 16..31).  An entry is  write | dir << 1 | next << 8, or 0x0F00 |
+//         bits 16..31).  An entry is  write | dir << 1 | next << 8, or 0x8F00 |
 //         kind << 12 for a cell that cannot be stepped (kind 1 halt, 2 unread
 //         editable cell, 3 free cell not chosen yet).  Because the row base has
 //         no bits in 0x0F00, the LDS address of the next row is
@@ -57,12 +57,13 @@ __device__ __forceinline__ uint32_t lds_base(const void *p) {
 
 // The 16-bit rule entry of the cell read on symbol `sym`: write | dir << 1 |
 // (write - sym) << 2 (two-bit signed: the ones-count change of the step, whose
-// low bit is the tape flip) | next << 8; or 0x0F00 | kind << 12 for a halt,
-// an unread editable cell or a free cell.
+// low bit is the tape flip) | next << 8; or 0x8F00 | kind << 12 for a halt,
+// an unread editable cell or a free cell (bit 15: a cell that cannot be
+// stepped, so the step loop finds it in the sign of entry << 16).
 __device__ __forceinline__ uint32_t enc_entry(int c, uint32_t sym) {
   if (c < 60) return (uint32_t)(c & 3) | ((((uint32_t)(c & 1) - sym) & 3u) << 2) | ((uint32_t)(c >> 2) << 8);
   uint32_t kind = c == E13_C_HALT ? 1u : (c == E13_C_EDIT ? 2u : 3u);
-  return 0x0F00u | (kind << 12);
+  return 0x8F00u | (kind << 12);
 }
 
 __device__ __forceinline__ int dec_entry(uint32_t e) {
@@ -91,10 +92,12 @@ struct Ctx {
   const uint64_t *key;
   const uint64_t *tgt;
   const int32_t *dl;
+  int dlb;            // >= 0: LDS byte address of the stage deadlines dl[0..9] (used instead of dl)
 };
 
 __device__ __forceinline__ int lane_deadline(const Lane &L, const Ctx &c, int n) {
-  return L.dlcap >= 0 ? L.dlcap : c.dl[n];
+  if (L.dlcap >= 0) return L.dlcap;
+  return c.dlb >= 0 ? (int)lds_ld(c.sb, c.dlb + 4 * n) : c.dl[n];
 }
 
 // Gate registers for target stage nt (after a load or an accepted stage).
@@ -227,39 +230,54 @@ __device__ __forceinline__ int step_once(Lane &L, const Ctx &c, int &evcell) {
 
 // Up to `slice` steps.  Returns an event; for EV_STOP the cell is in evcell.
 //
-// The loop body has a single exit test, made on the state before the step
+// Every step has a single exit test, made on the state before the step
 // changes anything: the head left the window on the previous step, the stage
 // gate may fire (ones count and stage-1 state/head prefilter, or the block
 // hash), the deadline/cap is due, or the cell read is unbound or a halt.  Any
 // of them ends the loop and step_once settles the step exactly; everything
-// else runs branch-free.  A lane that ran the whole round has t = its start
-// count + slice; one that left early has fewer.
-template <int GATE>
-__device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int &evcell) {
+// else runs branch-free.  The loop is tested at its bottom: a step's reads and
+// exit test are made at the end of the previous iteration, so the loop is one
+// block with one exit (no separate masked body per iteration).  A lane that
+// ran the whole round has t = its start count + slice; one that left early
+// has fewer.
+//
+// CUT: the round also ends, for every lane still running, once fewer than
+// min_live lanes of the wave are still in the loop; such a lane makes its due
+// step in step_once (the exact slow path: EV_NONE unless that step ends the
+// machine) and goes on next round, while the wave refills its idle lanes.
+template <int GATE, bool CUT>
+__device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int min_live, int &evcell) {
   const int t_end = L.t + slice;
   const int t_lim = min(L.tstop, t_end); // the round's end or the deadline/cap test, whichever comes first
   const bool nt1 = L.nt == 1;
-  int t = L.t;
+  // steps left before the deadline/cap test or the round's end: the loop runs
+  // while left >= 0 (so t = t_lim - 1 - left)
+  int left = t_lim - 1 - L.t;
   // tape word address (q >> 5) * 256 + tbl as one shift-add and one and-or
   const uint32_t tbl_lo = (uint32_t)c.tbl & 0xFFu, tbl_hi = (uint32_t)c.tbl - tbl_lo;
-  for (;;) {
-    // q may be one step outside the window here (q = -1 or 1362, caught by the
-    // exit test below): the address is then word -1 or 42, still inside this
-    // lane's tape area
-    const int wa = (int)(((((uint32_t)L.q << 3) + tbl_hi) & 0xFFFFFF00u) | tbl_lo);
-    uint32_t word = lds_ld(c.sb, wa);
-    const uint32_t pair = lds_ld(c.sb, L.row);
-    // bit extract and shift use the low 5 bits of q (the hardware masks them)
-    const uint32_t sym = __builtin_amdgcn_ubfe(word, (uint32_t)L.q, 1u);
-    const uint32_t e = pair >> (sym << 4);
-    const uint32_t nx = e & 0x0F00u;
+  // The exit test in one sign bit: (e << 16) is negative for a cell that
+  // cannot be stepped (halt, unread), q | (E13_QMAX - q) for a head outside
+  // the window, left for the deadline/cap or round end; the stage gate is
+  // tested beside it.
+  // q may be one step outside the window here (q = -1 or 1362): the address
+  // is then word -1 or 42, inside the wave's LDS area, and the word read is
+  // not used.
+  int wa = (int)(((((uint32_t)L.q << 3) + tbl_hi) & 0xFFFFFF00u) | tbl_lo);
+  uint32_t word = lds_ld(c.sb, wa);
+  uint32_t pair = lds_ld(c.sb, L.row);
+  // bit extract and shift use the low 5 bits of q (the hardware masks them)
+  uint32_t e = pair >> (__builtin_amdgcn_ubfe(word, (uint32_t)L.q, 1u) << 4);
+  bool stop;
+  {
     bool gate;
     if (GATE == GATE_COUNT)
       gate = (L.ones == L.target) & (nt1 | ((L.row == L.row0) & ((L.q == L.qA) | (L.q == L.qR))));
     else
       gate = L.hb == L.tgt;
-    const bool out = (unsigned)L.q > (unsigned)E13_QMAX;
-    if (out | gate | (t >= t_lim) | (nx == 0x0F00u)) break;
+    stop = (int)((e << 16) | (uint32_t)L.q | (uint32_t)(E13_QMAX - L.q) | (uint32_t)left) < 0 || gate;
+  }
+  while (!stop) {
+    // the step
     const uint32_t flip = __builtin_amdgcn_ubfe(e, 2u, 1u); // the write changes the cell
     word ^= flip << ((uint32_t)L.q & 31u);
     lds_st(c.sb, wa, word);
@@ -270,9 +288,24 @@ __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int &
       L.hb ^= kb & (0ull - (uint64_t)flip);
     }
     L.q += (int)(e & 2u) - 1;
-    L.row = (int)(nx | (uint32_t)c.rbl);
-    ++t;
+    L.row = (int)((e & 0x0F00u) | (uint32_t)c.rbl);
+    --left;
+    // the next step's reads and exit test
+    wa = (int)(((((uint32_t)L.q << 3) + tbl_hi) & 0xFFFFFF00u) | tbl_lo);
+    word = lds_ld(c.sb, wa);
+    pair = lds_ld(c.sb, L.row);
+    e = pair >> (__builtin_amdgcn_ubfe(word, (uint32_t)L.q, 1u) << 4);
+    bool gate;
+    if (GATE == GATE_COUNT)
+      gate = (L.ones == L.target) & (nt1 | ((L.row == L.row0) & ((L.q == L.qA) | (L.q == L.qR))));
+    else
+      gate = L.hb == L.tgt;
+    stop = (int)((e << 16) | (uint32_t)L.q | (uint32_t)(E13_QMAX - L.q) | (uint32_t)left) < 0 || gate;
+    // with too few lanes left in the loop (exec is wave-uniform), every lane
+    // leaves it; a lane whose step was due makes it in step_once below
+    if (CUT) stop |= __popcll(__builtin_amdgcn_read_exec()) < min_live;
   }
+  const int t = t_lim - 1 - left;
   L.t = t;
   if (t == t_end) return EV_NONE; // a full round (a deadline due exactly now is tested next round)
   if ((unsigned)L.q > (unsigned)E13_QMAX) { // the last step left the window: it does not count
@@ -633,6 +666,7 @@ __global__ void __launch_bounds__(256) k_expand(E13ExpandArgs a) {
   c.key = a.key;
   c.tgt = a.tgt;
   c.dl = a.pb->dl;
+  c.dlb = -1;
   if (lds_base(smem) != 0u) { // the absolute LDS addressing needs the dynamic area at address 0
     if (threadIdx.x == 0) atomicOr(&a.ctr->overflow, 4u);
     return;
@@ -772,7 +806,7 @@ __global__ void __launch_bounds__(256) k_expand(E13ExpandArgs a) {
     ts = tn;
     // ---- a round of steps ----
     int ev = EV_NONE, evcell = -1;
-    if (active) ev = run_slice<GATE>(L, c, a.slice, evcell);
+    if (active) ev = run_slice<GATE, false>(L, c, a.slice, 0, evcell);
     tn = stamp();
     cyc1 += tn - ts;
     ts = tn;
@@ -885,6 +919,7 @@ __global__ void __launch_bounds__(256) k_batch(E13BatchArgs a) {
   c.key = a.key;
   c.tgt = a.tgt;
   c.dl = a.dl;
+  c.dlb = -1;
   if (lds_base(smem) != 0u) { // the absolute LDS addressing needs the dynamic area at address 0
     if (threadIdx.x == 0) atomicOr(&a.ctr->overflow, 4u);
     return;
@@ -961,7 +996,7 @@ __global__ void __launch_bounds__(256) k_batch(E13BatchArgs a) {
       ev = run_solo(L, c, exhausted ? a.solo : a.slice, evcell, __ffsll((long long)act) - 1);
       if (!active) ev = EV_NONE;
     } else if (active) {
-      ev = run_slice<GATE>(L, c, a.slice, evcell);
+      ev = run_slice<GATE, false>(L, c, a.slice, 0, evcell);
     }
     if (active && ev != EV_NONE) {
       active = 0;
@@ -1045,6 +1080,24 @@ namespace {
 
 #define STG_WORDS 112 // staged node (76) + rows as stored (16) + rows with edits exhausted (16) + 4 spare
 
+// Per-workgroup copy of the problems in LDS (read by the refill, which would
+// otherwise wait on global loads): problem k at word k * PT_WORDS holds its 32
+// base codes as bytes (words 0..7), then nq, nslot, sym_lo, sym_hi; the stage
+// deadlines dl[0..9] (the same for every problem) follow the last problem.
+#define PT_WORDS 16
+#define PT_DL_WORDS 16
+__device__ __forceinline__ int pt_word(int ptb, int k, int w) {
+  return (int)*(const lds_word_t *)(size_t)(uint32_t)(ptb + (k * PT_WORDS + w) * 4);
+}
+__device__ __forceinline__ int pt_base(int ptb, int k, int i) {
+  return (int)*(const __attribute__((address_space(3))) int8_t *)(size_t)(uint32_t)(ptb + k * PT_WORDS * 4 + i);
+}
+// The entry of a move code (0..59) read on symbol sym (enc_entry without the
+// halt / unread kinds).
+__device__ __forceinline__ uint32_t enc_move(int c, uint32_t sym) {
+  return (uint32_t)(c & 3) | ((((uint32_t)(c & 1) - sym) & 3u) << 2) | ((uint32_t)(c >> 2) << 8);
+}
+
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -1090,19 +1143,19 @@ __device__ void broadcast_done(const E13SearchArgs &a, int lane) {
 
 // Encode both row sets of the staged node: A as stored, B with unread
 // editable cells bound to their base values (a child that spends the last edit).
-__device__ void stage_rows(uint32_t *stg, const E13Problem *pbs, int lane) {
+__device__ void stage_rows(uint32_t *stg, int ptb, int lane) {
   wave_lds_sync();
   const int prob = uni((int)stg[12]);
-  const E13Problem *pb = pbs + prob;
-  const int nq = pb->nq;
+  const int nq = pt_word(ptb, prob, 8);
   if (lane < 16) {
     uint32_t A = 0, B = 0;
     if (lane < nq) {
       const uint8_t *tab = (const uint8_t *)(stg + E13_NODE_W_TAB);
       int c0 = tab[2 * lane], c1 = tab[2 * lane + 1];
       A = enc_entry(c0, 0u) | (enc_entry(c1, 1u) << 16);
-      if (c0 == E13_C_EDIT) c0 = pb->base[2 * lane] < 0 ? E13_C_HALT : pb->base[2 * lane];
-      if (c1 == E13_C_EDIT) c1 = pb->base[2 * lane + 1] < 0 ? E13_C_HALT : pb->base[2 * lane + 1];
+      const int b0 = pt_base(ptb, prob, 2 * lane), b1 = pt_base(ptb, prob, 2 * lane + 1);
+      if (c0 == E13_C_EDIT) c0 = b0 < 0 ? E13_C_HALT : b0;
+      if (c1 == E13_C_EDIT) c1 = b1 < 0 ? E13_C_HALT : b1;
       B = enc_entry(c0, 0u) | (enc_entry(c1, 1u) << 16);
     }
     stg[76 + lane] = A;
@@ -1112,12 +1165,12 @@ __device__ void stage_rows(uint32_t *stg, const E13Problem *pbs, int lane) {
 }
 
 // Stage node `nd` (76 words) from global memory into LDS words stg[0..75].
-__device__ void stage_node(uint32_t *stg, const uint32_t *nd, const E13Problem *pbs, int lane) {
+__device__ void stage_node(uint32_t *stg, const uint32_t *nd, int ptb, int lane) {
   const uint32_t v0 = nd[lane];
   const uint32_t v1 = lane < E13_NODE_WORDS - 64 ? nd[64 + lane] : 0u;
   stg[lane] = v0;
   if (lane < E13_NODE_WORDS - 64) stg[64 + lane] = v1;
-  stage_rows(stg, pbs, lane);
+  stage_rows(stg, ptb, lane);
 }
 
 // This lane's machine as a node record in LDS (the staging area): the same
@@ -1229,6 +1282,22 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
   uint32_t *stg = (uint32_t *)(c.sb + after_keys) + wave * STG_WORDS;
   unsigned long long *pc = (unsigned long long *)(c.sb + after_keys + nwaves * STG_WORDS * 4); // np x 16 per WG
   for (int i = threadIdx.x; i < a.np * E13_PCNT; i += blockDim.x) pc[i] = 0ull;
+  // the problems and the stage deadlines, in LDS
+  const int ptb = after_keys + nwaves * STG_WORDS * 4 + a.np * E13_PCNT * 8;
+  for (int i = threadIdx.x; i < a.np * PT_WORDS; i += blockDim.x) {
+    const int k = i / PT_WORDS, w = i - k * PT_WORDS;
+    const E13Problem *pb = a.pbs + k;
+    uint32_t v = 0;
+    if (w < 8) v = ((const uint32_t *)pb->base)[w];
+    else if (w == 8) v = (uint32_t)pb->nq;
+    else if (w == 9) v = (uint32_t)pb->nslot;
+    else if (w == 10) v = (uint32_t)pb->sym_lo;
+    else if (w == 11) v = (uint32_t)pb->sym_hi;
+    lds_st(c.sb, ptb + i * 4, v);
+  }
+  c.dlb = ptb + a.np * PT_WORDS * 4;
+  for (int i = threadIdx.x; i < PT_DL_WORDS; i += blockDim.x)
+    lds_st(c.sb, c.dlb + i * 4, i <= E13_NS ? (uint32_t)a.pbs[0].dl[i] : 0u);
   if (GATE == GATE_HASH) fill_keyb(c.sb, c.keyb, a.key);
   __syncthreads();
 
@@ -1276,100 +1345,114 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
         if (staged >= bot && staged < top && lane == 0)
           stk_node(a, wid, staged)->next_slot = slot_range(cur_slot, cur_end);
         wave_mem_sync();
-        stage_node(stg, (const uint32_t *)stk_node(a, wid, tidx), a.pbs, lane);
+        stage_node(stg, (const uint32_t *)stk_node(a, wid, tidx), ptb, lane);
         staged = tidx;
         const uint32_t sw = (uint32_t)uni((int)stg[15]);
         cur_slot = (int)(sw & 0xFFFFu);
         cur_end = (int)(sw >> 16);
-        if (cur_end == 0) cur_end = a.pbs[uni((int)stg[12])].nslot;
+        if (cur_end == 0) cur_end = pt_word(ptb, uni((int)stg[12]), 9);
       }
       const int sprob = uni((int)stg[12]);
-      const E13Problem *pb = a.pbs + sprob;
       const int cnt = __popcll(need);
       const int give = min(cnt, cur_end - cur_slot);
       if (!active) {
         const int r = (int)rank_in(need);
         if (r < give) {
+          // child slot -> the value bound to the pending cell (explore(),
+          // e13search.c:236-251): slot 0 keeps the base value, slot 1+v binds
+          // move v; stage words: 9 budget, 10 symmetric states in use, 11 pending cell
           const int slot = cur_slot + r;
-          const int pend = (int)stg[11];
-          int nb = (int)stg[9], nin2 = (int)stg[10], val = -1, ok = 1;
-          if (pend < 0) {
+          const int pend = uni((int)stg[11]);
+          const int nb0 = uni((int)stg[9]), ni0 = uni((int)stg[10]);
+          int nb = nb0, nin2 = ni0, val = -1, ok;
+          if (pend < 0) { // a machine paused at a cap: the node is its own single child
             ok = slot == 0;
           } else {
-            const int orig = pb->base[pend];
+            const int orig = pt_base(ptb, sprob, pend); // move, -1 halt, -2 free (FORCED)
             if (slot == 0) {
-              if (orig == -2) ok = 0;
-              else if (orig == -1) { // keeping a halt: one leaf at the node's stage, 0 steps
+              ok = orig >= 0;
+              val = orig;
+              if (orig == -1) { // keeping a halt: one leaf at the node's stage, 0 steps
                 atomicAdd(&pc[sprob * E13_PCNT + 0], 1ull);
                 atomicAdd(&pc[sprob * E13_PCNT + 3 + (int)stg[4]], 1ull);
                 const uint32_t sd = stg[E13_NODE_W_SEED];
                 if (sd < a.nseed_cnt) atomicAdd((unsigned long long *)&a.seed_cnt[2 * sd], 1ull);
-                ok = 0;
-              } else val = orig;
+              }
             } else {
               const int v = slot - 1;
-              if (!(orig == -2 || nb > 0) || v == orig) ok = 0;
-              else {
-                const int n2 = sym_next(v, nin2, pb->sym_lo, pb->sym_hi);
-                if (n2 < 0) ok = 0;
-                else {
-                  if (orig != -2) nb -= 1;
-                  nin2 = n2;
-                  val = v;
-                }
-              }
+              const int n2 = sym_next(v, ni0, pt_word(ptb, sprob, 10), pt_word(ptb, sprob, 11));
+              ok = (orig == -2 || nb0 > 0) && v != orig && n2 >= 0;
+              nb = nb0 - (orig != -2);
+              nin2 = n2;
+              val = v;
             }
           }
           if (ok) {
             prob = sprob;
             seed = stg[E13_NODE_W_SEED];
-            nq = pb->nq;
-            c.dl = pb->dl;
-            L.q = (int)stg[0];
-            L.row = c.rbl + (int)stg[1] * 256;
-            L.t = (int)stg[2];
-            L.nt = (int)stg[3];
-            L.stage = (int)stg[4];
-            L.st0 = (int)stg[5];
-            L.q0 = (int)stg[6];
-            L.flags = (int)stg[7];
-            L.ones = (int)stg[8];
-            L.hb = (uint64_t)stg[16] | ((uint64_t)stg[17] << 32);
-            L.ch0 = (uint64_t)stg[18] | ((uint64_t)stg[19] << 32);
-            L.ch1 = (uint64_t)stg[20] | ((uint64_t)stg[21] << 32);
-            L.ch2 = (uint64_t)stg[22] | ((uint64_t)stg[23] << 32);
+            nq = pt_word(ptb, sprob, 8);
+            // the machine state, words 0..23 of the staged node
+            const uint4 s0 = *(const uint4 *)(stg + 0), s1 = *(const uint4 *)(stg + 4);
+            const uint4 s2 = *(const uint4 *)(stg + 8), s3 = *(const uint4 *)(stg + 12);
+            const uint4 s4 = *(const uint4 *)(stg + 16), s5 = *(const uint4 *)(stg + 20);
+            L.q = (int)s0.x;
+            L.row = c.rbl + (int)s0.y * 256;
+            L.t = (int)s0.z;
+            L.nt = (int)s0.w;
+            L.stage = (int)s1.x;
+            L.st0 = (int)s1.y;
+            L.q0 = (int)s1.z;
+            L.flags = (int)s1.w;
+            L.ones = (int)s2.x;
+            L.hb = (uint64_t)s4.x | ((uint64_t)s4.y << 32);
+            L.ch0 = (uint64_t)s4.z | ((uint64_t)s4.w << 32);
+            L.ch1 = (uint64_t)s5.x | ((uint64_t)s5.y << 32);
+            L.ch2 = (uint64_t)s5.z | ((uint64_t)s5.w << 32);
             arm_stage(L, c);
             t0 = L.t;
             budget = nb;
             nintro = nin2;
-            depth = (int)stg[13] + (pend >= 0);
-            nbound = (int)stg[14] + (pend >= 0);
-            const int rs = (nb == 0 && (int)stg[9] > 0) ? 92 : 76; // edits used up: unread cells keep their value
-            // rows and tape from the stage into this lane's columns: each chunk
-            // of stage words is read into registers before any is written, so
-            // the reads are not serialised behind the writes (the compiler
-            // cannot tell the stage from the lane columns)
-            uint32_t rw[16];
+            depth = (int)s3.y + (pend >= 0);
+            nbound = (int)s3.z + (pend >= 0);
+            // rows: as stored (A), or with edits used up (B: unread cells keep
+            // their value), then the pending cell's entry; every stage word is
+            // read before any lane column is written (the compiler cannot
+            // tell the stage from the columns and would serialise them)
+            const uint4 *rsrc = (const uint4 *)(stg + ((nb == 0 && nb0 > 0) ? 92 : 76));
+            uint4 rw[4];
 #pragma unroll
-            for (int r2 = 0; r2 < 16; ++r2) rw[r2] = stg[rs + r2];
+            for (int k4 = 0; k4 < 4; ++k4) rw[k4] = rsrc[k4];
+            uint32_t pr = 0;
+            if (pend >= 0) pr = ((const uint32_t *)rsrc)[pend >> 1];
 #pragma unroll
-            for (int r2 = 0; r2 < 16; ++r2) {
-              uint32_t pr = rw[r2];
-              if (pend >> 1 == r2) {
-                const uint32_t e = enc_entry(val, (uint32_t)pend & 1u);
-                pr = (pend & 1) ? ((pr & 0xFFFFu) | (e << 16)) : ((pr & 0xFFFF0000u) | e);
-              }
-              if (r2 < nq) lds_st(c.sb, c.rbl + r2 * 256, pr);
+            for (int k4 = 0; k4 < 4; ++k4) {
+              lds_st(c.sb, c.rbl + (4 * k4 + 0) * 256, rw[k4].x);
+              lds_st(c.sb, c.rbl + (4 * k4 + 1) * 256, rw[k4].y);
+              lds_st(c.sb, c.rbl + (4 * k4 + 2) * 256, rw[k4].z);
+              lds_st(c.sb, c.rbl + (4 * k4 + 3) * 256, rw[k4].w);
             }
+            if (pend >= 0) { // val is a move here (a base value or one of 0..4nq-1)
+              const uint32_t sh = ((uint32_t)pend & 1u) << 4;
+              const uint32_t e = enc_move(val, (uint32_t)pend & 1u);
+              lds_st(c.sb, c.rbl + (pend >> 1) * 256, (pr & ~(0xFFFFu << sh)) | (e << sh));
+            }
+            // tape: words 32..74 of the stage (word 75, the seed, comes along)
 #pragma unroll
-            for (int w0 = 0; w0 < E13_TW; w0 += 11) {
-              uint32_t tw[11];
+            for (int k4 = 0; k4 < 11; k4 += 4) {
+              uint4 tw[4];
 #pragma unroll
-              for (int j = 0; j < 11; ++j)
-                if (w0 + j < E13_TW) tw[j] = stg[E13_NODE_W_TAPE + w0 + j];
+              for (int j = 0; j < 4; ++j)
+                if (k4 + j < 11) tw[j] = *(const uint4 *)(stg + E13_NODE_W_TAPE + 4 * (k4 + j));
 #pragma unroll
-              for (int j = 0; j < 11; ++j)
-                if (w0 + j < E13_TW) lds_st(c.sb, c.tbl + (w0 + j) * 256, tw[j]);
+              for (int j = 0; j < 4; ++j) {
+                const int w = 4 * (k4 + j);
+                if (k4 + j < 11) {
+                  lds_st(c.sb, c.tbl + (w + 0) * 256, tw[j].x);
+                  lds_st(c.sb, c.tbl + (w + 1) * 256, tw[j].y);
+                  lds_st(c.sb, c.tbl + (w + 2) * 256, tw[j].z);
+                  if (w + 3 < E13_TW) lds_st(c.sb, c.tbl + (w + 3) * 256, tw[j].w);
+                }
+              }
             }
             active = 1;
           }
@@ -1508,7 +1591,9 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
       ev = run_solo(L, c, a.solo, evcell, __ffsll((long long)act) - 1);
       if (!active) ev = EV_NONE;
     } else if (active) {
-      ev = run_slice<GATE>(L, c, a.slice, evcell);
+      // with nodes to hand out, the round may end early once most lanes have stopped
+      if (a.cut_live > 1 && top > bot) ev = run_slice<GATE, true>(L, c, a.slice, a.cut_live, evcell);
+      else ev = run_slice<GATE, false>(L, c, a.slice, 0, evcell);
     }
     tn = stamp();
     cyc1 += tn - ts;
@@ -1583,11 +1668,11 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
         if (r == n - 1)
           stage_own(c, stg, L, budget, nintro, ev == EV_STOP ? evcell : -1, prob, depth, nbound, nq, seed);
       }
-      stage_rows(stg, a.pbs, lane);
+      stage_rows(stg, ptb, lane);
       top += n;
       staged = top - 1;
       cur_slot = 0;
-      cur_end = a.pbs[uni((int)stg[12])].nslot;
+      cur_end = pt_word(ptb, uni((int)stg[12]), 9);
     }
     if (fin) active = 0;
     if (requested && top > bot) { // work of its own again: withdraw the request unless a donor took it
@@ -1900,6 +1985,7 @@ __global__ void __launch_bounds__(256) k_probe(E13ProbeArgs a) {
   c.key = a.key;
   c.tgt = a.tgt;
   c.dl = a.pb->dl;
+  c.dlb = -1;
   if (lds_base(smem) != 0u) { // the absolute LDS addressing needs the dynamic area at address 0
     if (threadIdx.x == 0) atomicOr(&a.ctr->overflow, 4u);
     return;
@@ -2104,7 +2190,7 @@ __global__ void __launch_bounds__(256) k_probe(E13ProbeArgs a) {
       ev = run_solo(L, c, a.solo, evcell, __ffsll((long long)act) - 1);
       if (!active) ev = EV_NONE;
     } else if (active) {
-      ev = run_slice<GATE>(L, c, a.slice, evcell);
+      ev = run_slice<GATE, false>(L, c, a.slice, 0, evcell);
     }
     // ---- finished runs ----
     if (active && ev != EV_NONE) {
@@ -2207,7 +2293,7 @@ int e13k_launch_chain_push(E13Chain *ch, E13Node *stack, uint64_t stack_cap, con
 size_t e13k_search_lds_bytes(int wg_threads, int gate, int np) {
   const int nw = wg_threads / WAVE;
   return (size_t)nw * (ROWS_BYTES + TAPE_BYTES + STG_WORDS * 4) + (gate == GATE_HASH ? KEYB_WORDS * 8 : 0) +
-         (size_t)np * E13_PCNT * 8;
+         (size_t)np * E13_PCNT * 8 + (size_t)np * PT_WORDS * 4 + PT_DL_WORDS * 4;
 }
 
 // Workgroups of k_search that fit on one CU at once (registers, LDS, wave

@@ -23,9 +23,12 @@
 
 #include "e13_ctx.h"
 
-#define NKNOB 10
+#define NKNOB 11
 static const char *knob_name[NKNOB] = {"wgs_per_cu", "slice",       "refill_tries", "solo_steps", "seed_per_wave",
-                                       "donate_max", "sched_every", "wg_threads",   "idle_naps",  "evict_after"};
+                                       "donate_max", "sched_every", "wg_threads",   "idle_naps",  "evict_after",
+                                       "cut_live"};
+/* knobs whose "off" reads back as 0 and is set (and stored) as -1 */
+static int off_knob(int k) { return k == 3 || k == 4 || k == 9 || k == 10; }
 
 static int32_t *knob(e13_tuning *t, int k) {
   switch (k) {
@@ -38,14 +41,16 @@ static int32_t *knob(e13_tuning *t, int k) {
   case 6: return &t->sched_every;
   case 7: return &t->wg_threads;
   case 8: return &t->idle_naps;
-  default: return &t->evict_after;
+  case 9: return &t->evict_after;
+  default: return &t->cut_live;
   }
 }
 
 /* Candidate values (-1: off / none, as tuning_merge reads it). */
 static const int32_t cand[NKNOB][6] = {
     {1, 2, 3, 4, 0},  {16, 32, 64, 128, 0}, {1, 2, 4, 0, 0},     {-1, 1024, 4096, 16384, 65536, 0}, {-1, 2, 8, 0, 0},
-    {2, 4, 8, 16, 0}, {8, 32, 128, 0, 0},   {128, 256, 0, 0, 0}, {1, 4, 16, 0, 0},                  {-1, 4096, 16384, 65536, 0}};
+    {2, 4, 8, 16, 0}, {8, 32, 128, 0, 0},   {128, 256, 0, 0, 0}, {1, 4, 16, 0, 0},                  {-1, 4096, 16384, 65536, 0},
+    {-1, 16, 32, 48, 0}};
 
 static int same_counts(const e13_counts *a, const e13_counts *b, int np) {
   for (int k = 0; k < np; k++) {
@@ -69,9 +74,8 @@ static double now_ms(void) {
 static int trial(e13_ctx *c, const e13_tuning *t, const e13_problem *ps, int np, int reps, e13_counts *ref,
                  int *have_ref, double *ms) {
   e13_tuning t2 = *t; /* "off" is 0 in a read-back tuning, -1 when set */
-  if (t2.solo_steps == 0) t2.solo_steps = -1;
-  if (t2.seed_per_wave == 0) t2.seed_per_wave = -1;
-  if (t2.evict_after == 0) t2.evict_after = -1;
+  for (int k = 0; k < NKNOB; k++)
+    if (off_knob(k) && *knob(&t2, k) == 0) *knob(&t2, k) = -1;
   int rc = e13_set_tuning(c, &t2);
   if (rc) return rc;
   e13_counts cnt[E13_MAXPROB];
@@ -140,9 +144,8 @@ int e13_autotune(e13_ctx *c, const e13_problem *ps, int np, int reps, e13_tuning
     }
     if (!changed) break;
   }
-  if (cur.solo_steps == 0) cur.solo_steps = -1;
-  if (cur.seed_per_wave == 0) cur.seed_per_wave = -1;
-  if (cur.evict_after == 0) cur.evict_after = -1;
+  for (int k = 0; k < NKNOB; k++)
+    if (off_knob(k) && *knob(&cur, k) == 0) *knob(&cur, k) = -1;
   if ((rc = e13_set_tuning(c, &cur))) goto fail;
   e13_get_tuning(c, &cur);
   if (best) *best = cur;
@@ -166,7 +169,7 @@ int e13_save_tuning(const e13_tuning *t, const char *path) {
   fprintf(f, "# e13 tuning (autotuned): name value\n");
   for (int k = 0; k < NKNOB; k++) {
     int32_t v = *knob(&tmp, k);
-    if (k == 3 || k == 4 || k == 9) v = v > 0 ? v : -1; /* off is written as -1, which tuning_merge reads as off */
+    if (off_knob(k)) v = v > 0 ? v : -1; /* off is written as -1, which tuning_merge reads as off */
     fprintf(f, "%s %d\n", knob_name[k], (int)v);
   }
   fprintf(f, "gate %d\n", (int)t->gate);

@@ -114,6 +114,9 @@ typedef struct {
                              this many steps from the blank tape, and a wave whose only running machine is
                              one keeps it alone (register loop); negative: off */
   int32_t probe_slice;    /* probe estimator (e13_estimate): steps per round between hand-outs of child runs */
+  int32_t cut_live;       /* persistent form: > 1: a round of the multi-lane loop ends once fewer than this many
+                             of the wave's lanes still run while the wave has nodes to hand out, so the idle lanes
+                             are refilled sooner (the running ones go on next round); negative: off */
 } e13_tuning;
 
 typedef struct {

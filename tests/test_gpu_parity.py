@@ -237,7 +237,10 @@ def test_counter_3state_symmetric(e13, gpu_ctx, base):
                                   # more workgroups per CU than can be resident, with a frontier
                                   # dealt to every wave: the grid is clamped to the resident ones
                                   dict(form=3, wgs_per_cu=4, seed_per_wave=8),
-                                  dict(form=3, wg_threads=64, wgs_per_cu=1, stack_nodes=1 << 16)])
+                                  dict(form=3, wg_threads=64, wgs_per_cu=1, stack_nodes=1 << 16),
+                                  # rounds cut short once few lanes still run (the rest refilled)
+                                  dict(form=3, cut_live=32), dict(form=3, cut_live=64, slice=256),
+                                  dict(form=3, cut_live=2, slice=7, solo_steps=-1)])
 def test_results_invariant_under_tuning(e13, gpu_ctx, base, tune):
     say(f"search: delete FMAX=2 X=7,8,12 tuning={tune}")
     gpu_ctx.set_tuning(**tune)
