@@ -100,7 +100,8 @@ class Tuning(ctypes.Structure):
                 ("idle_naps", ctypes.c_int32), ("mail_every", ctypes.c_int32), ("trace_events", ctypes.c_int32),
                 ("graph_chain", ctypes.c_int32), ("cost_log", ctypes.c_int32),
                 ("learned", ctypes.c_int32), ("solo_steps", ctypes.c_int32), ("evict_after", ctypes.c_int32),
-                ("probe_slice", ctypes.c_int32), ("cut_live", ctypes.c_int32)]
+                ("probe_slice", ctypes.c_int32), ("cut_live", ctypes.c_int32),
+                ("phase_cycles", ctypes.c_int32), ("nt1_skip", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -441,7 +442,7 @@ class Context:
 
     # fields where 0 reads back as "off" but a set of 0 means "keep"
     _OFF_FIELDS = ("seed_per_wave", "trace_events", "graph_chain", "cost_log", "learned", "solo_steps", "evict_after",
-                   "cut_live")
+                   "cut_live", "phase_cycles")
 
     def restore_tuning(self, t):
         """Set every field of a tuning dict read back by tuning() (off fields included)."""

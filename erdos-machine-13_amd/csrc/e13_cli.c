@@ -31,7 +31,7 @@
  * E13_GATE (1 count, 2 hash), E13_WG, E13_WGS_PER_CU, E13_DEVICE,
  * E13_REFILL_TRIES, E13_SEED_PER_WAVE, E13_SCHED_EVERY, E13_DONATE_MAX,
  * E13_IDLE_NAPS, E13_MAIL_EVERY, E13_GRAPH_CHAIN, E13_SOLO_STEPS, E13_EVICT_AFTER, E13_CUT_LIVE (negative: off),
- * E13_PROBE_SLICE;
+ * E13_PROBE_SLICE, E13_NT1_SKIP (1 on, 2 off), E13_PHASE_CYCLES=1 (the persistent search's per-phase wave cycles);
  * E13_TUNING_FILE (stored settings; <name>_small.<ext> beside it for small searches); E13_TRACE=N with
  * E13_TRACE_FILE=path writes the persistent search's wave event log (N events per wave) to path; the
  * library prints its host phase times with E13_TIMING=1 and a line per autotuner trial with E13_TUNE_LOG=1.
@@ -471,6 +471,8 @@ int main(int argc, char **argv) {
   tu.solo_steps = env_int("E13_SOLO_STEPS");
   tu.evict_after = env_int("E13_EVICT_AFTER");
   tu.cut_live = env_int("E13_CUT_LIVE");
+  tu.phase_cycles = env_int("E13_PHASE_CYCLES");
+  tu.nt1_skip = env_int("E13_NT1_SKIP");
   tu.probe_slice = env_int("E13_PROBE_SLICE");
   e13_ctx *ctx = NULL;
   int rc = e13_open(env_int("E13_DEVICE"), &tu, &ctx);

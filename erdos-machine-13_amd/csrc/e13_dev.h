@@ -151,7 +151,7 @@ typedef struct E13Mail {
 #define E13_PCNT 16       /* per-problem counters: leaves, steps, survivors, hist[0..9] */
 
 typedef struct E13SearchArgs {
-  E13Node *stacks;        /* nwaves x stk_cap private stacks */
+  E13Node *stacks;        /* nwaves x stk_cap private stacks (rings; stk_cap a power of two) */
   E13Mail *mail;          /* one mailbox per wave */
   uint64_t *hungry;       /* hungry map: bit w set = wave w asks for a node (hungry_words words) */
   E13Sched *sch;
@@ -172,7 +172,10 @@ typedef struct E13SearchArgs {
   int32_t solo;           /* > 0: steps per round of the scalar loop for a wave's lone running lane */
   int32_t evict_after;    /* > 0: a machine this many steps from the blank tape may be handed to a hungry wave */
   int32_t cut_live;       /* > 1: a round ends once fewer lanes than this still run while the wave has nodes */
-  int32_t pad_cut;
+  int32_t phase_cycles;   /* nonzero: count wave clock cycles per phase (ctr->cyc) */
+  int32_t skip_settled;   /* nonzero: the step loop tests the stage-1 gate only after a step that changed
+                             the ones count (Lane.dm) */
+  int32_t pad_skip;
   uint32_t nseed_cnt;     /* cost logging: seeds with counters (0: off) */
   uint64_t *seed_cnt;     /* cost logging: per seed, leaves and steps of its subtree */
   uint64_t *trace;        /* optional: trace_cap words per wave, word 0 = events logged, then

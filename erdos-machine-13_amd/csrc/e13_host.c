@@ -215,6 +215,10 @@ static void tuning_merge(e13_tuning *dst, const e13_tuning *src) {
   if (src->probe_slice > 0) dst->probe_slice = src->probe_slice;
   if (src->cut_live > 0) dst->cut_live = src->cut_live;
   if (src->cut_live < 0) dst->cut_live = 0;
+  if (src->phase_cycles > 0) dst->phase_cycles = 1;
+  if (src->phase_cycles < 0) dst->phase_cycles = 0;
+  if (src->nt1_skip > 0) dst->nt1_skip = src->nt1_skip;
+  if (src->nt1_skip < 0) dst->nt1_skip = 0;
 }
 
 static int tuning_valid(const e13_tuning *t) {
@@ -233,6 +237,7 @@ static int tuning_valid(const e13_tuning *t) {
   if (t->trace_events < 0 || t->trace_events > 1 << 16) return 0;
   if (t->graph_chain < 0 || t->graph_chain > 4096) return 0;
   if (t->cut_live < 0 || t->cut_live > 64) return 0;
+  if (t->nt1_skip < 0 || t->nt1_skip > 2) return 0;
   return 1;
 }
 
@@ -943,7 +948,11 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
   if (fit < 1) return E13_EINVAL;
   const int grid = c->ncu * (tu->wgs_per_cu < fit ? tu->wgs_per_cu : fit);
   const int total_waves = grid * (wg / 64);
-  const int stk_cap = tu->stack_nodes >= 64 * (int64_t)total_waves ? (int)(tu->stack_nodes / total_waves) : 64;
+  /* each wave's stack is a ring of a power-of-two number of nodes, at least its
+     share of stack_nodes (a mask, not a 64-bit remainder, maps the ring index
+     to a slot) */
+  int stk_cap = 64;
+  while ((int64_t)stk_cap * total_waves < tu->stack_nodes && stk_cap < (1 << 24)) stk_cap *= 2;
   const size_t need_stacks = (size_t)total_waves * (size_t)stk_cap;
   if (total_waves > 64 * 64) return E13_EINVAL; /* the hungry map is read as at most 64 words */
   const int hungry_words = (total_waves + 63) / 64;
@@ -1128,6 +1137,12 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
   a.solo = tu->solo_steps;
   a.evict_after = tu->evict_after;
   a.cut_live = tu->cut_live;
+  a.phase_cycles = tu->phase_cycles;
+  if (tu->nt1_skip > 0) {
+    a.skip_settled = tu->nt1_skip == 1;
+  } else { /* automatic: delete-mode searches, whose edits break the start, have many such machines */
+    for (int k = 0; k < np; k++) a.skip_settled |= ps[k].budget > 0;
+  }
   a.pool_min_idle = 16;
   a.total_waves = total_waves;
   a.surv_cap = (uint32_t)c->surv_cap;

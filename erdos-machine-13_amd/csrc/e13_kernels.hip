@@ -80,6 +80,14 @@ __device__ __forceinline__ int cval(int n) { return (n >= 3) + (n >= 9); } // e1
 struct Lane {
   int q, row, t, tstop, deadline, tcap, dlcap;
   int ones, target, row0, qA, qR;
+  // The change of `ones` made by the last step (0: no cell at q >= 61
+  // flipped), or 1 when not known.  Before stage 1 the gate has no state or
+  // head test, and the block's exactness can change only with `ones`; the
+  // step loop's SKIP form therefore tests the stage-1 gate only after a step
+  // with dm != 0 (the gate of a configuration reached without such a step was
+  // already settled), so a machine whose two ones sit in the wrong cells does
+  // not leave the loop at every step.
+  int dm;
   int nt, stage, st0, q0, flags;
   uint64_t hb, tgt;
   uint64_t ch0, ch1, ch2;
@@ -93,7 +101,9 @@ struct Ctx {
   const uint64_t *tgt;
   const int32_t *dl;
   int dlb;            // >= 0: LDS byte address of the stage deadlines dl[0..9] (used instead of dl)
+  int ckb;            // LDS byte address of the counter keys: KEY[q+1] for q 0..60 (CKEY_BYTES)
 };
+#define CKEY_BYTES 512 // 64 u64 counter keys per workgroup (61 used)
 
 __device__ __forceinline__ int lane_deadline(const Lane &L, const Ctx &c, int n) {
   if (L.dlcap >= 0) return L.dlcap;
@@ -125,22 +135,29 @@ __device__ __forceinline__ bool block_exact(const Ctx &c, int n) {
 }
 
 // hc of the reference: XOR of KEY[cell+64] over the set cells -63..-3
-// (q 0..60, KEY index q+1), e13search.c:170.
+// (q 0..60, KEY index q+1), e13search.c:170.  The 61 keys are in LDS.
+typedef __attribute__((address_space(3))) uint64_t lds_u64_t;
 __device__ __forceinline__ uint64_t counter_hash(const Ctx &c) {
   uint32_t w0 = lds_ld(c.sb, c.tbl);
   uint32_t w1 = lds_ld(c.sb, c.tbl + 256) & 0x1FFFFFFFu;
   uint64_t h = 0;
   while (w0) {
     int b = __builtin_ctz(w0);
-    h ^= c.key[b + 1];
+    h ^= *(const lds_u64_t *)(size_t)(uint32_t)(c.ckb + 8 * b);
     w0 &= w0 - 1;
   }
   while (w1) {
     int b = __builtin_ctz(w1);
-    h ^= c.key[b + 33];
+    h ^= *(const lds_u64_t *)(size_t)(uint32_t)(c.ckb + 8 * (b + 32));
     w1 &= w1 - 1;
   }
   return h;
+}
+
+// The counter keys into LDS (the caller synchronises the workgroup after).
+__device__ __forceinline__ void fill_ckeys(int ckb, const uint64_t *key) {
+  for (int q = threadIdx.x; q < CKEY_BYTES / 8; q += blockDim.x)
+    *(lds_u64_t *)(size_t)(uint32_t)(ckb + 8 * q) = q <= 60 ? key[q + 1] : 0ull;
 }
 
 // The stage-start rule (e13search.c:136-161) at a gate hit.
@@ -216,7 +233,8 @@ __device__ __forceinline__ int step_once(Lane &L, const Ctx &c, int &evcell) {
   word ^= (w ^ sym) << qb;
   lds_st(c.sb, wa, word);
   if (GATE == GATE_COUNT) {
-    L.ones += L.q >= E13_QBLK ? (int)w - (int)sym : 0;
+    L.dm = L.q >= E13_QBLK ? (int)w - (int)sym : 0;
+    L.ones += L.dm;
   } else {
     const uint64_t kb = *(const uint64_t *)(c.sb + c.keyb + L.q * 8);
     L.hb ^= kb & (0ull - (uint64_t)(w ^ sym));
@@ -241,11 +259,13 @@ __device__ __forceinline__ int step_once(Lane &L, const Ctx &c, int &evcell) {
 // ran the whole round has t = its start count + slice; one that left early
 // has fewer.
 //
+// SKIP: the stage-1 gate is tested only after a step that flipped a cell at
+// q >= 61 (Lane.dm).
 // CUT: the round also ends, for every lane still running, once fewer than
 // min_live lanes of the wave are still in the loop; such a lane makes its due
 // step in step_once (the exact slow path: EV_NONE unless that step ends the
 // machine) and goes on next round, while the wave refills its idle lanes.
-template <int GATE, bool CUT>
+template <int GATE, bool CUT, bool SKIP>
 __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int min_live, int &evcell) {
   const int t_end = L.t + slice;
   const int t_lim = min(L.tstop, t_end); // the round's end or the deadline/cap test, whichever comes first
@@ -271,7 +291,8 @@ __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int m
   {
     bool gate;
     if (GATE == GATE_COUNT)
-      gate = (L.ones == L.target) & (nt1 | ((L.row == L.row0) & ((L.q == L.qA) | (L.q == L.qR))));
+      gate = (L.ones == L.target) & ((SKIP ? nt1 & (L.dm != 0) : nt1) |
+                                     ((L.row == L.row0) & ((L.q == L.qA) | (L.q == L.qR))));
     else
       gate = L.hb == L.tgt;
     stop = (int)((e << 16) | (uint32_t)L.q | (uint32_t)(E13_QMAX - L.q) | (uint32_t)left) < 0 || gate;
@@ -282,7 +303,9 @@ __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int m
     word ^= flip << ((uint32_t)L.q & 31u);
     lds_st(c.sb, wa, word);
     if (GATE == GATE_COUNT) {
-      L.ones += L.q >= E13_QBLK ? __builtin_amdgcn_sbfe((int)e, 2u, 2u) : 0; // write - read
+      const int dm = L.q >= E13_QBLK ? __builtin_amdgcn_sbfe((int)e, 2u, 2u) : 0; // write - read
+      L.ones += dm;
+      if (SKIP) L.dm = dm;
     } else {
       const uint64_t kb = *(const uint64_t *)(c.sb + c.keyb + L.q * 8);
       L.hb ^= kb & (0ull - (uint64_t)flip);
@@ -297,7 +320,8 @@ __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int m
     e = pair >> (__builtin_amdgcn_ubfe(word, (uint32_t)L.q, 1u) << 4);
     bool gate;
     if (GATE == GATE_COUNT)
-      gate = (L.ones == L.target) & (nt1 | ((L.row == L.row0) & ((L.q == L.qA) | (L.q == L.qR))));
+      gate = (L.ones == L.target) & ((SKIP ? nt1 & (L.dm != 0) : nt1) |
+                                     ((L.row == L.row0) & ((L.q == L.qA) | (L.q == L.qR))));
     else
       gate = L.hb == L.tgt;
     stop = (int)((e << 16) | (uint32_t)L.q | (uint32_t)(E13_QMAX - L.q) | (uint32_t)left) < 0 || gate;
@@ -553,6 +577,7 @@ __device__ __forceinline__ int run_solo(Lane &L, const Ctx &c, int nsteps_in, in
     L.row = rbl + st * 256;
     L.ones = ones;
     L.t = t_round + r;
+    L.dm = 1; // the loop's steps are not tracked: the gate is tested again
     if (why == 2) ev = EV_DEAD;
     else if (r >= left) ev = step_once<GATE_COUNT>(L, c, evcell); // the deadline or cap is due
   }
@@ -570,6 +595,9 @@ __device__ __forceinline__ long long uni64(long long v) {
 
 // Core-clock time stamp for the per-phase cycle counters.
 __device__ __forceinline__ uint64_t stamp() { return __builtin_readcyclecounter(); }
+// The same when the per-phase cycle counts are asked for, else 0 (a clock read
+// costs its latency on every phase boundary, about 2% of the search).
+__device__ __forceinline__ uint64_t stamp_if(int on) { return on ? __builtin_readcyclecounter() : 0ull; }
 
 // Wave event log of the persistent search (when a.trace is set): lane 0 appends
 // (constant-rate real time << 16 | aux << 8 | event) to the wave's own region.
@@ -675,6 +703,8 @@ __global__ void __launch_bounds__(256) k_expand(E13ExpandArgs a) {
   uint32_t *whist = (uint32_t *)(c.sb + c.keyb + (GATE == GATE_HASH ? KEYB_WORDS * 8 : 0)) + wave * 16;
   if (lane < 16) whist[lane] = 0;
   if (GATE == GATE_HASH) fill_keyb(c.sb, c.keyb, a.key);
+  c.ckb = c.keyb + (GATE == GATE_HASH ? KEYB_WORDS * 8 : 0) + nwaves * 64;
+  fill_ckeys(c.ckb, a.key);
   __syncthreads();
 
   uint32_t nin = a.nin;
@@ -770,6 +800,7 @@ __global__ void __launch_bounds__(256) k_expand(E13ExpandArgs a) {
             L.ch1 = nd->ch[1];
             L.ch2 = nd->ch[2];
             L.tcap = L.t + a.cap;
+            L.dm = 1; // not known: the first step tests the gate
             arm_stage(L, c);
             t0 = L.t;
             budget = nb;
@@ -806,7 +837,7 @@ __global__ void __launch_bounds__(256) k_expand(E13ExpandArgs a) {
     ts = tn;
     // ---- a round of steps ----
     int ev = EV_NONE, evcell = -1;
-    if (active) ev = run_slice<GATE, false>(L, c, a.slice, 0, evcell);
+    if (active) ev = run_slice<GATE, false, false>(L, c, a.slice, 0, evcell);
     tn = stamp();
     cyc1 += tn - ts;
     ts = tn;
@@ -925,6 +956,8 @@ __global__ void __launch_bounds__(256) k_batch(E13BatchArgs a) {
     return;
   }
   if (GATE == GATE_HASH) fill_keyb(c.sb, c.keyb, a.key);
+  c.ckb = c.keyb + (GATE == GATE_HASH ? KEYB_WORDS * 8 : 0) + nwaves * 64;
+  fill_ckeys(c.ckb, a.key);
   __syncthreads();
   Lane L;
   int active = 0;
@@ -973,6 +1006,7 @@ __global__ void __launch_bounds__(256) k_batch(E13BatchArgs a) {
         L.ch0 = L.ch1 = L.ch2 = 0;
         L.dlcap = a.caps ? a.caps[k] : -1;
         L.tcap = 0x7FFFFFFF;
+        L.dm = 1; // not known: the first step tests the gate
         arm_stage(L, c);
         active = 1;
       }
@@ -996,7 +1030,7 @@ __global__ void __launch_bounds__(256) k_batch(E13BatchArgs a) {
       ev = run_solo(L, c, exhausted ? a.solo : a.slice, evcell, __ffsll((long long)act) - 1);
       if (!active) ev = EV_NONE;
     } else if (active) {
-      ev = run_slice<GATE, false>(L, c, a.slice, 0, evcell);
+      ev = run_slice<GATE, false, false>(L, c, a.slice, 0, evcell);
     }
     if (active && ev != EV_NONE) {
       active = 0;
@@ -1112,7 +1146,9 @@ __device__ __forceinline__ void wave_mem_sync() { // this wave's global stores v
 __device__ __forceinline__ int slot_range(int first, int end) { return first | (end << 16); }
 
 __device__ __forceinline__ E13Node *stk_node(const E13SearchArgs &a, int wid, long long idx) {
-  return a.stacks + (size_t)wid * (size_t)a.stk_cap + (size_t)(idx % a.stk_cap);
+  // stk_cap is a power of two: the ring slot is the index's low bits
+  const uint32_t slot = (uint32_t)wid * (uint32_t)a.stk_cap + ((uint32_t)idx & (uint32_t)(a.stk_cap - 1));
+  return a.stacks + slot;
 }
 
 __device__ __forceinline__ uint64_t ld_agent64(const uint64_t *p) {
@@ -1298,6 +1334,8 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
   c.dlb = ptb + a.np * PT_WORDS * 4;
   for (int i = threadIdx.x; i < PT_DL_WORDS; i += blockDim.x)
     lds_st(c.sb, c.dlb + i * 4, i <= E13_NS ? (uint32_t)a.pbs[0].dl[i] : 0u);
+  c.ckb = c.dlb + PT_DL_WORDS * 4; // 8-byte aligned: every area before it is a multiple of 8 bytes
+  fill_ckeys(c.ckb, a.key);
   if (GATE == GATE_HASH) fill_keyb(c.sb, c.keyb, a.key);
   __syncthreads();
 
@@ -1333,7 +1371,7 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
   int refresh_in = (wid % 8) + 1;
   uint64_t w_slots = 0, w_rounds = 0, w_takes = 0, w_gives = 0;
   unsigned long long w_steps = 0;
-  uint64_t cyc0 = 0, cyc1 = 0, cyc2 = 0, cyc3 = 0, ts = stamp(), tn;
+  uint64_t cyc0 = 0, cyc1 = 0, cyc2 = 0, cyc3 = 0, ts = stamp_if(a.phase_cycles), tn;
 
   for (;;) {
     // ---- refill idle lanes from the private stack ----
@@ -1408,6 +1446,7 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
             L.ch0 = (uint64_t)s4.z | ((uint64_t)s4.w << 32);
             L.ch1 = (uint64_t)s5.x | ((uint64_t)s5.y << 32);
             L.ch2 = (uint64_t)s5.z | ((uint64_t)s5.w << 32);
+            L.dm = 1; // not known: the first step tests the gate
             arm_stage(L, c);
             t0 = L.t;
             budget = nb;
@@ -1465,7 +1504,7 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
         staged = -1;
       }
     }
-    tn = stamp();
+    tn = stamp_if(a.phase_cycles);
     cyc0 += tn - ts;
     ts = tn;
     // ---- this wave's mailbox: a donor may have delivered a node (time: pool/idle) ----
@@ -1557,7 +1596,7 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
         break;
       }
       for (int z = 0; z < a.idle_naps; ++z) __builtin_amdgcn_s_sleep(8);
-      tn = stamp();
+      tn = stamp_if(a.phase_cycles);
       cyc3 += tn - ts;
       ts = tn;
       continue;
@@ -1579,7 +1618,7 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
                             (a.evict_after > 0 && top == bot && __popcll(act) >= 2);
       if (lane < a.hungry_words && can_give) hw = ld_agent64(&a.hungry[lane]);
     }
-    tn = stamp();
+    tn = stamp_if(a.phase_cycles);
     cyc3 += tn - ts;
     ts = tn;
     // ---- a round of steps (time: stepping) ----
@@ -1592,10 +1631,16 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
       if (!active) ev = EV_NONE;
     } else if (active) {
       // with nodes to hand out, the round may end early once most lanes have stopped
-      if (a.cut_live > 1 && top > bot) ev = run_slice<GATE, true>(L, c, a.slice, a.cut_live, evcell);
-      else ev = run_slice<GATE, false>(L, c, a.slice, 0, evcell);
+      const bool cut = a.cut_live > 1 && top > bot;
+      if (a.skip_settled) {
+        if (cut) ev = run_slice<GATE, true, true>(L, c, a.slice, a.cut_live, evcell);
+        else ev = run_slice<GATE, false, true>(L, c, a.slice, 0, evcell);
+      } else {
+        if (cut) ev = run_slice<GATE, true, false>(L, c, a.slice, a.cut_live, evcell);
+        else ev = run_slice<GATE, false, false>(L, c, a.slice, 0, evcell);
+      }
     }
-    tn = stamp();
+    tn = stamp_if(a.phase_cycles);
     cyc1 += tn - ts;
     ts = tn;
     // ---- a clean stop: every running machine is paused as a resumable node, as at
@@ -1681,7 +1726,7 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
       if (lane == 0) withdrawn = (atomicAnd((unsigned long long *)&a.hungry[wid >> 6], ~bit) & bit) != 0;
       if (uni(withdrawn)) requested = 0; // else a delivery is on its way: keep polling the mailbox
     }
-    tn = stamp();
+    tn = stamp_if(a.phase_cycles);
     cyc2 += tn - ts;
     ts = tn;
     if (stopping) { // the staged node's cursor goes with it; the stack is this wave's whole state
@@ -1752,7 +1797,7 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
       }
       refresh_in = given > 0 || nz != 0 ? 2 : (int)a.sched_every; // look again soon while waves wait
     }
-    tn = stamp();
+    tn = stamp_if(a.phase_cycles);
     cyc3 += tn - ts;
     ts = tn;
   }
@@ -1769,7 +1814,7 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
   }
   TRACE_EV(TR_EXIT);
   if (a.trace != nullptr && lane == 0) a.trace[(size_t)wid * (size_t)a.trace_cap] = tr_n;
-  cyc3 += stamp() - ts;
+  cyc3 += stamp_if(a.phase_cycles) - ts;
   if (lane == 0) {
     atomicAdd((unsigned long long *)&a.ctr->cyc[0], (unsigned long long)cyc0);
     atomicAdd((unsigned long long *)&a.ctr->cyc[1], (unsigned long long)cyc1);
@@ -1994,6 +2039,8 @@ __global__ void __launch_bounds__(256) k_probe(E13ProbeArgs a) {
   // this wave's PK slots, after the tapes, the block keys and 64 bytes per wave
   const int slots = c.keyb + (GATE == GATE_HASH ? KEYB_WORDS * 8 : 0) + nwaves * 64 + wave * (PK * PS_WORDS * 4);
   for (int i = lane; i < PK * PS_WORDS; i += 64) lds_st(c.sb, slots + i * 4, 0u); // every slot empty
+  c.ckb = c.keyb + (GATE == GATE_HASH ? KEYB_WORDS * 8 : 0) + nwaves * 64 + nwaves * (PK * PS_WORDS * 4);
+  fill_ckeys(c.ckb, a.key);
   __syncthreads();
   uint8_t *sbytes = (uint8_t *)smem;
   const int nq = a.nq, nslot = 4 * nq + 1;
@@ -2167,6 +2214,7 @@ __global__ void __launch_bounds__(256) k_probe(E13ProbeArgs a) {
       L.ch0 = lds_ld64(c, r0 + 44);
       L.ch1 = lds_ld64(c, r0 + 52);
       L.ch2 = lds_ld64(c, r0 + 60);
+      L.dm = 1; // not known: the first step tests the gate
       arm_stage(L, c);
       t0 = L.t;
       active = 1;
@@ -2190,7 +2238,7 @@ __global__ void __launch_bounds__(256) k_probe(E13ProbeArgs a) {
       ev = run_solo(L, c, a.solo, evcell, __ffsll((long long)act) - 1);
       if (!active) ev = EV_NONE;
     } else if (active) {
-      ev = run_slice<GATE, false>(L, c, a.slice, 0, evcell);
+      ev = run_slice<GATE, false, false>(L, c, a.slice, 0, evcell);
     }
     // ---- finished runs ----
     if (active && ev != EV_NONE) {
@@ -2258,7 +2306,7 @@ extern "C" {
 
 size_t e13k_lds_bytes(int wg_threads, int gate) {
   const int nw = wg_threads / WAVE;
-  return (size_t)nw * (ROWS_BYTES + TAPE_BYTES + 64) + (gate == GATE_HASH ? KEYB_WORDS * 8 : 0);
+  return (size_t)nw * (ROWS_BYTES + TAPE_BYTES + 64) + (gate == GATE_HASH ? KEYB_WORDS * 8 : 0) + CKEY_BYTES;
 }
 
 int e13k_launch_expand(const E13ExpandArgs *a, int grid, int wg_threads, int gate, int refill, hipStream_t s) {
@@ -2293,7 +2341,7 @@ int e13k_launch_chain_push(E13Chain *ch, E13Node *stack, uint64_t stack_cap, con
 size_t e13k_search_lds_bytes(int wg_threads, int gate, int np) {
   const int nw = wg_threads / WAVE;
   return (size_t)nw * (ROWS_BYTES + TAPE_BYTES + STG_WORDS * 4) + (gate == GATE_HASH ? KEYB_WORDS * 8 : 0) +
-         (size_t)np * E13_PCNT * 8 + (size_t)np * PT_WORDS * 4 + PT_DL_WORDS * 4;
+         (size_t)np * E13_PCNT * 8 + (size_t)np * PT_WORDS * 4 + PT_DL_WORDS * 4 + CKEY_BYTES;
 }
 
 // Workgroups of k_search that fit on one CU at once (registers, LDS, wave

@@ -117,6 +117,13 @@ typedef struct {
   int32_t cut_live;       /* persistent form: > 1: a round of the multi-lane loop ends once fewer than this many
                              of the wave's lanes still run while the wave has nodes to hand out, so the idle lanes
                              are refilled sooner (the running ones go on next round); negative: off */
+  int32_t phase_cycles;   /* persistent form: > 0: count wave clock cycles per phase into e13_stats.cycles (a
+                             diagnostic that costs about 2%); negative: off */
+  int32_t nt1_skip;       /* persistent form, step loop: 1: test the stage-1 gate only after a step that changed
+                             the ones count (before stage 1 the block can become exact in no other way), so a
+                             machine whose two ones sit in the wrong cells does not leave the loop at every
+                             step; 2: test it at every step; 0 or negative: 1 when some problem has an edit
+                             budget (delete mode), else 2.  Never changes results. */
 } e13_tuning;
 
 typedef struct {
@@ -129,7 +136,8 @@ typedef struct {
   int32_t profile;       /* settings profile of the last persistent search (E13_PROFILE_*) */
   int64_t donations, pool_takes; /* persistent form: nodes moved between waves */
   int64_t cycles[4];             /* wave clock cycles spent refilling lanes, stepping, finishing
-                                    machines, and on the work pool / idle (summed over waves) */
+                                    machines, and on the work pool / idle (summed over waves; the
+                                    persistent form counts them with tuning.phase_cycles > 0 only) */
   int64_t graphs;                /* hipGraph launches (graph_chain > 0) */
   double search_ms;              /* the persistent kernel alone (k_search): HIP-event time, */
   int64_t search_launches;       /* launches, */

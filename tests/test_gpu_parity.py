@@ -240,7 +240,9 @@ def test_counter_3state_symmetric(e13, gpu_ctx, base):
                                   dict(form=3, wg_threads=64, wgs_per_cu=1, stack_nodes=1 << 16),
                                   # rounds cut short once few lanes still run (the rest refilled)
                                   dict(form=3, cut_live=32), dict(form=3, cut_live=64, slice=256),
-                                  dict(form=3, cut_live=2, slice=7, solo_steps=-1)])
+                                  dict(form=3, cut_live=2, slice=7, solo_steps=-1),
+                                  # the stage-1 gate at every step, or only after a change of the ones count
+                                  dict(form=3, nt1_skip=2), dict(form=3, nt1_skip=1, cut_live=-1, slice=3)])
 def test_results_invariant_under_tuning(e13, gpu_ctx, base, tune):
     say(f"search: delete FMAX=2 X=7,8,12 tuning={tune}")
     gpu_ctx.set_tuning(**tune)
