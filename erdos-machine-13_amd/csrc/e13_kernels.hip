@@ -102,6 +102,8 @@ struct Ctx {
   const int32_t *dl;
   int dlb;            // >= 0: LDS byte address of the stage deadlines dl[0..9] (used instead of dl)
   int ckb;            // LDS byte address of the counter keys: KEY[q+1] for q 0..60 (CKEY_BYTES)
+  int cbl;            // >= 0: LDS byte address of this lane's table codes (8 words, a column of stride 256),
+                      // kept beside the rows by the persistent search; -1: decode the rows
 };
 #define CKEY_BYTES 512 // 64 u64 counter keys per workgroup (61 used)
 
@@ -265,6 +267,7 @@ __device__ __forceinline__ int step_once(Lane &L, const Ctx &c, int &evcell) {
 // min_live lanes of the wave are still in the loop; such a lane makes its due
 // step in step_once (the exact slow path: EV_NONE unless that step ends the
 // machine) and goes on next round, while the wave refills its idle lanes.
+__device__ __forceinline__ uint64_t pos_key(int row, int q) { return ((uint64_t)(uint32_t)row << 32) | (uint32_t)q; }
 template <int GATE, bool CUT, bool SKIP>
 __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int min_live, int &evcell) {
   const int t_end = L.t + slice;
@@ -273,6 +276,8 @@ __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int m
   // steps left before the deadline/cap test or the round's end: the loop runs
   // while left >= 0 (so t = t_lim - 1 - left)
   int left = t_lim - 1 - L.t;
+  // the gate's state and head tests as one 64-bit compare each (row address, head)
+  const uint64_t pA = pos_key(L.row0, L.qA), pR = pos_key(L.row0, L.qR);
   // tape word address (q >> 5) * 256 + tbl as one shift-add and one and-or
   const uint32_t tbl_lo = (uint32_t)c.tbl & 0xFFu, tbl_hi = (uint32_t)c.tbl - tbl_lo;
   // The exit test in one sign bit: (e << 16) is negative for a cell that
@@ -290,11 +295,12 @@ __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int m
   bool stop;
   {
     bool gate;
-    if (GATE == GATE_COUNT)
-      gate = (L.ones == L.target) & ((SKIP ? nt1 & (L.dm != 0) : nt1) |
-                                     ((L.row == L.row0) & ((L.q == L.qA) | (L.q == L.qR))));
-    else
+    if (GATE == GATE_COUNT) {
+      const uint64_t pk = pos_key(L.row, L.q);
+      gate = (L.ones == L.target) & ((SKIP ? nt1 & (L.dm != 0) : nt1) | (pk == pA) | (pk == pR));
+    } else {
       gate = L.hb == L.tgt;
+    }
     stop = (int)((e << 16) | (uint32_t)L.q | (uint32_t)(E13_QMAX - L.q) | (uint32_t)left) < 0 || gate;
   }
   while (!stop) {
@@ -319,11 +325,12 @@ __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int m
     pair = lds_ld(c.sb, L.row);
     e = pair >> (__builtin_amdgcn_ubfe(word, (uint32_t)L.q, 1u) << 4);
     bool gate;
-    if (GATE == GATE_COUNT)
-      gate = (L.ones == L.target) & ((SKIP ? nt1 & (L.dm != 0) : nt1) |
-                                     ((L.row == L.row0) & ((L.q == L.qA) | (L.q == L.qR))));
-    else
+    if (GATE == GATE_COUNT) {
+      const uint64_t pk = pos_key(L.row, L.q);
+      gate = (L.ones == L.target) & ((SKIP ? nt1 & (L.dm != 0) : nt1) | (pk == pA) | (pk == pR));
+    } else {
       gate = L.hb == L.tgt;
+    }
     stop = (int)((e << 16) | (uint32_t)L.q | (uint32_t)(E13_QMAX - L.q) | (uint32_t)left) < 0 || gate;
     // with too few lanes left in the loop (exec is wave-uniform), every lane
     // leaves it; a lane whose step was due makes it in step_once below
@@ -656,6 +663,7 @@ __device__ __forceinline__ void build_rows(const Ctx &c, const uint8_t *tab, int
 
 // Codes of cells 4i..4i+3 packed little-endian (rows past nq read as free).
 __device__ __forceinline__ uint32_t codes_word(const Ctx &c, int nq, int i) {
+  if (c.cbl >= 0) return lds_ld(c.sb, c.cbl + i * 256);
   uint32_t v = 0;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
@@ -695,6 +703,7 @@ __global__ void __launch_bounds__(256) k_expand(E13ExpandArgs a) {
   c.tgt = a.tgt;
   c.dl = a.pb->dl;
   c.dlb = -1;
+  c.cbl = -1;
   if (lds_base(smem) != 0u) { // the absolute LDS addressing needs the dynamic area at address 0
     if (threadIdx.x == 0) atomicOr(&a.ctr->overflow, 4u);
     return;
@@ -951,6 +960,7 @@ __global__ void __launch_bounds__(256) k_batch(E13BatchArgs a) {
   c.tgt = a.tgt;
   c.dl = a.dl;
   c.dlb = -1;
+  c.cbl = -1;
   if (lds_base(smem) != 0u) { // the absolute LDS addressing needs the dynamic area at address 0
     if (threadIdx.x == 0) atomicOr(&a.ctr->overflow, 4u);
     return;
@@ -1112,7 +1122,10 @@ __global__ void __launch_bounds__(256) k_chain_push(E13Chain *ch, E13Node *stack
 // ---------------------------------------------------------------------------
 namespace {
 
-#define STG_WORDS 112 // staged node (76) + rows as stored (16) + rows with edits exhausted (16) + 4 spare
+#define STG_WORDS 120 // staged node (76) + rows as stored (16) + rows with edits exhausted (16)
+                      // + codes with edits exhausted (8) + 4 spare
+#define STG_W_CODESB 108
+#define CODES_BYTES (8 * 256) // per wave: each lane's 32 table codes, word i at i * 256 + lane * 4
 
 // Per-workgroup copy of the problems in LDS (read by the refill, which would
 // otherwise wait on global loads): problem k at word k * PT_WORDS holds its 32
@@ -1196,6 +1209,15 @@ __device__ void stage_rows(uint32_t *stg, int ptb, int lane) {
     }
     stg[76 + lane] = A;
     stg[92 + lane] = B;
+  }
+  if (lane < E13_MAXCELLS) { // codes with edits exhausted: unread editable cells keep their base value
+    const uint8_t *tab = (const uint8_t *)(stg + E13_NODE_W_TAB);
+    int cd = tab[lane];
+    if (cd == E13_C_EDIT) {
+      const int b = pt_base(ptb, prob, lane);
+      cd = b < 0 ? E13_C_HALT : b;
+    }
+    ((uint8_t *)(stg + STG_W_CODESB))[lane] = (uint8_t)cd;
   }
   wave_lds_sync();
 }
@@ -1336,6 +1358,7 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
     lds_st(c.sb, c.dlb + i * 4, i <= E13_NS ? (uint32_t)a.pbs[0].dl[i] : 0u);
   c.ckb = c.dlb + PT_DL_WORDS * 4; // 8-byte aligned: every area before it is a multiple of 8 bytes
   fill_ckeys(c.ckb, a.key);
+  c.cbl = c.ckb + CKEY_BYTES + wave * CODES_BYTES + lane * 4;
   if (GATE == GATE_HASH) fill_keyb(c.sb, c.keyb, a.key);
   __syncthreads();
 
@@ -1474,6 +1497,22 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
               const uint32_t sh = ((uint32_t)pend & 1u) << 4;
               const uint32_t e = enc_move(val, (uint32_t)pend & 1u);
               lds_st(c.sb, c.rbl + (pend >> 1) * 256, (pr & ~(0xFFFFu << sh)) | (e << sh));
+            }
+            // table codes: as stored (words 24..31) or with edits used up, then the pending cell
+            {
+              const uint4 *csrc = (const uint4 *)(stg + ((nb == 0 && nb0 > 0) ? STG_W_CODESB : E13_NODE_W_TAB));
+              const uint4 c0 = csrc[0], c1 = csrc[1];
+              lds_st(c.sb, c.cbl + 0 * 256, c0.x);
+              lds_st(c.sb, c.cbl + 1 * 256, c0.y);
+              lds_st(c.sb, c.cbl + 2 * 256, c0.z);
+              lds_st(c.sb, c.cbl + 3 * 256, c0.w);
+              lds_st(c.sb, c.cbl + 4 * 256, c1.x);
+              lds_st(c.sb, c.cbl + 5 * 256, c1.y);
+              lds_st(c.sb, c.cbl + 6 * 256, c1.z);
+              lds_st(c.sb, c.cbl + 7 * 256, c1.w);
+              if (pend >= 0)
+                *(__attribute__((address_space(3))) uint8_t *)(size_t)(uint32_t)(c.cbl + (pend >> 2) * 256 +
+                                                                                  (pend & 3)) = (uint8_t)val;
             }
             // tape: words 32..74 of the stage (word 75, the seed, comes along)
 #pragma unroll
@@ -2031,6 +2070,7 @@ __global__ void __launch_bounds__(256) k_probe(E13ProbeArgs a) {
   c.tgt = a.tgt;
   c.dl = a.pb->dl;
   c.dlb = -1;
+  c.cbl = -1;
   if (lds_base(smem) != 0u) { // the absolute LDS addressing needs the dynamic area at address 0
     if (threadIdx.x == 0) atomicOr(&a.ctr->overflow, 4u);
     return;
@@ -2341,7 +2381,8 @@ int e13k_launch_chain_push(E13Chain *ch, E13Node *stack, uint64_t stack_cap, con
 size_t e13k_search_lds_bytes(int wg_threads, int gate, int np) {
   const int nw = wg_threads / WAVE;
   return (size_t)nw * (ROWS_BYTES + TAPE_BYTES + STG_WORDS * 4) + (gate == GATE_HASH ? KEYB_WORDS * 8 : 0) +
-         (size_t)np * E13_PCNT * 8 + (size_t)np * PT_WORDS * 4 + PT_DL_WORDS * 4 + CKEY_BYTES;
+         (size_t)np * E13_PCNT * 8 + (size_t)np * PT_WORDS * 4 + PT_DL_WORDS * 4 + CKEY_BYTES +
+         (size_t)nw * CODES_BYTES;
 }
 
 // Workgroups of k_search that fit on one CU at once (registers, LDS, wave
