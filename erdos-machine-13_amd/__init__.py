@@ -30,7 +30,11 @@ ERRORS = {
     -4: "forced cells exceed fmax", -5: "symmetric state is named by a fixed cell or has one",
     -6: "start state in the symmetric states", -7: "node stack overflow", -8: "no HIP device",
     -9: "stopped cleanly; the checkpoint is saved", -10: "checkpoint file unreadable, damaged or for another search",
+    -11: "a mailbox delivery never arrived complete", -12: "a wave waited longer than the idle watchdog",
+    -13: "dynamic LDS does not start at LDS address 0",
+    -14: "more probes than the static scratch holds",
 }
+STATIC_PROBES = 1 << 16  # e13_estimate_static's limit (E13_STATIC_PROBES)
 
 
 class E13Error(RuntimeError):
@@ -148,7 +152,7 @@ EXPORTS = ("e13_open", "e13_close", "e13_strerror", "e13_set_tuning", "e13_get_t
            "e13_synth_table", "e13_get_trace", "e13_batch_digest", "e13_synth_run", "e13_get_cost_log",
            "e13_cost_targets", "e13_mlp_init", "e13_mlp_forward", "e13_mlp_grad", "e13_mlp_train",
            "e13_set_cost_model", "e13_autotune", "e13_save_tuning", "e13_load_tuning", "e13_tuning_file",
-           "e13_tuning_file_small", "e13_search_profile", "e13_estimate", "e13_set_checkpoint",
+           "e13_tuning_file_small", "e13_search_profile", "e13_estimate", "e13_estimate_static", "e13_set_checkpoint",
            "e13_deep_check_line", "e13_deep_check", "e13_recheck_counter_line", "e13_recheck_counter",
            "e13_check_survivor", "e13_classify_deep")
 
@@ -229,6 +233,7 @@ def lib():
     L.e13_set_checkpoint.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_double, ctypes.c_int64]
     L.e13_estimate.argtypes = [ctypes.c_void_p, P(Problem), ctypes.c_uint64, ctypes.c_int64, P(ctypes.c_double),
                                P(EstimateSum)]
+    L.e13_estimate_static.argtypes = L.e13_estimate.argtypes
     L.e13_get_trace.argtypes = [ctypes.c_void_p, P(ctypes.c_uint64), ctypes.c_uint64, P(ctypes.c_int32),
                                 P(ctypes.c_int32)]
     _lib = L
@@ -576,6 +581,17 @@ class Context:
         if per_probe:
             return es, [tuple(buf[4 * i:4 * i + 4]) for i in range(probes)]
         return es
+
+    def estimate_static(self, p, probes, seed=0):
+        """The probe estimator over the device's static scratch (no allocation
+        in the library call; probes <= STATIC_PROBES, else E13Error -14):
+        (EstimateSum, list of each probe's four sums).  Calls that overlap on
+        one GPU race on that scratch."""
+        es = EstimateSum()
+        buf = (ctypes.c_double * (4 * max(probes, 1)))()
+        _check(lib().e13_estimate_static(self._h, ctypes.byref(p), seed, probes, buf, ctypes.byref(es)),
+               "e13_estimate_static")
+        return es, [tuple(buf[4 * i:4 * i + 4]) for i in range(probes)]
 
     def sync(self):
         _check(lib().e13_sync(self._h), "e13_sync")

@@ -32,6 +32,8 @@ int e13k_launch_chain_push(E13Chain *ch, E13Node *stack, uint64_t stack_cap, con
 size_t e13k_probe_lds_bytes(int wg_threads, int gate);
 int e13k_probe_max_wgs(int wg_threads, int gate);
 int e13k_launch_probe(const E13ProbeArgs *a, int grid, int wg_threads, int gate, hipStream_t s);
+int e13k_scratch_addr(int device, void **ptr);
+int e13k_launch_zero(void *p, uint32_t nwords, hipStream_t s);
 
 _Static_assert(sizeof(E13Node) % 16 == 0, "node records are copied as 16-byte vectors");
 
@@ -88,6 +90,7 @@ struct e13_ctx {
   /* probe estimator (e13_estimate.c) */
   double *d_probe_out;
   size_t probe_cap;      /* probes the output buffer holds */
+  void *scratch;         /* this device's static scratch (e13_estimate_static), NULL when not found */
   /* checkpoints of the host-loop search (e13_ckpt.c) */
   char ckpt_path[1024];  /* empty: off */
   double ckpt_every_s;

@@ -147,6 +147,15 @@ typedef struct E13Mail {
   uint64_t w[E13_NODE_WORDS];
 } E13Mail;
 
+/* Static device scratch of the estimator entry point that allocates nothing:
+   per-probe sums for up to E13_SCRATCH_PROBES probes (4 doubles each), then the
+   launch counters (E13Counters at E13_SCRATCH_CTR) and the problem record
+   (E13Problem at E13_SCRATCH_PB). */
+#define E13_SCRATCH_PROBES (1 << 16)
+#define E13_SCRATCH_CTR (E13_SCRATCH_PROBES * 32)
+#define E13_SCRATCH_PB (E13_SCRATCH_CTR + 1024)
+#define E13_SCRATCH_BYTES (E13_SCRATCH_PB + 1024)
+
 #define E13_MAXPROB 16
 #define E13_PCNT 16       /* per-problem counters: leaves, steps, survivors, hist[0..9] */
 

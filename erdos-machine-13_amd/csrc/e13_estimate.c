@@ -16,58 +16,80 @@
 
 #define PROBES_PER_LAUNCH (1 << 22) /* 128 MiB of per-probe sums */
 
+_Static_assert(E13_STATIC_PROBES == E13_SCRATCH_PROBES, "the static estimator's limit is its scratch's size");
+_Static_assert(sizeof(E13Counters) <= E13_SCRATCH_PB - E13_SCRATCH_CTR && sizeof(E13Counters) % 4 == 0,
+               "the counters fit their scratch slot and clear as words");
+_Static_assert(sizeof(E13Problem) <= E13_SCRATCH_BYTES - E13_SCRATCH_PB, "the problem record fits its scratch slot");
+
 void e13i_free_estimate(e13_ctx *c) {
   hipFree(c->d_probe_out);
   c->d_probe_out = NULL;
   c->probe_cap = 0;
 }
 
-int e13_estimate(e13_ctx *c, const e13_problem *p, uint64_t seed, int64_t probes, double *per_probe,
-                 e13_estimate_sum *out) {
+/* Device buffers one estimate runs over: per-probe sums for chunk probes, the
+   launch counters and the problem record. */
+typedef struct {
+  double *out;
+  E13Counters *ctr;
+  E13Problem *pb;
+  size_t chunk;
+  int static_scratch; /* counters cleared by a fill kernel (no memset node) */
+} est_bufs;
+
+static int check_problem(const e13_ctx *c, const e13_problem *p, int64_t probes, const e13_estimate_sum *out) {
   if (!c || !p || !out || probes < 1 || p->nq < 1 || p->nq > E13_MAXNQ || p->budget < 0) return E13_EINVAL;
   for (int i = 0; i < 2 * p->nq; i++)
     if (p->tab[i] < E13_FORCED || p->tab[i] >= 4 * p->nq) return E13_EINVAL;
-  memset(out, 0, sizeof *out);
-  HIPCHK(hipSetDevice(c->device));
-  const int gate = c->tune.gate == 2;
-  const int wg = c->tune.wg_threads >= 64 && c->tune.wg_threads <= 256 ? c->tune.wg_threads / 64 * 64 : 256;
-  if (e13k_probe_lds_bytes(wg, gate) > c->lds_max) return E13_EINVAL;
-  /* every workgroup resident at once: waves take probes from one cursor */
-  int per_cu = e13k_probe_max_wgs(wg, gate);
-  if (per_cu < 1) per_cu = 1;
-  const int grid = c->ncu * per_cu;
-  const size_t chunk = probes < PROBES_PER_LAUNCH ? (size_t)probes : PROBES_PER_LAUNCH;
-  int rc = e13i_ensure_buf((void **)&c->d_probe_out, &c->probe_cap, chunk, 4 * sizeof(double));
-  if (rc) return rc;
-  double *host = per_probe ? NULL : (double *)malloc(chunk * 4 * sizeof(double));
-  if (!per_probe && !host) return E13_ENOMEM;
+  return E13_OK;
+}
 
+/* Launch shape: every workgroup resident at once, waves take probes from one cursor. */
+static int probe_shape(const e13_ctx *c, int *gate, int *wg, int *grid) {
+  *gate = c->tune.gate == 2;
+  *wg = c->tune.wg_threads >= 64 && c->tune.wg_threads <= 256 ? c->tune.wg_threads / 64 * 64 : 256;
+  if (e13k_probe_lds_bytes(*wg, *gate) > c->lds_max) return E13_EINVAL;
+  int per_cu = e13k_probe_max_wgs(*wg, *gate);
+  if (per_cu < 1) per_cu = 1;
+  *grid = c->ncu * per_cu;
+  return E13_OK;
+}
+
+/* The probes in launches of at most b->chunk, each launch's per-probe sums copied
+   to per_probe + 4*p0 (or to host, reused, when per_probe is NULL) and added in
+   probe order. */
+static int estimate_run(e13_ctx *c, const e13_problem *p, uint64_t seed, int64_t probes, double *per_probe,
+                        double *host, const est_bufs *b, e13_estimate_sum *out) {
+  int gate, wg, grid;
+  int rc = probe_shape(c, &gate, &wg, &grid);
+  if (rc) return rc;
   E13Problem pb;
   e13i_fill_problem(p, &pb);
   E13ProbeArgs a;
   memset(&a, 0, sizeof a);
-  a.pb = c->d_pb;
+  a.pb = b->pb;
   a.key = c->d_key;
   a.tgt = c->d_tgt;
-  a.out = c->d_probe_out;
-  a.ctr = c->d_ctr;
+  a.out = b->out;
+  a.ctr = b->ctr;
   a.seed = seed;
   a.nq = p->nq;
   a.slice = c->tune.probe_slice;
   a.budget = p->budget;
   a.nintro = p->sym_n0;
   a.solo = c->tune.solo_steps;
-  rc = E13_OK;
-  if (hipMemcpyAsync(c->d_pb, &pb, sizeof pb, hipMemcpyHostToDevice, c->stream) != hipSuccess) rc = E13_EHIP;
-  for (int64_t p0 = 0; rc == E13_OK && p0 < probes; p0 += (int64_t)chunk) {
-    const size_t n = (size_t)(probes - p0) < chunk ? (size_t)(probes - p0) : chunk;
+  if (hipMemcpyAsync(b->pb, &pb, sizeof pb, hipMemcpyHostToDevice, c->stream) != hipSuccess) return E13_EHIP;
+  for (int64_t p0 = 0; rc == E13_OK && p0 < probes; p0 += (int64_t)b->chunk) {
+    const size_t n = (size_t)(probes - p0) < b->chunk ? (size_t)(probes - p0) : b->chunk;
     a.p0 = (uint64_t)p0;
     a.n = (uint32_t)n;
     E13Counters ctr;
     float ms = 0.f;
     double *dst = per_probe ? per_probe + 4 * p0 : host;
-    if (hipMemsetAsync(c->d_ctr, 0, sizeof(E13Counters), c->stream) != hipSuccess ||
-        hipEventRecord(c->ev0, c->stream) != hipSuccess) {
+    const int zr = b->static_scratch
+                       ? e13k_launch_zero(b->ctr, (uint32_t)(sizeof(E13Counters) / 4), c->stream)
+                       : (int)hipMemsetAsync(b->ctr, 0, sizeof(E13Counters), c->stream);
+    if (zr != 0 || hipEventRecord(c->ev0, c->stream) != hipSuccess) {
       rc = E13_EHIP;
       break;
     }
@@ -78,9 +100,8 @@ int e13_estimate(e13_ctx *c, const e13_problem *p, uint64_t seed, int64_t probes
       break;
     }
     if (hipEventRecord(c->ev1, c->stream) != hipSuccess ||
-        hipMemcpyAsync(dst, c->d_probe_out, n * 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream) !=
-            hipSuccess ||
-        hipMemcpyAsync(&ctr, c->d_ctr, sizeof ctr, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipMemcpyAsync(dst, b->out, n * 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipMemcpyAsync(&ctr, b->ctr, sizeof ctr, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess || hipEventElapsedTime(&ms, c->ev0, c->ev1) != hipSuccess) {
       rc = E13_EHIP;
       break;
@@ -103,7 +124,6 @@ int e13_estimate(e13_ctx *c, const e13_problem *p, uint64_t seed, int64_t probes
         out->sq[j] += v * v;
       }
   }
-  free(host);
   if (rc) return rc;
   out->probes = probes;
   for (int j = 0; j < 4; j++) { /* e13search.c:503-506 */
@@ -120,4 +140,46 @@ int e13_estimate(e13_ctx *c, const e13_problem *p, uint64_t seed, int64_t probes
   c->stats.slice = a.slice;
   c->stats.gate = c->tune.gate;
   return E13_OK;
+}
+
+int e13_estimate(e13_ctx *c, const e13_problem *p, uint64_t seed, int64_t probes, double *per_probe,
+                 e13_estimate_sum *out) {
+  int rc = check_problem(c, p, probes, out);
+  if (rc) return rc;
+  memset(out, 0, sizeof *out);
+  HIPCHK(hipSetDevice(c->device));
+  int gate, wg, grid;
+  if ((rc = probe_shape(c, &gate, &wg, &grid)) != 0) return rc;
+  const size_t chunk = probes < PROBES_PER_LAUNCH ? (size_t)probes : PROBES_PER_LAUNCH;
+  rc = e13i_ensure_buf((void **)&c->d_probe_out, &c->probe_cap, chunk, 4 * sizeof(double));
+  if (rc) return rc;
+  double *host = per_probe ? NULL : (double *)malloc(chunk * 4 * sizeof(double));
+  if (!per_probe && !host) return E13_ENOMEM;
+  const est_bufs b = {c->d_probe_out, c->d_ctr, c->d_pb, chunk, 0};
+  rc = estimate_run(c, p, seed, probes, per_probe, host, &b, out);
+  free(host);
+  return rc;
+}
+
+/* e13_estimate over the device's static scratch: one launch, nothing allocated. */
+int e13_estimate_static(e13_ctx *c, const e13_problem *p, uint64_t seed, int64_t probes, double *per_probe,
+                        e13_estimate_sum *out) {
+  int rc = check_problem(c, p, probes, out);
+  if (rc) return rc;
+  if (!per_probe) return E13_EINVAL;
+  if (probes > E13_STATIC_PROBES) {
+    fprintf(stderr, "libe13: e13_estimate_static: %lld probes exceed the static scratch's %d\n", (long long)probes,
+            E13_STATIC_PROBES);
+    return E13_ESCRATCH;
+  }
+  if (!c->scratch) {
+    fprintf(stderr, "libe13: e13_estimate_static: no static scratch address on device %d\n", c->device);
+    return E13_EHIP;
+  }
+  memset(out, 0, sizeof *out);
+  HIPCHK(hipSetDevice(c->device));
+  unsigned char *s = (unsigned char *)c->scratch;
+  const est_bufs b = {(double *)s, (E13Counters *)(s + E13_SCRATCH_CTR), (E13Problem *)(s + E13_SCRATCH_PB),
+                      (size_t)E13_SCRATCH_PROBES, 1};
+  return estimate_run(c, p, seed, probes, per_probe, NULL, &b, out);
 }

@@ -40,6 +40,7 @@ const char *e13_strerror(int code) {
   case E13_ELOST: return "persistent search: a mailbox delivery never arrived complete";
   case E13_EWATCHDOG: return "persistent search: idle watchdog (no work and no end mark)";
   case E13_ELDS: return "dynamic LDS does not start at LDS address 0";
+  case E13_ESCRATCH: return "the request exceeds the static scratch (more than E13_STATIC_PROBES probes)";
   default: return "unknown error";
   }
 }
@@ -290,6 +291,8 @@ int e13_open(int device, const e13_tuning *t, e13_ctx **out) {
   }
   c->tune_open = c->tune;
   c->tune_small_open = c->tune_small;
+  /* the static scratch of e13_estimate_static: its address on this device */
+  if (e13k_scratch_addr(device, &c->scratch) != 0) c->scratch = NULL;
   if (hipStreamCreate(&c->stream) != hipSuccess || hipEventCreate(&c->ev0) != hipSuccess ||
       hipEventCreate(&c->ev1) != hipSuccess) {
     free(c);

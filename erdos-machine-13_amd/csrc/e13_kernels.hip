@@ -2439,3 +2439,51 @@ int e13k_launch_probe(const E13ProbeArgs *a, int grid, int wg_threads, int gate,
 }
 
 } // extern "C"
+
+// ---------------------------------------------------------------------------
+// Device scratch of the estimator entry point that allocates nothing
+// (e13_estimate_static): per-probe sums (E13_SCRATCH_PROBES x 4 doubles), then
+// the launch counters and the problem record.  Each GPU holds its own copy at
+// its own address, found once per device with hipGetSymbolAddress; host code
+// uses only that address (the array's name is not a device address there).
+// Every call on a GPU shares it: calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[E13_SCRATCH_BYTES];
+
+namespace {
+#define E13_SCRATCH_DEVICES 64
+void *g_scratch_ptr[E13_SCRATCH_DEVICES];
+int g_scratch_err[E13_SCRATCH_DEVICES]; // 0 not looked up yet, 1 found, -1 the lookup failed
+}
+
+__global__ void __launch_bounds__(256) k_zero_words(uint32_t *p, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+extern "C" {
+
+// This device's scratch address (the device must be current); 0 or a HIP error.
+int e13k_scratch_addr(int device, void **ptr) {
+  if (device < 0 || device >= E13_SCRATCH_DEVICES) return (int)hipErrorInvalidDevice;
+  if (g_scratch_err[device] == 0) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) {
+      g_scratch_err[device] = -1;
+      return (int)e;
+    }
+    g_scratch_ptr[device] = p;
+    g_scratch_err[device] = 1;
+  }
+  if (g_scratch_err[device] < 0) return (int)hipErrorInvalidSymbol;
+  *ptr = g_scratch_ptr[device];
+  return 0;
+}
+
+// Zero nwords words at p with a fill kernel (no memset node, safe under capture).
+int e13k_launch_zero(void *p, uint32_t nwords, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_words, dim3(8), dim3(256), 0, s, (uint32_t *)p, nwords);
+  return (int)hipGetLastError();
+}
+
+} // extern "C"

@@ -38,6 +38,7 @@ extern "C" {
 #define E13_ELOST (-11)     /* a mailbox delivery never arrived complete */
 #define E13_EWATCHDOG (-12) /* a wave waited longer than the idle watchdog for work or the end mark */
 #define E13_ELDS (-13)      /* dynamic LDS does not start at LDS address 0 */
+#define E13_ESCRATCH (-14)  /* e13_estimate_static: more probes than its static scratch holds */
 
 /* Result codes of a run (e13search.c:64-65). */
 #define E13_DEAD (-1)
@@ -246,6 +247,19 @@ typedef struct {
 } e13_estimate_sum;
 int e13_estimate(e13_ctx *ctx, const e13_problem *p, uint64_t seed, int64_t probes, double *per_probe,
                  e13_estimate_sum *out);
+
+/* e13_estimate with no allocation in the call (no hipMalloc/hipFree, no host
+   malloc): for callers that cannot allocate -- code under HIP graph capture,
+   tight serving loops, harnesses that forbid it.  No reference counterpart.
+   The device buffers (per-probe sums, counters, problem record) live in a
+   static array of E13_SCRATCH_BYTES on each GPU, found by e13_open; the caller
+   passes per_probe (4 * probes doubles, required).  probes <= E13_STATIC_PROBES
+   (E13_ESCRATCH otherwise).  Results equal e13_estimate's for the same
+   (problem, seed, probes).  Every call on one GPU shares its array: calls that
+   overlap on one GPU (two host threads, two contexts) race. */
+#define E13_STATIC_PROBES (1 << 16)
+int e13_estimate_static(e13_ctx *ctx, const e13_problem *p, uint64_t seed, int64_t probes, double *per_probe,
+                        e13_estimate_sum *out);
 
 /* Plain, unhashed re-check of a survivor over stages 1..9 with the search's
    deadlines and window (after recheck.c:57-92): 1 consistent, 0 not.  The
