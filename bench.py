@@ -188,7 +188,8 @@ def learned_vs_static(e13, ctx, base):
                 np.array([e13.cost_targets(r.leaves, r.steps) for r in recs], dtype=np.float32))
 
     f4 = [e13.setup(base, X, 4) for X in XS]
-    out = {"workload": "delete FMAX=4 X=0..13: plain per-leaf-launch form vs persistent static split vs learned split"}
+    out = {"workload": "delete FMAX=4 X=0..13: plain per-leaf-launch form vs persistent static split vs learned "
+                       "split vs random order (the three persistent runs interleaved, median of 3)"}
     try:
         x2, t2 = log(2)
         x3, t3 = log(3)
@@ -198,14 +199,24 @@ def learned_vs_static(e13, ctx, base):
         out["training"] = {"samples": len(x), "epochs": 150, "seconds": time.perf_counter() - t0,
                            "loss_first_epoch": losses[0], "loss_last_epoch": losses[-1]}
         ctx.set_cost_model(params)
-        out["static_split"] = run(f4, seed_per_wave=4, learned=-1)
-        out["learned_split"] = run(f4, seed_per_wave=4, learned=1)
+        # static, learned and random frontier orders, interleaved, median of 3 each
+        runs = {"static_split": [], "learned_split": [], "random_order": []}
+        for _ in range(3):
+            for name, learned in (("static_split", -1), ("learned_split", 1), ("random_order", 2)):
+                ctx.reset_tuning()
+                runs[name].append(run(f4, seed_per_wave=4, learned=learned))
+        for name, rs in runs.items():
+            med = sorted(rs, key=lambda r: r["seconds"])[1]
+            out[name] = dict(med, runs_s=[round(r["seconds"], 4) for r in rs],
+                             bit_identical_counts=all(r["bit_identical_counts"] for r in rs))
+        ctx.reset_tuning()
         out["plain_form1"] = run(f4, form=1, seed_per_wave=-1, learned=-1)
     finally:
         ctx.reset_tuning()
         ctx.set_cost_model(None)
     out["learned_over_plain"] = out["plain_form1"]["seconds"] / out["learned_split"]["seconds"]
     out["learned_over_static"] = out["static_split"]["seconds"] / out["learned_split"]["seconds"]
+    out["random_over_static"] = out["static_split"]["seconds"] / out["random_order"]["seconds"]
     return out
 
 
