@@ -16,8 +16,8 @@ beside it.  Once each, with counts checked: the symmetric 3-state counter
 search (3,078,656,272 leaves), the FMAX=3 search (36,383,467 leaves), the
 2^22-table synthetic stress batch (refilling vs plain form, digest checked),
 the counter search in the plain per-leaf-launch form vs the persistent form,
-FMAX=4 in the plain form and with the static, probe-ordered and learned
-frontier orders, and the probe estimator (the reference's estimate mode) on
+FMAX=4 in the plain form and with the static, learned and random frontier
+orders (interleaved, median of 3), and the probe estimator (the reference's estimate mode) on
 the counter search, its first probes matched bit for bit against the dump of
 the reference's probe().
 

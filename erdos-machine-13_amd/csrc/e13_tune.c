@@ -48,7 +48,7 @@ static int32_t *knob(e13_tuning *t, int k) {
 
 /* Candidate values (-1: off / none, as tuning_merge reads it). */
 static const int32_t cand[NKNOB][6] = {
-    {1, 2, 3, 4, 0},  {16, 32, 64, 128, 0}, {1, 2, 4, 0, 0},     {-1, 1024, 4096, 16384, 65536, 0}, {-1, 2, 8, 0, 0},
+    {1, 2, 3, 4, 0},  {16, 32, 64, 128, 256, 0}, {1, 2, 4, 0, 0},     {-1, 1024, 4096, 16384, 65536, 0}, {-1, 2, 8, 0, 0},
     {2, 4, 8, 16, 0}, {8, 32, 128, 0, 0},   {128, 256, 0, 0, 0}, {1, 4, 16, 0, 0},                  {-1, 4096, 16384, 65536, 0},
     {-1, 16, 32, 48, 0}};
 
