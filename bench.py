@@ -16,8 +16,9 @@ beside it.  Once each, with counts checked: the symmetric 3-state counter
 search (3,078,656,272 leaves), the FMAX=3 search (36,383,467 leaves), the
 2^22-table synthetic stress batch (refilling vs plain form, digest checked),
 the counter search in the plain per-leaf-launch form vs the persistent form,
-FMAX=4 in the plain form and with the static, learned and random frontier
-orders (interleaved, median of 3), and the probe estimator (the reference's estimate mode) on
+FMAX=4 in the plain form and with the static, learned, random and
+probe-estimated frontier orders (interleaved, median of 3), and the probe
+estimator (the reference's estimate mode) on
 the counter search, its first probes matched bit for bit against the dump of
 the reference's probe().
 
@@ -189,7 +190,8 @@ def learned_vs_static(e13, ctx, base):
 
     f4 = [e13.setup(base, X, 4) for X in XS]
     out = {"workload": "delete FMAX=4 X=0..13: plain per-leaf-launch form vs persistent static split vs learned "
-                       "split vs random order (the three persistent runs interleaved, median of 3)"}
+                       "split vs random order vs probe-estimated split (16 probes per frontier node; the four "
+                       "persistent runs interleaved, median of 3)"}
     try:
         x2, t2 = log(2)
         x3, t3 = log(3)
@@ -200,9 +202,9 @@ def learned_vs_static(e13, ctx, base):
                            "loss_first_epoch": losses[0], "loss_last_epoch": losses[-1]}
         ctx.set_cost_model(params)
         # static, learned and random frontier orders, interleaved, median of 3 each
-        runs = {"static_split": [], "learned_split": [], "random_order": []}
+        runs = {"static_split": [], "learned_split": [], "random_order": [], "probe_order": []}
         for _ in range(3):
-            for name, learned in (("static_split", -1), ("learned_split", 1), ("random_order", 2)):
+            for name, learned in (("static_split", -1), ("learned_split", 1), ("random_order", 2), ("probe_order", 3)):
                 ctx.reset_tuning()
                 runs[name].append(run(f4, seed_per_wave=4, learned=learned))
         for name, rs in runs.items():
@@ -217,6 +219,8 @@ def learned_vs_static(e13, ctx, base):
     out["learned_over_plain"] = out["plain_form1"]["seconds"] / out["learned_split"]["seconds"]
     out["learned_over_static"] = out["static_split"]["seconds"] / out["learned_split"]["seconds"]
     out["random_over_static"] = out["static_split"]["seconds"] / out["random_order"]["seconds"]
+    out["probe_over_static"] = out["static_split"]["seconds"] / out["probe_order"]["seconds"]
+    out["learned_over_probe"] = out["probe_order"]["seconds"] / out["learned_split"]["seconds"]
     return out
 
 

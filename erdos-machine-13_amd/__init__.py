@@ -88,7 +88,7 @@ MLP_PARAMS = 6402
 
 class CostRec(ctypes.Structure):
     _fields_ = [("x", ctypes.c_float * NFEAT), ("leaves", ctypes.c_uint64), ("steps", ctypes.c_uint64),
-                ("prob", ctypes.c_int32), ("depth", ctypes.c_int32)]
+                ("prob", ctypes.c_int32), ("depth", ctypes.c_int32), ("est_steps", ctypes.c_double)]
 
 
 class TuneReport(ctypes.Structure):

@@ -33,6 +33,10 @@ size_t e13k_probe_lds_bytes(int wg_threads, int gate);
 int e13k_probe_max_wgs(int wg_threads, int gate);
 int e13k_launch_probe(const E13ProbeArgs *a, int grid, int wg_threads, int gate, hipStream_t s);
 int e13k_scratch_addr(int device, void **ptr);
+#define E13_FRONTIER_PROBES 16 /* probes per frontier node when the frontier is ordered by probes */
+void e13i_probe_root(const e13_problem *p, E13Node *nd);
+int e13i_probe_nodes(e13_ctx *c, const e13_problem *p, const E13Problem *pb_dev, const E13Node *nodes, size_t n,
+                     int per_node, uint64_t seed, double *est);
 int e13k_launch_zero(void *p, uint32_t nwords, hipStream_t s);
 
 _Static_assert(sizeof(E13Node) % 16 == 0, "node records are copied as 16-byte vectors");
@@ -52,6 +56,11 @@ struct e13_ctx {
   E13Counters *d_ctr;
   uint64_t *d_key, *d_tgt;
   E13Problem *d_pb;
+  E13Node *d_proot;      /* the probe estimator's root node */
+  E13Counters *d_pctr;   /* counters of the frontier's probe launches (the search keeps d_ctr) */
+  double *seed_est;      /* subtree-steps estimate of each frontier node, in frontier order (the
+                            order's cost: learned = 1 the network, 3 the probes), for the cost log */
+  size_t seed_est_n, seed_est_cap;
   int32_t *d_dl;
   E13Node *d_stack;
   size_t stack_cap;

@@ -149,12 +149,13 @@ typedef struct E13Mail {
 
 /* Static device scratch of the estimator entry point that allocates nothing:
    per-probe sums for up to E13_SCRATCH_PROBES probes (4 doubles each), then the
-   launch counters (E13Counters at E13_SCRATCH_CTR) and the problem record
-   (E13Problem at E13_SCRATCH_PB). */
+   launch counters (E13Counters at E13_SCRATCH_CTR), the problem record
+   (E13Problem at E13_SCRATCH_PB) and the root node (E13Node at E13_SCRATCH_ROOT). */
 #define E13_SCRATCH_PROBES (1 << 16)
 #define E13_SCRATCH_CTR (E13_SCRATCH_PROBES * 32)
 #define E13_SCRATCH_PB (E13_SCRATCH_CTR + 1024)
-#define E13_SCRATCH_BYTES (E13_SCRATCH_PB + 1024)
+#define E13_SCRATCH_ROOT (E13_SCRATCH_PB + 1024)
+#define E13_SCRATCH_BYTES (E13_SCRATCH_ROOT + 512)
 
 #define E13_MAXPROB 16
 #define E13_PCNT 16       /* per-problem counters: leaves, steps, survivors, hist[0..9] */
@@ -220,8 +221,12 @@ typedef struct E13ProbeArgs {
   uint64_t seed;          /* probe p0 + i draws from the stream of (seed, p0 + i) */
   uint64_t p0;
   uint32_t n;             /* probes in this launch */
-  int32_t nq, slice, budget, nintro;
+  int32_t nq, slice;
   int32_t solo;           /* > 0: steps per round of the scalar loop for a wave's lone running lane */
+  int32_t pad_p;
+  const E13Node *roots;   /* probe p0 + i starts at roots[(p0 + i) / per_root]: a stopped machine with its
+                             pending cell (a root without one, paused at a cap, gets sums of -1) */
+  uint64_t per_root;
 } E13ProbeArgs;
 
 #endif

@@ -19,7 +19,25 @@
 _Static_assert(E13_STATIC_PROBES == E13_SCRATCH_PROBES, "the static estimator's limit is its scratch's size");
 _Static_assert(sizeof(E13Counters) <= E13_SCRATCH_PB - E13_SCRATCH_CTR && sizeof(E13Counters) % 4 == 0,
                "the counters fit their scratch slot and clear as words");
-_Static_assert(sizeof(E13Problem) <= E13_SCRATCH_BYTES - E13_SCRATCH_PB, "the problem record fits its scratch slot");
+_Static_assert(sizeof(E13Problem) <= E13_SCRATCH_ROOT - E13_SCRATCH_PB, "the problem record fits its scratch slot");
+_Static_assert(sizeof(E13Node) <= E13_SCRATCH_BYTES - E13_SCRATCH_ROOT, "the root node fits its scratch slot");
+
+/* The estimate's root as a node: the blank tape stopped before the read of
+   cell 0 at t = 0, every cell unread (e13search.c:265-271: probe() starts with
+   fixed[] all zero, so editable cells branch and free cells take any move). */
+void e13i_probe_root(const e13_problem *p, E13Node *nd) {
+  memset(nd, 0, sizeof *nd);
+  nd->q = E13_QOFF;
+  nd->q0 = E13_QOFF;
+  nd->nt = 1;
+  nd->st0 = -1;
+  nd->flags = 3; /* abs_ok, rel_ok */
+  nd->budget = p->budget;
+  nd->nintro = p->sym_n0;
+  nd->pend = 0;
+  for (int i = 0; i < E13_MAXCELLS; i++)
+    nd->tab[i] = i < 2 * p->nq && p->tab[i] != E13_FORCED ? E13_C_EDIT : E13_C_FREE;
+}
 
 void e13i_free_estimate(e13_ctx *c) {
   hipFree(c->d_probe_out);
@@ -33,6 +51,7 @@ typedef struct {
   double *out;
   E13Counters *ctr;
   E13Problem *pb;
+  E13Node *root;
   size_t chunk;
   int static_scratch; /* counters cleared by a fill kernel (no memset node) */
 } est_bufs;
@@ -75,10 +94,14 @@ static int estimate_run(e13_ctx *c, const e13_problem *p, uint64_t seed, int64_t
   a.seed = seed;
   a.nq = p->nq;
   a.slice = c->tune.probe_slice;
-  a.budget = p->budget;
-  a.nintro = p->sym_n0;
   a.solo = c->tune.solo_steps;
-  if (hipMemcpyAsync(b->pb, &pb, sizeof pb, hipMemcpyHostToDevice, c->stream) != hipSuccess) return E13_EHIP;
+  a.roots = b->root;
+  a.per_root = (uint64_t)probes; /* every probe starts at the one root */
+  E13Node root;
+  e13i_probe_root(p, &root);
+  if (hipMemcpyAsync(b->pb, &pb, sizeof pb, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipMemcpyAsync(b->root, &root, sizeof root, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+    return E13_EHIP;
   for (int64_t p0 = 0; rc == E13_OK && p0 < probes; p0 += (int64_t)b->chunk) {
     const size_t n = (size_t)(probes - p0) < b->chunk ? (size_t)(probes - p0) : b->chunk;
     a.p0 = (uint64_t)p0;
@@ -155,7 +178,7 @@ int e13_estimate(e13_ctx *c, const e13_problem *p, uint64_t seed, int64_t probes
   if (rc) return rc;
   double *host = per_probe ? NULL : (double *)malloc(chunk * 4 * sizeof(double));
   if (!per_probe && !host) return E13_ENOMEM;
-  const est_bufs b = {c->d_probe_out, c->d_ctr, c->d_pb, chunk, 0};
+  const est_bufs b = {c->d_probe_out, c->d_ctr, c->d_pb, c->d_proot, chunk, 0};
   rc = estimate_run(c, p, seed, probes, per_probe, host, &b, out);
   free(host);
   return rc;
@@ -180,6 +203,66 @@ int e13_estimate_static(e13_ctx *c, const e13_problem *p, uint64_t seed, int64_t
   HIPCHK(hipSetDevice(c->device));
   unsigned char *s = (unsigned char *)c->scratch;
   const est_bufs b = {(double *)s, (E13Counters *)(s + E13_SCRATCH_CTR), (E13Problem *)(s + E13_SCRATCH_PB),
-                      (size_t)E13_SCRATCH_PROBES, 1};
+                      (E13Node *)(s + E13_SCRATCH_ROOT), (size_t)E13_SCRATCH_PROBES, 1};
   return estimate_run(c, p, seed, probes, per_probe, NULL, &b, out);
+}
+
+/* Subtree steps below n device nodes of one problem (pb_dev its device
+   record): the mean step sum of per_node probes from each node, the
+   reference's probe() started at that node instead of the root; -1 for a node
+   with no pending cell (a machine paused at a cap). */
+int e13i_probe_nodes(e13_ctx *c, const e13_problem *p, const E13Problem *pb_dev, const E13Node *nodes, size_t n,
+                     int per_node, uint64_t seed, double *est) {
+  if (n == 0) return E13_OK;
+  if (per_node < 1 || (uint64_t)n * (uint64_t)per_node > 0xFFFFFFFFull) return E13_EINVAL;
+  int gate, wg, grid;
+  int rc = probe_shape(c, &gate, &wg, &grid);
+  if (rc) return rc;
+  const size_t total = n * (size_t)per_node;
+  if ((rc = e13i_ensure_buf((void **)&c->d_probe_out, &c->probe_cap, total, 4 * sizeof(double)))) return rc;
+  double *host = (double *)malloc(total * 4 * sizeof(double));
+  if (!host) return E13_ENOMEM;
+  E13ProbeArgs a;
+  memset(&a, 0, sizeof a);
+  a.pb = pb_dev;
+  a.key = c->d_key;
+  a.tgt = c->d_tgt;
+  a.out = c->d_probe_out;
+  a.ctr = c->d_pctr;
+  a.seed = seed;
+  a.n = (uint32_t)total;
+  a.nq = p->nq;
+  a.slice = c->tune.probe_slice;
+  a.solo = c->tune.solo_steps;
+  a.roots = nodes;
+  a.per_root = (uint64_t)per_node;
+  E13Counters ctr;
+  rc = E13_OK;
+  if (hipMemsetAsync(c->d_pctr, 0, sizeof(E13Counters), c->stream) != hipSuccess) rc = E13_EHIP;
+  int lr = rc ? 0 : e13k_launch_probe(&a, grid, wg, gate, c->stream);
+  if (lr != 0) {
+    fprintf(stderr, "libe13: probe launch failed: %s\n", hipGetErrorString((hipError_t)lr));
+    rc = E13_EHIP;
+  }
+  if (!rc && (hipMemcpyAsync(host, c->d_probe_out, total * 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream) !=
+                  hipSuccess ||
+              hipMemcpyAsync(&ctr, c->d_pctr, sizeof ctr, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+              hipStreamSynchronize(c->stream) != hipSuccess))
+    rc = E13_EHIP;
+  if (!rc && ctr.overflow) {
+    fprintf(stderr, "libe13: probe kernel error flags %u\n", ctr.overflow);
+    rc = E13_EHIP;
+  }
+  for (size_t i = 0; !rc && i < n; i++) {
+    double s = 0.0;
+    int paused = 0;
+    for (int j = 0; j < per_node; j++) {
+      const double v = host[4 * (i * (size_t)per_node + (size_t)j) + 2];
+      paused |= v < 0.0;
+      s += v;
+    }
+    est[i] = paused ? -1.0 : s / (double)per_node;
+  }
+  free(host);
+  return rc;
 }

@@ -207,7 +207,7 @@ static void tuning_merge(e13_tuning *dst, const e13_tuning *src) {
   if (src->graph_chain < 0) dst->graph_chain = 0; /* negative: host loop */
   if (src->cost_log > 0) dst->cost_log = 1;
   if (src->cost_log < 0) dst->cost_log = 0;
-  if (src->learned > 0) dst->learned = src->learned > 2 ? 2 : src->learned;
+  if (src->learned > 0) dst->learned = src->learned > 3 ? 3 : src->learned;
   if (src->solo_steps > 0) dst->solo_steps = src->solo_steps;
   if (src->solo_steps < 0) dst->solo_steps = 0;
   if (src->evict_after > 0) dst->evict_after = src->evict_after;
@@ -307,6 +307,8 @@ int e13_open(int device, const e13_tuning *t, e13_ctx **out) {
       hipMalloc((void **)&c->d_key, sizeof key) != hipSuccess ||
       hipMalloc((void **)&c->d_tgt, sizeof tgt) != hipSuccess ||
       hipMalloc((void **)&c->d_pb, sizeof(E13Problem)) != hipSuccess ||
+      hipMalloc((void **)&c->d_proot, sizeof(E13Node)) != hipSuccess ||
+      hipMalloc((void **)&c->d_pctr, sizeof(E13Counters)) != hipSuccess ||
       hipMalloc((void **)&c->d_dl, sizeof dl) != hipSuccess) {
     e13_close(c);
     return E13_ENOMEM;
@@ -329,6 +331,9 @@ void e13_close(e13_ctx *c) {
   hipFree(c->d_key);
   hipFree(c->d_tgt);
   hipFree(c->d_pb);
+  hipFree(c->d_proot);
+  hipFree(c->d_pctr);
+  free(c->seed_est);
   hipFree(c->d_dl);
   hipFree(c->d_stack);
   hipFree(c->d_out);
@@ -764,64 +769,88 @@ static void finish_survivor(const e13_problem *p, const E13SurvRec *r, e13_survi
   s->read_mask = mask;
 }
 
-/* ---- the frontier and the cost model ---- */
-static int frontier_predict(e13_ctx *c, const e13_problem *ps, const E13Node *hn, size_t n, float *cost) {
+/* ---- the frontier and its cost estimates ---- */
+/* Estimated subtree steps below the frontier nodes d[0..n) (host copy hn): the
+   installed network's prediction (mode 1) or the mean of E13_FRONTIER_PROBES
+   probes from each node (mode 3, the non-learned estimate: the reference's
+   probe() started at the node).  A node paused at a cap (no pending cell) is
+   ranked costliest: it has run past the cap already. */
+static int frontier_estimate(e13_ctx *c, const e13_problem *ps, int mode, const E13Node *d, const E13Node *hn,
+                             size_t n, double *est) {
+  if (mode == 3) {
+    for (size_t i0 = 0; i0 < n;) { /* one launch per run of nodes of one problem */
+      size_t i1 = i0 + 1;
+      while (i1 < n && hn[i1].prob == hn[i0].prob) i1++;
+      const int k = hn[i0].prob;
+      const int rc = e13i_probe_nodes(c, &ps[k], c->d_pbs + k, d + i0, i1 - i0, E13_FRONTIER_PROBES,
+                                      0xF20E5EEDull + (uint64_t)i0, est + i0);
+      if (rc) return rc;
+      i0 = i1;
+    }
+    for (size_t i = 0; i < n; i++)
+      if (est[i] < 0.0) est[i] = 1e300;
+    return E13_OK;
+  }
   float *x = (float *)malloc(n * E13_NFEAT * sizeof(float)), *y = (float *)malloc(n * 2 * sizeof(float));
   int rc = E13_ENOMEM;
   if (x && y) {
     for (size_t i = 0; i < n; i++) e13i_node_features(&ps[hn[i].prob], &hn[i], x + i * E13_NFEAT);
     rc = e13i_mlp_predict_dev(c, x, (int)n, y);
-    if (!rc)
-      for (size_t i = 0; i < n; i++) cost[i] = y[2 * i + 1]; /* predicted log steps of the subtree */
+    if (!rc) /* the network's second output is log1p(subtree steps) / 24 */
+      for (size_t i = 0; i < n; i++) est[i] = expm1(24.0 * (double)y[2 * i + 1]);
   }
   free(x);
   free(y);
   return rc;
 }
 
-static const float *g_sort_cost;
-static int by_cost_desc(const void *a, const void *b) {
-  const float ca = g_sort_cost[*(const int32_t *)a], cb = g_sort_cost[*(const int32_t *)b];
+static const double *g_sort_est;
+static int by_est_desc(const void *a, const void *b) {
+  const double ca = g_sort_est[*(const int32_t *)a], cb = g_sort_est[*(const int32_t *)b];
   if (ca != cb) return ca > cb ? -1 : 1;
   return *(const int32_t *)a - *(const int32_t *)b;
 }
 
-/* Nodes d[0..n) by predicted cost, costliest first (host copy in *hn_out). */
-static int frontier_sorted(e13_ctx *c, const e13_problem *ps, const E13Node *d, size_t n, E13Node **hn_out,
-                           int32_t **ord_out) {
+/* Nodes d[0..n) by estimated cost, costliest first: host copy *hn_out, order
+   *ord_out, estimates *est_out (indexed like hn). */
+static int frontier_sorted(e13_ctx *c, const e13_problem *ps, int mode, const E13Node *d, size_t n,
+                           E13Node **hn_out, int32_t **ord_out, double **est_out) {
   E13Node *hn = (E13Node *)malloc(n * sizeof(E13Node));
-  float *cost = (float *)malloc(n * sizeof(float));
+  double *est = (double *)malloc(n * sizeof(double));
   int32_t *ord = (int32_t *)malloc(n * sizeof(int32_t));
   int rc = E13_ENOMEM;
-  if (hn && cost && ord) {
+  if (hn && est && ord) {
     rc = E13_OK;
     if (hipMemcpyAsync(hn, d, n * sizeof(E13Node), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess)
       rc = E13_EHIP;
-    if (!rc) rc = frontier_predict(c, ps, hn, n, cost);
+    if (!rc) rc = frontier_estimate(c, ps, mode, d, hn, n, est);
     if (!rc) {
       for (size_t i = 0; i < n; i++) ord[i] = (int32_t)i;
-      g_sort_cost = cost;
-      qsort(ord, n, sizeof(int32_t), by_cost_desc);
+      g_sort_est = est;
+      qsort(ord, n, sizeof(int32_t), by_est_desc);
     }
   }
-  free(cost);
   if (rc) {
     free(hn);
+    free(est);
     free(ord);
     return rc;
   }
   *hn_out = hn;
   *ord_out = ord;
+  *est_out = est;
   return E13_OK;
 }
 
-/* Split step of the learned seeding: the costlier half of the frontier d[0..n)
+/* Split step of the ordered seeding: the costlier half of the frontier d[0..n)
    stays in d (to be expanded), the rest goes to `seeds`; *ne = nodes kept. */
-static int learned_split(e13_ctx *c, const e13_problem *ps, E13Node *d, size_t n, E13Node *seeds, size_t *ne) {
+static int learned_split(e13_ctx *c, const e13_problem *ps, int mode, E13Node *d, size_t n, E13Node *seeds,
+                         size_t *ne) {
   E13Node *hn = NULL;
   int32_t *ord = NULL;
-  int rc = frontier_sorted(c, ps, d, n, &hn, &ord);
+  double *est = NULL;
+  int rc = frontier_sorted(c, ps, mode, d, n, &hn, &ord, &est);
   if (rc) return rc;
   const size_t keep = (n + 1) / 2;
   E13Node *tmp = (E13Node *)malloc(n * sizeof(E13Node));
@@ -838,21 +867,36 @@ static int learned_split(e13_ctx *c, const e13_problem *ps, E13Node *d, size_t n
   free(tmp);
   free(hn);
   free(ord);
+  free(est);
   *ne = keep;
   return rc;
 }
 
 /* Order step: the whole frontier costliest first, so the waves that are dealt
-   seeds round-robin each start with one of the largest subtrees. */
-static int learned_order(e13_ctx *c, const e13_problem *ps, E13Node *d, size_t n) {
+   seeds round-robin each start with one of the largest subtrees; the estimates
+   are kept in the new order for the cost log. */
+static int learned_order(e13_ctx *c, const e13_problem *ps, int mode, E13Node *d, size_t n) {
   E13Node *hn = NULL;
   int32_t *ord = NULL;
-  int rc = frontier_sorted(c, ps, d, n, &hn, &ord);
+  double *est = NULL;
+  int rc = frontier_sorted(c, ps, mode, d, n, &hn, &ord, &est);
   if (rc) return rc;
   E13Node *tmp = (E13Node *)malloc(n * sizeof(E13Node));
   if (!tmp) rc = E13_ENOMEM;
+  if (!rc && n > c->seed_est_cap) {
+    double *se = (double *)realloc(c->seed_est, n * sizeof(double));
+    if (!se) rc = E13_ENOMEM;
+    else {
+      c->seed_est = se;
+      c->seed_est_cap = n;
+    }
+  }
   if (!rc) {
-    for (size_t i = 0; i < n; i++) tmp[i] = hn[ord[i]];
+    for (size_t i = 0; i < n; i++) {
+      tmp[i] = hn[ord[i]];
+      c->seed_est[i] = est[ord[i]];
+    }
+    c->seed_est_n = n;
     if (hipMemcpyAsync(d, tmp, n * sizeof(E13Node), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess)
       rc = E13_EHIP;
@@ -860,6 +904,7 @@ static int learned_order(e13_ctx *c, const e13_problem *ps, E13Node *d, size_t n
   free(tmp);
   free(hn);
   free(ord);
+  free(est);
   return rc;
 }
 
@@ -914,6 +959,7 @@ static int collect_cost_log(e13_ctx *c, const e13_problem *ps, size_t n) {
       r->steps = cnt[2 * i + 1];
       r->prob = hn[i].prob;
       r->depth = hn[i].depth;
+      r->est_steps = c->seed_est_n == n ? c->seed_est[i] : 0.0;
     }
     c->cost_n = (int64_t)n;
   }
@@ -983,7 +1029,9 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
      of about four nodes per wave; leaves met on the way are counted here.  The
      persistent launch then starts with the frontier dealt to the waves. */
   const size_t seed_target = (size_t)tu->seed_per_wave * (size_t)total_waves;
-  const int learned = tu->learned == 1 && c->have_model;
+  /* frontier order: 1 by the installed network, 3 by probes from each node (non-learned) */
+  const int order_mode = tu->learned == 1 && c->have_model ? 1 : (tu->learned == 3 ? 3 : 0);
+  c->seed_est_n = 0;
   if ((rc = e13i_ensure_buf((void **)&c->d_seeds, &c->seeds_cap, 64 * (size_t)total_waves + (size_t)np, sizeof(E13Node))))
     return rc;
   if ((rc = e13i_ensure_buf((void **)&c->d_stack, &c->stack_cap, c->seeds_cap, sizeof(E13Node)))) return rc;
@@ -1027,9 +1075,9 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
     ea.refill_tries = tu->refill_tries;
     ea.chunk = 64;
     for (int level = 0; level < 12 && top > 0 && top < target_k && seed_target > 0; level++) {
-      if (learned && top > 1) { /* split by predicted cost: expand the costlier half, keep the rest as seeds */
+      if (order_mode && top > 1) { /* split by estimated cost: expand the costlier half, keep the rest as seeds */
         size_t ne = top;
-        if ((rc = learned_split(c, ps, c->d_stack, top, c->d_seeds + nseeds, &ne))) return rc;
+        if ((rc = learned_split(c, ps, order_mode, c->d_stack, top, c->d_seeds + nseeds, &ne))) return rc;
         nseeds += top - ne;
         top = ne;
       }
@@ -1077,7 +1125,7 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
     prev[2] = h1.survivors;
     for (int j = 0; j <= E13_NS; j++) prev[3 + j] = h1.hist[j];
   }
-  if (!resumed && learned && nseeds > 1 && (rc = learned_order(c, ps, c->d_seeds, nseeds))) return rc;
+  if (!resumed && order_mode && nseeds > 1 && (rc = learned_order(c, ps, order_mode, c->d_seeds, nseeds))) return rc;
   if (!resumed && tu->learned == 2 && nseeds > 1 && (rc = random_order(c, c->d_seeds, nseeds))) return rc;
   const int logging = tu->cost_log == 1 && !ckpt; /* subtree costs need the frontier of one launch */
   if (logging) { /* number the frontier and count each subtree */

@@ -2102,25 +2102,25 @@ __global__ void __launch_bounds__(256) k_probe(E13ProbeArgs a) {
         uint32_t p = 0;
         if (lane == 0) p = atomicAdd(&a.ctr->item, 1u);
         p = (uint32_t)uni((int)p);
+        const E13Node *rt = a.roots + (size_t)((a.p0 + p) / a.per_root);
+        const int rpend = p < a.n ? uni((int)rt->pend) : 0;
         if (p >= a.n) {
           exhausted = true;
+        } else if (rpend < 0 || rpend >= 2 * nq) { // a machine paused at a cap has no pending cell: no probe
+          if (lane < 4) a.out[4 * (size_t)p + lane] = -1.0;
+          progress = true;
         } else {
-          // the root: the blank tape stopped before the read of cell 0 at t = 0
-          // (e13search.c:268-271: nodes 1, steps 0)
-          if (lane < E13_TW) lds_st(c.sb, sb + (PS_TAPE + lane) * 4, 0u);
-          if (lane < 20) {
-            int v = 0;
-            if (lane == 0 || lane == 6) v = E13_QOFF; // q, q0
-            if (lane == 3) v = 1;                      // nt
-            if (lane == 5) v = -1;                     // st0
-            if (lane == 7) v = 3;                      // abs_ok, rel_ok
-            if (lane == 17) v = a.budget;
-            if (lane == 18) v = a.nintro;
-            lds_st(c.sb, sb + (PS_REG + lane) * 4, (uint32_t)v);
+          // the probe's first node: root (a.p0 + p) / per_root, a stopped
+          // machine with its pending cell (the estimate's root is the blank
+          // tape stopped before the read of cell 0 at t = 0, e13search.c:268-271)
+          const uint32_t *nw = (const uint32_t *)rt;
+          if (lane < E13_TW) lds_st(c.sb, sb + (PS_TAPE + lane) * 4, nw[E13_NODE_W_TAPE + lane]);
+          if (lane < 20) { // q st t nt stage st0 q0 flags ones | hb ch0 ch1 ch2 | budget nintro pend
+            const int w = lane < 9 ? lane : (lane < 17 ? lane + 7 : lane - 8);
+            lds_st(c.sb, sb + (PS_REG + lane) * 4, nw[w]);
           }
-          if (lane < E13_MAXCELLS)
-            sbytes[sb + PS_TAB * 4 + lane] = lane < 2 * nq && base[lane] != -2 ? E13_C_EDIT : E13_C_FREE;
-          if (lane == 0) {
+          if (lane < E13_MAXCELLS) sbytes[sb + PS_TAB * 4 + lane] = rt->tab[lane];
+          if (lane == 0) { // nodes 1, steps 0 (e13search.c:268-271)
             lds_st(c.sb, sb + PS_P * 4, p);
             lds_st64(c, sb + PS_RNG * 4, probe_stream(a.seed, a.p0 + p));
             lds_st64(c, sb + PS_W * 4, __builtin_bit_cast(uint64_t, 1.0));

@@ -107,7 +107,9 @@ typedef struct {
   /* persistent form: the frontier and the cost model (e13_set_cost_model) */
   int32_t cost_log;       /* 1: log each frontier node's features and subtree leaves/steps (e13_get_cost_log) */
   int32_t learned;        /* 1: split and order the frontier by the installed cost model; 2: a seeded
-                             random order (a check); 0 or negative: the static split */
+                             random order (a check); 3: split and order it by probe estimates of each
+                             node's subtree (the reference's probe() from the node, 16 probes: the
+                             non-learned baseline); 0 or negative: the static split */
   /* tape placement: a wave whose only running lane has nothing to refill keeps that lane's
      rule rows and current tape word in registers and steps it on the scalar unit */
   int32_t solo_steps;     /* > 0: steps per such round (registers); negative: off (the LDS loop only) */
@@ -394,6 +396,8 @@ typedef struct {
   float x[E13_NFEAT];     /* features of the frontier node */
   uint64_t leaves, steps; /* of its subtree */
   int32_t prob, depth;
+  double est_steps;       /* the subtree-steps estimate the frontier order used (learned = 1: the
+                             network's, 3: the probes' mean; 1e300 for a node paused at a cap), else 0 */
 } e13_cost_rec;
 
 /* Frontier log of the last persistent search run with tuning.cost_log = 1. */

@@ -53,10 +53,10 @@ def test_forward_kernel_matches_oracle(e13, gpu_ctx):
     assert np.max(np.abs(y - yref)) <= 1e-5 * (1 + np.max(np.abs(yref)))
 
 
-def cost_log(e13, ctx, base, fmax, want):
+def cost_log(e13, ctx, base, fmax, want, learned=0):
     """A frontier cost log of the local-edit search (counts checked)."""
-    say(f"search: delete FMAX={fmax} X=0..13 with cost logging")
-    ctx.set_tuning(seed_per_wave=4, cost_log=1)
+    say(f"search: delete FMAX={fmax} X=0..13 with cost logging, learned={learned}")
+    ctx.set_tuning(seed_per_wave=4, cost_log=1, learned=learned if learned else -1)
     try:
         cs, sv, ns = ctx.search_many([e13.setup(base, X, fmax) for X in XS], surv_cap=4096)
         log = ctx.cost_log()
@@ -67,6 +67,8 @@ def cost_log(e13, ctx, base, fmax, want):
     x = np.array([list(r.x) for r in log], dtype=np.float32)
     t = np.array([e13.cost_targets(r.leaves, r.steps) for r in log], dtype=np.float32)
     steps = np.array([r.steps for r in log], dtype=np.float64)
+    if learned:
+        return x, t, steps, np.array([r.est_steps for r in log], dtype=np.float64)
     return x, t, steps
 
 
@@ -80,18 +82,25 @@ def ranks(v):
 # FMAX=4 search slower than the static order by more than this factor (median
 # of 5 interleaved runs; run-to-run spread on one box is about 2-4%).
 ORDER_SLOWDOWN_MAX = 1.04
+# The probe-estimated order pays for its probes inside the timed search (16
+# probes from every frontier node at each seeding level, about 60 ms here).
+PROBE_SLOWDOWN_MAX = 1.10
 
 
 def test_trained_model_ranks_subtrees_and_its_order_is_timed(e13, gpu_ctx, base):
     """Train on the FMAX=2 and FMAX=3 logs (GPU kernels) and predict the
     held-out FMAX=4 frontier: the prediction must rank subtrees (rank
-    correlation > 0.3 against the logged subtree steps).  Then the FMAX=4
-    search is timed on the GPU with the static frontier, the learned split and
-    order, and a random order, interleaved, 5 runs each: every order gives the
-    reference's counts, and neither the learned nor the random order may be
-    slower than the static one by more than ORDER_SLOWDOWN_MAX (medians).  The
-    measured ratios are printed: with push scheduling and the eviction of long
-    runners the frontier order does not decide the makespan."""
+    correlation > 0.3 against the logged subtree steps), and so must the
+    non-learned estimate, 16 probes of the reference's probe() from each
+    frontier node (both correlations printed).  Then the FMAX=4 search is timed
+    on the GPU with the static frontier, the learned split and order, a random
+    order and the probe-estimated split and order, interleaved, 5 runs each:
+    every order gives the reference's counts, and neither the learned nor the
+    random order may be slower than the static one by more than
+    ORDER_SLOWDOWN_MAX (medians), the probe-estimated one (which pays for its
+    probes) by more than PROBE_SLOWDOWN_MAX.  The measured ratios
+    are printed: with push scheduling and the eviction of long runners the
+    frontier order does not decide the makespan."""
     import statistics
     import time
     f2 = parse_summaries(read_text(REF_DUMP, "delete_f2.txt"))
@@ -112,13 +121,24 @@ def test_trained_model_ranks_subtrees_and_its_order_is_timed(e13, gpu_ctx, base)
     rho = np.corrcoef(ranks(y[:, 1]), ranks(t4[:, 1]))[0, 1]
     say(f"held-out FMAX=4 frontier ({len(x4)} subtrees): rank correlation {rho:.3f}")
     assert rho > 0.3
+    gpu_ctx.set_cost_model(P)
+    try:
+        _, _, s1, e1 = cost_log(e13, gpu_ctx, base, 4, f4, learned=1)
+        _, _, s3, e3 = cost_log(e13, gpu_ctx, base, 4, f4, learned=3)
+    finally:
+        gpu_ctx.set_cost_model(None)
+    rho1 = np.corrcoef(ranks(e1), ranks(s1))[0, 1]
+    rho3 = np.corrcoef(ranks(e3), ranks(s3))[0, 1]
+    say(f"frontier order's estimates against the subtree steps: network {rho1:.3f} ({len(s1)} nodes), "
+        f"probes {rho3:.3f} ({len(s3)} nodes)")
+    assert rho1 > 0.3 and rho3 > 0.3
     ps = [e13.setup(base, X, 4) for X in XS]
     want = sum(f4[X]["steps"] for X in XS)
-    times = {0: [], 1: [], 2: []}
+    times = {0: [], 1: [], 2: [], 3: []}
     gpu_ctx.set_cost_model(P)
     try:
         for rep in range(5):
-            for learned in (0, 1, 2):
+            for learned in (0, 1, 2, 3):
                 say(f"search: delete FMAX=4 X=0..13 learned={learned} run {rep}")
                 gpu_ctx.reset_tuning()
                 if learned:
@@ -132,16 +152,19 @@ def test_trained_model_ranks_subtrees_and_its_order_is_timed(e13, gpu_ctx, base)
         gpu_ctx.reset_tuning()
         gpu_ctx.set_cost_model(None)
     med = {k: statistics.median(v) for k, v in times.items()}
-    say(f"FMAX=4 medians (s): static {med[0]:.4f}, learned {med[1]:.4f}, random {med[2]:.4f}; "
-        f"static/learned {med[0] / med[1]:.4f}, static/random {med[0] / med[2]:.4f}")
-    assert med[1] <= ORDER_SLOWDOWN_MAX * med[0], med
-    assert med[2] <= ORDER_SLOWDOWN_MAX * med[0], med
+    say(f"FMAX=4 medians (s): static {med[0]:.4f}, learned {med[1]:.4f}, random {med[2]:.4f}, "
+        f"probes {med[3]:.4f}; static/learned {med[0] / med[1]:.4f}, static/random {med[0] / med[2]:.4f}, "
+        f"static/probes {med[0] / med[3]:.4f}")
+    for k in (1, 2):
+        assert med[k] <= ORDER_SLOWDOWN_MAX * med[0], med
+    assert med[3] <= PROBE_SLOWDOWN_MAX * med[0], med
 
 
-@pytest.mark.parametrize("learned", [1, 2])
+@pytest.mark.parametrize("learned", [1, 2, 3])
 def test_frontier_order_never_changes_results(e13, gpu_ctx, base, learned):
-    """Learned split/order (1) and a random order (2) give the reference's
-    counts and survivors, as the static split does."""
+    """Learned split/order (1), a random order (2) and the probe-estimated
+    split/order (3) give the reference's counts and survivors, as the static
+    split does."""
     say(f"search: delete FMAX=3 X=0..13 learned={learned}")
     gpu_ctx.set_cost_model(e13.mlp_init(4))  # any model: the order may be arbitrary
     gpu_ctx.set_tuning(seed_per_wave=4, learned=learned)
