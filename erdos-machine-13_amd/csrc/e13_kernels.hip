@@ -6,9 +6,10 @@
 // LDS, per wave:
 //   rows  16 x 64 words at a 4096-byte aligned base: the word of state s for
 //         lane l holds the two rule entries (symbol 0 in bits 0..15, symbol 1 in
-//         bits 16..31).  An entry is  write | dir << 1 | next << 8, or 0x8F00 |
-//         kind << 12 for a cell that cannot be stepped (kind 1 halt, 2 unread
-//         editable cell, 3 free cell not chosen yet).  Because the row base has
+//         bits 16..31).  An entry is  write | dir << 1 | ones change << 2 |
+//         next << 8 (enc_move), or 0x8F00 | kind << 12 for a cell that cannot
+//         be stepped (kind 1 halt, 2 unread editable cell,
+//         3 free cell not chosen yet).  Because the row base has
 //         no bits in 0x0F00, the LDS address of the next row is
 //         (entry & 0x0F00) | lane_base: one VALU op.
 //   tape  43 x 64 words: word w of lane l at w*256 + l*4 (every lane in its own
@@ -60,8 +61,12 @@ __device__ __forceinline__ uint32_t lds_base(const void *p) {
 // low bit is the tape flip) | next << 8; or 0x8F00 | kind << 12 for a halt,
 // an unread editable cell or a free cell (bit 15: a cell that cannot be
 // stepped, so the step loop finds it in the sign of entry << 16).
+// The entry of a move code (0..59).
+__device__ __forceinline__ uint32_t enc_move(int c, uint32_t sym) {
+  return (uint32_t)(c & 3) | ((((uint32_t)(c & 1) - sym) & 3u) << 2) | ((uint32_t)(c >> 2) << 8);
+}
 __device__ __forceinline__ uint32_t enc_entry(int c, uint32_t sym) {
-  if (c < 60) return (uint32_t)(c & 3) | ((((uint32_t)(c & 1) - sym) & 3u) << 2) | ((uint32_t)(c >> 2) << 8);
+  if (c < 60) return enc_move(c, sym);
   uint32_t kind = c == E13_C_HALT ? 1u : (c == E13_C_EDIT ? 2u : 3u);
   return 0x8F00u | (kind << 12);
 }
@@ -1138,11 +1143,6 @@ __device__ __forceinline__ int pt_word(int ptb, int k, int w) {
 }
 __device__ __forceinline__ int pt_base(int ptb, int k, int i) {
   return (int)*(const __attribute__((address_space(3))) int8_t *)(size_t)(uint32_t)(ptb + k * PT_WORDS * 4 + i);
-}
-// The entry of a move code (0..59) read on symbol sym (enc_entry without the
-// halt / unread kinds).
-__device__ __forceinline__ uint32_t enc_move(int c, uint32_t sym) {
-  return (uint32_t)(c & 3) | ((((uint32_t)(c & 1) - sym) & 3u) << 2) | ((uint32_t)(c >> 2) << 8);
 }
 
 __device__ __forceinline__ void wave_lds_sync() {
