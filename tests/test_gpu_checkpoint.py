@@ -7,11 +7,12 @@ reference's log for the same problem).  Each test prints its case before
 touching the GPU.
 """
 import os
+import re
 import subprocess
 
 import pytest
 
-from conftest import PKG_DIR, REF_DUMP, machine_path, parse_summaries, read_machine, read_text
+from conftest import PKG_DIR, REF_DUMP, REF_RESULTS, machine_path, parse_summaries, read_machine, read_text
 
 pytestmark = pytest.mark.gpu
 
@@ -249,3 +250,30 @@ def test_cli_persistent_multi_x_checkpoint(tmp_path):
         assert (got[X]["leaves"], got[X]["steps"], got[X]["survivors"], got[X]["hist"]) == (
             ref[X]["leaves"], ref[X]["steps"], ref[X]["survivors"], ref[X]["hist"]), X
 
+
+
+def test_cli_gpu_driver_v3_checkpointed(tmp_path):
+    """The division search V3 (freesym X=9 3..9 E0..I1: 53,282,072,641 leaves,
+    5.8e12 steps) through the GPU driver's argv with E13_CKPT, the way the
+    reference runs V1/V2 (src/e13gpu.py:293-300, 349-409): stopped after every
+    two launches of one second (exit code 3, clean stop, nodes saved) and re-run
+    until it ends, each run a new process resuming from the last slot.  The
+    final summary line equals the driver's own log (g_v3.log) up to the wall
+    time."""
+    ck = str(tmp_path / "v3")
+    env = dict(os.environ, E13_CKPT=ck, E13_CKPT_SEC="1", E13_CKPT_LAUNCHES="2")
+    args = [CLI, "gpu", machine_path(), "9", "3", "9"] + "E0 E1 F0 F1 G0 G1 H0 H1 I0 I1".split()
+    codes = []
+    for _ in range(60):
+        say(f"e13search_gpu gpu erdos_14 9 3 9 E0..I1 with E13_CKPT, run {len(codes) + 1}")
+        r = subprocess.run(args, env=env, capture_output=True, text=True, timeout=120)
+        codes.append(r.returncode)
+        if r.returncode != 3:
+            break
+        assert "STOPPED cleanly" in r.stderr
+    assert codes[-1] == 0 and codes.count(3) >= 2, (codes, r.stderr[-2000:])
+    strip = lambda l: re.sub(r" wall=[0-9.]+s", "", l).strip()
+    got = [strip(l) for l in r.stderr.splitlines() if " gpu leaves=" in l]
+    want = [strip(l) for l in read_text(REF_RESULTS, "4_division_6_v3_g_v3.txt").splitlines() if " gpu leaves=" in l]
+    assert got == want
+    assert not [l for l in r.stdout.splitlines() if l.startswith("SURVIVOR")]
