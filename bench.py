@@ -22,10 +22,13 @@ estimator (the reference's estimate mode) on
 the counter search, its first probes matched bit for bit against the dump of
 the reference's probe().
 
-roofline: the dominant kernel is the persistent search kernel (k_search).
-Its algorithmic work is 13 integer lane-ops and 8 LDS bytes per simulated
-step; achieved = steps simulated inside it x 13 / its time, the time from HIP
-events on the library's own stream around every launch of the timed region.
+roofline: the dominant kernel is the persistent search kernel (k_search) on
+its short lane tape (k_search<0, true>, three workgroups per CU); the
+machines that go past that tape finish in a second launch of k_search on the
+whole tape (k_search<0, false>), reported beside it.  Its algorithmic work is
+13 integer lane-ops and 8 LDS bytes per simulated step; achieved = steps
+simulated inside it x 13 / its time, the time from HIP events on the
+library's own stream around every launch of the timed region.
 peak = the gfx950 VALU int32 issue rate, 256 CU x 64 lanes/clk x 2.4 GHz =
 3.93e13 op/s (LDS: 256 CU x 128 B/clk x 2.4 GHz).  traffic = HBM bytes per
 launch from the committed rocprofv3 counter summary of the same workload.
@@ -97,13 +100,15 @@ def extra_workload(e13, ctx, name, problems, want_leaves, want_steps, want_surv)
     steps = sum(c.steps for c in cs)
     ok = leaves == want_leaves and ns == want_surv and (want_steps is None or steps == want_steps)
     ok = ok and all(s.recheck_ok == 1 for s in sv)
-    ks = (s1["search_ms"] - s0["search_ms"]) / 1e3
-    ksteps = s1["search_steps"] - s0["search_steps"]
+    ws = (s1["wide_ms"] - s0["wide_ms"]) / 1e3
+    ks = (s1["search_ms"] - s0["search_ms"]) / 1e3 - ws
+    ksteps = s1["search_steps"] - s0["search_steps"] - (s1["wide_steps"] - s0["wide_steps"])
     return {"workload": name, "leaves": leaves, "steps": steps, "survivors": ns, "bit_identical_counts": ok,
             "seconds": dt, "steps_per_s": steps / dt, "leaves_per_s": leaves / dt,
             "kernel_ms": s1["kernel_ms"] - s0["kernel_ms"],
             "settings_profile": "small-search" if s1["profile"] == 1 else "main",
-            "k_search": {"ms": ks * 1e3, "steps": ksteps, "steps_per_s": ksteps / ks,
+            "k_search": {"lane_tape_words": s1["tape_words"], "ms": ks * 1e3, "steps": ksteps, "steps_per_s": ksteps / ks,
+                         "whole_tape_rest_ms": ws * 1e3,
                          "valu_frac": ksteps * OPS_PER_STEP / ks / VALU_PEAK,
                          "lds_frac": ksteps * LDS_BYTES_PER_STEP / ks / LDS_PEAK,
                          "waves_per_simd": s1["grid"] * s1["wg_threads"] / 64 / (256 * 4),
@@ -429,12 +434,20 @@ def main():
     cyc = [b - a for a, b in zip(c0, ctx.stats()["cycles"])]
     ctx.reset_tuning()
     # the roofline is the persistent kernel's own: its HIP-event time, launches and
-    # the steps simulated inside it (the frontier launches before it are apart)
-    kernel_s = (s1["search_ms"] - s0["search_ms"]) / 1e3
-    launches = s1["search_launches"] - s0["search_launches"]
-    ksteps = s1["search_steps"] - s0["search_steps"]
-    frontier_ms = (s1["kernel_ms"] - s0["kernel_ms"]) - kernel_s * 1e3
+    # the steps simulated inside it (the frontier launches before it are apart).
+    # With a short lane tape the search is two launches of k_search per pass: the
+    # dominant short-tape one, and one on the whole tape for the machines that
+    # went past it (reported beside it)
+    wide_s = (s1["wide_ms"] - s0["wide_ms"]) / 1e3
+    wide_launches = s1["wide_launches"] - s0["wide_launches"]
+    wide_steps = s1["wide_steps"] - s0["wide_steps"]
+    kernel_s = (s1["search_ms"] - s0["search_ms"]) / 1e3 - wide_s
+    launches = s1["search_launches"] - s0["search_launches"] - wide_launches
+    ksteps = s1["search_steps"] - s0["search_steps"] - wide_steps
+    frontier_ms = (s1["kernel_ms"] - s0["kernel_ms"]) - (kernel_s + wide_s) * 1e3
     achieved = ksteps * OPS_PER_STEP / kernel_s
+    short_tape = s1["tape_words"] < 43
+    kname = "k_search<0, true>" if short_tape else "k_search<0, false>"
     extra = []
     if not args.no_extra:
         extra.append(extra_workload(e13, ctx, "3-state counter, freesym X=13 10..13 D0 G0 K0 K1 L1 M1", [counter],
@@ -485,11 +498,15 @@ def main():
                      "frac": achieved / VALU_PEAK, "traffic": traffic_from_profiles(args.workload),
                      "lds": {"achieved_GBps": ksteps * LDS_BYTES_PER_STEP / kernel_s / 1e9,
                              "peak_GBps": LDS_PEAK / 1e9, "frac": ksteps * LDS_BYTES_PER_STEP / kernel_s / LDS_PEAK},
-                     "kernel": "k_search", "launches": launches,
+                     "kernel": kname, "lane_tape_words": s1["tape_words"], "launches": launches,
                      "kernel_ms_per_launch": kernel_s * 1e3 / max(launches, 1),
                      "steps_per_launch": ksteps / max(launches, 1),
                      "steps_per_s_kernel": ksteps / kernel_s,
                      "frontier_kernels_ms_per_pass": frontier_ms / args.steps,
+                     "whole_tape_rest": {"kernel": "k_search<0, false>", "launches": wide_launches,
+                                         "ms_per_pass": wide_s * 1e3 / args.steps,
+                                         "steps_per_pass": wide_steps / args.steps,
+                                         "nodes_per_pass": (s1["wide_nodes"] - s0["wide_nodes"]) / args.steps},
                      # occupancy and divergence: resident waves per SIMD (256 CUs x 4 SIMDs),
                      # running lanes per wave round (of 64), steps per lane per round
                      "waves_per_simd": s1["grid"] * s1["wg_threads"] / 64 / (256 * 4),

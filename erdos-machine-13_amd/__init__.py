@@ -105,7 +105,8 @@ class Tuning(ctypes.Structure):
                 ("graph_chain", ctypes.c_int32), ("cost_log", ctypes.c_int32),
                 ("learned", ctypes.c_int32), ("solo_steps", ctypes.c_int32), ("evict_after", ctypes.c_int32),
                 ("probe_slice", ctypes.c_int32), ("cut_live", ctypes.c_int32),
-                ("phase_cycles", ctypes.c_int32), ("nt1_skip", ctypes.c_int32)]
+                ("phase_cycles", ctypes.c_int32), ("nt1_skip", ctypes.c_int32), ("tape_words", ctypes.c_int32),
+                ("wide_nodes", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -118,7 +119,9 @@ class Stats(ctypes.Structure):
                 ("form", ctypes.c_int32), ("profile", ctypes.c_int32), ("donations", ctypes.c_int64),
                 ("pool_takes", ctypes.c_int64), ("cycles", ctypes.c_int64 * 4), ("graphs", ctypes.c_int64),
                 ("search_ms", ctypes.c_double), ("search_launches", ctypes.c_int64), ("search_steps", ctypes.c_int64),
-                ("frontier", ctypes.c_int64)]
+                ("frontier", ctypes.c_int64), ("tape_words", ctypes.c_int32), ("pad_s", ctypes.c_int32),
+                ("wide_nodes", ctypes.c_int64), ("wide_launches", ctypes.c_int64), ("wide_ms", ctypes.c_double),
+                ("wide_steps", ctypes.c_int64)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k != "cycles"}
@@ -447,7 +450,7 @@ class Context:
 
     # fields where 0 reads back as "off" but a set of 0 means "keep"
     _OFF_FIELDS = ("seed_per_wave", "trace_events", "graph_chain", "cost_log", "learned", "solo_steps", "evict_after",
-                   "cut_live", "phase_cycles")
+                   "cut_live", "phase_cycles", "tape_words")
 
     def restore_tuning(self, t):
         """Set every field of a tuning dict read back by tuning() (off fields included)."""

@@ -200,8 +200,11 @@ int e13i_ckpt3_prepare(e13_ctx *c, int total_waves) {
   return e13i_ensure_buf((void **)&c->d_wave_state, &c->wave_state_cap, 3 * (size_t)total_waves, sizeof(int64_t));
 }
 
-/* The nodes a stopped launch left, gathered into d_seeds (count in *nseeds). */
-int e13i_ckpt3_gather(e13_ctx *c, int total_waves, int stk_cap, size_t *nseeds) {
+/* The nodes a stopped launch left, gathered into d_seeds (count in *nseeds):
+   every wave's stack range and the deliveries still in mailboxes (*n_left of
+   them), then n_extra nodes from extra (the narrow form's wide list). */
+int e13i_ckpt3_gather(e13_ctx *c, int total_waves, int stk_cap, const E13Node *extra, size_t n_extra, size_t *n_left,
+                      size_t *nseeds) {
   const size_t W = (size_t)total_waves;
   int64_t *ws = (int64_t *)malloc(3 * W * sizeof(int64_t));
   E13Mail *mail = (E13Mail *)malloc(W * sizeof(E13Mail));
@@ -217,7 +220,7 @@ int e13i_ckpt3_gather(e13_ctx *c, int total_waves, int stk_cap, size_t *nseeds) 
     const uint64_t f = mail[w].flag;
     if (f != 0 && f != E13_MAIL_DONE && f != (uint64_t)ws[3 * w + 2]) deliveries++; /* not taken */
   }
-  if (!rc) rc = e13i_ensure_buf((void **)&c->d_seeds, &c->seeds_cap, total + deliveries + 1, sizeof(E13Node));
+  if (!rc) rc = e13i_ensure_buf((void **)&c->d_seeds, &c->seeds_cap, total + deliveries + n_extra + 1, sizeof(E13Node));
   size_t off = 0;
   for (size_t w = 0; !rc && w < W; w++) {
     const int64_t bot = ws[3 * w], top = ws[3 * w + 1];
@@ -243,6 +246,12 @@ int e13i_ckpt3_gather(e13_ctx *c, int total_waves, int stk_cap, size_t *nseeds) 
     if (!rc && (hipMemcpy(c->d_seeds + off, &nd, sizeof nd, hipMemcpyHostToDevice) != hipSuccess)) rc = E13_EHIP;
     off++;
   }
+  *n_left = off;
+  if (!rc && n_extra &&
+      hipMemcpyAsync(c->d_seeds + off, extra, n_extra * sizeof(E13Node), hipMemcpyDeviceToDevice, c->stream) !=
+          hipSuccess)
+    rc = E13_EHIP;
+  off += n_extra;
   if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = E13_EHIP;
   free(ws);
   free(mail);

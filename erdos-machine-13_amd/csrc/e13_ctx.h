@@ -22,8 +22,8 @@
 size_t e13k_lds_bytes(int wg_threads, int gate);
 int e13k_launch_expand(const E13ExpandArgs *a, int grid, int wg_threads, int gate, int refill, hipStream_t s);
 int e13k_launch_batch(const E13BatchArgs *a, int grid, int wg_threads, int gate, int refill, hipStream_t s);
-size_t e13k_search_lds_bytes(int wg_threads, int gate, int np);
-int e13k_search_max_wgs(int wg_threads, int gate, int np);
+size_t e13k_search_lds_bytes(int wg_threads, int gate, int np, int tw);
+int e13k_search_max_wgs(int wg_threads, int gate, int np, int tw);
 int e13k_launch_search(const E13SearchArgs *a, int grid, int wg_threads, int gate, hipStream_t s);
 int e13k_launch_chain_pop(E13Chain *ch, E13Counters *ctr, hipStream_t s);
 int e13k_launch_chain_push(E13Chain *ch, E13Node *stack, uint64_t stack_cap, const E13Node *out, uint32_t out_cap,
@@ -46,6 +46,8 @@ struct e13_ctx {
   int device, ncu;
   char arch[256]; /* gcnArchName */
   size_t lds_max; /* largest dynamic LDS a workgroup may use */
+  size_t lds_cu;  /* LDS of one CU */
+  int narrow_off; /* a short-tape persistent launch found workgroups not resident: whole tape only */
   hipStream_t stream;
   hipEvent_t ev0, ev1;
   e13_tuning tune;
@@ -108,6 +110,8 @@ struct e13_ctx {
   uint32_t *h_stop, *d_stop; /* the persistent form's clean-stop flag: pinned host memory, its device address */
   int64_t *d_wave_state;     /* per wave: stack range and last delivery tag, written when a wave leaves */
   size_t wave_state_cap;
+  E13Node *d_wide;           /* the narrow form's list of machines past its tape, for the whole-tape launch */
+  size_t wide_cap;
 };
 
 /* The MLP kernels (e13_mlp.hip). */
@@ -134,7 +138,8 @@ E13_INTERNAL int e13i_ckpt_save(e13_ctx *c, const e13_problem *p, size_t top, si
 E13_INTERNAL int e13i_ckpt_stop_requested(e13_ctx *c);
 E13_INTERNAL void e13i_ckpt_stop_clear(e13_ctx *c);
 E13_INTERNAL int e13i_ckpt3_prepare(e13_ctx *c, int total_waves);
-E13_INTERNAL int e13i_ckpt3_gather(e13_ctx *c, int total_waves, int stk_cap, size_t *nseeds);
+E13_INTERNAL int e13i_ckpt3_gather(e13_ctx *c, int total_waves, int stk_cap, const E13Node *extra, size_t n_extra,
+                                  size_t *n_left, size_t *nseeds);
 E13_INTERNAL int e13i_ckpt3_save(e13_ctx *c, const e13_problem *ps, int np, const e13_counts *out, size_t nseeds);
 E13_INTERNAL int e13i_ckpt3_load(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, size_t *nseeds);
 E13_INTERNAL int e13i_ckpt_load(e13_ctx *c, const e13_problem *p, size_t *top, size_t *peak, int64_t *launches,

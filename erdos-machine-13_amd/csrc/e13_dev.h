@@ -19,6 +19,7 @@
 
 #define E13_NS 9
 #define E13_TW 43           /* tape words per machine */
+#define E13_TW_MIN 18       /* shortest lane tape of the narrow form: the stage-9 block (q <= 573) fits */
 #define E13_TW_PAD 44       /* words stored in a node record */
 #define E13_QMAX 1361       /* largest live q */
 #define E13_QOFF 63         /* q = cell + 63 */
@@ -122,11 +123,17 @@ typedef struct E13Sched {
   uint64_t pad1[2];
   int32_t busy;         /* waves holding work, plus deliveries not yet taken */
   int32_t error;        /* nonzero: abort (1 private stack full, 2 incomplete delivery, 3 idle watchdog,
-                           4 LDS base) */
+                           4 LDS base, 5 not every workgroup resident: no work was done) */
   int32_t donations, pool_takes; /* nodes delivered / taken */
   int32_t exited;       /* waves that have left */
-  int32_t pad0;
-  uint64_t pad[3];
+  int32_t halt;         /* narrow form: nonzero once the list of wide nodes is full: every wave stops
+                           cleanly, as at a stop request */
+  uint32_t wide_n;      /* narrow form: nodes put on (or refused by) the list of wide nodes */
+  uint32_t wide_cap;    /* narrow form: capacity of that list */
+  E13Node *wide;        /* narrow form: nodes whose head or written cells lie past the lane's LDS tape,
+                           with their slot cursors, for a launch with the whole window */
+  int32_t started;      /* waves that have started (the check that every workgroup is resident) */
+  int32_t pad3;
 } E13Sched;
 
 /* A wave's mailbox.  A donor that won the wave's bit in the hungry map takes
@@ -185,7 +192,8 @@ typedef struct E13SearchArgs {
   int32_t phase_cycles;   /* nonzero: count wave clock cycles per phase (ctr->cyc) */
   int32_t skip_settled;   /* nonzero: the step loop tests the stage-1 gate only after a step that changed
                              the ones count (Lane.dm) */
-  int32_t pad_skip;
+  int32_t tw;             /* tape words per lane in LDS: E13_TW, or fewer (the narrow form; its list of wide
+                             nodes is in sch) */
   uint32_t nseed_cnt;     /* cost logging: seeds with counters (0: off) */
   uint64_t *seed_cnt;     /* cost logging: per seed, leaves and steps of its subtree */
   uint64_t *trace;        /* optional: trace_cap words per wave, word 0 = events logged, then

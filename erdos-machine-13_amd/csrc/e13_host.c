@@ -41,6 +41,7 @@ const char *e13_strerror(int code) {
   case E13_EWATCHDOG: return "persistent search: idle watchdog (no work and no end mark)";
   case E13_ELDS: return "dynamic LDS does not start at LDS address 0";
   case E13_ESCRATCH: return "the request exceeds the static scratch (more than E13_STATIC_PROBES probes)";
+  case E13_ERESIDENT: return "persistent search: not every workgroup became resident at once";
   default: return "unknown error";
   }
 }
@@ -220,6 +221,9 @@ static void tuning_merge(e13_tuning *dst, const e13_tuning *src) {
   if (src->phase_cycles < 0) dst->phase_cycles = 0;
   if (src->nt1_skip > 0) dst->nt1_skip = src->nt1_skip;
   if (src->nt1_skip < 0) dst->nt1_skip = 0;
+  if (src->tape_words > 0) dst->tape_words = src->tape_words;
+  if (src->tape_words < 0) dst->tape_words = 0;
+  if (src->wide_nodes > 0) dst->wide_nodes = src->wide_nodes;
 }
 
 static int tuning_valid(const e13_tuning *t) {
@@ -239,6 +243,8 @@ static int tuning_valid(const e13_tuning *t) {
   if (t->graph_chain < 0 || t->graph_chain > 4096) return 0;
   if (t->cut_live < 0 || t->cut_live > 64) return 0;
   if (t->nt1_skip < 0 || t->nt1_skip > 2) return 0;
+  if (t->tape_words != 0 && (t->tape_words < E13_TW_MIN || t->tape_words > E13_TW)) return 0;
+  if (t->wide_nodes < 0 || t->wide_nodes > (1 << 24)) return 0;
   return 1;
 }
 
@@ -262,6 +268,13 @@ int e13_open(int device, const e13_tuning *t, e13_ctx **out) {
   }
   c->ncu = prop.multiProcessorCount;
   c->lds_max = prop.sharedMemPerBlock;
+  {
+    int lds_cu = 0;
+    if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) != hipSuccess ||
+        lds_cu <= 0)
+      lds_cu = (int)prop.sharedMemPerBlock;
+    c->lds_cu = (size_t)lds_cu;
+  }
   snprintf(c->arch, sizeof c->arch, "%s", prop.gcnArchName);
   /* defaults, then this device's stored (autotuned) settings, then the caller's;
      the small-search profile likewise, over the stored main settings */
@@ -352,6 +365,7 @@ void e13_close(e13_ctx *c) {
   e13i_free_cost(c);
   e13i_free_estimate(c);
   hipFree(c->d_wave_state);
+  hipFree(c->d_wide);
   if (c->h_stop) hipHostFree(c->h_stop);
   hipFree(c->d_chain);
   if (c->ev0) hipEventDestroy(c->ev0);
@@ -975,6 +989,19 @@ static double host_ms(void) {
 }
 
 /* The persistent form: every problem in one launch (see k_search). */
+/* Workgroups of the persistent kernel that fit a CU: the runtime's figure, and
+   no more than the CU's LDS holds with each workgroup's share rounded up to
+   the allocation unit, taken as 1/128 of the CU's LDS (1280 bytes on gfx950:
+   measured, the runtime's figure ignores it; 512 bytes on 64 KB parts).  The
+   kernel also checks at start that every workgroup is resident. */
+static int search_fit(const e13_ctx *c, int wg, int gate_hash, int np, int tw) {
+  const int f = e13k_search_max_wgs(wg, gate_hash, np, tw);
+  const size_t unit = c->lds_cu >= 128 * 512 ? c->lds_cu / 128 : 512;
+  const size_t lds = (e13k_search_lds_bytes(wg, gate_hash, np, tw) + unit - 1) / unit * unit;
+  const int f_lds = (int)(c->lds_cu / lds);
+  return f < f_lds ? f : f_lds;
+}
+
 int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, e13_survivor *surv,
                     uint64_t surv_cap, uint64_t *n_surv) {
   if (!c || !ps || !out || np < 1 || np > E13_MAXPROB) return E13_EINVAL;
@@ -987,36 +1014,72 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
   const int profile = e13_search_profile(c, ps, np, NULL);
   const e13_tuning *tu = profile == E13_PROFILE_SMALL ? &c->tune_small : &c->tune;
   c->stats.profile = profile;
+  const int gh = tu->gate == 2;
   int wg = tu->wg_threads;
-  while (wg > 64 && e13k_search_lds_bytes(wg, tu->gate == 2, np) > c->lds_max) wg -= 64;
-  if (e13k_search_lds_bytes(wg, tu->gate == 2, np) > c->lds_max) return E13_EINVAL;
+  while (wg > 64 && e13k_search_lds_bytes(wg, gh, np, E13_TW) > c->lds_max) wg -= 64;
+  if (e13k_search_lds_bytes(wg, gh, np, E13_TW) > c->lds_max) return E13_EINVAL;
   /* every workgroup of the persistent kernel must be resident at once: a wave
      that could start only after others exit would hold its frontier share and
      its place in the busy count while they wait for it */
-  const int fit = e13k_search_max_wgs(wg, tu->gate == 2, np);
-  if (fit < 1) return E13_EINVAL;
-  const int grid = c->ncu * (tu->wgs_per_cu < fit ? tu->wgs_per_cu : fit);
-  const int total_waves = grid * (wg / 64);
-  /* each wave's stack is a ring of a power-of-two number of nodes, at least its
-     share of stack_nodes (a mask, not a 64-bit remainder, maps the ring index
-     to a slot) */
-  int stk_cap = 64;
-  while ((int64_t)stk_cap * total_waves < tu->stack_nodes && stk_cap < (1 << 24)) stk_cap *= 2;
-  const size_t need_stacks = (size_t)total_waves * (size_t)stk_cap;
-  if (total_waves > 64 * 64) return E13_EINVAL; /* the hungry map is read as at most 64 words */
-  const int hungry_words = (total_waves + 63) / 64;
+  const int fit_wide = search_fit(c, wg, gh, np, E13_TW);
+  if (fit_wide < 1) return E13_EINVAL;
+  /* The narrow form: with a shorter lane tape in LDS more workgroups fit a CU.
+     tape_words 0: when the whole tape does not fit wgs_per_cu workgroups, the
+     widest tape with which the most of them (up to wgs_per_cu) fit; 18..42:
+     that tape; 43: the whole tape.  Machines that go past the short tape
+     finish in a launch with the whole tape (the wide launch below). */
+  int tw = E13_TW, fit = fit_wide < tu->wgs_per_cu ? fit_wide : tu->wgs_per_cu;
+  if (c->narrow_off) {
+    /* a short-tape launch of this context found workgroups not resident: whole tape only */
+  } else if (!gh && tu->tape_words >= E13_TW_MIN && tu->tape_words < E13_TW) {
+    const int f = search_fit(c, wg, 0, np, tu->tape_words);
+    if (f < 1) return E13_EINVAL;
+    tw = tu->tape_words;
+    fit = f < tu->wgs_per_cu ? f : tu->wgs_per_cu;
+  } else if (!gh && tu->tape_words == 0) {
+    for (int t = E13_TW - 1; t >= E13_TW_MIN && fit < tu->wgs_per_cu; t--) {
+      int f = search_fit(c, wg, 0, np, t);
+      if (f > tu->wgs_per_cu) f = tu->wgs_per_cu;
+      if (f > fit) {
+        fit = f;
+        tw = t;
+      }
+    }
+  }
+  /* launch shapes: [0] the first launch (narrow when tw < E13_TW), [1] the whole tape */
+  int grid_k[2], waves_k[2], stk_k[2];
+  grid_k[0] = c->ncu * fit;
+  grid_k[1] = c->ncu * (fit_wide < tu->wgs_per_cu ? fit_wide : tu->wgs_per_cu);
+  size_t need_stacks = 0;
+  for (int k = 0; k < 2; k++) {
+    waves_k[k] = grid_k[k] * (wg / 64);
+    /* each wave's stack is a ring of a power-of-two number of nodes, at least its
+       share of stack_nodes (a mask, not a 64-bit remainder, maps the ring index
+       to a slot) */
+    stk_k[k] = 64;
+    while ((int64_t)stk_k[k] * waves_k[k] < tu->stack_nodes && stk_k[k] < (1 << 24)) stk_k[k] *= 2;
+    if ((size_t)waves_k[k] * (size_t)stk_k[k] > need_stacks) need_stacks = (size_t)waves_k[k] * (size_t)stk_k[k];
+    if (waves_k[k] > 64 * 64) return E13_EINVAL; /* the hungry map is read as at most 64 words */
+  }
+  const int max_waves = waves_k[0] > waves_k[1] ? waves_k[0] : waves_k[1];
   int rc;
   if ((rc = e13i_ensure_buf((void **)&c->d_stacks, &c->stacks_cap, need_stacks, sizeof(E13Node)))) return rc;
-  if (c->mail_cap < (size_t)total_waves) {
+  if (c->mail_cap < (size_t)max_waves) {
     hipFree(c->d_mail);
     hipFree(c->d_hungry);
     c->d_mail = NULL;
     c->d_hungry = NULL;
     c->mail_cap = 0;
-    if (hipMalloc((void **)&c->d_mail, (size_t)total_waves * sizeof(E13Mail)) != hipSuccess ||
+    if (hipMalloc((void **)&c->d_mail, (size_t)max_waves * sizeof(E13Mail)) != hipSuccess ||
         hipMalloc((void **)&c->d_hungry, 64 * sizeof(uint64_t)) != hipSuccess)
       return E13_ENOMEM;
-    c->mail_cap = (size_t)total_waves;
+    c->mail_cap = (size_t)max_waves;
+  }
+  const size_t wide_cap = tu->wide_nodes > 0 ? (size_t)tu->wide_nodes : ((size_t)1 << 19);
+  if (tw < E13_TW) { /* the wide list, and the wave records a stop of the narrow launch leaves */
+    if ((rc = e13i_ensure_buf((void **)&c->d_wide, &c->wide_cap, wide_cap, sizeof(E13Node)))) return rc;
+    if ((rc = e13i_ensure_buf((void **)&c->d_wave_state, &c->wave_state_cap, 3 * (size_t)max_waves, sizeof(int64_t))))
+      return rc;
   }
   if (!c->d_sch && hipMalloc((void **)&c->d_sch, sizeof(E13Sched)) != hipSuccess) return E13_ENOMEM;
   if (!c->d_pbs && hipMalloc((void **)&c->d_pbs, E13_MAXPROB * sizeof(E13Problem)) != hipSuccess) return E13_ENOMEM;
@@ -1028,11 +1091,11 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
   /* Breadth-first seeding: a few k_expand launches per problem grow a frontier
      of about four nodes per wave; leaves met on the way are counted here.  The
      persistent launch then starts with the frontier dealt to the waves. */
-  const size_t seed_target = (size_t)tu->seed_per_wave * (size_t)total_waves;
+  const size_t seed_target = (size_t)tu->seed_per_wave * (size_t)waves_k[0];
   /* frontier order: 1 by the installed network, 3 by probes from each node (non-learned) */
   const int order_mode = tu->learned == 1 && c->have_model ? 1 : (tu->learned == 3 ? 3 : 0);
   c->seed_est_n = 0;
-  if ((rc = e13i_ensure_buf((void **)&c->d_seeds, &c->seeds_cap, 64 * (size_t)total_waves + (size_t)np, sizeof(E13Node))))
+  if ((rc = e13i_ensure_buf((void **)&c->d_seeds, &c->seeds_cap, 64 * (size_t)max_waves + (size_t)np, sizeof(E13Node))))
     return rc;
   if ((rc = e13i_ensure_buf((void **)&c->d_stack, &c->stack_cap, c->seeds_cap, sizeof(E13Node)))) return rc;
   E13Problem pbs[E13_MAXPROB];
@@ -1136,13 +1199,21 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
     if (e13k_launch_seed_ids(c->d_seeds, (uint32_t)nseeds, c->stream)) return E13_EHIP;
   }
   E13Sched sch;
-  if (ckpt && (rc = e13i_ckpt3_prepare(c, total_waves))) return rc;
+  if (ckpt && (rc = e13i_ckpt3_prepare(c, max_waves))) return rc;
   int64_t call_launches = 0;
   double t_ck = host_ms(), last_stop_check = 0;
   E13Counters h;
   float ms = 0;
   if (!resumed) c->stats.frontier = (int64_t)nseeds;
-  for (;;) { /* one launch; with checkpoints, again from the nodes a clean stop left */
+  c->stats.tape_words = tw;
+  const size_t n_logged = nseeds;
+  /* One launch at a time: the narrow form (or the whole tape when tw is
+     E13_TW), then, when machines went past the narrow tape, a launch with the
+     whole tape from the list of them; with checkpoints, again from the nodes a
+     clean stop left. */
+  const E13Node *seeds_dev = c->d_seeds;
+  int wide_next = tw == E13_TW;
+  for (;;) {
   if (nseeds == 0) {
     /* the breadth-first seeding finished every tree (or a resumed checkpoint
        holds no nodes): the counts are complete, and a launch with no work
@@ -1151,6 +1222,10 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
     HIPCHK(hipStreamSynchronize(c->stream));
     break;
   }
+  const int narrow = !wide_next;
+  const int kk = narrow ? 0 : 1;
+  const int grid = grid_k[kk], total_waves = waves_k[kk], stk_cap = stk_k[kk];
+  const int hungry_words = (total_waves + 63) / 64;
   if (nseeds > (size_t)total_waves * (size_t)stk_cap) return E13_ESTACK;
   if (call_launches > 0) { /* per-launch figures restart (the survivor count goes on) */
     HIPCHK(hipMemsetAsync(&c->d_ctr->steps, 0, sizeof(uint64_t), c->stream));
@@ -1159,6 +1234,10 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
   }
   memset(&sch, 0, sizeof sch);
   sch.busy = nseeds < (size_t)total_waves ? (int32_t)nseeds : total_waves; /* waves dealt a seed */
+  if (narrow && tw < E13_TW) {
+    sch.wide = c->d_wide;
+    sch.wide_cap = (uint32_t)wide_cap;
+  }
   HIPCHK(hipMemcpyAsync(c->d_sch, &sch, sizeof sch, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemsetAsync(c->d_mail, 0, (size_t)total_waves * sizeof(E13Mail), c->stream));
   HIPCHK(hipMemsetAsync(c->d_hungry, 0, 64 * sizeof(uint64_t), c->stream));
@@ -1199,11 +1278,13 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
   a.surv_cap = (uint32_t)c->surv_cap;
   a.spin_limit = 1u << 20; /* mailbox re-reads before a delivery counts as lost */
   a.idle_limit = 3000000000u; /* idle watchdog: 30 s of the 100 MHz constant clock */
-  a.seeds = c->d_seeds;
+  a.seeds = seeds_dev;
   a.nseeds = (uint32_t)nseeds;
+  a.tw = narrow ? tw : E13_TW;
+  if (narrow && tw < E13_TW) a.wave_state = c->d_wave_state; /* a full wide list stops the launch cleanly */
   if (logging) {
     a.seed_cnt = c->d_seed_cnt;
-    a.nseed_cnt = (uint32_t)nseeds;
+    a.nseed_cnt = (uint32_t)n_logged;
   }
   if (ckpt) {
     *c->h_stop = 0u;
@@ -1226,7 +1307,7 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
     tm[1] = host_ms();
   }
   HIPCHK(hipEventRecord(c->ev0, c->stream));
-  int lr = e13k_launch_search(&a, grid, wg, tu->gate == 2, c->stream);
+  int lr = e13k_launch_search(&a, grid, wg, gh, c->stream);
   if (lr != 0) {
     fprintf(stderr, "libe13: search launch failed: %s\n", hipGetErrorString((hipError_t)lr));
     return E13_EHIP;
@@ -1266,35 +1347,72 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
   for (int j = 0; j < 4; j++) c->stats.cycles[j] += (int64_t)h.cyc[j];
   c->stats.slice = tu->slice;
   c->stats.wg_threads = wg;
-  c->stats.grid = grid;
+  if (narrow || tw == E13_TW) c->stats.grid = grid; /* the search's main launch, not its whole-tape rest */
   c->stats.gate = tu->gate;
   c->stats.form = 3;
   c->stats.donations += sch.donations;
   c->stats.pool_takes += sch.pool_takes;
+  if (sch.error == 5 && narrow && tw < E13_TW) {
+    /* not every workgroup became resident (every wave left before any work):
+       this context runs the whole tape from now on, and the launch is redone */
+    fprintf(stderr, "libe13: short-tape launch of %d workgroups not all resident: whole tape from now on\n", grid);
+    c->narrow_off = 1;
+    tw = E13_TW;
+    c->stats.tape_words = tw;
+    wide_next = 1;
+    continue;
+  }
   if (sch.error) {
     fprintf(stderr, "libe13: persistent search aborted (%s)\n",
             sch.error == 1   ? "a private node stack is full"
             : sch.error == 2 ? "a mailbox delivery never arrived complete"
             : sch.error == 3 ? "idle watchdog"
+            : sch.error == 5 ? "not every workgroup resident at once"
                              : "dynamic LDS does not start at LDS address 0");
-    return sch.error == 1 ? E13_ESTACK : sch.error == 2 ? E13_ELOST : sch.error == 3 ? E13_EWATCHDOG : E13_ELDS;
+    return sch.error == 1   ? E13_ESTACK
+           : sch.error == 2 ? E13_ELOST
+           : sch.error == 3 ? E13_EWATCHDOG
+           : sch.error == 5 ? E13_ERESIDENT
+                            : E13_ELDS;
   }
-  if (logging && (rc = collect_cost_log(c, ps, nseeds))) return rc;
   c->stats.search_ms += ms;
   c->stats.search_launches++;
+  if (!narrow && tw < E13_TW) {
+    c->stats.wide_launches++;
+    c->stats.wide_ms += ms;
+  }
   for (int k = 0; k < np; k++) {
     const uint64_t *q = pcnt + k * E13_PCNT;
     c->stats.search_steps += (int64_t)q[1];
+    if (!narrow && tw < E13_TW) c->stats.wide_steps += (int64_t)q[1];
     out[k].leaves += q[0];
     out[k].steps += q[1];
     out[k].survivors += q[2];
     for (int j = 0; j <= E13_NS; j++) out[k].dead_at_stage[j] += q[3 + j];
   }
-  if (!ckpt) break;
-  /* what a clean stop left: every wave's stack and the deliveries still in mailboxes */
   call_launches++;
-  if ((rc = e13i_ckpt3_gather(c, total_waves, stk_cap, &nseeds))) return rc;
+  /* machines a narrow launch left for the whole tape (a full list stopped it) */
+  const size_t n_wide = narrow && tw < E13_TW ? (sch.wide_n < wide_cap ? (size_t)sch.wide_n : wide_cap) : 0;
+  const int halted = narrow && tw < E13_TW && sch.halt != 0;
+  c->stats.wide_nodes += (int64_t)n_wide;
+  if (!ckpt && !halted) {
+    if (n_wide == 0) break;
+    seeds_dev = c->d_wide; /* the next launch runs them, on the whole tape */
+    nseeds = n_wide;
+    wide_next = 1;
+    continue;
+  }
+  /* what a clean stop left: every wave's stack, the deliveries still in
+     mailboxes, and the wide list */
+  size_t n_left = 0;
+  if ((rc = e13i_ckpt3_gather(c, total_waves, stk_cap, c->d_wide, n_wide, &n_left, &nseeds))) return rc;
+  seeds_dev = c->d_seeds;
+  /* the next launch has the whole tape when only wide nodes are left, or when
+     the full list stopped this one (a narrow launch would stop again at once);
+     else it is narrow, and its refill sends wide nodes to the list again */
+  wide_next = tw == E13_TW || (narrow && (halted || (n_left == 0 && n_wide > 0)));
   if (nseeds == 0) break; /* the search ended before the flag was seen */
+  if (!ckpt) continue;
   if ((rc = e13i_ckpt3_save(c, ps, np, out, nseeds))) return rc;
   t_ck = host_ms();
   if (stop_file) {
@@ -1303,6 +1421,7 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
   }
   if (c->ckpt_max_launches > 0 && call_launches >= c->ckpt_max_launches) return E13_ESTOPPED;
   } /* next launch, from the gathered nodes */
+  if (logging && (rc = collect_cost_log(c, ps, n_logged))) return rc;
   uint64_t ns = h.nsurv;
   if (n_surv) *n_surv = ns;
   uint64_t keep = ns < c->surv_cap ? ns : c->surv_cap;

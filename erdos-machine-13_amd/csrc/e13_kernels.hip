@@ -109,6 +109,12 @@ struct Ctx {
   int ckb;            // LDS byte address of the counter keys: KEY[q+1] for q 0..60 (CKEY_BYTES)
   int cbl;            // >= 0: LDS byte address of this lane's table codes (8 words, a column of stride 256),
                       // kept beside the rows by the persistent search; -1: decode the rows
+  int tw;             // tape words of a lane in LDS (E13_TW: the whole window)
+  int qlim;           // largest head position the lane's LDS tape holds (32 * tw - 1; E13_QMAX for the whole
+                      // window): a machine past it is paused as a resumable node (EV_CAP) for a launch with
+                      // the whole window
+  int mvb;            // LDS byte address of word 0, lane 0, of the lone-lane memo's value words (a column of
+                      // 16 words per idle lane, stride 256)
 };
 #define CKEY_BYTES 512 // 64 u64 counter keys per workgroup (61 used)
 
@@ -286,12 +292,11 @@ __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int m
   // tape word address (q >> 5) * 256 + tbl as one shift-add and one and-or
   const uint32_t tbl_lo = (uint32_t)c.tbl & 0xFFu, tbl_hi = (uint32_t)c.tbl - tbl_lo;
   // The exit test in one sign bit: (e << 16) is negative for a cell that
-  // cannot be stepped (halt, unread), q | (E13_QMAX - q) for a head outside
-  // the window, left for the deadline/cap or round end; the stage gate is
-  // tested beside it.
-  // q may be one step outside the window here (q = -1 or 1362): the address
-  // is then word -1 or 42, inside the wave's LDS area, and the word read is
-  // not used.
+  // cannot be stepped (halt, unread), q | (qlim - q) for a head outside the
+  // window or past the lane's LDS tape, left for the deadline/cap or round
+  // end; the stage gate is tested beside it.
+  // q may be one step outside here (q = -1 or qlim + 1): the address is then
+  // word -1 or tw, inside the workgroup's LDS, and the word read is not used.
   int wa = (int)(((((uint32_t)L.q << 3) + tbl_hi) & 0xFFFFFF00u) | tbl_lo);
   uint32_t word = lds_ld(c.sb, wa);
   uint32_t pair = lds_ld(c.sb, L.row);
@@ -306,7 +311,7 @@ __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int m
     } else {
       gate = L.hb == L.tgt;
     }
-    stop = (int)((e << 16) | (uint32_t)L.q | (uint32_t)(E13_QMAX - L.q) | (uint32_t)left) < 0 || gate;
+    stop = (int)((e << 16) | (uint32_t)L.q | (uint32_t)(c.qlim - L.q) | (uint32_t)left) < 0 || gate;
   }
   while (!stop) {
     // the step
@@ -336,7 +341,7 @@ __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int m
     } else {
       gate = L.hb == L.tgt;
     }
-    stop = (int)((e << 16) | (uint32_t)L.q | (uint32_t)(E13_QMAX - L.q) | (uint32_t)left) < 0 || gate;
+    stop = (int)((e << 16) | (uint32_t)L.q | (uint32_t)(c.qlim - L.q) | (uint32_t)left) < 0 || gate;
     // with too few lanes left in the loop (exec is wave-uniform), every lane
     // leaves it; a lane whose step was due makes it in step_once below
     if (CUT) stop |= __popcll(__builtin_amdgcn_read_exec()) < min_live;
@@ -348,6 +353,7 @@ __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int m
     L.t -= 1;
     return EV_DEAD;
   }
+  if (L.q > c.qlim) return EV_CAP; // past the lane's LDS tape (inside the window): paused, goes on wide
   return step_once<GATE>(L, c, evcell);
 }
 
@@ -368,10 +374,11 @@ __device__ __forceinline__ int lane_of(int v, int sl) { return __builtin_amdgcn_
 // aligned 16-cell block of tape, up to the step that leaves it or for at most
 // SOLO_STAY steps, is a pure function of (state, block contents, entry cell).
 // It is memoised in a tagged table of 1024 slots hashed from that key (24
-// bits), kept in tape words 0..31 of the wave's idle lanes (free while one
-// lane runs alone): slot s has its tag (1 << 31 | key) in word s >> 6 and its
-// value in word 16 + (s >> 6) of lane s & 63 (lane sl's slots move to lane
-// (s & 63) ^ 32).  value = new contents | state after << 16 | exit right << 20
+// bits), kept in 32 words of the wave's idle lanes (free while one lane runs
+// alone): slot s has its tag (1 << 31 | key) in tape word s >> 6 and its value
+// in word s >> 6 of the value area (Ctx.mvb: tape words 16..31, or the rule
+// rows when the lane's tape is shorter than 32 words) of lane s & 63 (lane
+// sl's slots move to lane (s & 63) ^ 32).  value = new contents | state after << 16 | exit right << 20
 // | code << 21, where code 1..SOLO_STAY: the run leaves the block after that
 // many steps, on the side the right bit names; code SOLO_STAY+1+p: it is still
 // inside after SOLO_STAY steps, at block cell p (a machine caught in a loop
@@ -405,6 +412,7 @@ __device__ __forceinline__ int run_solo(Lane &L, const Ctx &c, int nsteps_in, in
     if (lane == sl) L.t = t_round - 1;    // that step does not count
     return EV_DEAD;
   }
+  if (q > c.qlim) return EV_CAP; // a round ended on the step past the lane's LDS tape: paused, goes on wide
   int left = lane_of(L.tstop, sl) - t_round;
   int target = lane_of(L.target, sl), qA = lane_of(L.qA, sl), qR = lane_of(L.qR, sl);
   int st0r = (lane_of(L.row0, sl) - rbl) >> 8;
@@ -447,13 +455,13 @@ __device__ __forceinline__ int run_solo(Lane &L, const Ctx &c, int nsteps_in, in
         const int wj = wi + ((e & 2u) ? 1 + lane : -1 - lane);
         const uint32_t fill = 0u - sym;
         uint32_t wv = ~fill; // past the tape: the run ends there
-        if (wj >= 0 && wj < E13_TW) wv = lds_ld(c.sb, (wj << 8) + tbl);
+        if (wj >= 0 && wj < c.tw) wv = lds_ld(c.sb, (wj << 8) + tbl);
         const uint64_t mism = __ballot(wv != fill);
         const int j = __builtin_ctzll(mism); // the first word where the run ends (always one)
         const uint32_t wd = (uint32_t)__builtin_amdgcn_readlane((int)wv, j) ^ fill;
         k += 32 * j + ((e & 2u) ? __builtin_ctz(wd) : __builtin_clz(wd));
       }
-      k = (e & 2u) ? min(k, E13_QMAX - q) : min(k, q);
+      k = (e & 2u) ? min(k, c.qlim - q) : min(k, q);
       k = min(k, n_eff - r);
       if (ones == target) { // the gate can fire only in the stage-1 state at qA or qR (or anywhere at nt 1)
         if (nt1 | ((st == st0r) & ((q == qA) | (q == qR)))) k = 0;
@@ -486,13 +494,13 @@ __device__ __forceinline__ int run_solo(Lane &L, const Ctx &c, int nsteps_in, in
     const int qn = q & ~15;
     const bool far = (ones - target > 16) | (target - ones > 16);
     const bool no_gate_cell = !nt1 & ((qA >> 4) != (qn >> 4)) & ((qR >> 4) != (qn >> 4));
-    if ((qn >= 16) & (qn + 16 <= E13_QMAX) & (far | no_gate_cell)) {
+    if ((qn >= 16) & (qn + 16 <= c.qlim) & (far | no_gate_cell)) {
       const uint32_t nb = qb & ~15u, byt = (word >> nb) & 0xFFFFu, pos = qb & 15u;
       const uint32_t key = ((uint32_t)st << 20) | (byt << 4) | pos;
       const uint32_t slot = (key * 0x9E3779B1u) >> 22;
       int col = (int)(slot & 63u);
       if (col == sl) col ^= 32;
-      const int ta = tbase + (int)(slot >> 6) * 256 + col * 4, va = ta + SOLO_MEMO_ROWS * 256;
+      const int ta = tbase + (int)(slot >> 6) * 256 + col * 4, va = c.mvb + (int)(slot >> 6) * 256 + col * 4;
       const uint32_t tag = 0x80000000u | key;
       const uint32_t t_ld = lds_ld(c.sb, ta), v_ld = lds_ld(c.sb, va);
       uint32_t val = (uint32_t)solo_uni((int)v_ld);
@@ -562,6 +570,13 @@ __device__ __forceinline__ int run_solo(Lane &L, const Ctx &c, int nsteps_in, in
         nt1 = lane_of(L.nt, sl) == 1;
         left = lane_of(L.tstop, sl) - t_round;
         n_eff = min(nsteps, left);
+        if (q > c.qlim) { // the step went past the lane's LDS tape: paused
+          // (step_once wrote the cell in word wi: the cached word is stale)
+          word = (uint32_t)solo_uni((int)lds_ld(c.sb, (wi << 8) + tbl));
+          r += 1;
+          why = 3;
+          break;
+        }
         wi = q >> 5;
         word = (uint32_t)solo_uni((int)lds_ld(c.sb, (wi << 8) + tbl));
         continue;
@@ -576,6 +591,11 @@ __device__ __forceinline__ int run_solo(Lane &L, const Ctx &c, int nsteps_in, in
       break;
     }
     st = (int)(nx >> 8);
+    if (q > c.qlim) { // past the lane's LDS tape (the step counts): paused, goes on wide
+      r += 1;
+      why = 3;
+      break;
+    }
     if ((q >> 5) != wi) { // into another tape word
       lds_st(c.sb, (wi << 8) + tbl, word);
       wi = q >> 5;
@@ -591,6 +611,7 @@ __device__ __forceinline__ int run_solo(Lane &L, const Ctx &c, int nsteps_in, in
     L.t = t_round + r;
     L.dm = 1; // the loop's steps are not tracked: the gate is tested again
     if (why == 2) ev = EV_DEAD;
+    else if (why == 3) ev = EV_CAP;
     else if (r >= left) ev = step_once<GATE_COUNT>(L, c, evcell); // the deadline or cap is due
   }
   return lane_of(ev, sl);
@@ -709,6 +730,9 @@ __global__ void __launch_bounds__(256) k_expand(E13ExpandArgs a) {
   c.dl = a.pb->dl;
   c.dlb = -1;
   c.cbl = -1;
+  c.tw = E13_TW;
+  c.qlim = E13_QMAX;
+  c.mvb = c.tbl - lane * 4 + SOLO_MEMO_ROWS * 256; // memo values in tape words 16..31
   if (lds_base(smem) != 0u) { // the absolute LDS addressing needs the dynamic area at address 0
     if (threadIdx.x == 0) atomicOr(&a.ctr->overflow, 4u);
     return;
@@ -966,6 +990,9 @@ __global__ void __launch_bounds__(256) k_batch(E13BatchArgs a) {
   c.dl = a.dl;
   c.dlb = -1;
   c.cbl = -1;
+  c.tw = E13_TW;
+  c.qlim = E13_QMAX;
+  c.mvb = c.tbl - lane * 4 + SOLO_MEMO_ROWS * 256; // memo values in tape words 16..31
   if (lds_base(smem) != 0u) { // the absolute LDS addressing needs the dynamic area at address 0
     if (threadIdx.x == 0) atomicOr(&a.ctr->overflow, 4u);
     return;
@@ -1127,6 +1154,7 @@ __global__ void __launch_bounds__(256) k_chain_push(E13Chain *ch, E13Node *stack
 // ---------------------------------------------------------------------------
 namespace {
 
+#define START_LIMIT 2000000ull // the residency check's wait: 20 ms of the 100 MHz constant clock
 #define STG_WORDS 120 // staged node (76) + rows as stored (16) + rows with edits exhausted (16)
                       // + codes with edits exhausted (8) + 4 spare
 #define STG_W_CODESB 108
@@ -1185,8 +1213,11 @@ __device__ __forceinline__ void vm_drain() {
 }
 
 // The search is over (or aborted): every wave's mailbox says so.
-__device__ void broadcast_done(const E13SearchArgs &a, int lane) {
-  for (int w = lane; w < a.total_waves; w += 64) st_agent64(&a.mail[w].flag, E13_MAIL_DONE);
+// (the fields, not the argument block: a reference to the kernel's argument
+// block in a call that is not inlined would make the compiler copy the block
+// to scratch memory)
+__device__ void broadcast_done(E13Mail *mail, int total_waves, int lane) {
+  for (int w = lane; w < total_waves; w += 64) st_agent64(&mail[w].flag, E13_MAIL_DONE);
   vm_drain();
 }
 
@@ -1272,7 +1303,8 @@ __device__ __forceinline__ void stage_own(const Ctx &c, uint32_t *stg, const Lan
     uint32_t tw[11];
 #pragma unroll
     for (int j = 0; j < 11; ++j)
-      if (w0 + j < E13_TW_PAD) tw[j] = w0 + j < E13_TW ? lds_ld(c.sb, c.tbl + (w0 + j) * 256) : seed;
+      if (w0 + j < E13_TW_PAD)
+        tw[j] = w0 + j < c.tw ? lds_ld(c.sb, c.tbl + (w0 + j) * 256) : (w0 + j < E13_TW ? 0u : seed);
 #pragma unroll
     for (int j = 0; j < 11; ++j)
       if (w0 + j < E13_TW_PAD) stg[E13_NODE_W_TAPE + w0 + j] = tw[j];
@@ -1308,26 +1340,39 @@ __device__ __forceinline__ void write_node(const Ctx &c, E13Node *o, const Lane 
   uint4 *dst = (uint4 *)o->tape;
   for (int w4 = 0; w4 < E13_TW_PAD / 4; ++w4) {
     uint4 v4;
-    v4.x = lds_ld(c.sb, c.tbl + (4 * w4 + 0) * 256);
-    v4.y = lds_ld(c.sb, c.tbl + (4 * w4 + 1) * 256);
-    v4.z = lds_ld(c.sb, c.tbl + (4 * w4 + 2) * 256);
-    v4.w = (4 * w4 + 3 < E13_TW) ? lds_ld(c.sb, c.tbl + (4 * w4 + 3) * 256) : seed;
+    // words past the lane's LDS tape are blank (the machine never went there); word 43 is the seed
+    v4.x = 4 * w4 + 0 < c.tw ? lds_ld(c.sb, c.tbl + (4 * w4 + 0) * 256) : 0u;
+    v4.y = 4 * w4 + 1 < c.tw ? lds_ld(c.sb, c.tbl + (4 * w4 + 1) * 256) : 0u;
+    v4.z = 4 * w4 + 2 < c.tw ? lds_ld(c.sb, c.tbl + (4 * w4 + 2) * 256) : 0u;
+    v4.w = 4 * w4 + 3 < c.tw ? lds_ld(c.sb, c.tbl + (4 * w4 + 3) * 256) : (4 * w4 + 3 < E13_TW ? 0u : seed);
     dst[w4] = v4;
   }
 }
 
 } // namespace
 
-template <int GATE>
-__global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
+// NARROW: each lane's tape in LDS is a.tw < E13_TW words (cells up to
+// q = 32 * a.tw - 1), so more waves fit in a CU.  A machine whose head passes
+// that cell is paused as a resumable node (as at a launch cap), and a node
+// whose head or written cells lie past it is not loaded: it goes, with its
+// slot cursor, to the list a.wide, which the host runs afterwards in a launch
+// with the whole window.  When that list is full the launch stops cleanly
+// (every wave pauses its machines and leaves), so nothing is lost.
+template <int GATE, bool NARROW>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NARROW ? 3 : 2))) k_search(E13SearchArgs a) {
   extern __shared__ __align__(16) uint32_t smem[];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
   const int wid = blockIdx.x * nwaves + wave;
+  const int tw = NARROW ? a.tw : E13_TW;
   Ctx c;
   c.sb = (char *)smem;
   c.rbl = wave * ROWS_BYTES + lane * 4;
-  c.tbl = nwaves * ROWS_BYTES + wave * TAPE_BYTES + lane * 4;
-  c.keyb = nwaves * (ROWS_BYTES + TAPE_BYTES);
+  c.tbl = nwaves * ROWS_BYTES + wave * tw * 256 + lane * 4;
+  c.keyb = nwaves * (ROWS_BYTES + tw * 256);
+  c.tw = tw;
+  c.qlim = NARROW ? 32 * tw - 1 : E13_QMAX;
+  // the lone-lane memo's values: tape words 16..31 of idle lanes, or their rule rows on a shorter tape
+  c.mvb = tw >= 2 * SOLO_MEMO_ROWS ? c.tbl - lane * 4 + SOLO_MEMO_ROWS * 256 : wave * ROWS_BYTES;
   c.key = a.key;
   c.tgt = a.tgt;
   c.dl = a.pbs[0].dl;
@@ -1336,6 +1381,35 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
     return;
   }
   if (a.nseeds == 0) return; // no work: no wave would ever count busy down to the end mark
+  // Every workgroup must be resident at once (seeds are dealt to every wave,
+  // and a wave that started only after others left would hold its share and
+  // its place in the busy count).  Each workgroup counts itself in and waits,
+  // bounded, for all of them; if they do not all arrive in time, or a
+  // workgroup arrives after others gave up, every workgroup leaves before any
+  // work (error 5) and the host redoes the launch in a shape that fits.
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    atomicAdd(&a.sch->started, 1);
+    const uint64_t t_start = __builtin_readsteadycounter();
+    for (;;) {
+      if (ld_agent_i(&a.sch->error) != 0) { // another workgroup gave up (or failed): leave too
+        ok = 0;
+        break;
+      }
+      if (ld_agent_i(&a.sch->started) >= (int)gridDim.x) break;
+      if (__builtin_readsteadycounter() - t_start > START_LIMIT) {
+        atomicCAS(&a.sch->error, 0, 5);
+        ok = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    smem[0] = (uint32_t)ok;
+  }
+  __syncthreads();
+  const bool resident = smem[0] != 0u;
+  __syncthreads(); // smem[0] is wave 0's first rule word: read by all before it is written
+  if (!resident) return;
   const int after_keys = c.keyb + (GATE == GATE_HASH ? KEYB_WORDS * 8 : 0);
   uint32_t *stg = (uint32_t *)(c.sb + after_keys) + wave * STG_WORDS;
   unsigned long long *pc = (unsigned long long *)(c.sb + after_keys + nwaves * STG_WORDS * 4); // np x 16 per WG
@@ -1395,6 +1469,7 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
   uint64_t w_slots = 0, w_rounds = 0, w_takes = 0, w_gives = 0;
   unsigned long long w_steps = 0;
   uint64_t cyc0 = 0, cyc1 = 0, cyc2 = 0, cyc3 = 0, ts = stamp_if(a.phase_cycles), tn;
+  bool halt_self = false; // NARROW: the list of wide nodes is full; this wave stops (and so do all)
 
   for (;;) {
     // ---- refill idle lanes from the private stack ----
@@ -1412,6 +1487,35 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
         cur_slot = (int)(sw & 0xFFFFu);
         cur_end = (int)(sw >> 16);
         if (cur_end == 0) cur_end = pt_word(ptb, uni((int)stg[12]), 9);
+      }
+      if (NARROW) {
+        // a node whose head or written cells lie past the lane's LDS tape goes, with its slot
+        // cursor, to the list of the launch with the whole window
+        const bool past = lane < E13_TW - tw && stg[E13_NODE_W_TAPE + tw + lane] != 0u;
+        if (uni((int)stg[0]) > c.qlim || ballot(past) != 0) {
+          uint32_t idx = 0;
+          uint32_t cap = 0;
+          E13Node *wl = nullptr;
+          if (lane == 0) {
+            idx = atomicAdd(&a.sch->wide_n, 1u);
+            cap = a.sch->wide_cap;
+            wl = a.sch->wide;
+          }
+          idx = (uint32_t)uni((int)idx);
+          cap = (uint32_t)uni((int)cap);
+          if (idx >= cap) { // full: this launch stops cleanly, the host runs the list
+            if (lane == 0) atomicMax(&a.sch->halt, 1);
+            halt_self = true;
+            break;
+          }
+          uint32_t *o = (uint32_t *)((E13Node *)uni64((long long)wl) + idx);
+          const uint32_t v0 = lane == E13_NODE_W_SLOTS ? (uint32_t)slot_range(cur_slot, cur_end) : stg[lane];
+          o[lane] = v0;
+          if (lane < E13_NODE_WORDS - 64) o[64 + lane] = stg[64 + lane];
+          top -= 1; // the node leaves this wave's stack
+          staged = -1;
+          continue;
+        }
       }
       const int sprob = uni((int)stg[12]);
       const int cnt = __popcll(need);
@@ -1514,21 +1618,22 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
                 *(__attribute__((address_space(3))) uint8_t *)(size_t)(uint32_t)(c.cbl + (pend >> 2) * 256 +
                                                                                   (pend & 3)) = (uint8_t)val;
             }
-            // tape: words 32..74 of the stage (word 75, the seed, comes along)
+            // tape: words 32..74 of the stage (word 75, the seed, comes along), the
+            // first tw of them (a node that fits the lane's tape is blank past them)
 #pragma unroll
             for (int k4 = 0; k4 < 11; k4 += 4) {
-              uint4 tw[4];
+              uint4 tv[4];
 #pragma unroll
               for (int j = 0; j < 4; ++j)
-                if (k4 + j < 11) tw[j] = *(const uint4 *)(stg + E13_NODE_W_TAPE + 4 * (k4 + j));
+                if (k4 + j < 11) tv[j] = *(const uint4 *)(stg + E13_NODE_W_TAPE + 4 * (k4 + j));
 #pragma unroll
               for (int j = 0; j < 4; ++j) {
                 const int w = 4 * (k4 + j);
                 if (k4 + j < 11) {
-                  lds_st(c.sb, c.tbl + (w + 0) * 256, tw[j].x);
-                  lds_st(c.sb, c.tbl + (w + 1) * 256, tw[j].y);
-                  lds_st(c.sb, c.tbl + (w + 2) * 256, tw[j].z);
-                  if (w + 3 < E13_TW) lds_st(c.sb, c.tbl + (w + 3) * 256, tw[j].w);
+                  if (w + 0 < tw) lds_st(c.sb, c.tbl + (w + 0) * 256, tv[j].x);
+                  if (w + 1 < tw) lds_st(c.sb, c.tbl + (w + 1) * 256, tv[j].y);
+                  if (w + 2 < tw) lds_st(c.sb, c.tbl + (w + 2) * 256, tv[j].z);
+                  if (w + 3 < tw) lds_st(c.sb, c.tbl + (w + 3) * 256, tv[j].w);
                 }
               }
             }
@@ -1556,7 +1661,7 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
       if (f != last_tag) {
         if (top - bot + 1 > a.stk_cap) {
           if (lane == 0) atomicMax(&a.sch->error, 1);
-          broadcast_done(a, lane);
+          broadcast_done(a.mail, a.total_waves, lane);
           break;
         }
         // read until every element carries this delivery's tag
@@ -1574,7 +1679,7 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
         }
         if (spins > a.spin_limit) {
           if (lane == 0) atomicMax(&a.sch->error, 2);
-          broadcast_done(a, lane);
+          broadcast_done(a.mail, a.total_waves, lane);
           break;
         }
         uint32_t *dst = (uint32_t *)stk_node(a, wid, top);
@@ -1606,6 +1711,11 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
     }
     const uint64_t act = ballot(active);
     if (act == 0) {
+      if (NARROW && halt_self) { // nothing runs: keep the staged node's cursor and leave
+        if (staged >= bot && staged < top && lane == 0)
+          stk_node(a, wid, staged)->next_slot = slot_range(cur_slot, cur_end);
+        break;
+      }
       if (top != bot) continue;
       // no work here: leave the busy count; the wave that takes it to zero ends the search
       if (busy) {
@@ -1615,7 +1725,7 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
         busy = 0;
         TRACE_EV(TR_IDLE);
         if (uni(last)) {
-          broadcast_done(a, lane);
+          broadcast_done(a.mail, a.total_waves, lane);
           break;
         }
       }
@@ -1626,12 +1736,13 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
         if (lane == 0) err = ld_agent_i(&a.sch->error);
         if (a.stop != nullptr && lane == 0)
           stp = (int)__hip_atomic_load(a.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        if (NARROW && lane == 0) stp |= ld_agent_i(&a.sch->halt);
         if (uni(err) || uni(stp)) break; // a clean stop: nothing here to keep (a delivery still on its way
                                          // stays in the mailbox for the host)
       }
       if (now - idle_since > (uint64_t)a.idle_limit) { // watchdog: no delivery and no end for this long
         if (lane == 0) atomicMax(&a.sch->error, 3);
-        broadcast_done(a, lane);
+        broadcast_done(a.mail, a.total_waves, lane);
         break;
       }
       for (int z = 0; z < a.idle_naps; ++z) __builtin_amdgcn_s_sleep(8);
@@ -1653,6 +1764,7 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
       if (lane == 0) s_err = ld_agent_i(&a.sch->error);
       if (a.stop != nullptr && lane == 0)
         s_stop = (int)__hip_atomic_load(a.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      if (NARROW && lane == 0) s_stop |= ld_agent_i(&a.sch->halt);
       const bool can_give = top - bot >= 2 || (top - bot == 1 && staged == top - 1 && cur_end - cur_slot >= 2) ||
                             (a.evict_after > 0 && top == bot && __popcll(act) >= 2);
       if (lane < a.hungry_words && can_give) hw = ld_agent64(&a.hungry[lane]);
@@ -1684,7 +1796,7 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
     ts = tn;
     // ---- a clean stop: every running machine is paused as a resumable node, as at
     // a launch cap, and goes onto this wave's stack below ----
-    const bool stopping = refresh && uni(s_stop) != 0;
+    const bool stopping = (refresh && uni(s_stop) != 0) || (NARROW && halt_self);
     if (stopping && active && ev == EV_NONE) ev = EV_CAP;
     // ---- finished machines ----
     // ---- long runners to hungry waves: with nothing on its stack to give, a wave
@@ -1740,7 +1852,7 @@ __global__ void __launch_bounds__(256) k_search(E13SearchArgs a) {
       const int n = __popcll(em);
       if (top - bot + n > a.stk_cap) {
         if (lane == 0) atomicMax(&a.sch->error, 1);
-        broadcast_done(a, lane);
+        broadcast_done(a.mail, a.total_waves, lane);
         break;
       }
       if (staged >= bot && staged < top && lane == 0)
@@ -2071,6 +2183,9 @@ __global__ void __launch_bounds__(256) k_probe(E13ProbeArgs a) {
   c.dl = a.pb->dl;
   c.dlb = -1;
   c.cbl = -1;
+  c.tw = E13_TW;
+  c.qlim = E13_QMAX;
+  c.mvb = c.tbl - lane * 4 + SOLO_MEMO_ROWS * 256; // memo values in tape words 16..31
   if (lds_base(smem) != 0u) { // the absolute LDS addressing needs the dynamic area at address 0
     if (threadIdx.x == 0) atomicOr(&a.ctr->overflow, 4u);
     return;
@@ -2378,29 +2493,34 @@ int e13k_launch_chain_push(E13Chain *ch, E13Node *stack, uint64_t stack_cap, con
   return (int)hipGetLastError();
 }
 
-size_t e13k_search_lds_bytes(int wg_threads, int gate, int np) {
+// tw: tape words per lane in LDS (E13_TW: the whole window; fewer: the
+// narrow form, count gate only).
+size_t e13k_search_lds_bytes(int wg_threads, int gate, int np, int tw) {
   const int nw = wg_threads / WAVE;
-  return (size_t)nw * (ROWS_BYTES + TAPE_BYTES + STG_WORDS * 4) + (gate == GATE_HASH ? KEYB_WORDS * 8 : 0) +
+  return (size_t)nw * (ROWS_BYTES + (size_t)tw * 256 + STG_WORDS * 4) + (gate == GATE_HASH ? KEYB_WORDS * 8 : 0) +
          (size_t)np * E13_PCNT * 8 + (size_t)np * PT_WORDS * 4 + PT_DL_WORDS * 4 + CKEY_BYTES +
          (size_t)nw * CODES_BYTES;
 }
 
 // Workgroups of k_search that fit on one CU at once (registers, LDS, wave
 // slots); -1 when the runtime cannot tell.
-int e13k_search_max_wgs(int wg_threads, int gate, int np) {
-  const size_t lds = e13k_search_lds_bytes(wg_threads, gate, np);
+int e13k_search_max_wgs(int wg_threads, int gate, int np, int tw) {
+  const size_t lds = e13k_search_lds_bytes(wg_threads, gate, np, tw);
   int n = 0;
-  const hipError_t e =
-      gate == GATE_HASH
-          ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_search<GATE_HASH>, wg_threads, lds)
-          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_search<GATE_COUNT>, wg_threads, lds);
+  hipError_t e;
+  if (gate == GATE_HASH) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_search<GATE_HASH, false>, wg_threads, lds);
+  else if (tw < E13_TW) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_search<GATE_COUNT, true>, wg_threads, lds);
+  else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_search<GATE_COUNT, false>, wg_threads, lds);
   return e == hipSuccess ? n : -1;
 }
 
 int e13k_launch_search(const E13SearchArgs *a, int grid, int wg_threads, int gate, hipStream_t s) {
-  const size_t lds = e13k_search_lds_bytes(wg_threads, gate, a->np);
-  if (gate == GATE_HASH) hipLaunchKernelGGL((k_search<GATE_HASH>), dim3(grid), dim3(wg_threads), lds, s, *a);
-  else hipLaunchKernelGGL((k_search<GATE_COUNT>), dim3(grid), dim3(wg_threads), lds, s, *a);
+  const int tw = gate == GATE_HASH ? E13_TW : a->tw;
+  if (tw < E13_TW_MIN || tw > E13_TW) return (int)hipErrorInvalidValue;
+  const size_t lds = e13k_search_lds_bytes(wg_threads, gate, a->np, tw);
+  if (gate == GATE_HASH) hipLaunchKernelGGL((k_search<GATE_HASH, false>), dim3(grid), dim3(wg_threads), lds, s, *a);
+  else if (tw < E13_TW) hipLaunchKernelGGL((k_search<GATE_COUNT, true>), dim3(grid), dim3(wg_threads), lds, s, *a);
+  else hipLaunchKernelGGL((k_search<GATE_COUNT, false>), dim3(grid), dim3(wg_threads), lds, s, *a);
   return (int)hipGetLastError();
 }
 

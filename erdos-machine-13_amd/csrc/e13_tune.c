@@ -190,6 +190,8 @@ int e13_load_tuning(const char *path, e13_tuning *t) {
     for (int k = 0; k < NKNOB; k++)
       if (!strcmp(name, knob_name[k])) *knob(t, k) = v;
     if (!strcmp(name, "gate")) t->gate = v;
+    if (!strcmp(name, "tape_words")) t->tape_words = v;
+    if (!strcmp(name, "wide_nodes")) t->wide_nodes = v;
   }
   fclose(f);
   return E13_OK;

@@ -39,6 +39,7 @@ extern "C" {
 #define E13_EWATCHDOG (-12) /* a wave waited longer than the idle watchdog for work or the end mark */
 #define E13_ELDS (-13)      /* dynamic LDS does not start at LDS address 0 */
 #define E13_ESCRATCH (-14)  /* e13_estimate_static: more probes than its static scratch holds */
+#define E13_ERESIDENT (-15) /* not every workgroup of the persistent search became resident at once */
 
 /* Result codes of a run (e13search.c:64-65). */
 #define E13_DEAD (-1)
@@ -127,6 +128,14 @@ typedef struct {
                              machine whose two ones sit in the wrong cells does not leave the loop at every
                              step; 2: test it at every step; 0 or negative: 1 when some problem has an edit
                              budget (delete mode), else 2.  Never changes results. */
+  int32_t tape_words;     /* persistent form: tape words per lane in LDS.  0 or negative (automatic): when the
+                             whole tape (43 words, the window of cells -63..1298) does not let wgs_per_cu
+                             workgroups fit a CU, the widest shorter tape with which the most of them do; 18..42:
+                             that tape; 43: the whole tape.  With a shorter tape, machines whose head goes past
+                             it are paused and finish in a second launch with the whole tape.  Never changes
+                             results. */
+  int32_t wide_nodes;     /* capacity of the list of those machines (nodes; 0: 2^19); when it fills, the
+                             launch stops cleanly and the rest of the search runs on the whole tape */
 } e13_tuning;
 
 typedef struct {
@@ -148,6 +157,12 @@ typedef struct {
   int64_t frontier;              /* nodes the last search's parallel phase started from: the breadth-first
                                     frontier of the persistent form (the reference's front=, e13search.c:373),
                                     1 (the root) for the host-loop forms */
+  int32_t tape_words;            /* persistent form: lane tape words of the last search's first launch */
+  int32_t pad_s;
+  int64_t wide_nodes;            /* machines that went past the short tape (tape_words < 43) */
+  int64_t wide_launches;         /* launches with the whole tape after a short-tape launch, */
+  double wide_ms;                /* their HIP-event time */
+  int64_t wide_steps;            /* and the steps simulated in them (part of search_ms / search_steps) */
 } e13_stats;
 
 typedef struct e13_ctx e13_ctx;

@@ -7,8 +7,9 @@
  * aligned 16-cell block, to the step that leaves it or for at most STAY
  * steps).  Both must give exactly what plain stepping gives: the same steps,
  * the same exit (a stage-gate candidate or an exceptional rule before the
- * step, or the head leaving the window, which is not counted), the same tape,
- * head, state and ones count.  This program restates that loop in C (the
+ * step, the head leaving the window, which is not counted, or the head going
+ * past a short lane tape of tw words, which is counted), the same tape, head,
+ * state and ones count.  This program restates that loop in C (the
  * kernel's control flow and encodings, scalar) and compares it with a plain
  * one-step-at-a-time loop on random machines, tapes, heads, gate registers and
  * round lengths, biased towards the hard cases: few states (loops inside a
@@ -18,8 +19,9 @@
  * Usage: solo_loop_check SEED RUNS; prints "runs=N mismatches=M stays=S" and
  * exits 1 on a mismatch.  Built with -DPLANT=1 (ones mask of the boundary
  * block), -DPLANT=2 (gate margin 8 instead of 16), -DPLANT=4 (a word of a run
- * that crosses words counted as 33 cells) or -DPLANT=3 (stay position
- * off by one) it carries a deliberate slip, which the check must catch.
+ * that crosses words counted as 33 cells), -DPLANT=3 (stay position
+ * off by one) or -DPLANT=5 (a sweep allowed one cell past a short tape) it
+ * carries a deliberate slip, which the check must catch.
  */
 #include <stdint.h>
 #include <stdio.h>
@@ -49,10 +51,12 @@ typedef struct {
    0x0F00 | kind << 12 for a halt / unbound cell */
 static uint32_t rows[16];
 static int target, qA, qR, st0r, nt1;
+static int qlim = QMAX, tw = 43; /* the lane tape: cells q <= qlim = 32 * tw - 1 (QMAX for the whole window) */
 
 static uint32_t entry(int st, uint32_t sym) { return (rows[st & 15] >> (sym << 4)) & 0xFFFFu; }
 
-/* The plain loop: up to n steps; why = 1 exit before a step, 2 left the window. */
+/* The plain loop: up to n steps; why = 1 exit before a step, 2 left the window,
+   3 went past the lane tape (that step counts). */
 static int plain(Machine *m, int n, int *why) {
   *why = 0;
   for (int r = 0; r < n; ++r) {
@@ -72,6 +76,10 @@ static int plain(Machine *m, int n, int *why) {
       return r;
     }
     m->st = (int)((e >> 8) & 15u);
+    if (m->q > qlim) {
+      *why = 3;
+      return r + 1;
+    }
   }
   return n;
 }
@@ -114,15 +122,16 @@ static int fast(Machine *m, int n, int *why, int me) {
       if (to_end) { /* the run goes on through the following words while they are all sym */
         const uint32_t fill = 0u - sym;
         int wj = wi + ((e & 2u) ? 1 : -1);
-        while (wj >= 0 && wj < 43 && m->tape[wj] == fill) {
+        while (wj >= 0 && wj < tw && m->tape[wj] == fill) {
           k += PLANT == 4 ? 33 : 32;
           wj += (e & 2u) ? 1 : -1;
         }
-        const uint32_t wd = (wj >= 0 && wj < 43 ? m->tape[wj] : ~fill) ^ fill;
+        const uint32_t wd = (wj >= 0 && wj < tw ? m->tape[wj] : ~fill) ^ fill;
         k += (e & 2u) ? __builtin_ctz(wd) : __builtin_clz(wd);
       }
       if (e & 2u) {
-        if (k > QMAX - q) k = QMAX - q;
+        const int lim = qlim - q + (PLANT == 5);
+        if (k > lim) k = lim;
       } else {
         if (k > q) k = q;
       }
@@ -154,7 +163,7 @@ static int fast(Machine *m, int n, int *why, int me) {
     const int margin = PLANT == 2 ? 8 : 16;
     const int far = (ones - target > margin) | (target - ones > margin);
     const int no_gate_cell = !nt1 & ((qA >> 4) != (qn >> 4)) & ((qR >> 4) != (qn >> 4));
-    if ((qn >= 16) & (qn + 16 <= QMAX) & (far | no_gate_cell)) { /* a block macro-step */
+    if ((qn >= 16) & (qn + 16 <= qlim) & (far | no_gate_cell)) { /* a block macro-step */
       const uint32_t nb = qb & ~15u, byt = (word >> nb) & 0xFFFFu, pos = qb & 15u;
       const uint32_t key = ((uint32_t)st << 20) | (byt << 4) | pos;
       const uint32_t slot = (key * 0x9E3779B1u) >> 22;
@@ -219,6 +228,11 @@ static int fast(Machine *m, int n, int *why, int me) {
       break;
     }
     st = (int)(nx >> 8);
+    if (q > qlim) { /* past the lane tape: the step counts, the kernel pauses the machine */
+      r += 1;
+      *why = 3;
+      break;
+    }
     if ((q >> 5) != wi) {
       m->tape[wi] = word;
       wi = q >> 5;
@@ -260,11 +274,15 @@ int main(int argc, char **argv) {
         if (x) a.tape[w] = x == 1 ? ~0u : 0u;
       }
     a.tape[42] &= (1u << (QMAX + 1 - 1344)) - 1u;
+    /* half the runs on a short lane tape of tw words: the cells past it blank */
+    tw = it % 2 ? 18 + (int)(splitmix(&s) % 25) : 43;
+    qlim = tw < 43 ? 32 * tw - 1 : QMAX;
+    for (int w = tw; w < 43; w++) a.tape[w] = 0u;
     const int where = (int)(it % 8);
     a.q = where == 1 ? 52 + (int)(splitmix(&s) % 20)
           : where == 2 ? (int)(splitmix(&s) % 24)
-          : where == 3 ? QMAX - (int)(splitmix(&s) % 24)
-                       : (int)(splitmix(&s) % (QMAX + 1));
+          : where == 3 ? qlim - (int)(splitmix(&s) % 24)
+                       : (int)(splitmix(&s) % (qlim + 1));
     a.st = (int)(splitmix(&s) % (uint64_t)nq);
     for (int q = QBLK; q <= QMAX; q++) a.ones += (int)((a.tape[q >> 5] >> (q & 31)) & 1u);
     const int nt = 1 + (int)(splitmix(&s) % 9), tm = (int)(splitmix(&s) % 4);

@@ -3,7 +3,8 @@ context does not cover: fresh contexts with the stored settings profiles
 (including searches whose breadth-first frontier finishes the whole tree),
 free-mode searches with no deleted state (X = -1), the raw 3-state counter
 search without symmetry, the division search V3, and FMAX=3 in the plain
-per-leaf-launch form.  Exact comparisons throughout."""
+per-leaf-launch form, and the persistent search on a short lane tape.  Exact
+comparisons throughout."""
 import pytest
 
 from conftest import REF_DUMP, REF_RESULTS, XS, parse_summaries, read_machine, read_text, ref_hashes, survivor_lines
@@ -155,3 +156,67 @@ def test_local_edits_fmax3_plain_form(e13, gpu_ctx, base):
     finally:
         gpu_ctx.reset_tuning()
     assert sorted(lines) == survivor_lines(read_text(REF_RESULTS, "1_local_edits_surv_f3.txt"))
+
+
+def check_many(e13, ctx, problems, golden, surv_file):
+    cs, sv, ns = ctx.search_many([p for _, p in problems], surv_cap=4096)
+    assert ns == len(sv)
+    for (X, _), c in zip(problems, cs):
+        d = c.as_dict()
+        same({"leaves": d["leaves"], "survivors": d["survivors"], "steps": d["steps"], "hist": d["hist"]}, golden[X])
+    lines = []
+    for s in sv:
+        assert s.recheck_ok == 1
+        lines.append(e13.format_survivor(problems[s.prob][1], s))
+    assert sorted(lines) == survivor_lines(read_text(REF_RESULTS, surv_file))
+
+
+@pytest.mark.parametrize("tune", [dict(wgs_per_cu=2, tape_words=20), dict(wgs_per_cu=3, tape_words=18),
+                                  dict(wgs_per_cu=3, tape_words=-1), dict(wgs_per_cu=1, tape_words=24, slice=64)],
+                         ids=lambda t: "-".join(f"{k}{v}" for k, v in t.items()))
+def test_short_lane_tape(e13, base, tune):
+    """The persistent search on a short lane tape in LDS (more workgroups per
+    CU): machines whose head goes past it are paused and finish in a second
+    launch with the whole tape; FMAX=3 over X=0..13 gives the reference's
+    per-X counts and survivors, and the second launch did run."""
+    f3 = parse_summaries(read_text(REF_DUMP, "delete_f3.txt"))
+    with e13.Context(0) as ctx:
+        say(f"search: delete FMAX=3 X=0..13, one persistent search, {tune}, fresh context")
+        ctx.set_tuning(form=3, **tune)
+        check_many(e13, ctx, [(X, e13.setup(base, X, 3)) for X in XS], f3, "1_local_edits_surv_f3.txt")
+        st = ctx.stats()
+        assert 18 <= st["tape_words"] < 43
+        assert st["wide_nodes"] > 0 and st["wide_launches"] >= 1
+
+
+@pytest.mark.parametrize("cap", [1, 5])
+def test_short_lane_tape_full_list(e13, base, cap):
+    """A list of machines past the short tape that fills at once: the launch
+    stops cleanly, and the rest of the search runs on the whole tape; FMAX=3
+    over X=0..13 still gives the reference's counts and survivors."""
+    f3 = parse_summaries(read_text(REF_DUMP, "delete_f3.txt"))
+    with e13.Context(0) as ctx:
+        say(f"search: delete FMAX=3 X=0..13, tape_words=20, list of {cap} wide nodes, fresh context")
+        ctx.set_tuning(form=3, wgs_per_cu=3, tape_words=20, wide_nodes=cap)
+        check_many(e13, ctx, [(X, e13.setup(base, X, 3)) for X in XS], f3, "1_local_edits_surv_f3.txt")
+        st = ctx.stats()
+        assert st["tape_words"] == 20 and st["wide_launches"] >= 1
+
+
+def test_short_lane_tape_checkpoints(e13, base, tmp_path):
+    """Clean stops and resumes on a short lane tape: the counter search with a
+    checkpoint every 50 ms on a 20-word lane tape (machines past it wait in the
+    saved nodes and finish on the whole tape) gives the reference's counts and
+    61 survivors."""
+    want = parse_summaries(read_text(REF_RESULTS, "2_counter_3_log_modsym.txt"))[13]
+    p = e13.setup(base, 13, 0, "D0 G0 K0 K1 L1 M1".split(), (10, 13))
+    with e13.Context(0) as ctx:
+        say("search: freesym X=13 10..13 D0 G0 K0 K1 L1 M1, tape_words=20, checkpoint every 50 ms")
+        ctx.set_tuning(form=3, wgs_per_cu=3, tape_words=20)
+        ctx.set_checkpoint(str(tmp_path / "short"), every_sec=0.05)
+        got, lines, _ = run_one(e13, ctx, p)
+        ctx.set_checkpoint(None)
+        same(got, want)
+        assert lines == survivor_lines(read_text(REF_RESULTS, "2_counter_3_surv_modsym.txt"))
+        st = ctx.stats()
+        assert st["search_launches"] >= 3 and st["wide_nodes"] > 0

@@ -230,12 +230,14 @@ def test_lone_lane_loop_accelerations_match_plain_stepping(tmp_path):
     """The register loop's sweeps (within a tape word and across words) and
     16-cell block memo, restated in C (tests/solo_loop_check.c), give exactly
     what plain stepping gives on random machines, tapes (with runs of whole
-    words), heads and gate registers; each of four planted slips (ones mask of
-    the block at q 48..63, gate margin 8 instead of 16, a stay position off by
-    one, a word of a cross-word run counted as 33 cells) is caught."""
+    words), heads and gate registers, on the whole window and on short lane
+    tapes (a machine going past one stops after that counted step); each of
+    five planted slips (ones mask of the block at q 48..63, gate margin 8
+    instead of 16, a stay position off by one, a word of a cross-word run
+    counted as 33 cells, a sweep one cell past a short tape) is caught."""
     import subprocess
     src = os.path.join(ROOT, "tests", "solo_loop_check.c")
-    for plant in (0, 1, 2, 3, 4):
+    for plant in (0, 1, 2, 3, 4, 5):
         exe = str(tmp_path / f"solo_loop_check_{plant}")
         subprocess.run(["gcc", "-O2", f"-DPLANT={plant}", "-o", exe, src], check=True)
         for seed in ((7, 11) if plant == 0 else (7,)):
