@@ -52,6 +52,9 @@ __device__ __forceinline__ void lds_st(char *sb, int off, uint32_t v) {
   (void)sb;
   *(lds_word_t *)(size_t)(uint32_t)off = v;
 }
+__device__ __forceinline__ void lds_xor(int off, uint32_t v) {
+  __hip_atomic_fetch_xor((lds_word_t *)(size_t)(uint32_t)off, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
 __device__ __forceinline__ uint32_t lds_base(const void *p) {
   return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void *)p;
 }
@@ -316,8 +319,7 @@ __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int m
   while (!stop) {
     // the step
     const uint32_t flip = __builtin_amdgcn_ubfe(e, 2u, 1u); // the write changes the cell
-    word ^= flip << ((uint32_t)L.q & 31u);
-    lds_st(c.sb, wa, word);
+    lds_xor(wa, flip << ((uint32_t)L.q & 31u));
     if (GATE == GATE_COUNT) {
       const int dm = L.q >= E13_QBLK ? __builtin_amdgcn_sbfe((int)e, 2u, 2u) : 0; // write - read
       L.ones += dm;
