@@ -31,7 +31,9 @@
  * E13_GATE (1 count, 2 hash), E13_WG, E13_WGS_PER_CU, E13_DEVICE,
  * E13_REFILL_TRIES, E13_SEED_PER_WAVE, E13_SCHED_EVERY, E13_DONATE_MAX,
  * E13_IDLE_NAPS, E13_MAIL_EVERY, E13_GRAPH_CHAIN, E13_SOLO_STEPS, E13_EVICT_AFTER, E13_CUT_LIVE (negative: off),
- * E13_PROBE_SLICE, E13_NT1_SKIP (1 on, 2 off), E13_PHASE_CYCLES=1 (the persistent search's per-phase wave cycles);
+ * E13_PROBE_SLICE, E13_NT1_SKIP (1 on, 2 off), E13_PHASE_CYCLES=1 (the persistent search's per-phase wave cycles),
+ * E13_TAPE_WORDS (lane tape words of the persistent search: 18..42 short, 43 whole, negative automatic),
+ * E13_WIDE_NODES (capacity of the list of machines past a short tape);
  * E13_TUNING_FILE (stored settings; <name>_small.<ext> beside it for small searches); E13_TRACE=N with
  * E13_TRACE_FILE=path writes the persistent search's wave event log (N events per wave) to path; the
  * library prints its host phase times with E13_TIMING=1 and a line per autotuner trial with E13_TUNE_LOG=1.
@@ -474,6 +476,8 @@ int main(int argc, char **argv) {
   tu.phase_cycles = env_int("E13_PHASE_CYCLES");
   tu.nt1_skip = env_int("E13_NT1_SKIP");
   tu.probe_slice = env_int("E13_PROBE_SLICE");
+  tu.tape_words = env_int("E13_TAPE_WORDS");
+  tu.wide_nodes = env_int("E13_WIDE_NODES");
   e13_ctx *ctx = NULL;
   int rc = e13_open(env_int("E13_DEVICE"), &tu, &ctx);
   if (rc) {

@@ -319,6 +319,8 @@ __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int m
   while (!stop) {
     // the step
     const uint32_t flip = __builtin_amdgcn_ubfe(e, 2u, 1u); // the write changes the cell
+    // flipped in LDS by an atomic xor with no result: the word is read again
+    // for the next step, so the register copy needs no update
     lds_xor(wa, flip << ((uint32_t)L.q & 31u));
     if (GATE == GATE_COUNT) {
       const int dm = L.q >= E13_QBLK ? __builtin_amdgcn_sbfe((int)e, 2u, 2u) : 0; // write - read
