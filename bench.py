@@ -20,7 +20,9 @@ FMAX=4 in the plain form and with the static, learned, random and
 probe-estimated frontier orders (interleaved, median of 3), and the probe
 estimator (the reference's estimate mode) on
 the counter search, its first probes matched bit for bit against the dump of
-the reference's probe().
+the reference's probe(), and the deep stage check (the reference's deep.c) of
+all 2868 V1 survivors to stage 15 on the GPU, its lines equal to the
+reference's file, with the reference program on a sample of it on the host.
 
 roofline: the dominant kernel is the persistent search kernel (k_search) on
 its short lane tape (k_search<0, true>, three workgroups per CU); the
@@ -258,6 +260,57 @@ def estimator_workload(e13, ctx, base, probes=1 << 20):
                          "exact_leaves": 3078656272, "exact_steps": 319112357176}}
 
 
+def deep_workload(e13, ctx, nmax=15, sample=64, with_cpu=True):
+    """The deep stage check (the reference's src/deep.c) of all 2868 survivors
+    of V1's 1RD search to stage 15 on the GPU, its lines compared with the
+    reference's own output (results/4_division_6/v1/deep15_1RD.txt, whose
+    survivor numbers restart per chunk, so compared after the number); and on
+    the first `sample` survivors the GPU beside the reference program compiled
+    from its source (oracle/_ref/deep, one process per host core, each on its
+    share of the sample), or the library's host threads without it."""
+    import gzip
+    rr = os.path.join(ROOT, "tests", "golden", "ref_results")
+    surv = gzip.open(os.path.join(rr, "4_division_6_v1_surv_1RD.txt.gz"), "rt").read().splitlines()
+    want = gzip.open(os.path.join(rr, "4_division_6_v1_deep15_1RD.txt.gz"), "rt").read().splitlines()
+    s0 = ctx.stats()
+    t0 = time.perf_counter()
+    got = ctx.deep_check("\n".join(surv) + "\n", nmax).splitlines()
+    dt = time.perf_counter() - t0
+    s1 = ctx.stats()
+    ok = len(got) == len(want) and all(g.split(" ", 2)[2] == w.split(" ", 2)[2] for g, w in zip(got, want))
+    out = {"workload": f"deep stage check (deep.c) of the {len(surv)} V1 1RD survivors to stage {nmax}",
+           "kernel": "k_deep", "seconds": dt, "kernel_ms": s1["kernel_ms"] - s0["kernel_ms"],
+           "launches": s1["launches"] - s0["launches"], "survivors_per_s": len(surv) / dt,
+           "lines_equal_reference": ok}
+    text = "\n".join(surv[:sample]) + "\n"
+    t0 = time.perf_counter()
+    ctx.deep_check(text, nmax)
+    gdt = time.perf_counter() - t0
+    out["sample"] = {"survivors": sample, "gpu_seconds": gdt}
+    if with_cpu:
+        model, ncpu = host_cpu()
+        ref = os.path.join(ROOT, "oracle", "_ref", "deep")
+        t0 = time.perf_counter()
+        if os.path.exists(ref):
+            parts = [surv[i:sample:ncpu] for i in range(min(ncpu, sample))]
+            procs = [subprocess.Popen([ref, str(nmax)], stdin=subprocess.PIPE, stdout=subprocess.DEVNULL, text=True)
+                     for _ in parts]
+            for pr, part in zip(procs, parts):
+                pr.stdin.write("\n".join(part) + "\n")
+                pr.stdin.close()
+            for pr in procs:
+                pr.wait()
+            kind, cores = "reference", len(parts)
+        else:
+            e13.deep_check(text, nmax, threads=ncpu)
+            kind, cores = "port", ncpu
+        cdt = time.perf_counter() - t0
+        out["sample"]["cpu"] = {"seconds": cdt, "kind": kind, "cores": cores, "cpu_model": model,
+                                "survivors_per_s": sample / cdt}
+        out["sample"]["gpu_over_cpu"] = cdt / gdt
+    return out
+
+
 def cpu_estimate_baseline(probes=8000000, max_s=120):
     """The reference program's own estimate mode on the same problem, all host cores."""
     ref = os.path.join(ROOT, "oracle", "_ref", "e13search")
@@ -465,6 +518,9 @@ def main():
         if not args.no_cpu:
             est["cpu"] = cpu_estimate_baseline()
         extra.append(est)
+        print("bench: estimator done", file=sys.stderr, flush=True)
+        extra.append(deep_workload(e13, ctx, with_cpu=not args.no_cpu))
+        print("bench: deep check done", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu and args.workload == "fmax4":
         cpu = cpu_baseline()

@@ -157,7 +157,7 @@ EXPORTS = ("e13_open", "e13_close", "e13_strerror", "e13_set_tuning", "e13_get_t
            "e13_set_cost_model", "e13_autotune", "e13_save_tuning", "e13_load_tuning", "e13_tuning_file",
            "e13_tuning_file_small", "e13_search_profile", "e13_estimate", "e13_estimate_static", "e13_set_checkpoint",
            "e13_deep_check_line", "e13_deep_check", "e13_recheck_counter_line", "e13_recheck_counter",
-           "e13_check_survivor", "e13_classify_deep")
+           "e13_check_survivor", "e13_classify_deep", "e13_deep_check_gpu", "e13_recheck_counter_gpu")
 
 _lib = None
 
@@ -233,6 +233,9 @@ def lib():
     L.e13_check_survivor.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
     L.e13_deep_check.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
                                  ctypes.c_int64]
+    L.e13_deep_check_gpu.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int64]
+    L.e13_recheck_counter_gpu.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p,
+                                          ctypes.c_int64]
     L.e13_set_checkpoint.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_double, ctypes.c_int64]
     L.e13_estimate.argtypes = [ctypes.c_void_p, P(Problem), ctypes.c_uint64, ctypes.c_int64, P(ctypes.c_double),
                                P(EstimateSum)]
@@ -469,6 +472,23 @@ class Context:
         s = Stats()
         _check(lib().e13_get_stats(self._h, ctypes.byref(s)), "e13_get_stats")
         return s.as_dict()
+
+    def deep_check(self, text, nmax):
+        """src/deep.c's stage check of every SURVIVOR line of text on this GPU
+        (numbered from 1): its result lines."""
+        cap = 2048 * (len(text) // 64 + 16)
+        buf = ctypes.create_string_buffer(cap)
+        _check(min(lib().e13_deep_check_gpu(self._h, text.encode(), nmax, buf, cap), 0), "e13_deep_check_gpu")
+        return buf.value.decode()
+
+    def recheck_counter(self, text, nmax):
+        """src/recheck.c's counter re-check of every line with table= on this GPU:
+        its verdict lines."""
+        cap = 20480 * (len(text) // 64 + 16)
+        buf = ctypes.create_string_buffer(cap)
+        _check(min(lib().e13_recheck_counter_gpu(self._h, text.encode(), nmax, buf, cap), 0),
+               "e13_recheck_counter_gpu")
+        return buf.value.decode()
 
     def search(self, p, surv_cap=200000):
         """run_delete() for one problem: (Counts, [Survivor], n_survivors)."""

@@ -14,6 +14,8 @@
  *   e13search_gpu deep    NMAX < survivors.txt   (src/deep.c's stage check to NMAX, host threads)
  *   e13search_gpu deep_big NMAX < survivors.txt  (src/deep_big.c: 4M cells each side)
  *   e13search_gpu recheck NMAX < survivors.txt   (src/recheck.c's counter re-check, host threads)
+ *   e13search_gpu deep_gpu NMAX < survivors.txt, recheck_gpu NMAX < survivors.txt   (the same on the
+ *                                       GPU, one wave per survivor; NMAX <= 18)
  *   e13search_gpu check   survivors.txt          (src/check.py's crafted-start check, host only)
  *   e13search_gpu classify survivors.txt deep.txt...   (src/classify_deep.py, host only)
  *   e13search_gpu tune    machine.txt FMAX X...   (autotune on that search; saves the settings
@@ -309,14 +311,16 @@ static int run_many(e13_ctx *ctx, const int32_t *base, int nq_base, int nx, char
 int main(int argc, char **argv) {
   if (argc < 3) {
     fprintf(stderr, "usage: e13search_gpu base|mutant|delete|free|freesym|gpu|estimate|tune machine.txt [...]\n"
-                    "       e13search_gpu synth SEED0 K0 N | deep|deep_big|recheck NMAX < survivors.txt |\n"
+                    "       e13search_gpu synth SEED0 K0 N | deep|deep_big|recheck|deep_gpu|recheck_gpu NMAX < survivors.txt |\n"
                     "       e13search_gpu check survivors.txt | classify survivors.txt deep.txt...\n");
     return 1;
   }
-  if (!strcmp(argv[1], "deep") || !strcmp(argv[1], "deep_big") || !strcmp(argv[1], "recheck")) {
+  if (!strcmp(argv[1], "deep") || !strcmp(argv[1], "deep_big") || !strcmp(argv[1], "recheck") ||
+      !strcmp(argv[1], "deep_gpu") || !strcmp(argv[1], "recheck_gpu")) {
     /* deep NMAX < survivors.txt (src/deep.c; deep_big: src/deep_big.c), recheck NMAX < survivors.txt
-       (src/recheck.c): host threads */
-    const int rech = !strcmp(argv[1], "recheck"), big = !strcmp(argv[1], "deep_big");
+       (src/recheck.c): host threads; deep_gpu / recheck_gpu: the same on the GPU */
+    const int rech = !strcmp(argv[1], "recheck") || !strcmp(argv[1], "recheck_gpu"),
+              big = !strcmp(argv[1], "deep_big"), gpu = strstr(argv[1], "_gpu") != NULL;
     const int nmax = atoi(argv[2]);
     size_t cap = 1 << 20, len = 0;
     char *text = (char *)malloc(cap);
@@ -334,9 +338,15 @@ int main(int argc, char **argv) {
     const int threads = th && atoi(th) > 0 ? atoi(th) : (int)sysconf(_SC_NPROCESSORS_ONLN);
     const int64_t outlen = (rech ? 20480 : 2048) * (int64_t)(len / 64 + 16);
     char *out = (char *)malloc((size_t)outlen);
-    const int rc = !out ? E13_ENOMEM
-                        : rech ? e13_recheck_counter(text, nmax, threads, out, outlen)
-                               : e13_deep_check(text, nmax, big, threads, out, outlen);
+    e13_ctx *dctx = NULL;
+    int rc = !out ? E13_ENOMEM : E13_OK;
+    if (!rc && gpu) rc = e13_open(env_int("E13_DEVICE"), NULL, &dctx);
+    if (!rc)
+      rc = gpu ? (rech ? e13_recheck_counter_gpu(dctx, text, nmax, out, outlen)
+                       : e13_deep_check_gpu(dctx, text, nmax, out, outlen))
+               : (rech ? e13_recheck_counter(text, nmax, threads, out, outlen)
+                       : e13_deep_check(text, nmax, big, threads, out, outlen));
+    if (dctx) e13_close(dctx);
     if (rc < 0) {
       fprintf(stderr, "%s: %s\n", argv[1], e13_strerror(rc));
       return 1;

@@ -8,7 +8,8 @@
  * tape.  The output lines are the reference's, survivor by survivor.
  *
  * These are a few long sequential runs (a survivor to stage 15 is ~1e9
- * steps), so they run on host threads, one survivor at a time per thread.
+ * steps): on host threads, one survivor at a time per thread, or on the GPU
+ * (end of this file), one wave per survivor.
  */
 #include <pthread.h>
 #include <stdarg.h>
@@ -17,6 +18,7 @@
 #include <string.h>
 
 #include "e13.h"
+#include "e13_ctx.h"
 
 #define DEEP_LEFT 4096         /* cells left of cell 0 (deep.c:13) */
 #define DEEP_BIG_LEFT (1 << 22) /* deep_big.c:14: cells each side beyond the block */
@@ -576,4 +578,265 @@ int e13_classify_deep(const char *survivors, const char *deep, char *out, int64_
   free(dv);
   free(kinds);
   return rc ? rc : nd;
+}
+
+/* ------------------------------------------------------------------------ */
+/* The deep check (deep.c, 4096 cells left of cell 0) and the counter        */
+/* re-check (recheck.c) on the GPU: one wave per survivor, its tape in LDS   */
+/* (k_deep, e13_deepk.hip), the same lines as the host versions above.  A    */
+/* re-check whose machine leaves the tape held on the GPU (recheck.c's tape  */
+/* is 4e6 cells each side) runs on the host instead; so does a table whose   */
+/* moves name a state it does not have.                                      */
+/* ------------------------------------------------------------------------ */
+int e13k_launch_deep(const E13DeepArgs *a, int nblocks, hipStream_t s);
+
+#define DG_BUDGET (1LL << 20) /* loop iterations per survivor per launch */
+
+/* deep.c's counter at a stage start from the snapshot of the tape's left side
+   (bits 0 .. left-1): the ones at cells -3 down to -left+1, and the leftmost. */
+static void dg_count(const uint32_t *sn, int left, long long *cnt, long long *lo) {
+  *cnt = 0;
+  *lo = 0;
+  for (int q = left - 3; q >= 1; q--)
+    if ((sn[q >> 5] >> (q & 31)) & 1u) {
+      ++*cnt;
+      *lo = q - left;
+    }
+}
+
+/* recheck.c's counter set text (rc_counter) from the snapshot: cells left of
+   the leftmost head position are blank, so listing every one down to the
+   tape's end is the same list. */
+static void dg_set(const uint32_t *sn, int left, char *q) {
+  int len = 0;
+  q[0] = 0;
+  for (int b = left - 3; b >= 0 && len < 8000; b--)
+    if ((sn[b >> 5] >> (b & 31)) & 1u) len += snprintf(q + len, (size_t)(RC_SET - len), "%d,", b - left);
+}
+
+static int deep_gpu(e13_ctx *c, const char *text, int nmax, int recheck, char *out, int64_t outlen) {
+  if (!c || !text || !out || outlen < 1 || nmax < 1 || nmax > 18) return E13_EINVAL;
+  char *copy = strdup(text);
+  if (!copy) return E13_ENOMEM;
+  int cap = 1024, n = 0;
+  const char **lines = (const char **)malloc(sizeof(char *) * (size_t)cap);
+  for (char *save = NULL, *l = strtok_r(copy, "\n", &save); lines && l; l = strtok_r(NULL, "\n", &save)) {
+    if (!strstr(l, "table=") || (!recheck && !strstr(l, "st0="))) continue;
+    if (n == cap) {
+      cap *= 2;
+      const char **nl = (const char **)realloc(lines, sizeof(char *) * (size_t)cap);
+      if (!nl) {
+        free(lines);
+        lines = NULL;
+        break;
+      }
+      lines = nl;
+    }
+    lines[n++] = l;
+  }
+  const int left = DEEP_LEFT;
+  const int W = (int)((left + (1LL << nmax) + 4096 + 31) / 32), SW = left / 32;
+  deep_machine *ms = lines && n ? (deep_machine *)calloc((size_t)n, sizeof *ms) : NULL;
+  uint32_t *rows = n ? (uint32_t *)calloc((size_t)n * E13_DEEP_MAXQ, 4) : NULL;
+  int32_t *param = n ? (int32_t *)calloc((size_t)n * 4, 4) : NULL;
+  E13DeepState *st = n ? (E13DeepState *)calloc((size_t)n, sizeof *st) : NULL;
+  int32_t *list = n ? (int32_t *)malloc((size_t)n * 4) : NULL;
+  uint32_t *snap = n ? (uint32_t *)malloc((size_t)n * (size_t)(nmax + 1) * SW * 4) : NULL;
+  char *res = n ? (char *)calloc((size_t)n, DEEP_OUT) : NULL;
+  int rc = lines && (!n || (ms && rows && param && st && list && snap && res)) ? E13_OK : E13_ENOMEM;
+  const long long kmul = recheck ? 8 : 64;
+  const int timing = getenv("E13_TIMING") && *getenv("E13_TIMING") != '0'; /* a line per launch on stderr */
+  for (int i = 0; !rc && i < n; i++) {
+    deep_machine *m = &ms[i];
+    if (!(recheck ? deep_parse_recheck(lines[i], m) : deep_parse(lines[i], m))) rc = E13_EINVAL;
+    int host = 0; /* a move to a state the table does not have: the host's loop */
+    for (int q = 0; !rc && q < E13_DEEP_MAXQ; q++) {
+      uint32_t ent[2];
+      for (int a = 0; a < 2; a++) {
+        if (q >= m->nq || m->stop[q][a]) {
+          ent[a] = 0x8000u | ((q >= m->nq ? 1u : (uint32_t)m->stop[q][a]) << 13);
+        } else {
+          if (m->nx[q][a] < 0 || m->nx[q][a] >= m->nq) host = 1;
+          ent[a] = (uint32_t)m->wr[q][a] | ((m->dir[q][a] > 0 ? 1u : 0u) << 1) | ((uint32_t)(m->nx[q][a] & 31) << 8);
+        }
+      }
+      rows[(size_t)i * E13_DEEP_MAXQ + q] = ent[0] | (ent[1] << 16);
+    }
+    if (m->st0 < 0 || m->st0 >= E13_DEEP_MAXQ || m->head0 < -left + 2 || m->head0 > 4096) host = 1;
+    param[4 * i + 0] = m->st0;
+    param[4 * i + 1] = (int32_t)(m->head0 + left);
+    param[4 * i + 2] = m->rel;
+    st[i].q = left;
+    st[i].minq = left;
+    st[i].n = 1;
+    st[i].limit = kmul * 4 + 100000;
+    st[i].status = host ? 2 : 0;
+  }
+  /* device buffers, launches until every survivor is finished */
+  uint32_t *d_rows = NULL, *d_tape = NULL, *d_snap = NULL;
+  int32_t *d_param = NULL, *d_list = NULL;
+  E13DeepState *d_st = NULL;
+  float kms = 0;
+  int launches = 0;
+  if (!rc && n) {
+    if (hipSetDevice(c->device) != hipSuccess) rc = E13_EHIP;
+    if (!rc && (hipMalloc((void **)&d_rows, (size_t)n * E13_DEEP_MAXQ * 4) != hipSuccess ||
+                hipMalloc((void **)&d_param, (size_t)n * 16) != hipSuccess ||
+                hipMalloc((void **)&d_st, (size_t)n * sizeof(E13DeepState)) != hipSuccess ||
+                hipMalloc((void **)&d_list, (size_t)n * 4) != hipSuccess ||
+                hipMalloc((void **)&d_tape, (size_t)n * (size_t)W * 4) != hipSuccess ||
+                hipMalloc((void **)&d_snap, (size_t)n * (size_t)(nmax + 1) * SW * 4) != hipSuccess))
+      rc = E13_ENOMEM;
+    if (!rc && (hipMemcpy(d_rows, rows, (size_t)n * E13_DEEP_MAXQ * 4, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(d_param, param, (size_t)n * 16, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(d_st, st, (size_t)n * sizeof(E13DeepState), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemset(d_tape, 0, (size_t)n * (size_t)W * 4) != hipSuccess ||
+                hipMemset(d_snap, 0, (size_t)n * (size_t)(nmax + 1) * SW * 4) != hipSuccess))
+      rc = E13_EHIP;
+    E13DeepArgs a;
+    memset(&a, 0, sizeof a);
+    a.rows = d_rows;
+    a.param = d_param;
+    a.state = d_st;
+    a.tape = d_tape;
+    a.snap = d_snap;
+    a.list = d_list;
+    a.nmax = nmax;
+    a.W = W;
+    a.left = left;
+    a.mode = recheck;
+    a.kmul = kmul;
+    a.budget = DG_BUDGET;
+    for (;;) {
+      if (rc) break;
+      int m = 0;
+      for (int i = 0; i < n; i++)
+        if (st[i].status == 0) list[m++] = i;
+      if (m == 0) break;
+      if (hipMemcpyAsync(d_list, list, (size_t)m * 4, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+          hipEventRecord(c->ev0, c->stream) != hipSuccess || e13k_launch_deep(&a, m, c->stream) != 0 ||
+          hipEventRecord(c->ev1, c->stream) != hipSuccess ||
+          hipMemcpyAsync(st, d_st, (size_t)n * sizeof(E13DeepState), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+          hipStreamSynchronize(c->stream) != hipSuccess) {
+        rc = E13_EHIP;
+        break;
+      }
+      float ms1 = 0;
+      if (hipEventElapsedTime(&ms1, c->ev0, c->ev1) == hipSuccess) kms += ms1;
+      launches++;
+      if (timing) fprintf(stderr, "libe13: deep launch %d: %d survivors, %.1f ms\n", launches, m, ms1);
+      for (int i = 0; i < n; i++)
+        if (st[i].status == 3) rc = E13_ELDS;
+    }
+    if (!rc && hipMemcpy(snap, d_snap, (size_t)n * (size_t)(nmax + 1) * SW * 4, hipMemcpyDeviceToHost) != hipSuccess)
+      rc = E13_EHIP;
+    c->stats.kernel_ms += kms;
+    c->stats.launches += launches;
+  }
+  hipFree(d_rows);
+  hipFree(d_param);
+  hipFree(d_st);
+  hipFree(d_list);
+  hipFree(d_tape);
+  hipFree(d_snap);
+  /* the result lines, as deep_run / rc_run write them */
+  uint8_t *htape = NULL;
+  char (*ctr)[RC_SET] = recheck && n ? (char (*)[RC_SET])calloc(20, RC_SET) : NULL;
+  if (!rc && recheck && n && !ctr) rc = E13_ENOMEM;
+  for (int i = 0; !rc && i < n; i++) {
+    const deep_machine *m = &ms[i];
+    char *o = res + (size_t)i * DEEP_OUT;
+    const E13DeepState *s = &st[i];
+    const uint32_t *sn = snap + (size_t)i * (size_t)(nmax + 1) * SW;
+    const int reached = s->n - 1; /* stages whose start was found */
+    if (s->status == 2) { /* the host's loop */
+      if (!htape) htape = (uint8_t *)malloc(recheck ? (size_t)(2 * RC_SIDE + 1) : (size_t)deep_size(nmax, 0));
+      if (!htape) rc = E13_ENOMEM;
+      else if ((recheck ? rc_run(m, i + 1, nmax, htape, ctr, o, DEEP_OUT) : deep_run(m, i + 1, nmax, 0, htape, o, DEEP_OUT)) < 0)
+        rc = E13_EINVAL;
+      continue;
+    }
+    int pos = 0;
+    if (!recheck) {
+      if (appendf(o, DEEP_OUT, &pos, "survivor %d (nq=%d st0=%d%s):", i + 1, m->nq, m->st0, m->rel ? " rel" : "")) rc = E13_EINVAL;
+      for (int k = 1; !rc && k <= reached; k++) {
+        long long cnt, lo;
+        dg_count(sn + (size_t)k * SW, left, &cnt, &lo);
+        if (appendf(o, DEEP_OUT, &pos, " n%d:%lld", k, cnt) || (cnt && appendf(o, DEEP_OUT, &pos, "@%lld", lo)))
+          rc = E13_EINVAL;
+      }
+      if (!rc && s->fail) {
+        const int r = s->fail == 1   ? appendf(o, DEEP_OUT, &pos, " | stage %d NOT reached (limit, t=%lld)", s->fail_n,
+                                               (long long)s->fail_t)
+                      : s->fail == 2 ? appendf(o, DEEP_OUT, &pos, " | stage %d: off tape at %lld", s->fail_n,
+                                               (long long)s->fail_q - left)
+                      : s->fail == 3 ? appendf(o, DEEP_OUT, &pos, " | FREE cell %c%d read before stage %d (t=%lld)",
+                                               'A' + s->fail_st, s->fail_sym, s->fail_n, (long long)s->fail_t)
+                                     : appendf(o, DEEP_OUT, &pos, " | HALT before stage %d (t=%lld)", s->fail_n,
+                                               (long long)s->fail_t);
+        if (r) rc = E13_EINVAL;
+      }
+      if (!rc && appendf(o, DEEP_OUT, &pos, "\n")) rc = E13_EINVAL;
+      continue;
+    }
+    /* the counter re-check's verdict from the counter sets at the starts found */
+    char verdict[2 * RC_SET + 128];
+    verdict[0] = 0;
+    int failed = 0;
+    for (int k = 1; k <= reached && !failed; k++) {
+      dg_set(sn + (size_t)k * SW, left, ctr[k]);
+      const int cc = k <= 2 ? 0 : (k <= 8 ? 1 : 2), first = cc == 0 ? 1 : (cc == 1 ? 3 : 9);
+      if (k != first && strcmp(ctr[k], ctr[first])) {
+        snprintf(verdict, sizeof verdict, "counter changes at n=%d (%s -> %s)", k, ctr[first][0] ? ctr[first] : "none",
+                 ctr[k][0] ? ctr[k] : "none");
+        failed = 1;
+      }
+    }
+    if (!failed && s->fail) {
+      failed = 1;
+      if (s->fail == 1) snprintf(verdict, sizeof verdict, "stage %d not reached (limit)", s->fail_n);
+      else if (s->fail == 3)
+        snprintf(verdict, sizeof verdict, "free cell %c%d read before stage %d", 'A' + s->fail_st, s->fail_sym, s->fail_n);
+      else if (s->fail == 4) snprintf(verdict, sizeof verdict, "halt before stage %d", s->fail_n);
+      else rc = E13_EINVAL; /* off the tape in the re-check is the host's case */
+    }
+    if (!rc && !failed) {
+      if (nmax >= 9 && (!strcmp(ctr[1], ctr[3]) || !strcmp(ctr[3], ctr[9]) || !strcmp(ctr[1], ctr[9])))
+        snprintf(verdict, sizeof verdict, "counter equal for different c");
+      else
+        snprintf(verdict, sizeof verdict, "CONSISTENT to n=%d (c2 counter %s)", nmax, nmax >= 9 ? ctr[9] : "");
+    }
+    if (!rc && appendf(o, DEEP_OUT, &pos, "%d: %s\n", i + 1, verdict)) rc = E13_EINVAL;
+  }
+  free(htape);
+  free(ctr);
+  int64_t pos = 0;
+  for (int i = 0; !rc && i < n; i++) {
+    const size_t k = strlen(res + (size_t)i * DEEP_OUT);
+    if (pos + (int64_t)k >= outlen) {
+      rc = E13_EINVAL;
+      break;
+    }
+    memcpy(out + pos, res + (size_t)i * DEEP_OUT, k);
+    pos += (int64_t)k;
+  }
+  out[pos < outlen ? pos : outlen - 1] = 0;
+  free(ms);
+  free(rows);
+  free(param);
+  free(st);
+  free(list);
+  free(snap);
+  free(res);
+  free(lines);
+  free(copy);
+  return rc ? rc : n;
+}
+
+int e13_deep_check_gpu(e13_ctx *c, const char *text, int nmax, char *out, int64_t outlen) {
+  return deep_gpu(c, text, nmax, 0, out, outlen);
+}
+
+int e13_recheck_counter_gpu(e13_ctx *c, const char *text, int nmax, char *out, int64_t outlen) {
+  return deep_gpu(c, text, nmax, 1, out, outlen);
 }

@@ -237,4 +237,33 @@ typedef struct E13ProbeArgs {
   uint64_t per_root;
 } E13ProbeArgs;
 
+/* Deep stage check / counter re-check of survivors on the GPU (k_deep): one
+   wave per survivor, its whole tape in LDS (W words, cell k at bit k + left),
+   resumable over launches of a bounded number of loop iterations. */
+#define E13_DEEP_MAXQ 32       /* states of a survivor table */
+typedef struct E13DeepState {
+  int64_t t;                   /* steps so far */
+  int64_t limit;               /* step limit of the stage sought */
+  int64_t fail_t;              /* the step count of the failure */
+  int32_t q, st, ones, n;      /* head (cell + left), state, ones at cells >= -2, stage sought */
+  int32_t minq;                /* leftmost head so far */
+  int32_t status;              /* 0 running, 1 finished, 2 left the tape held in LDS (re-check: the host runs it),
+                                  3 dynamic LDS not at address 0 */
+  int32_t fail;                /* 0 every stage reached, 1 step limit, 2 off the tape, 3 free cell read, 4 halt read */
+  int32_t fail_n, fail_q, fail_st, fail_sym, pad;
+} E13DeepState;
+
+typedef struct E13DeepArgs {
+  const uint32_t *rows;        /* per survivor E13_DEEP_MAXQ words: entry(state, 0) | entry(state, 1) << 16 */
+  const int32_t *param;        /* per survivor: st0, head0 + left, relative start (0/1), 0 */
+  E13DeepState *state;
+  uint32_t *tape;              /* per survivor W words */
+  uint32_t *snap;              /* per survivor (nmax + 1) x (left / 32) words: the counter side of the tape at
+                                  each stage start */
+  const int32_t *list;         /* the survivors of this launch, one per workgroup */
+  int32_t nmax, W, left, mode; /* mode 0: deep.c (window -left+2 .. 2^n+2000); 1: recheck.c (the tape held) */
+  int64_t kmul;                /* step limit of stage n: (stage n-1's step) + kmul * 4^n + 100000 */
+  int64_t budget;              /* loop iterations per launch */
+} E13DeepArgs;
+
 #endif

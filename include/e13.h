@@ -322,6 +322,17 @@ int e13_recheck_counter_line(const char *line, int id, int nmax, char *out, int 
    the verdict lines into out in input order; returns the number of lines. */
 int e13_recheck_counter(const char *text, int nmax, int threads, char *out, int64_t outlen);
 
+/* The deep check (src/deep.c:15-84, big = 0) and the counter re-check
+   (src/recheck.c:22-104) on the GPU of ctx, nmax <= 18: the same result lines
+   as e13_deep_check / e13_recheck_counter, in input order; returns the number
+   of survivors.  One wave per survivor with its whole tape in LDS, launches of
+   a bounded number of loop iterations until every survivor is done.  A
+   re-check whose machine leaves the 4096 + 2^nmax + 4096 cells held on the GPU
+   (recheck.c's tape is 4e6 cells each side), and a table with a move to a state
+   it does not have, run on the host instead. */
+int e13_deep_check_gpu(e13_ctx *ctx, const char *text, int nmax, char *out, int64_t outlen);
+int e13_recheck_counter_gpu(e13_ctx *ctx, const char *text, int nmax, char *out, int64_t outlen);
+
 /* Independent check with crafted stage starts (replaces src/check.py): the
    stage starts to 11 from the blank tape with the counter depending only on
    c(n) = #{1 <= k < n : 2^k has no digit 2 in base 3}, then from crafted
