@@ -13,7 +13,8 @@ search: every X in one persistent launch after a breadth-first frontier, plus
 the survivor re-check), with the device's stored autotuned settings
 (erdos-machine-13_amd/tuning/).  Leaves/s and the CPU baseline are reported
 beside it.  Once each, with counts checked: the symmetric 3-state counter
-search (3,078,656,272 leaves), the FMAX=3 search (36,383,467 leaves), the
+search (3,078,656,272 leaves; with the reference program and the GPU side by
+side on a bounded sample of its family, freesym X=13 11..13 D0 G0 L1 M1), the FMAX=3 search (36,383,467 leaves), the
 2^22-table synthetic stress batch (refilling vs plain form, digest checked),
 the counter search in the plain per-leaf-launch form vs the persistent form,
 FMAX=4 in the plain form and with the static, learned, random and
@@ -378,6 +379,54 @@ def cpu_baseline(runs=3, max_s=150):
             "leaves_per_s": leaves / wall, "cpu_model": model, "nproc": ncpu}
 
 
+def counter_cpu_sample(e13, ctx, base, runs=3, max_s=150):
+    """The counter search's family on a bounded sample both ways: the reference
+    program (oracle/_ref/e13search, or the oracle port) on freesym X=13 with
+    states 11..13 symmetric and free cells D0 G0 L1 M1 (96,010,408 leaves,
+    10,716,181,112 steps), all host cores, median of `runs` wall times; the
+    GPU on the same problem beside it, its counts equal to the CPU's."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "e13search")
+    port = os.path.join(ROOT, "oracle", "_build", "e13oracle")
+    exe, kind = (ref, "reference") if os.path.exists(ref) else (port, "port")
+    if not os.path.exists(exe):
+        return None
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    cells = "D0 G0 L1 M1".split()
+    walls, summ = [], None
+    for _ in range(runs):
+        t0 = time.time()
+        try:
+            out = subprocess.run([exe, "freesym", MACHINE, "13", "11", "13"] + cells, capture_output=True, text=True,
+                                 env=env, timeout=max_s)
+        except subprocess.TimeoutExpired:
+            return None
+        walls.append(time.time() - t0)
+        summ = [l for l in out.stderr.splitlines() if "leaves=" in l][0]
+        print(f"bench: counter cpu sample run {len(walls)}: {walls[-1]:.1f} s", file=sys.stderr, flush=True)
+    m = re.search(r"leaves=(\d+) survivors=(\d+) steps=(\d+) .*dead-at-stage\(0\.\.9\):((?: \d+)+)", summ)
+    cpu = (int(m.group(1)), int(m.group(2)), int(m.group(3)), [int(x) for x in m.group(4).split()])
+    wall = sorted(walls)[len(walls) // 2]
+    p = e13.setup(base, 13, free_cells=cells, sym=(11, 13))
+    gw = []
+    for _ in range(runs):
+        ctx.sync()
+        t0 = time.perf_counter()
+        cs, sv, ns = ctx.search_many([p], surv_cap=1024)
+        gw.append(time.perf_counter() - t0)
+    c = cs[0]
+    same = (c.leaves, c.survivors, c.steps, list(c.dead_at_stage)) == cpu
+    if not same:
+        raise SystemExit(f"bench: counter sample parity failure: {c.leaves} {c.steps} != the CPU's {cpu}")
+    gwall = sorted(gw)[len(gw) // 2]
+    model, ncpu = host_cpu()
+    return {"sample": f"freesym X=13 11..13 D0 G0 L1 M1 ({cpu[0]} leaves, {cpu[2]} steps)",
+            "cpu": {"value": cpu[2] / wall, "unit": "steps/s", "cores": threads, "kind": kind,
+                    "walls_s": [round(w, 2) for w in walls], "cpu_model": model, "nproc": ncpu},
+            "gpu": {"value": cpu[2] / gwall, "unit": "steps/s", "walls_s": [round(w, 4) for w in gw]},
+            "counts_equal": same, "gpu_over_cpu": wall / gwall}
+
+
 def gpu_on_cpu_sample(e13, ctx, base, runs=3):
     """The GPU on the CPU baseline's sample (FMAX=4, X=CPU_SAMPLE_X alone):
     median of `runs` search calls, counts checked."""
@@ -505,6 +554,8 @@ def main():
     if not args.no_extra:
         extra.append(extra_workload(e13, ctx, "3-state counter, freesym X=13 10..13 D0 G0 K0 K1 L1 M1", [counter],
                                     3078656272, 319112357176, 61))
+        if not args.no_cpu:
+            extra[-1]["cpu_same_sample"] = counter_cpu_sample(e13, ctx, base)
         print("bench: counter search done", file=sys.stderr, flush=True)
         f3 = [e13.setup(base, X, 3) for X in XS]
         extra.append(extra_workload(e13, ctx, "delete FMAX=3 X=0..13", f3, 36383467, 4006276021, 11))

@@ -63,6 +63,13 @@ fi
 for f in "$HERE"/ref_dump/delete_f*.err; do
   grep 'leaves=' "$f" | sed -E 's/ wall=[0-9.]+s//' > "$f.tmp" && mv "$f.tmp" "${f%.err}.txt" && rm -f "$f"
 done
+# a bounded sample of the 3-state counter search's family (the bench's CPU leg
+# beside it): states 11..13 symmetric, free cells D0 G0 L1 M1; summary (wall
+# time removed) and the sorted survivor lines
+"$BIN/e13search" freesym "$M" 13 11 13 D0 G0 L1 M1 > "$HERE/ref_dump/_cs.out" 2> "$HERE/ref_dump/_cs.err"
+{ grep 'leaves=' "$HERE/ref_dump/_cs.err" | sed -E 's/ wall=[0-9.]+s//'; grep '^SURVIVOR' "$HERE/ref_dump/_cs.out" | sort; } \
+    > "$HERE/ref_dump/freesym_x13_s11_D0G0L1M1.txt"
+rm -f "$HERE/ref_dump/_cs.out" "$HERE/ref_dump/_cs.err"
 "$BIN/e13ref_dump" keys > "$HERE/ref_dump/keys_tgt.txt"
 cat "$REF"/results/1_local_edits/surv_f{2,3,4}.txt "$REF"/results/2_counter_3/surv_modsym.txt \
     "$REF"/results/2_counter_3/surv_mod.txt > "$HERE/ref_dump/_surv_in.txt"

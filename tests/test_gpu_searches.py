@@ -220,3 +220,16 @@ def test_short_lane_tape_checkpoints(e13, base, tmp_path):
         assert lines == survivor_lines(read_text(REF_RESULTS, "2_counter_3_surv_modsym.txt"))
         st = ctx.stats()
         assert st["search_launches"] >= 3 and st["wide_nodes"] > 0
+
+
+def test_counter_family_sample(e13, gpu_ctx, base):
+    """The counter search's family on the bench's CPU sample: freesym X=13
+    with states 11..13 symmetric and free cells D0 G0 L1 M1 -- the compiled
+    reference's counts (96,010,408 leaves, 10,716,181,112 steps) and its 19
+    survivor lines (tests/golden/ref_dump/freesym_x13_s11_D0G0L1M1.txt)."""
+    text = read_text(REF_DUMP, "freesym_x13_s11_D0G0L1M1.txt")
+    say("search: freesym X=13 11..13 D0 G0 L1 M1")
+    p = e13.setup(base, 13, 0, "D0 G0 L1 M1".split(), (11, 13))
+    got, lines, _ = run_one(e13, gpu_ctx, p)
+    same(got, parse_summaries(text)[13])
+    assert lines == sorted(l for l in text.splitlines() if l.startswith("SURVIVOR"))
