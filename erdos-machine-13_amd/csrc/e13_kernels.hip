@@ -1177,6 +1177,16 @@ __device__ __forceinline__ int pt_base(int ptb, int k, int i) {
   return (int)*(const __attribute__((address_space(3))) int8_t *)(size_t)(uint32_t)(ptb + k * PT_WORDS * 4 + i);
 }
 
+// Words W.. tw-1 of a loaded lane tape (tw at most E13_TW), word w at tb + w * 256.
+template <int W> __device__ __forceinline__ void tape_tail(const Ctx &c, int tb, const uint32_t (&tv)[44], int tw) {
+  if constexpr (W < E13_TW) {
+    if (tw > W) {
+      lds_st(c.sb, tb + W * 256, tv[W]);
+      tape_tail<W + 1>(c, tb, tv, tw);
+    }
+  }
+}
+
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -1447,6 +1457,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NARROW
   uint32_t seed = 0; // frontier node the lane's machine descends from
   uint64_t solo_last = ~0ull, mach = 0; // mach: lane slots this wave has handed out (wave-uniform)
   long long top = 0, bot = 0, staged = -1;
+  bool stg_checked = false; // NARROW: the staged node was tested against the lane tape
   int cur_slot = 0, cur_end = 0; // the staged node's child slots still to hand out: [cur_slot, cur_end)
   int busy = 0, requested = 0, mail_in = 1, idle_polls = 0;
   uint64_t last_tag = 0; // tag of the last delivery taken from this wave's mailbox
@@ -1487,14 +1498,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NARROW
         wave_mem_sync();
         stage_node(stg, (const uint32_t *)stk_node(a, wid, tidx), ptb, lane);
         staged = tidx;
+        stg_checked = false;
         const uint32_t sw = (uint32_t)uni((int)stg[15]);
         cur_slot = (int)(sw & 0xFFFFu);
         cur_end = (int)(sw >> 16);
         if (cur_end == 0) cur_end = pt_word(ptb, uni((int)stg[12]), 9);
       }
-      if (NARROW) {
+      if (NARROW && !stg_checked) {
         // a node whose head or written cells lie past the lane's LDS tape goes, with its slot
-        // cursor, to the list of the launch with the whole window
+        // cursor, to the list of the launch with the whole window (tested once per staging)
         const bool past = lane < E13_TW - tw && stg[E13_NODE_W_TAPE + tw + lane] != 0u;
         if (uni((int)stg[0]) > c.qlim || ballot(past) != 0) {
           uint32_t idx = 0;
@@ -1520,6 +1532,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NARROW
           staged = -1;
           continue;
         }
+        stg_checked = true;
       }
       const int sprob = uni((int)stg[12]);
       const int cnt = __popcll(need);
@@ -1594,17 +1607,23 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NARROW
             for (int k4 = 0; k4 < 4; ++k4) rw[k4] = rsrc[k4];
             uint32_t pr = 0;
             if (pend >= 0) pr = ((const uint32_t *)rsrc)[pend >> 1];
+            // the column base as an opaque value: known to be a multiple of 4
+            // under 256 apart from its wave part, the compiler would OR the row
+            // offsets into it and keep 16 addresses live across the loop
+            // instead of folding them into the stores' offsets
+            int rb = c.rbl;
+            asm volatile("" : "+v"(rb));
 #pragma unroll
             for (int k4 = 0; k4 < 4; ++k4) {
-              lds_st(c.sb, c.rbl + (4 * k4 + 0) * 256, rw[k4].x);
-              lds_st(c.sb, c.rbl + (4 * k4 + 1) * 256, rw[k4].y);
-              lds_st(c.sb, c.rbl + (4 * k4 + 2) * 256, rw[k4].z);
-              lds_st(c.sb, c.rbl + (4 * k4 + 3) * 256, rw[k4].w);
+              lds_st(c.sb, rb + (4 * k4 + 0) * 256, rw[k4].x);
+              lds_st(c.sb, rb + (4 * k4 + 1) * 256, rw[k4].y);
+              lds_st(c.sb, rb + (4 * k4 + 2) * 256, rw[k4].z);
+              lds_st(c.sb, rb + (4 * k4 + 3) * 256, rw[k4].w);
             }
             if (pend >= 0) { // val is a move here (a base value or one of 0..4nq-1)
               const uint32_t sh = ((uint32_t)pend & 1u) << 4;
               const uint32_t e = enc_move(val, (uint32_t)pend & 1u);
-              lds_st(c.sb, c.rbl + (pend >> 1) * 256, (pr & ~(0xFFFFu << sh)) | (e << sh));
+              lds_st(c.sb, rb + (pend >> 1) * 256, (pr & ~(0xFFFFu << sh)) | (e << sh));
             }
             // table codes: as stored (words 24..31) or with edits used up, then the pending cell
             {
@@ -1624,22 +1643,26 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NARROW
             }
             // tape: words 32..74 of the stage (word 75, the seed, comes along), the
             // first tw of them (a node that fits the lane's tape is blank past them)
+            uint32_t tv[44];
 #pragma unroll
-            for (int k4 = 0; k4 < 11; k4 += 4) {
-              uint4 tv[4];
+            for (int k4 = 0; k4 < 11; ++k4) {
+              const uint4 v = *(const uint4 *)(stg + E13_NODE_W_TAPE + 4 * k4);
+              tv[4 * k4 + 0] = v.x;
+              tv[4 * k4 + 1] = v.y;
+              tv[4 * k4 + 2] = v.z;
+              tv[4 * k4 + 3] = v.w;
+            }
+            int tb = c.tbl;
+            asm volatile("" : "+v"(tb));
 #pragma unroll
-              for (int j = 0; j < 4; ++j)
-                if (k4 + j < 11) tv[j] = *(const uint4 *)(stg + E13_NODE_W_TAPE + 4 * (k4 + j));
-#pragma unroll
-              for (int j = 0; j < 4; ++j) {
-                const int w = 4 * (k4 + j);
-                if (k4 + j < 11) {
-                  if (w + 0 < tw) lds_st(c.sb, c.tbl + (w + 0) * 256, tv[j].x);
-                  if (w + 1 < tw) lds_st(c.sb, c.tbl + (w + 1) * 256, tv[j].y);
-                  if (w + 2 < tw) lds_st(c.sb, c.tbl + (w + 2) * 256, tv[j].z);
-                  if (w + 3 < tw) lds_st(c.sb, c.tbl + (w + 3) * 256, tv[j].w);
-                }
-              }
+            for (int w = 0; w < (NARROW ? E13_TW_MIN : E13_TW); ++w) lds_st(c.sb, tb + w * 256, tv[w]);
+            // words E13_TW_MIN .. tw-1 of a short lane tape: nested tests on an
+            // opaque copy of tw (structured, so they stay uniform branches, and
+            // made here, so the compiler does not hold 25 test results across the loop)
+            if (NARROW) {
+              int twv = tw;
+              asm volatile("" : "+s"(twv));
+              tape_tail<E13_TW_MIN>(c, tb, tv, twv);
             }
             active = 1;
           }
@@ -1871,6 +1894,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NARROW
       stage_rows(stg, ptb, lane);
       top += n;
       staged = top - 1;
+      stg_checked = false; // a child paused past the lane tape may be on top
       cur_slot = 0;
       cur_end = pt_word(ptb, uni((int)stg[12]), 9);
     }
