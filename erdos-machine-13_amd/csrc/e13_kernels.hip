@@ -1177,6 +1177,21 @@ __device__ __forceinline__ int pt_base(int ptb, int k, int i) {
   return (int)*(const __attribute__((address_space(3))) int8_t *)(size_t)(uint32_t)(ptb + k * PT_WORDS * 4 + i);
 }
 
+// This lane's LDS tape column and tape length as opaque values (when the
+// length is not a constant): a test `w < tw` per word on the plain values is
+// hoisted out of the persistent loop, and the 43 test results held across it
+// spill; an opaque column base lets the words' offsets fold into the reads.
+struct LaneTape {
+  int tb, tw;
+};
+__device__ __forceinline__ LaneTape lane_tape(const Ctx &c) {
+  LaneTape t{c.tbl, c.tw};
+  asm volatile("" : "+v"(t.tb));
+  if (!__builtin_constant_p(t.tw)) asm volatile("" : "+s"(t.tw));
+  __builtin_assume(t.tw >= E13_TW_MIN && t.tw <= E13_TW);
+  return t;
+}
+
 // Words W.. tw-1 of a loaded lane tape (tw at most E13_TW), word w at tb + w * 256.
 template <int W> __device__ __forceinline__ void tape_tail(const Ctx &c, int tb, const uint32_t (&tv)[44], int tw) {
   if constexpr (W < E13_TW) {
@@ -1312,13 +1327,14 @@ __device__ __forceinline__ void stage_own(const Ctx &c, uint32_t *stg, const Lan
   for (int i = 0; i < E13_MAXCELLS / 4; ++i) cw[i] = codes_word(c, nq, i);
 #pragma unroll
   for (int i = 0; i < E13_MAXCELLS / 4; ++i) stg[E13_NODE_W_TAB + i] = cw[i];
+  const LaneTape lt = lane_tape(c);
 #pragma unroll
   for (int w0 = 0; w0 < E13_TW_PAD; w0 += 11) {
     uint32_t tw[11];
 #pragma unroll
     for (int j = 0; j < 11; ++j)
       if (w0 + j < E13_TW_PAD)
-        tw[j] = w0 + j < c.tw ? lds_ld(c.sb, c.tbl + (w0 + j) * 256) : (w0 + j < E13_TW ? 0u : seed);
+        tw[j] = w0 + j < lt.tw ? lds_ld(c.sb, lt.tb + (w0 + j) * 256) : (w0 + j < E13_TW ? 0u : seed);
 #pragma unroll
     for (int j = 0; j < 11; ++j)
       if (w0 + j < E13_TW_PAD) stg[E13_NODE_W_TAPE + w0 + j] = tw[j];
@@ -1352,13 +1368,15 @@ __device__ __forceinline__ void write_node(const Ctx &c, E13Node *o, const Lane 
 #pragma unroll
   for (int i = 0; i < E13_MAXCELLS / 4; ++i) tw[i] = codes_word(c, nq, i);
   uint4 *dst = (uint4 *)o->tape;
+  const LaneTape lt = lane_tape(c);
+#pragma unroll
   for (int w4 = 0; w4 < E13_TW_PAD / 4; ++w4) {
     uint4 v4;
     // words past the lane's LDS tape are blank (the machine never went there); word 43 is the seed
-    v4.x = 4 * w4 + 0 < c.tw ? lds_ld(c.sb, c.tbl + (4 * w4 + 0) * 256) : 0u;
-    v4.y = 4 * w4 + 1 < c.tw ? lds_ld(c.sb, c.tbl + (4 * w4 + 1) * 256) : 0u;
-    v4.z = 4 * w4 + 2 < c.tw ? lds_ld(c.sb, c.tbl + (4 * w4 + 2) * 256) : 0u;
-    v4.w = 4 * w4 + 3 < c.tw ? lds_ld(c.sb, c.tbl + (4 * w4 + 3) * 256) : (4 * w4 + 3 < E13_TW ? 0u : seed);
+    v4.x = 4 * w4 + 0 < lt.tw ? lds_ld(c.sb, lt.tb + (4 * w4 + 0) * 256) : 0u;
+    v4.y = 4 * w4 + 1 < lt.tw ? lds_ld(c.sb, lt.tb + (4 * w4 + 1) * 256) : 0u;
+    v4.z = 4 * w4 + 2 < lt.tw ? lds_ld(c.sb, lt.tb + (4 * w4 + 2) * 256) : 0u;
+    v4.w = 4 * w4 + 3 < lt.tw ? lds_ld(c.sb, lt.tb + (4 * w4 + 3) * 256) : (4 * w4 + 3 < E13_TW ? 0u : seed);
     dst[w4] = v4;
   }
 }
@@ -1652,18 +1670,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NARROW
               tv[4 * k4 + 2] = v.z;
               tv[4 * k4 + 3] = v.w;
             }
-            int tb = c.tbl;
-            asm volatile("" : "+v"(tb));
+            const LaneTape lt = lane_tape(c);
 #pragma unroll
-            for (int w = 0; w < (NARROW ? E13_TW_MIN : E13_TW); ++w) lds_st(c.sb, tb + w * 256, tv[w]);
-            // words E13_TW_MIN .. tw-1 of a short lane tape: nested tests on an
-            // opaque copy of tw (structured, so they stay uniform branches, and
-            // made here, so the compiler does not hold 25 test results across the loop)
-            if (NARROW) {
-              int twv = tw;
-              asm volatile("" : "+s"(twv));
-              tape_tail<E13_TW_MIN>(c, tb, tv, twv);
-            }
+            for (int w = 0; w < E13_TW_MIN; ++w) lds_st(c.sb, lt.tb + w * 256, tv[w]);
+            // words E13_TW_MIN .. tw-1: nested tests (structured, so they stay
+            // uniform branches, and only as many as the tape has words)
+            tape_tail<E13_TW_MIN>(c, lt.tb, tv, lt.tw);
             active = 1;
           }
         }
