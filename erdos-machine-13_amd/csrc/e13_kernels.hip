@@ -1291,10 +1291,11 @@ __device__ void stage_node(uint32_t *stg, const uint32_t *nd, int ptb, int lane)
   stage_rows(stg, ptb, lane);
 }
 
-// This lane's machine as a node record in LDS (the staging area): the same
-// words write_node stores, so the newest child needs no reload from HBM.
-__device__ __forceinline__ void stage_own(const Ctx &c, uint32_t *stg, const Lane &L, int budget, int nintro, int pend,
-                                          int prob, int depth, int nbound, int nq, uint32_t seed) {
+// The newest child as the staged node, without a reload from HBM: words
+// 0..23 (the machine's registers) from its own lane, the same words
+// write_node stores ...
+__device__ __forceinline__ void stage_head(const Ctx &c, uint32_t *stg, const Lane &L, int budget, int nintro, int pend,
+                                           int prob, int depth, int nbound) {
   stg[0] = (uint32_t)L.q;
   stg[1] = (uint32_t)((L.row - c.rbl) >> 8);
   stg[2] = (uint32_t)L.t;
@@ -1319,26 +1320,21 @@ __device__ __forceinline__ void stage_own(const Ctx &c, uint32_t *stg, const Lan
   stg[21] = (uint32_t)(L.ch1 >> 32);
   stg[22] = (uint32_t)L.ch2;
   stg[23] = (uint32_t)(L.ch2 >> 32);
-  // the lane's rows and tape are read into registers before the stage is
-  // written (the compiler cannot tell them apart and would serialise each
-  // read behind the previous write)
-  uint32_t cw[E13_MAXCELLS / 4];
-#pragma unroll
-  for (int i = 0; i < E13_MAXCELLS / 4; ++i) cw[i] = codes_word(c, nq, i);
-#pragma unroll
-  for (int i = 0; i < E13_MAXCELLS / 4; ++i) stg[E13_NODE_W_TAB + i] = cw[i];
+}
+
+// ... and words 24..75 (table codes, tape, seed) from lane src's LDS columns
+// by the whole wave, one word per lane (lanes 0..51): two reads and a store
+// per lane instead of 52 reads and stores by one lane.
+__device__ __forceinline__ void stage_body(const Ctx &c, uint32_t *stg, int src, uint32_t seed, int lane) {
+  const uint32_t sd = (uint32_t)__builtin_amdgcn_readlane((int)seed, src);
   const LaneTape lt = lane_tape(c);
-#pragma unroll
-  for (int w0 = 0; w0 < E13_TW_PAD; w0 += 11) {
-    uint32_t tw[11];
-#pragma unroll
-    for (int j = 0; j < 11; ++j)
-      if (w0 + j < E13_TW_PAD)
-        tw[j] = w0 + j < lt.tw ? lds_ld(c.sb, lt.tb + (w0 + j) * 256) : (w0 + j < E13_TW ? 0u : seed);
-#pragma unroll
-    for (int j = 0; j < 11; ++j)
-      if (w0 + j < E13_TW_PAD) stg[E13_NODE_W_TAPE + w0 + j] = tw[j];
-  }
+  const int d = (src - lane) * 4; // lane src's column from this lane's
+  const int k = lane - E13_MAXCELLS / 4; // tape word of this lane, if it has one
+  uint32_t v = 0u;
+  if (k < 0) v = lds_ld(c.sb, c.cbl + d + lane * 256);
+  else if (k < lt.tw) v = lds_ld(c.sb, lt.tb + d + k * 256);
+  else if (k == E13_TW) v = sd;
+  if (k < E13_TW_PAD) stg[E13_NODE_W_TAB + lane] = v;
 }
 
 // Copy this lane's machine into node record `o` (plain stores).
@@ -1900,9 +1896,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NARROW
         const int r = (int)rank_in(em);
         write_node(c, stk_node(a, wid, top + r), L, budget, nintro, ev == EV_STOP ? evcell : -1, prob, depth, nbound,
                    nq, seed);
-        if (r == n - 1)
-          stage_own(c, stg, L, budget, nintro, ev == EV_STOP ? evcell : -1, prob, depth, nbound, nq, seed);
+        if (r == n - 1) stage_head(c, stg, L, budget, nintro, ev == EV_STOP ? evcell : -1, prob, depth, nbound);
       }
+      stage_body(c, stg, 63 - __builtin_clzll(em), seed, lane);
       stage_rows(stg, ptb, lane);
       top += n;
       staged = top - 1;
