@@ -1292,8 +1292,10 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
     a.stop = c->d_stop;
     a.wave_state = c->d_wave_state;
   }
-  c->trace_waves = c->trace_per_wave = 0;
-  if (tu->trace_events > 0) {
+  /* the trace is of the call's first launch (with a short lane tape the main
+     one; the launches after it run its leftovers) */
+  if (call_launches == 0) c->trace_waves = c->trace_per_wave = 0;
+  if (tu->trace_events > 0 && call_launches == 0) {
     const size_t per = (size_t)tu->trace_events + 1, words = per * (size_t)total_waves;
     if ((rc = e13i_ensure_buf((void **)&c->d_trace, &c->trace_words, words, sizeof(uint64_t)))) return rc;
     HIPCHK(hipMemsetAsync(c->d_trace, 0, words * sizeof(uint64_t), c->stream));

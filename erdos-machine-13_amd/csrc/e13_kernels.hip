@@ -314,7 +314,7 @@ __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int m
     } else {
       gate = L.hb == L.tgt;
     }
-    stop = (int)((e << 16) | (uint32_t)L.q | (uint32_t)(c.qlim - L.q) | (uint32_t)left) < 0 || gate;
+    stop = ((int)((e << 16) | (uint32_t)left) < 0) | ((uint32_t)L.q > (uint32_t)c.qlim) | gate;
   }
   while (!stop) {
     // the step
@@ -345,7 +345,7 @@ __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int m
     } else {
       gate = L.hb == L.tgt;
     }
-    stop = (int)((e << 16) | (uint32_t)L.q | (uint32_t)(c.qlim - L.q) | (uint32_t)left) < 0 || gate;
+    stop = ((int)((e << 16) | (uint32_t)left) < 0) | ((uint32_t)L.q > (uint32_t)c.qlim) | gate;
     // with too few lanes left in the loop (exec is wave-uniform), every lane
     // leaves it; a lane whose step was due makes it in step_once below
     if (CUT) stop |= __popcll(__builtin_amdgcn_read_exec()) < min_live;

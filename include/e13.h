@@ -101,7 +101,8 @@ typedef struct {
   int32_t donate_max;     /* nodes a wave may deliver per read of the hungry map */
   int32_t idle_naps;      /* an idle wave's pause between mailbox polls, in units of s_sleep(8) */
   int32_t mail_every;     /* rounds between mailbox polls of a wave whose lanes still run */
-  int32_t trace_events;   /* > 0: log up to this many scheduling events per wave (e13_get_trace) */
+  int32_t trace_events;   /* > 0: log up to this many scheduling events per wave of a search call's
+                             first launch (e13_get_trace) */
   /* host-loop forms (1, 2) */
   int32_t graph_chain;    /* > 0: iterations (pop, expand, push) per captured hipGraph; 0 = one launch
                              and one host round trip per iteration */
