@@ -17,6 +17,10 @@
 //    every configuration it would see.
 // The reference's order inside a step (deep.c:51-71, recheck.c:64-79): stage
 // start, step limit, window, read (free cell / halt), write, move, state, t++.
+// Those tests run in full only where one can fire; between, a tight loop in
+// 32-bit counts makes the steps and sweeps (a lone wave issues about one
+// scalar instruction per 5 clocks, so the long survivors that end a deep check
+// alone are bound by the length of this loop).
 // A launch runs at most `budget` iterations per survivor, then saves the tape
 // and the registers; the host launches again until every survivor is done.
 
@@ -101,7 +105,9 @@ __global__ void __launch_bounds__(64) k_deep(E13DeepArgs a) {
   const int qb0 = left - 2; // ones are counted at cells >= -2
   int wi = q >> 5;
   uint32_t word = UW(ldw(wi));
-  for (long long it = 0; it < a.budget; ++it) {
+  for (long long it = 0; it < a.budget;) {
+    // ---- the reference's tests on this configuration, in its order (deep.c:51-60,
+    // recheck.c:64-72) ----
     // stage start (deep.c:51-56)
     if ((ones == blk) & (st == st0) & (q == at)) {
       stw(wi, word);
@@ -118,6 +124,7 @@ __global__ void __launch_bounds__(64) k_deep(E13DeepArgs a) {
         limit = t + a.kmul * (1ll << (2 * n)) + 100000;
         at = rel ? h0q + blk - 2 : h0q;
         if (MODE == 0) hi = left + blk + 2000;
+        ++it;
         continue;
       }
     }
@@ -132,76 +139,98 @@ __global__ void __launch_bounds__(64) k_deep(E13DeepArgs a) {
       fail_q = q;
       break;
     }
-    const uint32_t qb = (uint32_t)q & 31u;
-    const uint32_t sym = (word >> qb) & 1u;
-    const uint32_t e = ((uint32_t)__builtin_amdgcn_readlane((int)rowv, st) >> (sym << 4)) & 0xFFFFu;
-    if (e & DE_STOP) { // deep.c:59-60: a free cell never read, or a halt
-      status = 1;
-      fail = ((e >> 13) & 3u) == 2u ? 3 : 4;
-      fail_st = st;
-      fail_sym = (int)sym;
-      break;
+    {
+      const uint32_t sym = (word >> ((uint32_t)q & 31u)) & 1u;
+      const uint32_t e = ((uint32_t)__builtin_amdgcn_readlane((int)rowv, st) >> (sym << 4)) & 0xFFFFu;
+      if (e & DE_STOP) { // deep.c:59-60: a free cell never read, or a halt
+        status = 1;
+        fail = ((e >> 13) & 3u) == 2u ? 3 : 4;
+        fail_st = st;
+        fail_sym = (int)sym;
+        break;
+      }
     }
-    if ((((sym ? sw1 : sw0) >> st) & 1u) != 0u) {
-      // a sweep: k cells of the run of `sym` cells from q in the direction of travel
-      const uint32_t diff = word ^ (0u - sym);
-      int k;
-      bool to_end;
-      if (e & 2u) {
-        const uint32_t up = diff >> qb;
-        to_end = up == 0u;
-        k = up ? __builtin_ctz(up) : 32 - (int)qb;
-      } else {
-        const uint32_t dn = diff << (31u - qb);
-        to_end = dn == 0u;
-        k = dn ? __builtin_clz(dn) : (int)qb + 1;
-      }
-      if (to_end) { // the run may go on: the next 64 words, one per lane
-        stw(wi, word);
-        const int wj = wi + ((e & 2u) ? 1 + lane : -1 - lane);
-        const uint32_t fill = 0u - sym;
-        uint32_t wv = ~fill;
-        if ((wj >= 0) & (wj < W)) wv = ldw(wj);
-        const uint64_t mism = __ballot(wv != fill);
-        if (mism) {
-          const int j = __builtin_ctzll(mism);
-          const uint32_t wd = (uint32_t)__builtin_amdgcn_readlane((int)wv, j) ^ fill;
-          k += 32 * j + ((e & 2u) ? __builtin_ctz(wd) : __builtin_clz(wd));
+    // ---- none fires here: steps and sweeps in a tight loop, in 32-bit counts,
+    // back to the tests above on the stage-start cell, on a cell that stops the
+    // run, and once `room` steps are made: the steps left before the step limit
+    // and before the head could reach a window edge (it moves one cell a step),
+    // so the limit and window tests pass on every configuration in between ----
+    const int room = (int)min(min(limit - t, (long long)min(q - lo, hi - q) + 1), (long long)(1 << 30));
+    const int iters = (int)min(a.budget - it, (long long)(1 << 30));
+    int done = 0, nit = 0;
+    for (;;) {
+      const uint32_t qb = (uint32_t)q & 31u;
+      const uint32_t sym = (word >> qb) & 1u;
+      const uint32_t e = (uint32_t)__builtin_amdgcn_readlane((int)rowv, st) >> (sym << 4);
+      if (e & DE_STOP) break; // (never on the first pass: tested above)
+      int k = 0;
+      uint32_t wv = 0u; // for a run past word wi: word wi + 1 + lane (right) or wi - 1 - lane (left)
+      if ((((sym ? sw1 : sw0) >> st) & 1u) != 0u) {
+        // a sweep: k cells of the run of `sym` cells from q in the direction of travel
+        const uint32_t diff = word ^ (0u - sym);
+        bool to_end;
+        if (e & 2u) {
+          const uint32_t up = diff >> qb;
+          to_end = up == 0u;
+          k = up ? __builtin_ctz(up) : 32 - (int)qb;
         } else {
-          k += 32 * 64;
+          const uint32_t dn = diff << (31u - qb);
+          to_end = dn == 0u;
+          k = dn ? __builtin_clz(dn) : (int)qb + 1;
         }
-      }
-      k = (e & 2u) ? min(k, hi - q) : min(k, q - lo); // stays inside the window
-      if ((long long)k > limit - t) k = (int)(limit - t);
-      if ((ones == blk) & (st == st0)) { // stop on the start cell ahead: the next iteration tests it
-        if ((e & 2u) && at > q) k = min(k, at - q);
-        if (!(e & 2u) && at < q) k = min(k, q - at);
+        if (to_end) { // the run may go on: the next 64 words, one per lane
+          stw(wi, word);
+          const int wj = wi + ((e & 2u) ? 1 + lane : -1 - lane);
+          const uint32_t fill = 0u - sym;
+          wv = ~fill;
+          if ((wj >= 0) & (wj < W)) wv = ldw(wj);
+          const uint64_t mism = __ballot(wv != fill);
+          if (mism) {
+            const int j = __builtin_ctzll(mism);
+            const uint32_t wd = (uint32_t)__builtin_amdgcn_readlane((int)wv, j) ^ fill;
+            k += 32 * j + ((e & 2u) ? __builtin_ctz(wd) : __builtin_clz(wd));
+          } else {
+            k += 32 * 64;
+          }
+        }
+        k = min(k, room - done); // inside the window and the step limit
+        if ((ones == blk) & (st == st0)) { // stop on the start cell ahead: the tests above take it
+          if ((e & 2u) && at > q) k = min(k, at - q);
+          if (!(e & 2u) && at < q) k = min(k, q - at);
+        }
       }
       if (k > 0) {
         q += (e & 2u) ? k : -k;
-        t += k;
+        done += k;
+        minq = min(minq, q);
+        if ((q >> 5) != wi) { // only a run that reached the end of word wi (written back above)
+          const int wn = q >> 5;
+          const int jn = (e & 2u) ? wn - wi - 1 : wi - wn - 1; // lane that read word wn
+          wi = wn;
+          // the sweep wrote nothing: the word is the one read for the run, unless
+          // the run went on past all 64 of them
+          word = jn < 64 ? (uint32_t)__builtin_amdgcn_readlane((int)wv, jn) : UW(ldw(wi));
+        }
+      } else {
+        // one step (deep.c:62-66)
+        const uint32_t w1 = e & 1u;
+        word ^= (w1 ^ sym) << qb;
+        if (q >= qb0) ones += (int)w1 - (int)sym;
+        q += (e & 2u) ? 1 : -1;
+        st = (int)((e >> 8) & 31u);
+        ++done;
         minq = min(minq, q);
         if ((q >> 5) != wi) {
           stw(wi, word);
           wi = q >> 5;
           word = UW(ldw(wi));
         }
-        continue;
       }
+      ++nit;
+      if ((done >= room) | (nit >= iters) | (q == at)) break;
     }
-    // one step (deep.c:62-66)
-    const uint32_t w1 = e & 1u;
-    word ^= (w1 ^ sym) << qb;
-    if (q >= qb0) ones += (int)w1 - (int)sym;
-    q += (e & 2u) ? 1 : -1;
-    st = (int)((e >> 8) & 31u);
-    ++t;
-    minq = min(minq, q);
-    if ((q >> 5) != wi) {
-      stw(wi, word);
-      wi = q >> 5;
-      word = UW(ldw(wi));
-    }
+    t += done;
+    it += nit;
   }
   stw(wi, word);
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
