@@ -1324,15 +1324,32 @@ __device__ __forceinline__ void stage_head(const Ctx &c, uint32_t *stg, const La
 
 // ... and words 24..75 (table codes, tape, seed) from lane src's LDS columns
 // by the whole wave, one word per lane (lanes 0..51): two reads and a store
-// per lane instead of 52 reads and stores by one lane.
-__device__ __forceinline__ void stage_body(const Ctx &c, uint32_t *stg, int src, uint32_t seed, int lane) {
+// per lane instead of 52 reads and stores by one lane.  Without a codes
+// column (c.cbl < 0), lanes 0..7 decode codes word k from rows 2k, 2k+1.
+__device__ __forceinline__ void stage_body(const Ctx &c, uint32_t *stg, int src, uint32_t seed, int nq, int lane) {
   const uint32_t sd = (uint32_t)__builtin_amdgcn_readlane((int)seed, src);
+  const int nqs = __builtin_amdgcn_readlane(nq, src);
   const LaneTape lt = lane_tape(c);
   const int d = (src - lane) * 4; // lane src's column from this lane's
   const int k = lane - E13_MAXCELLS / 4; // tape word of this lane, if it has one
   uint32_t v = 0u;
-  if (k < 0) v = lds_ld(c.sb, c.cbl + d + lane * 256);
-  else if (k < lt.tw) v = lds_ld(c.sb, lt.tb + d + k * 256);
+  if (k < 0) {
+    if (c.cbl >= 0) {
+      v = lds_ld(c.sb, c.cbl + d + lane * 256);
+    } else { // as codes_word, on lane src's rows (rows past its nq read as free)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int r = 2 * lane + h;
+        uint32_t lo = E13_C_FREE, hi = E13_C_FREE;
+        if (r < nqs) {
+          const uint32_t pr = lds_ld(c.sb, c.rbl + d + r * 256);
+          lo = (uint32_t)dec_entry(pr);
+          hi = (uint32_t)dec_entry(pr >> 16);
+        }
+        v |= (lo | (hi << 8)) << (16 * h);
+      }
+    }
+  } else if (k < lt.tw) v = lds_ld(c.sb, lt.tb + d + k * 256);
   else if (k == E13_TW) v = sd;
   if (k < E13_TW_PAD) stg[E13_NODE_W_TAB + lane] = v;
 }
@@ -1385,9 +1402,13 @@ __device__ __forceinline__ void write_node(const Ctx &c, E13Node *o, const Lane 
 // whose head or written cells lie past it is not loaded: it goes, with its
 // slot cursor, to the list a.wide, which the host runs afterwards in a launch
 // with the whole window.  When that list is full the launch stops cleanly
-// (every wave pauses its machines and leaves), so nothing is lost.
+// (every wave pauses its machines and leaves), so nothing is lost.  The
+// NARROW form is built for four waves per SIMD (128 VGPRs: the step loop
+// keeps its registers, the refill and finish spill) and keeps no table codes
+// in LDS (they are decoded from the rule rows where a node or a survivor is
+// written), so four 4-wave workgroups fit a CU with an 18-word tape.
 template <int GATE, bool NARROW>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NARROW ? 3 : 2))) k_search(E13SearchArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NARROW ? 4 : 2))) k_search(E13SearchArgs a) {
   extern __shared__ __align__(16) uint32_t smem[];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
   const int wid = blockIdx.x * nwaves + wave;
@@ -1460,7 +1481,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NARROW
     lds_st(c.sb, c.dlb + i * 4, i <= E13_NS ? (uint32_t)a.pbs[0].dl[i] : 0u);
   c.ckb = c.dlb + PT_DL_WORDS * 4; // 8-byte aligned: every area before it is a multiple of 8 bytes
   fill_ckeys(c.ckb, a.key);
-  c.cbl = c.ckb + CKEY_BYTES + wave * CODES_BYTES + lane * 4;
+  c.cbl = NARROW ? -1 : c.ckb + CKEY_BYTES + wave * CODES_BYTES + lane * 4;
   if (GATE == GATE_HASH) fill_keyb(c.sb, c.keyb, a.key);
   __syncthreads();
 
@@ -1640,7 +1661,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NARROW
               lds_st(c.sb, rb + (pend >> 1) * 256, (pr & ~(0xFFFFu << sh)) | (e << sh));
             }
             // table codes: as stored (words 24..31) or with edits used up, then the pending cell
-            {
+            if (c.cbl >= 0) {
               const uint4 *csrc = (const uint4 *)(stg + ((nb == 0 && nb0 > 0) ? STG_W_CODESB : E13_NODE_W_TAB));
               const uint4 c0 = csrc[0], c1 = csrc[1];
               lds_st(c.sb, c.cbl + 0 * 256, c0.x);
@@ -1898,7 +1919,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NARROW
                    nq, seed);
         if (r == n - 1) stage_head(c, stg, L, budget, nintro, ev == EV_STOP ? evcell : -1, prob, depth, nbound);
       }
-      stage_body(c, stg, 63 - __builtin_clzll(em), seed, lane);
+      stage_body(c, stg, 63 - __builtin_clzll(em), seed, nq, lane);
       stage_rows(stg, ptb, lane);
       top += n;
       staged = top - 1;
@@ -2535,7 +2556,7 @@ size_t e13k_search_lds_bytes(int wg_threads, int gate, int np, int tw) {
   const int nw = wg_threads / WAVE;
   return (size_t)nw * (ROWS_BYTES + (size_t)tw * 256 + STG_WORDS * 4) + (gate == GATE_HASH ? KEYB_WORDS * 8 : 0) +
          (size_t)np * E13_PCNT * 8 + (size_t)np * PT_WORDS * 4 + PT_DL_WORDS * 4 + CKEY_BYTES +
-         (size_t)nw * CODES_BYTES;
+         (size_t)nw * (tw < E13_TW ? 0 : CODES_BYTES); // the short-tape form decodes the rows instead
 }
 
 // Workgroups of k_search that fit on one CU at once (registers, LDS, wave

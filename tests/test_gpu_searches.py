@@ -172,7 +172,8 @@ def check_many(e13, ctx, problems, golden, surv_file):
 
 
 @pytest.mark.parametrize("tune", [dict(wgs_per_cu=2, tape_words=20), dict(wgs_per_cu=3, tape_words=18),
-                                  dict(wgs_per_cu=3, tape_words=-1), dict(wgs_per_cu=1, tape_words=24, slice=64)],
+                                  dict(wgs_per_cu=3, tape_words=-1), dict(wgs_per_cu=4, tape_words=-1),
+                                  dict(wgs_per_cu=1, tape_words=24, slice=64)],
                          ids=lambda t: "-".join(f"{k}{v}" for k, v in t.items()))
 def test_short_lane_tape(e13, base, tune):
     """The persistent search on a short lane tape in LDS (more workgroups per
