@@ -669,7 +669,7 @@ int e13_search(e13_ctx *c, const e13_problem *p, e13_counts *out, e13_survivor *
     if (ckpt) {
       const double now = host_ms();
       int stop = c->ckpt_max_launches > 0 && call_launches >= c->ckpt_max_launches;
-      if (now - last_stop_check > 1000.0) { /* the stop file, at most once a second */
+      if (now - last_stop_check > 100.0) { /* the stop file, at most ten times a second */
         last_stop_check = now;
         stop |= access(stop_path, F_OK) == 0;
       }
@@ -1323,7 +1323,7 @@ int e13_search_many(e13_ctx *c, const e13_problem *ps, int np, e13_counts *out, 
       if (q == hipSuccess) break;
       if (q != hipErrorNotReady) return E13_EHIP;
       const double now = host_ms();
-      if (!raised && now - last_stop_check > 1000.0) {
+      if (!raised && now - last_stop_check > 100.0) {
         last_stop_check = now;
         stop_file = e13i_ckpt_stop_requested(c);
       }

@@ -229,7 +229,7 @@ int e13_sync(e13_ctx *ctx);
    path set, e13_search / e13_search_many save the remaining search to
    <path>.a / <path>.b in turn, the pointer file <path> naming the last complete
    one: every every_sec seconds (<= 0: 1800), when the file <path>.stop appears
-   (checked at most once a second; the file is removed), or after max_launches
+   (checked at most ten times a second; the file is removed), or after max_launches
    launches of this call (> 0; 0 = no limit).  Host-loop forms (1, 2): between
    launches, the node stack, counters and survivor records.  Persistent form
    (3): the launch is stopped cleanly (every wave pauses its running machines as
