@@ -60,13 +60,16 @@ __device__ __forceinline__ uint32_t lds_base(const void *p) {
 }
 
 // The 16-bit rule entry of the cell read on symbol `sym`: write | dir << 1 |
-// (write - sym) << 2 (two-bit signed: the ones-count change of the step, whose
-// low bit is the tape flip) | next << 8; or 0x8F00 | kind << 12 for a halt,
+// (write - sym) << 2 (three-bit signed, bits 2..4: the ones-count change of the
+// step, whose low bit is the tape flip; bits 3 and 4 are both its sign, so a
+// two- or a three-bit signed extract at bit 2 reads it) | next << 8; or
+// 0x8F00 | kind << 12 for a halt,
 // an unread editable cell or a free cell (bit 15: a cell that cannot be
 // stepped, so the step loop finds it in the sign of entry << 16).
 // The entry of a move code (0..59).
 __device__ __forceinline__ uint32_t enc_move(int c, uint32_t sym) {
-  return (uint32_t)(c & 3) | ((((uint32_t)(c & 1) - sym) & 3u) << 2) | ((uint32_t)(c >> 2) << 8);
+  const uint32_t d = ((uint32_t)(c & 1) - sym) & 7u; // the change as a 3-bit two's complement field
+  return (uint32_t)(c & 3) | (d << 2) | ((uint32_t)(c >> 2) << 8);
 }
 __device__ __forceinline__ uint32_t enc_entry(int c, uint32_t sym) {
   if (c < 60) return enc_move(c, sym);
@@ -281,6 +284,27 @@ __device__ __forceinline__ int step_once(Lane &L, const Ctx &c, int &evcell) {
 // min_live lanes of the wave are still in the loop; such a lane makes its due
 // step in step_once (the exact slow path: EV_NONE unless that step ends the
 // machine) and goes on next round, while the wave refills its idle lanes.
+// The step loop's address and entry arithmetic, written with the operations
+// gfx950 issues at twice the rate of three-operand ones (add, and, or, xor and
+// right shifts; left shifts, bit-field extracts, compares and selects take the
+// full four cycles per wave):
+// tape word (q >> 5) of the column tbl: (q & ~31) * 8 + tbl, an and and one
+// shift-add (q & ~31 kept opaque, so it is not folded back into a shift and a
+// mask of q << 3).  q = -1 gives word -1, an address inside the workgroup's LDS.
+__device__ __forceinline__ int tape_word_addr(int q, int tbl) {
+  uint32_t qa = (uint32_t)q & ~31u;
+  asm("" : "+v"(qa));
+  return (int)((qa << 3) + (uint32_t)tbl);
+}
+// The entry of the symbol at q: (word >> q) << 4 has the symbol at bit 4 and
+// zeros below it, so a shift of the pair by its low five bits (the hardware's
+// shift masks them) selects bits 0..15 or 16..31.
+__device__ __forceinline__ uint32_t entry_of(uint32_t pair, uint32_t word, int q) {
+  const uint32_t sh = (word >> ((uint32_t)q & 31u)) << 4;
+  uint32_t e;
+  asm("v_lshrrev_b32 %0, %1, %2" : "=v"(e) : "v"(sh), "v"(pair));
+  return e;
+}
 __device__ __forceinline__ uint64_t pos_key(int row, int q) { return ((uint64_t)(uint32_t)row << 32) | (uint32_t)q; }
 template <int GATE, bool CUT, bool SKIP>
 __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int min_live, int &evcell) {
@@ -292,19 +316,16 @@ __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int m
   int left = t_lim - 1 - L.t;
   // the gate's state and head tests as one 64-bit compare each (row address, head)
   const uint64_t pA = pos_key(L.row0, L.qA), pR = pos_key(L.row0, L.qR);
-  // tape word address (q >> 5) * 256 + tbl as one shift-add and one and-or
-  const uint32_t tbl_lo = (uint32_t)c.tbl & 0xFFu, tbl_hi = (uint32_t)c.tbl - tbl_lo;
   // The exit test in one sign bit: (e << 16) is negative for a cell that
   // cannot be stepped (halt, unread), q | (qlim - q) for a head outside the
   // window or past the lane's LDS tape, left for the deadline/cap or round
   // end; the stage gate is tested beside it.
   // q may be one step outside here (q = -1 or qlim + 1): the address is then
   // word -1 or tw, inside the workgroup's LDS, and the word read is not used.
-  int wa = (int)(((((uint32_t)L.q << 3) + tbl_hi) & 0xFFFFFF00u) | tbl_lo);
+  int wa = tape_word_addr(L.q, c.tbl);
   uint32_t word = lds_ld(c.sb, wa);
   uint32_t pair = lds_ld(c.sb, L.row);
-  // bit extract and shift use the low 5 bits of q (the hardware masks them)
-  uint32_t e = pair >> (__builtin_amdgcn_ubfe(word, (uint32_t)L.q, 1u) << 4);
+  uint32_t e = entry_of(pair, word, L.q);
   bool stop;
   {
     bool gate;
@@ -323,7 +344,9 @@ __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int m
     // for the next step, so the register copy needs no update
     lds_xor(wa, flip << ((uint32_t)L.q & 31u));
     if (GATE == GATE_COUNT) {
-      const int dm = L.q >= E13_QBLK ? __builtin_amdgcn_sbfe((int)e, 2u, 2u) : 0; // write - read
+      // write - read at q >= 61, else 0: a signed extract of width 3 or 0, the
+      // width being the top two bits of 60 - q (both set exactly when q > 60)
+      const int dm = __builtin_amdgcn_sbfe((int)e, 2u, (uint32_t)(E13_QBLK - 1 - L.q) >> 30);
       L.ones += dm;
       if (SKIP) L.dm = dm;
     } else {
@@ -334,10 +357,10 @@ __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int m
     L.row = (int)((e & 0x0F00u) | (uint32_t)c.rbl);
     --left;
     // the next step's reads and exit test
-    wa = (int)(((((uint32_t)L.q << 3) + tbl_hi) & 0xFFFFFF00u) | tbl_lo);
+    wa = tape_word_addr(L.q, c.tbl);
     word = lds_ld(c.sb, wa);
     pair = lds_ld(c.sb, L.row);
-    e = pair >> (__builtin_amdgcn_ubfe(word, (uint32_t)L.q, 1u) << 4);
+    e = entry_of(pair, word, L.q);
     bool gate;
     if (GATE == GATE_COUNT) {
       const uint64_t pk = pos_key(L.row, L.q);
