@@ -83,8 +83,9 @@ def ranks(v):
 # of 5 interleaved runs; run-to-run spread on one box is about 2-4%).
 ORDER_SLOWDOWN_MAX = 1.04
 # The probe-estimated order pays for its probes inside the timed search (16
-# probes from every frontier node at each seeding level, about 60 ms here).
-PROBE_SLOWDOWN_MAX = 1.10
+# probes from every frontier node at each seeding level, about 60 ms here,
+# 7% of a 0.92 s search with the main profile; the run-to-run spread on top).
+PROBE_SLOWDOWN_MAX = 1.15
 
 
 def test_trained_model_ranks_subtrees_and_its_order_is_timed(e13, gpu_ctx, base):
