@@ -89,8 +89,8 @@ __device__ __forceinline__ int dec_entry(uint32_t e) {
 __device__ __forceinline__ int cval(int n) { return (n >= 3) + (n >= 9); } // e13search.c:71
 
 struct Lane {
-  int q, row, t, tstop, deadline, tcap, dlcap;
-  int ones, target, row0, qA, qR;
+  int q, row, t, tstop, tcap, dlcap;
+  int ones;
   // The change of `ones` made by the last step (0: no cell at q >= 61
   // flipped), or 1 when not known.  Before stage 1 the gate has no state or
   // head test, and the block's exactness can change only with `ones`; the
@@ -129,14 +129,26 @@ __device__ __forceinline__ int lane_deadline(const Lane &L, const Ctx &c, int n)
   return c.dlb >= 0 ? (int)lds_ld(c.sb, c.dlb + 4 * n) : c.dl[n];
 }
 
-// Gate registers for target stage nt (after a load or an accepted stage).
+// The gate's targets for stage nt, derived where a round or a slow step needs
+// them (so they hold no registers across the persistent loop): the ones count
+// 2^nt and the stage-1 state's row address and the heads of either start kind
+// (-1000: that kind is ruled out).
+struct GatePos {
+  int target, row0, qA, qR;
+};
+__device__ __forceinline__ GatePos gate_pos(int nt, int st0, int q0, int flags, int rbl) {
+  GatePos g;
+  g.target = 1 << nt;
+  g.row0 = rbl + st0 * 256;
+  g.qA = (flags & 1) ? q0 : -1000;
+  g.qR = (flags & 2) ? q0 - 2 + (1 << nt) : -1000;
+  return g;
+}
+
+// The step count at which target stage nt's deadline/cap test is due (after a
+// load or an accepted stage).
 __device__ __forceinline__ void arm_stage(Lane &L, const Ctx &c) {
-  L.target = 1 << L.nt;
-  L.row0 = c.rbl + L.st0 * 256;
-  L.qA = (L.flags & 1) ? L.q0 : -1000;
-  L.qR = (L.flags & 2) ? L.q0 - 2 + (1 << L.nt) : -1000;
-  L.deadline = lane_deadline(L, c, L.nt);
-  L.tstop = min(L.deadline + 1, L.tcap);
+  L.tstop = min(lane_deadline(L, c, L.nt) + 1, L.tcap);
   L.tgt = c.tgt[L.nt];
 }
 
@@ -227,15 +239,16 @@ __device__ __forceinline__ int stage_event(Lane &L, const Ctx &c) {
 // step, so a lane never stalls on a configuration.
 template <int GATE>
 __device__ __forceinline__ int step_once(Lane &L, const Ctx &c, int &evcell) {
-  const bool hit = GATE == GATE_COUNT ? L.ones == L.target : L.hb == L.tgt;
+  const bool hit = GATE == GATE_COUNT ? L.ones == (1 << L.nt) : L.hb == L.tgt;
   if (hit) {
-    const bool pre = GATE == GATE_HASH || ((L.nt == 1) | ((L.row == L.row0) & ((L.q == L.qA) | (L.q == L.qR))));
+    const GatePos g = gate_pos(L.nt, L.st0, L.q0, L.flags, c.rbl);
+    const bool pre = GATE == GATE_HASH || ((L.nt == 1) | ((L.row == g.row0) & ((L.q == g.qA) | (L.q == g.qR))));
     if (pre) {
       const int ev = stage_event<GATE>(L, c);
       if (ev) return ev;
     }
   }
-  if (L.t >= L.tstop) return L.t > L.deadline ? EV_DEAD : EV_CAP; // deadline (e13search.c:162) or launch cap
+  if (L.t >= L.tstop) return L.t > lane_deadline(L, c, L.nt) ? EV_DEAD : EV_CAP; // deadline (e13search.c:162) or cap
   const uint32_t qb = (uint32_t)L.q & 31u;
   const int wa = ((L.q >> 5) << 8) + c.tbl;
   uint32_t word = lds_ld(c.sb, wa);
@@ -315,7 +328,8 @@ __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int m
   // while left >= 0 (so t = t_lim - 1 - left)
   int left = t_lim - 1 - L.t;
   // the gate's state and head tests as one 64-bit compare each (row address, head)
-  const uint64_t pA = pos_key(L.row0, L.qA), pR = pos_key(L.row0, L.qR);
+  const GatePos g = gate_pos(L.nt, L.st0, L.q0, L.flags, c.rbl);
+  const uint64_t pA = pos_key(g.row0, g.qA), pR = pos_key(g.row0, g.qR);
   // The exit test in one sign bit: (e << 16) is negative for a cell that
   // cannot be stepped (halt, unread), q | (qlim - q) for a head outside the
   // window or past the lane's LDS tape, left for the deadline/cap or round
@@ -331,7 +345,7 @@ __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int m
     bool gate;
     if (GATE == GATE_COUNT) {
       const uint64_t pk = pos_key(L.row, L.q);
-      gate = (L.ones == L.target) & ((SKIP ? nt1 & (L.dm != 0) : nt1) | (pk == pA) | (pk == pR));
+      gate = (L.ones == g.target) & ((SKIP ? nt1 & (L.dm != 0) : nt1) | (pk == pA) | (pk == pR));
     } else {
       gate = L.hb == L.tgt;
     }
@@ -364,7 +378,7 @@ __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int m
     bool gate;
     if (GATE == GATE_COUNT) {
       const uint64_t pk = pos_key(L.row, L.q);
-      gate = (L.ones == L.target) & ((SKIP ? nt1 & (L.dm != 0) : nt1) | (pk == pA) | (pk == pR));
+      gate = (L.ones == g.target) & ((SKIP ? nt1 & (L.dm != 0) : nt1) | (pk == pA) | (pk == pR));
     } else {
       gate = L.hb == L.tgt;
     }
@@ -441,8 +455,9 @@ __device__ __forceinline__ int run_solo(Lane &L, const Ctx &c, int nsteps_in, in
   }
   if (q > c.qlim) return EV_CAP; // a round ended on the step past the lane's LDS tape: paused, goes on wide
   int left = lane_of(L.tstop, sl) - t_round;
-  int target = lane_of(L.target, sl), qA = lane_of(L.qA, sl), qR = lane_of(L.qR, sl);
-  int st0r = (lane_of(L.row0, sl) - rbl) >> 8;
+  GatePos gs = gate_pos(lane_of(L.nt, sl), lane_of(L.st0, sl), lane_of(L.q0, sl), lane_of(L.flags, sl), 0);
+  int target = gs.target, qA = gs.qA, qR = gs.qR;
+  int st0r = lane_of(L.st0, sl);
   bool nt1 = lane_of(L.nt, sl) == 1;
   int ones = lane_of(L.ones, sl);
   // rule row k of lane sl in lane k (k < 16; lanes 16..63 hold copies)
@@ -590,10 +605,11 @@ __device__ __forceinline__ int run_solo(Lane &L, const Ctx &c, int nsteps_in, in
         st = (lane_of(L.row, sl) - rbl) >> 8;
         ones = lane_of(L.ones, sl);
         r = lane_of(L.t, sl) - t_round - 1; // the loop's ++r counts the step step_once made
-        target = lane_of(L.target, sl);
-        qA = lane_of(L.qA, sl);
-        qR = lane_of(L.qR, sl);
-        st0r = (lane_of(L.row0, sl) - rbl) >> 8;
+        gs = gate_pos(lane_of(L.nt, sl), lane_of(L.st0, sl), lane_of(L.q0, sl), lane_of(L.flags, sl), 0);
+        target = gs.target;
+        qA = gs.qA;
+        qR = gs.qR;
+        st0r = lane_of(L.st0, sl);
         nt1 = lane_of(L.nt, sl) == 1;
         left = lane_of(L.tstop, sl) - t_round;
         n_eff = min(nsteps, left);
@@ -860,7 +876,7 @@ __global__ void __launch_bounds__(256) k_expand(E13ExpandArgs a) {
             L.q0 = nd->q0;
             L.flags = nd->flags;
             L.ones = nd->ones;
-            L.hb = nd->hb;
+            L.hb = GATE == GATE_HASH ? nd->hb : 0ull;
             L.ch0 = nd->ch[0];
             L.ch1 = nd->ch[1];
             L.ch2 = nd->ch[2];
@@ -1430,6 +1446,19 @@ __device__ __forceinline__ void write_node(const Ctx &c, E13Node *o, const Lane 
 // keeps its registers, the refill and finish spill) and keeps no table codes
 // in LDS (they are decoded from the rule rows where a node or a survivor is
 // written), so four 4-wave workgroups fit a CU with an 18-word tape.
+// k_search's per-lane machine bookkeeping in one word: problem index (6
+// bits, < E13_MAXPROB), rows nq (6), symmetric states in use (6), binding
+// depth (7) and cells bound (7), each at most E13_MAXCELLS.
+__device__ __forceinline__ uint32_t meta_pack(int prob, int nq, int nintro, int depth, int nbound) {
+  return (uint32_t)prob | ((uint32_t)nq << 6) | ((uint32_t)nintro << 12) | ((uint32_t)depth << 18) |
+         ((uint32_t)nbound << 25);
+}
+__device__ __forceinline__ int meta_prob(uint32_t m) { return (int)(m & 63u); }
+__device__ __forceinline__ int meta_nq(uint32_t m) { return (int)((m >> 6) & 63u); }
+__device__ __forceinline__ int meta_nintro(uint32_t m) { return (int)((m >> 12) & 63u); }
+__device__ __forceinline__ int meta_depth(uint32_t m) { return (int)((m >> 18) & 127u); }
+__device__ __forceinline__ int meta_nbound(uint32_t m) { return (int)(m >> 25); }
+
 template <int GATE, bool NARROW>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NARROW ? 4 : 2))) k_search(E13SearchArgs a) {
   extern __shared__ __align__(16) uint32_t smem[];
@@ -1511,7 +1540,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NARROW
   Lane L;
   L.dlcap = -1;
   L.tcap = 0x7FFFFFFF;
-  int active = 0, t0 = 0, budget = 0, nintro = 0, prob = 0, depth = 0, nbound = 0, nq = 0;
+  int active = 0, t0 = 0, budget = 0;
+  // the machine's problem index, rows, symmetric states in use, binding depth
+  // and cells bound, packed in one register (read where a machine finishes)
+  uint32_t meta = 0;
   uint32_t seed = 0; // frontier node the lane's machine descends from
   uint64_t solo_last = ~0ull, mach = 0; // mach: lane slots this wave has handed out (wave-uniform)
   long long top = 0, bot = 0, staged = -1;
@@ -1628,9 +1660,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NARROW
             }
           }
           if (ok) {
-            prob = sprob;
             seed = stg[E13_NODE_W_SEED];
-            nq = pt_word(ptb, sprob, 8);
             // the machine state, words 0..23 of the staged node
             const uint4 s0 = *(const uint4 *)(stg + 0), s1 = *(const uint4 *)(stg + 4);
             const uint4 s2 = *(const uint4 *)(stg + 8), s3 = *(const uint4 *)(stg + 12);
@@ -1644,7 +1674,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NARROW
             L.q0 = (int)s1.z;
             L.flags = (int)s1.w;
             L.ones = (int)s2.x;
-            L.hb = (uint64_t)s4.x | ((uint64_t)s4.y << 32);
+            // the block hash only passes through the count gate's machines (always 0
+            // there): a constant, so it holds no registers across the loop
+            L.hb = GATE == GATE_HASH ? (uint64_t)s4.x | ((uint64_t)s4.y << 32) : 0ull;
             L.ch0 = (uint64_t)s4.z | ((uint64_t)s4.w << 32);
             L.ch1 = (uint64_t)s5.x | ((uint64_t)s5.y << 32);
             L.ch2 = (uint64_t)s5.z | ((uint64_t)s5.w << 32);
@@ -1652,9 +1684,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NARROW
             arm_stage(L, c);
             t0 = L.t;
             budget = nb;
-            nintro = nin2;
-            depth = (int)s3.y + (pend >= 0);
-            nbound = (int)s3.z + (pend >= 0);
+            meta = meta_pack(sprob, pt_word(ptb, sprob, 8), nin2, (int)s3.y + (pend >= 0), (int)s3.z + (pend >= 0));
             // rows: as stored (A), or with edits used up (B: unread cells keep
             // their value), then the pending cell's entry; every stage word is
             // read before any lane column is written (the compiler cannot
@@ -1892,6 +1922,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NARROW
       }
     }
     const bool fin = active && ev != EV_NONE;
+    const int prob = meta_prob(meta), nq = meta_nq(meta);
     if (fin) {
       const unsigned long long d = (unsigned long long)(L.t - t0);
       w_steps += d;
@@ -1938,6 +1969,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NARROW
         stk_node(a, wid, staged)->next_slot = slot_range(cur_slot, cur_end);
       if (emit) {
         const int r = (int)rank_in(em);
+        const int nintro = meta_nintro(meta), depth = meta_depth(meta), nbound = meta_nbound(meta);
         write_node(c, stk_node(a, wid, top + r), L, budget, nintro, ev == EV_STOP ? evcell : -1, prob, depth, nbound,
                    nq, seed);
         if (r == n - 1) stage_head(c, stg, L, budget, nintro, ev == EV_STOP ? evcell : -1, prob, depth, nbound);
@@ -2445,7 +2477,7 @@ __global__ void __launch_bounds__(256) k_probe(E13ProbeArgs a) {
       L.q0 = (int)lds_ld(c.sb, r0 + 24);
       L.flags = (int)lds_ld(c.sb, r0 + 28);
       L.ones = (int)lds_ld(c.sb, r0 + 32);
-      L.hb = lds_ld64(c, r0 + 36);
+      L.hb = GATE == GATE_HASH ? lds_ld64(c, r0 + 36) : 0ull;
       L.ch0 = lds_ld64(c, r0 + 44);
       L.ch1 = lds_ld64(c, r0 + 52);
       L.ch2 = lds_ld64(c, r0 + 60);
