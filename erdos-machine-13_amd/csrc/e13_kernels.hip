@@ -1459,8 +1459,11 @@ __device__ __forceinline__ int meta_nintro(uint32_t m) { return (int)((m >> 12) 
 __device__ __forceinline__ int meta_depth(uint32_t m) { return (int)((m >> 18) & 127u); }
 __device__ __forceinline__ int meta_nbound(uint32_t m) { return (int)(m >> 25); }
 
-template <int GATE, bool NARROW>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NARROW ? 4 : 2))) k_search(E13SearchArgs a) {
+// CLOCKS: the per-phase wave clock reads (tuning phase_cycles) are compiled in;
+// the short-tape count-gate search is also built without them (k_search), so
+// their accumulators hold no registers in the timed kernel.
+template <int GATE, bool NARROW, bool CLOCKS>
+__device__ __forceinline__ void search_body(const E13SearchArgs &a) {
   extern __shared__ __align__(16) uint32_t smem[];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
   const int wid = blockIdx.x * nwaves + wave;
@@ -1572,8 +1575,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NARROW
   if (busy) TRACE_EV(TR_SEEDED);
   int refresh_in = (wid % 8) + 1;
   uint64_t w_slots = 0, w_rounds = 0, w_takes = 0, w_gives = 0;
-  unsigned long long w_steps = 0;
-  uint64_t cyc0 = 0, cyc1 = 0, cyc2 = 0, cyc3 = 0, ts = stamp_if(a.phase_cycles), tn;
+  uint32_t w_steps = 0;       // this lane's steps since the last fold (below 2^31 + 2^31)
+  uint64_t w_steps_wave = 0;  // the wave's folded steps (uniform)
+  uint64_t cyc0 = 0, cyc1 = 0, cyc2 = 0, cyc3 = 0, ts = (CLOCKS ? stamp_if(a.phase_cycles) : 0ull), tn;
   bool halt_self = false; // NARROW: the list of wide nodes is full; this wave stops (and so do all)
 
   for (;;) {
@@ -1757,7 +1761,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NARROW
         staged = -1;
       }
     }
-    tn = stamp_if(a.phase_cycles);
+    tn = (CLOCKS ? stamp_if(a.phase_cycles) : 0ull);
     cyc0 += tn - ts;
     ts = tn;
     // ---- this wave's mailbox: a donor may have delivered a node (time: pool/idle) ----
@@ -1855,7 +1859,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NARROW
         break;
       }
       for (int z = 0; z < a.idle_naps; ++z) __builtin_amdgcn_s_sleep(8);
-      tn = stamp_if(a.phase_cycles);
+      tn = (CLOCKS ? stamp_if(a.phase_cycles) : 0ull);
       cyc3 += tn - ts;
       ts = tn;
       continue;
@@ -1878,7 +1882,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NARROW
                             (a.evict_after > 0 && top == bot && __popcll(act) >= 2);
       if (lane < a.hungry_words && can_give) hw = ld_agent64(&a.hungry[lane]);
     }
-    tn = stamp_if(a.phase_cycles);
+    tn = (CLOCKS ? stamp_if(a.phase_cycles) : 0ull);
     cyc3 += tn - ts;
     ts = tn;
     // ---- a round of steps (time: stepping) ----
@@ -1900,7 +1904,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NARROW
         else ev = run_slice<GATE, false, false>(L, c, a.slice, 0, evcell);
       }
     }
-    tn = stamp_if(a.phase_cycles);
+    tn = (CLOCKS ? stamp_if(a.phase_cycles) : 0ull);
     cyc1 += tn - ts;
     ts = tn;
     // ---- a clean stop: every running machine is paused as a resumable node, as at
@@ -1925,7 +1929,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NARROW
     const int prob = meta_prob(meta), nq = meta_nq(meta);
     if (fin) {
       const unsigned long long d = (unsigned long long)(L.t - t0);
-      w_steps += d;
+      w_steps += (uint32_t)d;
       if (ev == EV_DEAD) {
         atomicAdd(&pc[prob * E13_PCNT + 0], 1ull);
         atomicAdd(&pc[prob * E13_PCNT + 3 + L.stage], 1ull);
@@ -1954,6 +1958,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NARROW
         if (ev == EV_DEAD || ev == EV_SURV) atomicAdd((unsigned long long *)&a.seed_cnt[2 * seed], 1ull);
         atomicAdd((unsigned long long *)&a.seed_cnt[2 * seed + 1], d);
       }
+    }
+    if (ballot(w_steps >= 0x80000000u)) { // fold before a lane's 32-bit count could wrap (each d is below 2^31)
+      w_steps_wave += wave_sum64(w_steps);
+      w_steps = 0;
     }
     // ---- live children onto the private stack ----
     const bool emit = fin && (ev == EV_STOP || ev == EV_CAP);
@@ -1989,7 +1997,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NARROW
       if (lane == 0) withdrawn = (atomicAnd((unsigned long long *)&a.hungry[wid >> 6], ~bit) & bit) != 0;
       if (uni(withdrawn)) requested = 0; // else a delivery is on its way: keep polling the mailbox
     }
-    tn = stamp_if(a.phase_cycles);
+    tn = (CLOCKS ? stamp_if(a.phase_cycles) : 0ull);
     cyc2 += tn - ts;
     ts = tn;
     if (stopping) { // the staged node's cursor goes with it; the stack is this wave's whole state
@@ -2060,7 +2068,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NARROW
       }
       refresh_in = given > 0 || nz != 0 ? 2 : (int)a.sched_every; // look again soon while waves wait
     }
-    tn = stamp_if(a.phase_cycles);
+    tn = (CLOCKS ? stamp_if(a.phase_cycles) : 0ull);
     cyc3 += tn - ts;
     ts = tn;
   }
@@ -2077,7 +2085,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NARROW
   }
   TRACE_EV(TR_EXIT);
   if (a.trace != nullptr && lane == 0) a.trace[(size_t)wid * (size_t)a.trace_cap] = tr_n;
-  cyc3 += stamp_if(a.phase_cycles) - ts;
+  cyc3 += (CLOCKS ? stamp_if(a.phase_cycles) : 0ull) - ts;
   if (lane == 0) {
     atomicAdd((unsigned long long *)&a.ctr->cyc[0], (unsigned long long)cyc0);
     atomicAdd((unsigned long long *)&a.ctr->cyc[1], (unsigned long long)cyc1);
@@ -2086,11 +2094,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NARROW
     atomicAdd((unsigned long long *)&a.ctr->rounds, (unsigned long long)w_rounds);
     atomicAdd((unsigned long long *)&a.ctr->lane_slots, (unsigned long long)w_slots);
   }
-  const uint64_t s_steps = wave_sum64(w_steps);
+  const uint64_t s_steps = w_steps_wave + wave_sum64(w_steps);
   if (lane == 0) atomicAdd((unsigned long long *)&a.ctr->steps, (unsigned long long)s_steps);
   __syncthreads();
   for (int i = threadIdx.x; i < a.np * E13_PCNT; i += blockDim.x)
     if (pc[i]) atomicAdd((unsigned long long *)&a.pcnt[i], pc[i]);
+}
+
+template <int GATE, bool NARROW>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NARROW ? 4 : 2))) k_search(E13SearchArgs a) {
+  search_body<GATE, NARROW, !(NARROW && GATE == GATE_COUNT)>(a);
+}
+// The short-tape count-gate search with the per-phase clock reads.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_search_clocked(E13SearchArgs a) {
+  search_body<GATE_COUNT, true, true>(a);
 }
 
 // ---------------------------------------------------------------------------
@@ -2631,6 +2648,7 @@ int e13k_launch_search(const E13SearchArgs *a, int grid, int wg_threads, int gat
   if (tw < E13_TW_MIN || tw > E13_TW) return (int)hipErrorInvalidValue;
   const size_t lds = e13k_search_lds_bytes(wg_threads, gate, a->np, tw);
   if (gate == GATE_HASH) hipLaunchKernelGGL((k_search<GATE_HASH, false>), dim3(grid), dim3(wg_threads), lds, s, *a);
+  else if (tw < E13_TW && a->phase_cycles) hipLaunchKernelGGL(k_search_clocked, dim3(grid), dim3(wg_threads), lds, s, *a);
   else if (tw < E13_TW) hipLaunchKernelGGL((k_search<GATE_COUNT, true>), dim3(grid), dim3(wg_threads), lds, s, *a);
   else hipLaunchKernelGGL((k_search<GATE_COUNT, false>), dim3(grid), dim3(wg_threads), lds, s, *a);
   return (int)hipGetLastError();

@@ -223,6 +223,25 @@ def test_short_lane_tape_checkpoints(e13, base, tmp_path):
         assert st["search_launches"] >= 3 and st["wide_nodes"] > 0
 
 
+def test_phase_clocks(e13, base):
+    """The short-tape search built with the per-phase wave clock reads (the
+    kernel the bench's untimed pass runs) gives the reference's counts and
+    survivors for the counter search on the stored main profile, and counts
+    wave cycles in every phase."""
+    want = parse_summaries(read_text(REF_RESULTS, "2_counter_3_log_modsym.txt"))[13]
+    p = e13.setup(base, 13, 0, "D0 G0 K0 K1 L1 M1".split(), (10, 13))
+    with e13.Context(0) as ctx:
+        _, tuning = ctx.search_profile([p])
+        ctx.restore_tuning(dict(tuning, phase_cycles=1))
+        say("search: freesym X=13 10..13 D0 G0 K0 K1 L1 M1, main profile, per-phase clocks")
+        got, lines, _ = run_one(e13, ctx, p)
+        same(got, want)
+        assert lines == survivor_lines(read_text(REF_RESULTS, "2_counter_3_surv_modsym.txt"))
+        st = ctx.stats()
+        assert st["tape_words"] < 43
+        assert all(c > 0 for c in st["cycles"]), st["cycles"]
+
+
 def test_counter_family_sample(e13, gpu_ctx, base):
     """The counter search's family on the bench's CPU sample: freesym X=13
     with states 11..13 symmetric and free cells D0 G0 L1 M1 -- the compiled
