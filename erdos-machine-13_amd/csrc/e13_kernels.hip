@@ -91,18 +91,20 @@ __device__ __forceinline__ int cval(int n) { return (n >= 3) + (n >= 9); } // e1
 struct Lane {
   int q, row, t, tstop, tcap, dlcap;
   int ones;
-  // The change of `ones` made by the last step (0: no cell at q >= 61
-  // flipped), or 1 when not known.  Before stage 1 the gate has no state or
-  // head test, and the block's exactness can change only with `ones`; the
-  // step loop's SKIP form therefore tests the stage-1 gate only after a step
-  // with dm != 0 (the gate of a configuration reached without such a step was
-  // already settled), so a machine whose two ones sit in the wrong cells does
-  // not leave the loop at every step.
-  int dm;
+  // flags: bit 0 abs_ok, bit 1 rel_ok, bits 2..4 cs[0..2] (the counter hash
+  // at that class is set), bit 5 (F_DM) the last step changed `ones` (a cell
+  // at q >= 61 flipped), or set when not known.  Before stage 1 the gate has
+  // no state or head test, and the block's exactness can change only with
+  // `ones`; the step loop's SKIP form therefore tests the stage-1 gate only
+  // after a step with F_DM (the gate of a configuration reached without such
+  // a step was already settled), so a machine whose two ones sit in the wrong
+  // cells does not leave the loop at every step.
   int nt, stage, st0, q0, flags;
   uint64_t hb, tgt;
   uint64_t ch0, ch1, ch2;
 };
+
+#define F_DM 32
 
 struct Ctx {
   char *sb;
@@ -265,8 +267,9 @@ __device__ __forceinline__ int step_once(Lane &L, const Ctx &c, int &evcell) {
   word ^= (w ^ sym) << qb;
   lds_st(c.sb, wa, word);
   if (GATE == GATE_COUNT) {
-    L.dm = L.q >= E13_QBLK ? (int)w - (int)sym : 0;
-    L.ones += L.dm;
+    const int dm = L.q >= E13_QBLK ? (int)w - (int)sym : 0;
+    L.ones += dm;
+    L.flags = (L.flags & ~F_DM) | (dm != 0 ? F_DM : 0);
   } else {
     const uint64_t kb = *(const uint64_t *)(c.sb + c.keyb + L.q * 8);
     L.hb ^= kb & (0ull - (uint64_t)(w ^ sym));
@@ -292,7 +295,7 @@ __device__ __forceinline__ int step_once(Lane &L, const Ctx &c, int &evcell) {
 // has fewer.
 //
 // SKIP: the stage-1 gate is tested only after a step that flipped a cell at
-// q >= 61 (Lane.dm).
+// q >= 61 (F_DM in Lane.flags).
 // CUT: the round also ends, for every lane still running, once fewer than
 // min_live lanes of the wave are still in the loop; such a lane makes its due
 // step in step_once (the exact slow path: EV_NONE unless that step ends the
@@ -330,6 +333,7 @@ __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int m
   // the gate's state and head tests as one 64-bit compare each (row address, head)
   const GatePos g = gate_pos(L.nt, L.st0, L.q0, L.flags, c.rbl);
   const uint64_t pA = pos_key(g.row0, g.qA), pR = pos_key(g.row0, g.qR);
+  int dmv = L.flags & F_DM; // nonzero: the last step changed the ones count
   // The exit test in one sign bit: (e << 16) is negative for a cell that
   // cannot be stepped (halt, unread), q | (qlim - q) for a head outside the
   // window or past the lane's LDS tape, left for the deadline/cap or round
@@ -345,7 +349,7 @@ __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int m
     bool gate;
     if (GATE == GATE_COUNT) {
       const uint64_t pk = pos_key(L.row, L.q);
-      gate = (L.ones == g.target) & ((SKIP ? nt1 & (L.dm != 0) : nt1) | (pk == pA) | (pk == pR));
+      gate = (L.ones == g.target) & ((SKIP ? nt1 & (dmv != 0) : nt1) | (pk == pA) | (pk == pR));
     } else {
       gate = L.hb == L.tgt;
     }
@@ -362,7 +366,7 @@ __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int m
       // width being the top two bits of 60 - q (both set exactly when q > 60)
       const int dm = __builtin_amdgcn_sbfe((int)e, 2u, (uint32_t)(E13_QBLK - 1 - L.q) >> 30);
       L.ones += dm;
-      if (SKIP) L.dm = dm;
+      if (SKIP) dmv = dm;
     } else {
       const uint64_t kb = *(const uint64_t *)(c.sb + c.keyb + L.q * 8);
       L.hb ^= kb & (0ull - (uint64_t)flip);
@@ -378,7 +382,7 @@ __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int m
     bool gate;
     if (GATE == GATE_COUNT) {
       const uint64_t pk = pos_key(L.row, L.q);
-      gate = (L.ones == g.target) & ((SKIP ? nt1 & (L.dm != 0) : nt1) | (pk == pA) | (pk == pR));
+      gate = (L.ones == g.target) & ((SKIP ? nt1 & (dmv != 0) : nt1) | (pk == pA) | (pk == pR));
     } else {
       gate = L.hb == L.tgt;
     }
@@ -387,6 +391,7 @@ __device__ __forceinline__ int run_slice(Lane &L, const Ctx &c, int slice, int m
     // leaves it; a lane whose step was due makes it in step_once below
     if (CUT) stop |= __popcll(__builtin_amdgcn_read_exec()) < min_live;
   }
+  if (SKIP) L.flags = (L.flags & ~F_DM) | (dmv != 0 ? F_DM : 0);
   const int t = t_lim - 1 - left;
   L.t = t;
   if (t == t_end) return EV_NONE; // a full round (a deadline due exactly now is tested next round)
@@ -652,7 +657,7 @@ __device__ __forceinline__ int run_solo(Lane &L, const Ctx &c, int nsteps_in, in
     L.row = rbl + st * 256;
     L.ones = ones;
     L.t = t_round + r;
-    L.dm = 1; // the loop's steps are not tracked: the gate is tested again
+    L.flags |= F_DM; // the loop's steps are not tracked: the gate is tested again
     if (why == 2) ev = EV_DEAD;
     else if (why == 3) ev = EV_CAP;
     else if (r >= left) ev = step_once<GATE_COUNT>(L, c, evcell); // the deadline or cap is due
@@ -680,7 +685,7 @@ __device__ __forceinline__ uint64_t stamp_if(int on) { return on ? __builtin_rea
 enum { TR_SEEDED = 1, TR_TAKE = 2, TR_RUNNING = 3, TR_IDLE = 4, TR_EXIT = 5, TR_GIVE = 6, TR_ROUND = 7 };
 #define TRACE_EV2(ev, aux)                                                                            \
   do {                                                                                                \
-    if (a.trace != nullptr && lane == 0) {                                                            \
+    if (CLOCKS && a.trace != nullptr && lane == 0) {                                                  \
       if (tr_n + 1u < (uint32_t)a.trace_cap)                                                          \
         a.trace[(size_t)wid * (size_t)a.trace_cap + 1u + tr_n] =                                      \
             (__builtin_readsteadycounter() << 16) | ((uint64_t)((aux) & 0xFF) << 8) | (ev);           \
@@ -881,7 +886,7 @@ __global__ void __launch_bounds__(256) k_expand(E13ExpandArgs a) {
             L.ch1 = nd->ch[1];
             L.ch2 = nd->ch[2];
             L.tcap = L.t + a.cap;
-            L.dm = 1; // not known: the first step tests the gate
+            L.flags |= F_DM; // not known: the first step tests the gate
             arm_stage(L, c);
             t0 = L.t;
             budget = nb;
@@ -1091,7 +1096,7 @@ __global__ void __launch_bounds__(256) k_batch(E13BatchArgs a) {
         L.ch0 = L.ch1 = L.ch2 = 0;
         L.dlcap = a.caps ? a.caps[k] : -1;
         L.tcap = 0x7FFFFFFF;
-        L.dm = 1; // not known: the first step tests the gate
+        L.flags |= F_DM; // not known: the first step tests the gate
         arm_stage(L, c);
         active = 1;
       }
@@ -1459,9 +1464,10 @@ __device__ __forceinline__ int meta_nintro(uint32_t m) { return (int)((m >> 12) 
 __device__ __forceinline__ int meta_depth(uint32_t m) { return (int)((m >> 18) & 127u); }
 __device__ __forceinline__ int meta_nbound(uint32_t m) { return (int)(m >> 25); }
 
-// CLOCKS: the per-phase wave clock reads (tuning phase_cycles) are compiled in;
-// the short-tape count-gate search is also built without them (k_search), so
-// their accumulators hold no registers in the timed kernel.
+// CLOCKS: the per-phase wave clock reads (tuning phase_cycles) and the wave
+// event trace (trace_events) are compiled in; the short-tape count-gate search
+// is also built without them (k_search), so their accumulators and the trace
+// cursor hold no registers in the timed kernel.
 template <int GATE, bool NARROW, bool CLOCKS>
 __device__ __forceinline__ void search_body(const E13SearchArgs &a) {
   extern __shared__ __align__(16) uint32_t smem[];
@@ -1684,7 +1690,7 @@ __device__ __forceinline__ void search_body(const E13SearchArgs &a) {
             L.ch0 = (uint64_t)s4.z | ((uint64_t)s4.w << 32);
             L.ch1 = (uint64_t)s5.x | ((uint64_t)s5.y << 32);
             L.ch2 = (uint64_t)s5.z | ((uint64_t)s5.w << 32);
-            L.dm = 1; // not known: the first step tests the gate
+            L.flags |= F_DM; // not known: the first step tests the gate
             arm_stage(L, c);
             t0 = L.t;
             budget = nb;
@@ -2084,7 +2090,7 @@ __device__ __forceinline__ void search_body(const E13SearchArgs &a) {
     atomicAdd(&a.sch->pool_takes, (int)w_takes);
   }
   TRACE_EV(TR_EXIT);
-  if (a.trace != nullptr && lane == 0) a.trace[(size_t)wid * (size_t)a.trace_cap] = tr_n;
+  if (CLOCKS && a.trace != nullptr && lane == 0) a.trace[(size_t)wid * (size_t)a.trace_cap] = tr_n;
   cyc3 += (CLOCKS ? stamp_if(a.phase_cycles) : 0ull) - ts;
   if (lane == 0) {
     atomicAdd((unsigned long long *)&a.ctr->cyc[0], (unsigned long long)cyc0);
@@ -2105,7 +2111,7 @@ template <int GATE, bool NARROW>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NARROW ? 4 : 2))) k_search(E13SearchArgs a) {
   search_body<GATE, NARROW, !(NARROW && GATE == GATE_COUNT)>(a);
 }
-// The short-tape count-gate search with the per-phase clock reads.
+// The short-tape count-gate search with the per-phase clock reads and the trace.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_search_clocked(E13SearchArgs a) {
   search_body<GATE_COUNT, true, true>(a);
 }
@@ -2498,7 +2504,7 @@ __global__ void __launch_bounds__(256) k_probe(E13ProbeArgs a) {
       L.ch0 = lds_ld64(c, r0 + 44);
       L.ch1 = lds_ld64(c, r0 + 52);
       L.ch2 = lds_ld64(c, r0 + 60);
-      L.dm = 1; // not known: the first step tests the gate
+      L.flags |= F_DM; // not known: the first step tests the gate
       arm_stage(L, c);
       t0 = L.t;
       active = 1;
@@ -2648,7 +2654,8 @@ int e13k_launch_search(const E13SearchArgs *a, int grid, int wg_threads, int gat
   if (tw < E13_TW_MIN || tw > E13_TW) return (int)hipErrorInvalidValue;
   const size_t lds = e13k_search_lds_bytes(wg_threads, gate, a->np, tw);
   if (gate == GATE_HASH) hipLaunchKernelGGL((k_search<GATE_HASH, false>), dim3(grid), dim3(wg_threads), lds, s, *a);
-  else if (tw < E13_TW && a->phase_cycles) hipLaunchKernelGGL(k_search_clocked, dim3(grid), dim3(wg_threads), lds, s, *a);
+  else if (tw < E13_TW && (a->phase_cycles || a->trace != nullptr))
+    hipLaunchKernelGGL(k_search_clocked, dim3(grid), dim3(wg_threads), lds, s, *a);
   else if (tw < E13_TW) hipLaunchKernelGGL((k_search<GATE_COUNT, true>), dim3(grid), dim3(wg_threads), lds, s, *a);
   else hipLaunchKernelGGL((k_search<GATE_COUNT, false>), dim3(grid), dim3(wg_threads), lds, s, *a);
   return (int)hipGetLastError();
