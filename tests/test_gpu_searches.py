@@ -232,14 +232,28 @@ def test_phase_clocks(e13, base):
     p = e13.setup(base, 13, 0, "D0 G0 K0 K1 L1 M1".split(), (10, 13))
     with e13.Context(0) as ctx:
         _, tuning = ctx.search_profile([p])
-        ctx.restore_tuning(dict(tuning, phase_cycles=1))
-        say("search: freesym X=13 10..13 D0 G0 K0 K1 L1 M1, main profile, per-phase clocks")
+        ctx.restore_tuning(dict(tuning, phase_cycles=1, trace_events=4096))
+        say("search: freesym X=13 10..13 D0 G0 K0 K1 L1 M1, main profile, per-phase clocks and event trace")
         got, lines, _ = run_one(e13, ctx, p)
         same(got, want)
         assert lines == survivor_lines(read_text(REF_RESULTS, "2_counter_3_surv_modsym.txt"))
         st = ctx.stats()
         assert st["tape_words"] < 43
         assert all(c > 0 for c in st["cycles"]), st["cycles"]
+        # the event trace of the short-tape launch: every wave logged events and,
+        # where its log did not fill, ended with its exit event (5)
+        import ctypes
+        L = e13.lib()
+        waves, per = ctypes.c_int32(), ctypes.c_int32()
+        assert L.e13_get_trace(ctx._h, None, 0, ctypes.byref(waves), ctypes.byref(per)) == 0
+        assert waves.value > 0 and per.value == 4097
+        buf = (ctypes.c_uint64 * (waves.value * per.value))()
+        assert L.e13_get_trace(ctx._h, buf, len(buf), ctypes.byref(waves), ctypes.byref(per)) == 0
+        for w in range(waves.value):
+            n = buf[w * per.value]
+            assert n >= 1, w
+            if n < per.value - 1:
+                assert buf[w * per.value + n] & 0xFF == 5, (w, n)
 
 
 def test_counter_family_sample(e13, gpu_ctx, base):
